@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Headline benchmark: batched FP32 parallel tempering of the pyrochlore
+J1-J2-J3 Heisenberg model on one MI355X.
+
+Workload (the reference's tempering sweep, temper.cpp:418-474, with all
+replicas on one GPU): L = 12 (27,648 spins), 64 replicas at log-spaced T in
+[0.02, 2.0], J1 = 1, J2 = -0.05, J3 = 0.05, B = 0; each sweep is one
+Gaussian-kick Metropolis pass + two overrelaxation passes; one step = one
+tempering round = 4 sweeps, one SSF sample of all 9 correlators at every
+rung, the energy sample and the neighbour-pair exchange.
+Metric: site-updates/s = N * R * 3 passes * 4 sweeps * steps / wall time.
+
+Also reported: the plain FP64 one-replica GPU path on the same lattice and
+sweep (the north-star comparison), the sweep kernel's HBM roofline and the
+CPU restatement of the reference sweep timed on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+L_LAT = 12
+R_REP = 64
+J123 = (1.0, -0.05, 0.05)
+T_RANGE = (2.0, 0.02)
+N_OR = 2
+N_SWEEP = 4
+CORR = ("xx", "yy", "zz", "xy", "xz", "yz", "yx", "zx", "zy")
+PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E spec
+
+
+def bytes_per_update(prec_bytes: int, n_nbr: int, r_share: int, lifted_frac: float = 0.0) -> float:
+    """Compulsory HBM bytes per site-update: read + write own spin (3
+    components), the int32 neighbour row amortised over the replicas that
+    read it in one launch, and the lifted direction byte (read + write)."""
+    return 2 * 3 * prec_bytes + 4.0 * n_nbr / r_share + 2.0 * lifted_frac
+
+
+def make_c3(pyro, precision, R, seed=2024):
+    ctx = pyro.Context.pyrochlore(L_LAT, n_replicas=R, precision=precision, seed=seed)
+    ctx.define_J1J2J3(*J123)
+    ctx.setup()
+    ctx.randomize(seed + 1)
+    ladder = np.geomspace(T_RANGE[0], T_RANGE[1], R) if R > 1 else np.array([T_RANGE[1]])
+    ctx.pt_init(ladder, swap_seed=seed + 2)
+    return ctx
+
+
+def time_rounds(ctx, pyro, steps, warmup, sample_ssf=True):
+    ctx.pt_rounds(pyro.METROPOLIS, N_OR, N_SWEEP, sample_ssf, warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.mark(0)
+    ctx.pt_rounds(pyro.METROPOLIS, N_OR, N_SWEEP, sample_ssf, steps)
+    ctx.mark(1)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ctx.elapsed_ms() / 1e3
+
+
+def cpu_baseline(seconds: float = 12.0):
+    """The oracle's restatement of the reference sweep (sequential, AoS
+    spins with bond-shell pointer lists, Xoroshiro128++ + libstdc++
+    distributions; MC.cpp:87-163) on one host core, one L = 12 replica,
+    Metropolis + 2 OR per sweep at T = 0.5."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    o = pyoracle.Oracle.pyrochlore(L_LAT, *J123, seed=7)
+    rng = np.random.default_rng(0)
+    v = rng.normal(size=(o.N, 3))
+    o.set_spins(v / np.linalg.norm(v, axis=1, keepdims=True))
+    o.sweep_ref(0, 0.5, N_OR)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.sweep_ref(0, 0.5, N_OR)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt > seconds:
+            break
+    return {
+        "value": o.N * (1 + N_OR) * n / dt,
+        "unit": "site-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n} sweeps (Metropolis + {N_OR} OR) of one L={L_LAT} replica "
+                  f"({o.N} spins) at T=0.5, {dt:.1f} s on one core, "
+                  f"host {os.cpu_count()} logical CPUs",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1024)
+    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--strategy", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the tempering "
+              "path runs all replicas on a single device.", file=sys.stderr)
+        sys.exit(2)
+
+    import pyro_cmc_amd as pyro
+
+    if pyro.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    # ---- headline: batched FP32 tempering ----
+    ctx = make_c3(pyro, pyro.FP32, R_REP)
+    ctx.ssf_configure(CORR, n_slots=R_REP, store_error_term=True)
+    ctx.set_strategy(args.strategy)
+    wall, gpu_s = time_rounds(ctx, pyro, args.steps, args.warmup)
+    N = ctx.N
+    updates = N * R_REP * (1 + N_OR) * N_SWEEP * args.steps
+    value = updates / wall
+    strategy = ctx.strategy()
+    info = ctx.info()
+
+    # ---- per-kernel timing with HIP events on the context stream ----
+    prof_steps = max(8, min(args.steps, 256))
+    ctx.profile(True)
+    ctx.pt_rounds(pyro.METROPOLIS, N_OR, N_SWEEP, True, prof_steps)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    sweep_ms, sweep_launches = prof["sweep"]
+    prof_updates = N * R_REP * (1 + N_OR) * N_SWEEP * prof_steps
+    upd_per_launch = prof_updates / sweep_launches
+    avg_launch_s = sweep_ms / 1e3 / sweep_launches
+    b_alg = bytes_per_update(4, info["n_slot"], R_REP)
+    achieved = b_alg * upd_per_launch / avg_launch_s
+    st = ctx.pt_stats()
+    acc_rate = (st["accepts"] / np.maximum(st["attempts"], 1)).tolist()
+
+    # ---- FP64 one-replica GPU path, same lattice and sweep ----
+    fp64 = None
+    if not args.no_fp64:
+        c1 = make_c3(pyro, pyro.FP64, 1)
+        steps64 = max(4, args.steps // 16)
+        w64, _ = time_rounds(c1, pyro, steps64, 4, sample_ssf=False)
+        fp64 = c1.N * (1 + N_OR) * N_SWEEP * steps64 / w64
+        c1.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+
+    line = {
+        "metric": "site-updates/s (Metropolis+OR) at L=12x64 replicas",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: spins uniform on the sphere from the seed",
+        "config": {
+            "workload": "parallel tempering, pyrochlore L=12 (27648 spins) x 64 replicas, "
+                        "T log-spaced 2.0..0.02, J1=1 J2=-0.05 J3=0.05, sweep = 1 Metropolis + "
+                        "2 OR, exchange every 4 sweeps, SSF (9 correlators) every round",
+            "L": L_LAT, "replicas": R_REP, "passes_per_sweep": 1 + N_OR,
+            "sweeps_per_step": N_SWEEP, "strategy": strategy,
+        },
+        "gpu_event_s": gpu_s,
+        "fp64_one_replica_site_updates_per_s": fp64,
+        "speedup_vs_fp64_one_replica": (value / fp64) if fp64 else None,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM,
+            "traffic": None,
+            "kernel": "sweep (" + ("resident workgroup per replica" if strategy == 2 else "one launch per colour") + ")",
+            "bytes_per_update": b_alg,
+            "updates_per_launch": upd_per_launch,
+            "avg_launch_us": avg_launch_s * 1e6,
+        },
+        "kernel_ms": {k: v[0] for k, v in prof.items()},
+        "kernel_launches": {k: v[1] for k, v in prof.items()},
+        "swap_acceptance_min": float(min(acc_rate)) if acc_rate else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

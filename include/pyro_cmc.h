@@ -1,0 +1,234 @@
+/*
+ * pyro_cmc.h — C ABI of the MI355X sweep-and-measure library (libpyro_cmc.so).
+ *
+ * Plain C types only: opaque context, plain pointers and sizes, int status
+ * codes.  No exception crosses this boundary; every entry point returns
+ * PYRO_OK (0) or a non-zero status, and pyro_last_error() returns the
+ * message of the last failure on the calling thread.
+ *
+ * One context = one lattice geometry + coupling set, R replicas of it in
+ * device memory (HBM), one HIP stream.  A context is not thread-safe.
+ * Host buffers are caller-owned; device memory is context-owned.
+ *
+ * Site order everywhere is the reference's storage order: site index
+ * i = sl * n_cells + cell, cell = (n0 * L1 + n1) * L2 + n2.
+ * Host spin buffers are [N][3] doubles (x, y, z per site), as in the
+ * reference's HeisenbergSpin::S.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the pyro_cmc source tree).
+ */
+#ifndef PYRO_CMC_H
+#define PYRO_CMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pyro_ctx pyro_ctx;
+
+/* status codes */
+#define PYRO_OK 0
+#define PYRO_E_INVALID 1   /* bad argument: maps to std::invalid_argument */
+#define PYRO_E_STATE 2     /* call out of order (e.g. sweep before setup) */
+#define PYRO_E_LOGIC 3     /* duplicate coupling name: std::logic_error */
+#define PYRO_E_HIP 4       /* HIP runtime failure: std::runtime_error */
+#define PYRO_E_UNSUPPORTED 5
+
+/* precision of spins, local fields and acceptance; accumulators are FP64 */
+#define PYRO_FP32 32
+#define PYRO_FP64 64
+
+/* single-site update kinds */
+#define PYRO_METROPOLIS 0  /* MC.cpp:111-130  Gaussian-kick Metropolis      */
+#define PYRO_LIFTED 1      /* MC.cpp:249-281  lifted (displacement) Metropolis */
+#define PYRO_LIFTED_ROT 2  /* MC.cpp:194-244  lifted rotation Metropolis      */
+#define PYRO_OVERRELAX 3   /* MC.cpp:145-153  random-angle rotation about h   */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pyro_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int pyro_version(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int pyro_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* Construction.                                                        */
+/* Replaces: build_supercell(UnitCellSpecifier, imat33) + MC_runner    */
+/* ctor (MC.hpp:103-108), PyroCubicCell (pyrochlore_geometry.hpp:12-32) */
+/* Unit cell: orthorhombic with integer edges cell[3]; n_sl sites at    */
+/* integer positions sl_pos[n_sl][3]; sl_kind[s] is the reference's     */
+/* pyro_sl (which displacement list of a coupling applies, and which    */
+/* endpoint of a bond is the source).  Supercell: L[0] x L[1] x L[2].   */
+/* precision: PYRO_FP32 or PYRO_FP64.  seed keys the counter-based RNG. */
+/* ------------------------------------------------------------------ */
+int pyro_create(int n_sl, const int32_t* sl_pos, const int32_t* sl_kind,
+                const int32_t cell[3], const int32_t L[3], int n_replicas,
+                int precision, uint64_t seed, pyro_ctx** out);
+int pyro_destroy(pyro_ctx* ctx);
+
+/* MC_runner::define_coupling (MC.hpp:122-125, MC.cpp:56-66).
+ * rel_vecs holds, for kind k = 0..n_kinds-1, counts[k] displacement
+ * vectors (int32 triples) back to back.  J is a row-major 3x3 matrix
+ * applied by the source endpoint (lower pyro_sl); the target endpoint
+ * applies J^T.  Duplicate names give PYRO_E_LOGIC. */
+int pyro_define_coupling(pyro_ctx* ctx, const char* name, int n_kinds,
+                         const int32_t* counts, const int32_t* rel_vecs,
+                         const double J[9]);
+
+/* Convenience: the nine couplings of build_J1J2J3_h (cli_bits.hpp:144-207):
+ * J1 as six sublattice-pair XXZ specs J1*[I + (Delta-1) z_mu z_nu^T], then
+ * J2, J3a, J3b isotropic.  Requires the 16-site pyrochlore cubic cell. */
+int pyro_define_J1J2J3(pyro_ctx* ctx, double J1, double J2, double J3,
+                       double Delta);
+
+/* MC_runner::setup_lattice (MC.cpp:15-52): builds the int32 neighbour
+ * table, per-slot coupling table and the graph colouring, uploads them. */
+int pyro_setup_lattice(pyro_ctx* ctx);
+
+/* Geometry / topology queries.
+ * info[0]=N sites, [1]=n_cells, [2]=n_sl, [3]=replicas, [4]=precision,
+ * [5]=neighbour slots per site (after dropping zero couplings),
+ * [6]=number of colours, [7]=1 if every coupling is isotropic. */
+int pyro_info(pyro_ctx* ctx, int64_t info[8]);
+/* Site positions, [N][3] int32 (the reference's HeisenbergSpin::ipos). */
+int pyro_site_positions(pyro_ctx* ctx, int32_t* pos);
+/* Neighbour table [N][n_slot] and its per-slot 3x3 couplings [N][n_slot][9];
+ * padded slots hold the site itself with a zero matrix. */
+int pyro_neighbours(pyro_ctx* ctx, int32_t* nbr, double* J);
+/* colour_of_site[N]; sites of one colour share no bond. */
+int pyro_colouring(pyro_ctx* ctx, int32_t* colour_of_site);
+
+/* ------------------------------------------------------------------ */
+/* State.                                                               */
+/* ------------------------------------------------------------------ */
+/* MC_runner::set_global_field (MC.cpp:69-71); replica -1 = all. */
+int pyro_set_field(pyro_ctx* ctx, int replica, const double B[3]);
+/* MC_parameters::T_ref (MC.hpp:79-82). */
+int pyro_set_T_ref(pyro_ctx* ctx, double T_ref);
+/* Temperature of each replica (T[R]).  Also resets the tempering
+ * permutation to the identity (replica r sits in slot r). */
+int pyro_set_temperatures(pyro_ctx* ctx, const double* T);
+/* Spin upload/download for one replica, host [N][3] doubles
+ * (HeisenbergSpin::S; save_spin_state layout, MC.cpp:371-406). */
+int pyro_set_spins(pyro_ctx* ctx, int replica, const double* S);
+int pyro_get_spins(pyro_ctx* ctx, int replica, double* S);
+/* Lifted directions, int8 +-1 per site (HeisenbergSpin::lifted_dir). */
+int pyro_set_lifted_dir(pyro_ctx* ctx, int replica, const int8_t* dir);
+int pyro_get_lifted_dir(pyro_ctx* ctx, int replica, int8_t* dir);
+/* Uniform-on-sphere spins for every replica from `seed` (counter RNG);
+ * lifted directions reset to +1. */
+int pyro_randomize_spins(pyro_ctx* ctx, uint64_t seed);
+/* Number of single-site passes done so far (the RNG pass counter). */
+int pyro_pass_count(pyro_ctx* ctx, uint64_t* passes);
+
+/* ------------------------------------------------------------------ */
+/* Sweeps.                                                              */
+/* MC_runner::sweep_local_Metropolis / sweep_lifted_Metropolis /        */
+/* sweep_lifted_Metropolis_rot (MC.cpp:155-163, 283-303): one pass of   */
+/* `kind` over every site, then n_overrelax overrelaxation passes,      */
+/* repeated n_sweeps times, for all replicas at their temperatures.     */
+/* accepted[R] (may be NULL) receives the accepted-update count of the  */
+/* first pass kind summed over the n_sweeps sweeps.                      */
+/* ------------------------------------------------------------------ */
+int pyro_sweep(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweeps,
+               uint64_t* accepted);
+/* MC_runner::overrelax_all (MC.cpp:145-153), n_passes times. */
+int pyro_overrelax(pyro_ctx* ctx, int n_passes);
+
+/* ------------------------------------------------------------------ */
+/* Observables on the current configuration.                            */
+/* ------------------------------------------------------------------ */
+/* MC_runner::local_field (MC.cpp:87-96) for every site of one replica,
+ * host [N][3] doubles; the global field is NOT subtracted. */
+int pyro_local_fields(pyro_ctx* ctx, int replica, double* h);
+/* MC_runner::total_energy_per_unit_cell (MC.cpp:305-312) per replica,
+ * e[R], summed in FP64. */
+int pyro_energy_per_cell(pyro_ctx* ctx, double* e);
+
+/* ------------------------------------------------------------------ */
+/* Replica exchange (temper.cpp:418-474 without MPI).                   */
+/* T_ladder[R]: slot temperatures, slot 0 hottest (descending, like the */
+/* reference's T_grid / rank order).  Replica r starts in slot r.       */
+/* ------------------------------------------------------------------ */
+int pyro_pt_init(pyro_ctx* ctx, const double* T_ladder, uint64_t swap_seed);
+/* One tempering round: n_sweep sweeps (kind + n_overrelax OR passes) of
+ * every replica at its slot temperature; if sample_ssf, one SSF sample of
+ * every replica into its slot's accumulator; the energy per cell of every
+ * replica is added to its slot's E, E^2, n; then the checkerboard
+ * exchange of slot pairs (s, s+1) with s = round parity (mod 2).
+ * n_rounds rounds are run back to back (captured once as a HIP graph). */
+int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep,
+                   int sample_ssf, int n_rounds);
+/* Per-pair swap statistics attempts[R-1], accepts[R-1] (pair s <-> s+1),
+ * and per-slot energy sums E[R], E2[R], n[R] (temper.cpp:432-435,
+ * 478-508).  Any pointer may be NULL. */
+int pyro_pt_stats(pyro_ctx* ctx, uint64_t* attempts, uint64_t* accepts,
+                  double* E, double* E2, uint64_t* n);
+/* replica_of_slot[R]. */
+int pyro_pt_permutation(pyro_ctx* ctx, int32_t* replica_of_slot);
+/* Zero the swap statistics and energy sums (burn-in -> sampling). */
+int pyro_pt_reset_stats(pyro_ctx* ctx);
+/* Accumulate the current energy per cell of every replica into the
+ * energy sums of its slot (energy_manager::sample, energy_manager.hpp:38-44). */
+int pyro_energy_sample(pyro_ctx* ctx);
+
+/* ------------------------------------------------------------------ */
+/* Static structure factor (ssf_manager.hpp:42-57, 260-293).            */
+/* ------------------------------------------------------------------ */
+/* n_corr correlators (alpha[c], beta[c] in {0,1,2}); n_slots accumulator
+ * slots (temperatures).  store_error_term keeps the (Re S)^2, (Im S)^2
+ * sums (static_corr_2).  munu != 0 also accumulates the sublattice-
+ * resolved C^{ab}_{mu nu}(K) (temper.cpp:98-159 layout) through buffered
+ * FP32-MFMA products flushed into FP64; munu_buffer = samples per flush. */
+int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha,
+                       const int32_t* beta, int n_slots, int store_error_term,
+                       int munu, int munu_buffer);
+/* One SSF sample of every replica r with slot_of_replica[r] >= 0 into that
+ * slot.  slot_of_replica == NULL: use each replica's tempering slot. */
+int pyro_ssf_sample(pyro_ctx* ctx, const int32_t* slot_of_replica);
+/* Read slot accumulators: S[n_corr][n_k][2] and S2[n_corr][n_k][2]
+ * (either may be NULL), n_samples.  k order: (K0*L1 + K1)*L2 + K2. */
+int pyro_ssf_read(pyro_ctx* ctx, int slot, double* S, double* S2,
+                  uint64_t* n_samples);
+/* Read sublattice-resolved accumulators of one slot for correlator c:
+ * C[n_k][n_sl][n_sl][2] (requires munu). */
+int pyro_ssf_read_munu(pyro_ctx* ctx, int slot, int c, double* C);
+/* Zero one slot's accumulators (slot -1 = all). */
+int pyro_ssf_reset(pyro_ctx* ctx, int slot);
+/* Raw per-sublattice DFT of one replica's spins, ft[3][n_sl][n_k][2]
+ * (save_ft_spin_state layout, MC.cpp:320-368). */
+int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft);
+
+/* ------------------------------------------------------------------ */
+/* Execution control and timing.                                        */
+/* ------------------------------------------------------------------ */
+int pyro_synchronize(pyro_ctx* ctx);
+/* Sweep launch strategy: 0 = autotuned (default), 1 = one launch per
+ * colour class, 2 = one resident workgroup per replica looping over all
+ * colours of a sweep.  Returns PYRO_E_UNSUPPORTED where a strategy does
+ * not apply. */
+int pyro_set_strategy(pyro_ctx* ctx, int strategy);
+int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
+/* Kernel timing with HIP events on the context's stream.  When enabled,
+ * every launch is bracketed by events and its duration is added to the
+ * counter of its kernel family:
+ *   0 = sweep passes, 1 = energy, 2 = exchange, 3 = SSF transforms,
+ *   4 = SSF contraction, 5 = sublattice-resolved products.
+ * ms[6], launches[6]. */
+int pyro_profile_enable(pyro_ctx* ctx, int on);
+int pyro_profile_read(pyro_ctx* ctx, double* ms, uint64_t* launches);
+/* Elapsed milliseconds between two marks recorded on the context stream. */
+int pyro_mark(pyro_ctx* ctx, int which);
+int pyro_elapsed_ms(pyro_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYRO_CMC_H */

@@ -1,0 +1,1338 @@
+// C ABI of libpyro_cmc.so: context management, host<->device layout
+// conversion, launch sequencing (eager or HIP-graph), autotuning of the sweep
+// launch strategy and kernel timing.  Declarations and the reference
+// interface each entry point replaces are in include/pyro_cmc.h.
+
+#include "pyro_cmc.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "rng.hpp"
+#include "topology.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Status : std::runtime_error {
+    int code;
+    Status(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw Status(PYRO_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+void require(bool ok, int code, const std::string& msg) {
+    if (!ok) throw Status(code, msg);
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PYRO_OK;
+    } catch (const Status& s) {
+        g_last_error = s.what();
+        return s.code;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return PYRO_E_INVALID;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return PYRO_E_HIP;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PYRO_E_HIP;
+    }
+}
+
+// Owning device allocation.
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        release();
+        if (b == 0) b = 8;
+        HIPCHK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr int kFamilies = 6;
+enum Family { kFamSweep = 0, kFamEnergy, kFamExchange, kFamDft, kFamContract, kFamMunu };
+
+}  // namespace
+
+struct pyro_ctx {
+    pyro::LatticeGeom geom;
+    std::vector<pyro::CouplingDef> couplings;
+    std::set<std::string> names;
+    pyro::Topology topo;
+    bool is_setup = false;     // device state allocated
+    bool topo_ready = false;   // host topology built (needs no device)
+
+    int R = 0, prec = 32, NS = 0, device = 0;
+    uint64_t seed = 0;
+    double T_ref = 1.0;
+    hipStream_t stream = nullptr;
+
+    DevMem S, dir, nbr, cpl, T_slot, slot_of, rep_of, B, acc, cls, counters;
+    DevMem epart, e, Esum, E2sum, nE, att, accp;
+    DevMem S_save, dir_save, fields;
+    std::vector<int32_t> h_cls;          // all class descriptors, colour-major
+    std::vector<int32_t> h_colour_begin; // [n_colours + 1]
+    DevMem colour_begin;
+
+    std::vector<double> h_T;             // slot temperatures
+    std::vector<double> h_B;             // [R][3]
+    uint64_t pass_count = 0;
+    uint64_t round_count = 0;
+
+    bool pt_ready = false;
+    uint32_t swap_k0 = 0, swap_k1 = 0;
+
+    int strategy = 0;          // requested: 0 auto, 1 per-colour, 2 resident
+    int tuned_key = -1;        // (kind, n_or) the auto choice was made for
+    int tuned_strategy = 1;
+
+    // SSF
+    bool ssf_ready = false;
+    int n_corr = 0, n_slots = 0, munu_B = 0, munu_T = 0, ssf_chunk = 1;
+    bool store2 = true, munu = false;
+    std::vector<int32_t> h_alpha, h_beta;
+    DevMem F, G, tw0, tw1, tw2, phase, d_alpha, d_beta, sacc, sacc2, sn, ssf_slot;
+    DevMem mbuf, mcnt, macc;
+    int munu_pending = 0;      // samples staged since the last flush (PT rounds)
+    std::vector<int> host_munu_count;
+
+    // graph of one tempering round
+    hipGraphExec_t gexec = nullptr;
+    int g_kind = -1, g_nor = -1, g_nsw = -1, g_ssf = -1, g_strategy = -1;
+
+    // timing
+    bool prof = false;
+    double prof_ms[kFamilies] = {};
+    uint64_t prof_n[kFamilies] = {};
+    struct Pending { int fam; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t marks[2] = {nullptr, nullptr};
+
+    ~pyro_ctx() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto ev : event_pool) (void)hipEventDestroy(ev);
+        for (auto ev : marks)
+            if (ev) (void)hipEventDestroy(ev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int64_t N() const { return geom.n_sites(); }
+    int64_t nc() const { return geom.n_cells(); }
+    size_t real_size() const { return prec == 64 ? 8 : 4; }
+
+    // Copies ordered on the context stream (kernels queued earlier finish
+    // first); synchronous so host buffers may be temporaries.
+    void h2d(void* dst, const void* src, size_t n) {
+        HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    void d2h(void* dst, const void* src, size_t n) {
+        HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    hipEvent_t get_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    // Run `launch` bracketed by timing events when profiling.
+    template <typename L>
+    void timed(int fam, L&& launch) {
+        if (!prof) {
+            HIPCHK(launch());
+            return;
+        }
+        hipEvent_t a = get_event(), b = get_event();
+        HIPCHK(hipEventRecord(a, stream));
+        HIPCHK(launch());
+        HIPCHK(hipEventRecord(b, stream));
+        pending.push_back({fam, a, b});
+    }
+    void collect_timing() {
+        if (pending.empty()) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto& p : pending) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+            prof_ms[p.fam] += ms;
+            prof_n[p.fam] += 1;
+            event_pool.push_back(p.a);
+            event_pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+
+    template <typename Real>
+    pyro::DeviceState<Real> state(const unsigned long long* pass_base) const {
+        pyro::DeviceState<Real> st;
+        st.S = S.as<Real>();
+        st.dir = dir.as<int8_t>();
+        st.nbr = nbr.as<int32_t>();
+        st.cpl = cpl.as<Real>();
+        st.T_slot = T_slot.as<double>();
+        st.slot_of = slot_of.as<int32_t>();
+        st.B = B.as<Real>();
+        st.acc = acc.as<unsigned long long>();
+        st.pass_base = pass_base;
+        st.k0 = uint32_t(seed) ^ pyro::kTagSweep;
+        st.k1 = uint32_t(seed >> 32);
+        st.T_ref = T_ref;
+        st.N = int(N());
+        st.n_cells = int(nc());
+        st.R = R;
+        st.NS = NS;
+        st.L0 = geom.L[0];
+        st.L1 = geom.L[1];
+        st.L2 = geom.L[2];
+        st.p0 = topo.period[0];
+        st.p1 = topo.period[1];
+        st.p2 = topo.period[2];
+        st.iso = topo.isotropic;
+        return st;
+    }
+    template <typename Real>
+    pyro::SsfState<Real> ssf_state() const {
+        pyro::SsfState<Real> st;
+        st.S = S.as<Real>();
+        st.F = F.as<Real>();
+        st.G = G.as<Real>();
+        st.tw0 = tw0.as<Real>();
+        st.tw1 = tw1.as<Real>();
+        st.tw2 = tw2.as<Real>();
+        st.phase = phase.as<Real>();
+        st.R = R;
+        st.n_sl = geom.n_sl;
+        st.n_cells = int(nc());
+        st.L0 = geom.L[0];
+        st.L1 = geom.L[1];
+        st.L2 = geom.L[2];
+        return st;
+    }
+
+    void need_setup() const { require(is_setup, PYRO_E_STATE, "setup_lattice() has not been called"); }
+    void need_topo() const { require(topo_ready, PYRO_E_STATE, "setup_lattice() has not been called"); }
+    void check_replica(int r, bool allow_all) const {
+        require((allow_all && r == -1) || (r >= 0 && r < R), PYRO_E_INVALID,
+                "replica index out of range");
+    }
+
+    // ------------------------------------------------------------------
+    void upload_temperatures() {
+        h2d(T_slot.p, h_T.data(), sizeof(double) * R);
+    }
+    void upload_field() {
+        if (prec == 64) {
+            h2d(B.p, h_B.data(), sizeof(double) * 3 * R);
+        } else {
+            std::vector<float> f(h_B.begin(), h_B.end());
+            h2d(B.p, f.data(), sizeof(float) * 3 * R);
+        }
+    }
+    void identity_permutation() {
+        std::vector<int32_t> id(R);
+        for (int r = 0; r < R; ++r) id[r] = r;
+        h2d(slot_of.p, id.data(), sizeof(int32_t) * R);
+        h2d(rep_of.p, id.data(), sizeof(int32_t) * R);
+    }
+
+    // One pass of `kind` over all colours (strategy 1).
+    template <typename Real>
+    void pass_colours(int kind, const unsigned long long* pass_base, unsigned long long pass_index) {
+        const auto st = state<Real>(pass_base);
+        for (int c = 0; c < topo.n_colours; ++c) {
+            pyro::ColourList col;
+            col.cls = cls.as<int32_t>() + h_colour_begin[c];
+            col.n_cls = h_colour_begin[c + 1] - h_colour_begin[c];
+            timed(kFamSweep, [&] { return pyro::launch_sweep_colour<Real>(st, col, kind, pass_index, stream); });
+        }
+    }
+    // n_sweeps sweeps of (kind + n_or OR passes) with the given strategy.
+    template <typename Real>
+    void sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
+                unsigned long long pass_index) {
+        if (strat == 2) {
+            const auto st = state<Real>(pass_base);
+            timed(kFamSweep, [&] {
+                return pyro::launch_sweep_resident<Real>(st, cls.as<int32_t>(), colour_begin.as<int32_t>(),
+                                                         topo.n_colours, kind, n_or, n_sweeps,
+                                                         pass_index, stream);
+            });
+            return;
+        }
+        unsigned long long p = pass_index;
+        for (int s = 0; s < n_sweeps; ++s) {
+            pass_colours<Real>(kind, pass_base, p++);
+            for (int o = 0; o < n_or; ++o) pass_colours<Real>(3, pass_base, p++);
+        }
+    }
+    void run_sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
+                    unsigned long long pass_index) {
+        if (prec == 64) sweeps<double>(strat, kind, n_or, n_sweeps, pass_base, pass_index);
+        else sweeps<float>(strat, kind, n_or, n_sweeps, pass_base, pass_index);
+    }
+
+    // Pick the faster launch strategy for (kind, n_or) by timing both on a
+    // saved copy of the state, then restore it (results do not depend on the
+    // strategy: both use the same per-site counters and colour order).
+    int choose_strategy(int kind, int n_or) {
+        if (strategy != 0) return strategy;
+        const int key = kind * 64 + n_or;
+        if (tuned_key == key) return tuned_strategy;
+        const size_t sbytes = size_t(R) * 3 * N() * real_size();
+        const size_t dbytes = size_t(R) * N();
+        if (!S_save.p) {
+            S_save.alloc(sbytes);
+            dir_save.alloc(dbytes);
+        }
+        HIPCHK(hipMemcpyAsync(S_save.p, S.p, sbytes, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dir_save.p, dir.p, dbytes, hipMemcpyDeviceToDevice, stream));
+        hipEvent_t a = get_event(), b = get_event();
+        const bool was_prof = prof;
+        prof = false;
+        float best = 1e30f;
+        int best_s = 1;
+        for (int s : {1, 2}) {
+            run_sweeps(s, kind, n_or, 1, nullptr, pass_count);  // warm-up
+            HIPCHK(hipEventRecord(a, stream));
+            run_sweeps(s, kind, n_or, 3, nullptr, pass_count);
+            HIPCHK(hipEventRecord(b, stream));
+            HIPCHK(hipEventSynchronize(b));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) {
+                best = ms;
+                best_s = s;
+            }
+        }
+        prof = was_prof;
+        event_pool.push_back(a);
+        event_pool.push_back(b);
+        HIPCHK(hipMemcpyAsync(S.p, S_save.p, sbytes, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dir.p, dir_save.p, dbytes, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        tuned_key = key;
+        tuned_strategy = best_s;
+        return best_s;
+    }
+
+    void energy(bool accumulate) {
+        double* es = accumulate ? Esum.as<double>() : nullptr;
+        double* e2 = accumulate ? E2sum.as<double>() : nullptr;
+        unsigned long long* ne = accumulate ? nE.as<unsigned long long>() : nullptr;
+        timed(kFamEnergy, [&] {
+            return prec == 64 ? pyro::launch_energy<double>(state<double>(nullptr), epart.as<double>(),
+                                                            e.as<double>(), es, e2, ne, stream)
+                              : pyro::launch_energy<float>(state<float>(nullptr), epart.as<double>(),
+                                                           e.as<double>(), es, e2, ne, stream);
+        });
+    }
+
+    // SSF sample of replicas [0, R) using the device slot array `slots`.
+    void ssf_sample_device(const int32_t* slots) {
+        for (int r0 = 0; r0 < R; r0 += ssf_chunk) {
+            const int rc = std::min(ssf_chunk, R - r0);
+            pyro::CorrArgs c;
+            c.slot_of = slots;
+            c.alpha = d_alpha.as<int32_t>();
+            c.beta = d_beta.as<int32_t>();
+            c.acc = sacc.as<double>();
+            c.acc2 = store2 ? sacc2.as<double>() : nullptr;
+            c.n_samples = sn.as<unsigned long long>();
+            c.n_corr = n_corr;
+            if (prec == 64) {
+                const auto st = ssf_state<double>();
+                timed(kFamDft, [&] { return pyro::launch_dft<double>(st, r0, rc, stream); });
+                timed(kFamContract, [&] { return pyro::launch_contract<double>(st, c, r0, rc, stream); });
+                if (munu) timed(kFamMunu, [&] { return pyro::launch_munu_stage<double>(st, slots, r0, rc, munu_args(), stream); });
+            } else {
+                const auto st = ssf_state<float>();
+                timed(kFamDft, [&] { return pyro::launch_dft<float>(st, r0, rc, stream); });
+                timed(kFamContract, [&] { return pyro::launch_contract<float>(st, c, r0, rc, stream); });
+                if (munu) timed(kFamMunu, [&] { return pyro::launch_munu_stage<float>(st, slots, r0, rc, munu_args(), stream); });
+            }
+        }
+    }
+    pyro::MunuArgs munu_args() const {
+        pyro::MunuArgs m;
+        m.buf = mbuf.as<float>();
+        m.cnt = mcnt.as<unsigned int>();
+        m.acc = macc.as<double>();
+        m.Bmax = munu_B;
+        m.n_k = int(nc());
+        m.n_slots = n_slots;
+        m.n_f = 3 * geom.n_sl;
+        m.T = munu_T;
+        return m;
+    }
+    void munu_flush() {
+        if (!munu) return;
+        timed(kFamMunu, [&] { return pyro::launch_munu_flush(munu_args(), stream); });
+        munu_pending = 0;
+        std::fill(host_munu_count.begin(), host_munu_count.end(), 0);
+    }
+
+    void invalidate_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
+};
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* pyro_last_error(void) { return g_last_error.c_str(); }
+
+int pyro_version(void) { return 10000; }
+
+int pyro_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pyro_create(int n_sl, const int32_t* sl_pos, const int32_t* sl_kind, const int32_t cell[3],
+                const int32_t L[3], int n_replicas, int precision, uint64_t seed, pyro_ctx** out) {
+    return guarded([&] {
+        require(out != nullptr, PYRO_E_INVALID, "out is NULL");
+        *out = nullptr;
+        require(n_sl > 0 && sl_pos && sl_kind && cell && L, PYRO_E_INVALID, "lattice description incomplete");
+        require(n_replicas > 0 && n_replicas <= 65535, PYRO_E_INVALID, "n_replicas must be in [1, 65535]");
+        require(precision == PYRO_FP32 || precision == PYRO_FP64, PYRO_E_INVALID, "precision must be 32 or 64");
+        auto ctx = std::make_unique<pyro_ctx>();
+        ctx->geom.n_sl = n_sl;
+        for (int s = 0; s < n_sl; ++s) {
+            ctx->geom.sl_pos.push_back({sl_pos[3 * s], sl_pos[3 * s + 1], sl_pos[3 * s + 2]});
+            ctx->geom.sl_kind.push_back(sl_kind[s]);
+        }
+        ctx->geom.cell = {cell[0], cell[1], cell[2]};
+        ctx->geom.L = {L[0], L[1], L[2]};
+        ctx->geom.validate();
+        ctx->R = n_replicas;
+        ctx->prec = precision;
+        ctx->seed = seed;
+        ctx->h_T.assign(n_replicas, 1.0);
+        ctx->h_B.assign(size_t(3) * n_replicas, 0.0);
+        *out = ctx.release();
+    });
+}
+
+int pyro_destroy(pyro_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    });
+}
+
+int pyro_define_coupling(pyro_ctx* ctx, const char* name, int n_kinds, const int32_t* counts,
+                         const int32_t* rel_vecs, const double J[9]) {
+    return guarded([&] {
+        require(ctx && name && J && (n_kinds == 0 || counts), PYRO_E_INVALID, "NULL argument");
+        require(!ctx->topo_ready, PYRO_E_STATE, "couplings must be defined before setup_lattice()");
+        require(!ctx->names.count(name), PYRO_E_LOGIC, "Coupling names must be unique");
+        pyro::CouplingDef c;
+        c.name = name;
+        size_t off = 0;
+        for (int k = 0; k < n_kinds; ++k) {
+            require(counts[k] >= 0, PYRO_E_INVALID, "negative displacement count");
+            std::vector<pyro::ivec3> v;
+            for (int i = 0; i < counts[k]; ++i, ++off)
+                v.push_back({rel_vecs[3 * off], rel_vecs[3 * off + 1], rel_vecs[3 * off + 2]});
+            c.rel.push_back(std::move(v));
+        }
+        for (int e = 0; e < 9; ++e) c.J[e] = J[e];
+        ctx->names.insert(c.name);
+        ctx->couplings.push_back(std::move(c));
+    });
+}
+
+int pyro_define_J1J2J3(pyro_ctx* ctx, double J1, double J2, double J3, double Delta) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(!ctx->topo_ready, PYRO_E_STATE, "couplings must be defined before setup_lattice()");
+        const auto ref = pyro::pyrochlore_cell(1, 1, 1);
+        bool pyro_cell = ctx->geom.n_sl == 16 && ctx->geom.cell == ref.cell;
+        for (int s = 0; pyro_cell && s < 16; ++s)
+            pyro_cell = ctx->geom.sl_pos[s] == ref.sl_pos[s] && ctx->geom.sl_kind[s] == ref.sl_kind[s];
+        require(pyro_cell, PYRO_E_INVALID, "define_J1J2J3 needs the 16-site pyrochlore cubic cell");
+        for (auto& c : pyro::pyrochlore_J1J2J3(J1, J2, J3, Delta)) {
+            require(!ctx->names.count(c.name), PYRO_E_LOGIC, "Coupling names must be unique");
+            ctx->names.insert(c.name);
+            ctx->couplings.push_back(std::move(c));
+        }
+    });
+}
+
+int pyro_setup_lattice(pyro_ctx* ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(!ctx->is_setup, PYRO_E_STATE, "setup_lattice() already called");
+        if (!ctx->topo_ready) {
+            ctx->topo = pyro::build_topology(ctx->geom, ctx->couplings);
+            ctx->topo_ready = true;
+        }
+        auto& t = ctx->topo;
+        const int ns_list[] = {6, 8, 18, 30, 32};
+        ctx->NS = 0;
+        for (int v : ns_list)
+            if (t.n_slot <= v) {
+                ctx->NS = v;
+                break;
+            }
+        require(ctx->NS > 0, PYRO_E_UNSUPPORTED, "more than 32 coupled neighbours per site");
+        // the topology above is host-only; everything below needs a device
+        int ndev = 0;
+        require(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, PYRO_E_HIP, "no HIP device available");
+        HIPCHK(hipGetDevice(&ctx->device));
+        if (!ctx->stream) HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        const int64_t N = ctx->N();
+        const int R = ctx->R;
+        const int NS = ctx->NS;
+        const size_t rs = ctx->real_size();
+
+        // neighbour table, slot-major and padded to NS slots with the site itself
+        std::vector<int32_t> nbr(size_t(NS) * N);
+        for (int64_t i = 0; i < N; ++i)
+            for (int k = 0; k < NS; ++k)
+                nbr[size_t(k) * N + i] = k < t.n_slot ? t.nbr[size_t(i) * t.n_slot + k] : int32_t(i);
+        ctx->nbr.alloc(nbr.size() * sizeof(int32_t));
+        ctx->h2d(ctx->nbr.p, nbr.data(), nbr.size() * sizeof(int32_t));
+
+        // per-slot couplings
+        const int n_sl = ctx->geom.n_sl;
+        std::vector<double> cp;
+        if (t.isotropic) {
+            cp.assign(size_t(n_sl) * NS, 0.0);
+            for (int s = 0; s < n_sl; ++s)
+                for (int k = 0; k < t.n_slot; ++k) cp[size_t(s) * NS + k] = t.cslot[size_t(s) * t.n_slot + k];
+        } else {
+            cp.assign(size_t(n_sl) * NS * 9, 0.0);
+            for (int s = 0; s < n_sl; ++s)
+                for (int k = 0; k < t.n_slot; ++k)
+                    for (int e = 0; e < 9; ++e)
+                        cp[(size_t(s) * NS + k) * 9 + e] = t.Jslot[(size_t(s) * t.n_slot + k) * 9 + e];
+        }
+        ctx->cpl.alloc(cp.size() * rs);
+        if (rs == 8) {
+            ctx->h2d(ctx->cpl.p, cp.data(), cp.size() * 8);
+        } else {
+            std::vector<float> f(cp.begin(), cp.end());
+            ctx->h2d(ctx->cpl.p, f.data(), f.size() * 4);
+        }
+
+        // colour classes, colour-major
+        const int P = t.period[0] * t.period[1] * t.period[2];
+        ctx->h_cls.clear();
+        ctx->h_colour_begin.assign(1, 0);
+        for (int c = 0; c < t.n_colours; ++c) {
+            for (int cl : t.colour_classes[c]) {
+                const int sl = cl / P;
+                int res = cl % P;
+                const int r2 = res % t.period[2];
+                res /= t.period[2];
+                const int r1 = res % t.period[1];
+                const int r0 = res / t.period[1];
+                ctx->h_cls.push_back(sl | (r0 << 16) | (r1 << 20) | (r2 << 24));
+            }
+            ctx->h_colour_begin.push_back(int32_t(ctx->h_cls.size()));
+        }
+        ctx->cls.alloc(ctx->h_cls.size() * sizeof(int32_t));
+        ctx->h2d(ctx->cls.p, ctx->h_cls.data(), ctx->h_cls.size() * sizeof(int32_t));
+        ctx->colour_begin.alloc(ctx->h_colour_begin.size() * sizeof(int32_t));
+        ctx->h2d(ctx->colour_begin.p, ctx->h_colour_begin.data(), ctx->h_colour_begin.size() * sizeof(int32_t));
+
+        // state
+        ctx->S.alloc(size_t(R) * 3 * N * rs);
+        ctx->dir.alloc(size_t(R) * N);
+        ctx->T_slot.alloc(sizeof(double) * R);
+        ctx->slot_of.alloc(sizeof(int32_t) * R);
+        ctx->rep_of.alloc(sizeof(int32_t) * R);
+        ctx->B.alloc(rs * 3 * R);
+        ctx->acc.alloc(sizeof(unsigned long long) * R);
+        ctx->counters.alloc(sizeof(unsigned long long) * 2);
+        ctx->epart.alloc(sizeof(double) * R * pyro::energy_blocks(int(N)));
+        ctx->e.alloc(sizeof(double) * R);
+        ctx->Esum.alloc(sizeof(double) * R);
+        ctx->E2sum.alloc(sizeof(double) * R);
+        ctx->nE.alloc(sizeof(unsigned long long) * R);
+        ctx->att.alloc(sizeof(unsigned long long) * R);
+        ctx->accp.alloc(sizeof(unsigned long long) * R);
+        ctx->fields.alloc(size_t(3) * N * rs);
+        HIPCHK(hipMemset(ctx->S.p, 0, ctx->S.bytes));
+        HIPCHK(hipMemset(ctx->dir.p, 1, ctx->dir.bytes));
+        HIPCHK(hipMemset(ctx->counters.p, 0, ctx->counters.bytes));
+        HIPCHK(hipMemset(ctx->Esum.p, 0, ctx->Esum.bytes));
+        HIPCHK(hipMemset(ctx->E2sum.p, 0, ctx->E2sum.bytes));
+        HIPCHK(hipMemset(ctx->nE.p, 0, ctx->nE.bytes));
+        HIPCHK(hipMemset(ctx->att.p, 0, ctx->att.bytes));
+        HIPCHK(hipMemset(ctx->accp.p, 0, ctx->accp.bytes));
+        HIPCHK(hipDeviceSynchronize());
+        ctx->identity_permutation();
+        ctx->upload_temperatures();
+        ctx->upload_field();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->is_setup = true;
+    });
+}
+
+int pyro_info(pyro_ctx* ctx, int64_t info[8]) {
+    return guarded([&] {
+        require(ctx && info, PYRO_E_INVALID, "NULL argument");
+        info[0] = ctx->N();
+        info[1] = ctx->nc();
+        info[2] = ctx->geom.n_sl;
+        info[3] = ctx->R;
+        info[4] = ctx->prec;
+        info[5] = ctx->topo_ready ? ctx->topo.n_slot : -1;
+        info[6] = ctx->topo_ready ? ctx->topo.n_colours : -1;
+        info[7] = ctx->topo_ready ? (ctx->topo.isotropic ? 1 : 0) : -1;
+    });
+}
+
+int pyro_site_positions(pyro_ctx* ctx, int32_t* pos) {
+    return guarded([&] {
+        require(ctx && pos, PYRO_E_INVALID, "NULL argument");
+        for (int64_t i = 0; i < ctx->N(); ++i) {
+            const auto p = ctx->geom.position(i);
+            pos[3 * i] = p[0];
+            pos[3 * i + 1] = p[1];
+            pos[3 * i + 2] = p[2];
+        }
+    });
+}
+
+int pyro_neighbours(pyro_ctx* ctx, int32_t* nbr, double* J) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_topo();
+        const auto& t = ctx->topo;
+        const int64_t N = ctx->N();
+        if (nbr) std::copy(t.nbr.begin(), t.nbr.end(), nbr);
+        if (J) {
+            for (int64_t i = 0; i < N; ++i) {
+                const int s = int(i / ctx->nc());
+                for (int k = 0; k < t.n_slot; ++k)
+                    for (int e = 0; e < 9; ++e)
+                        J[(size_t(i) * t.n_slot + k) * 9 + e] = t.Jslot[(size_t(s) * t.n_slot + k) * 9 + e];
+            }
+        }
+    });
+}
+
+int pyro_colouring(pyro_ctx* ctx, int32_t* colour_of_site) {
+    return guarded([&] {
+        require(ctx && colour_of_site, PYRO_E_INVALID, "NULL argument");
+        ctx->need_topo();
+        const auto& t = ctx->topo;
+        const auto& L = ctx->geom.L;
+        const int P = t.period[0] * t.period[1] * t.period[2];
+        for (int64_t i = 0; i < ctx->N(); ++i) {
+            const int s = int(i / ctx->nc());
+            int64_t c = i % ctx->nc();
+            const int n2 = int(c % L[2]);
+            c /= L[2];
+            const int n1 = int(c % L[1]);
+            const int n0 = int(c / L[1]);
+            const int cl = s * P + ((n0 % t.period[0]) * t.period[1] + (n1 % t.period[1])) * t.period[2] +
+                           (n2 % t.period[2]);
+            colour_of_site[i] = t.class_colour[cl];
+        }
+    });
+}
+
+int pyro_set_field(pyro_ctx* ctx, int replica, const double B[3]) {
+    return guarded([&] {
+        require(ctx && B, PYRO_E_INVALID, "NULL argument");
+        ctx->check_replica(replica, true);
+        for (int r = 0; r < ctx->R; ++r)
+            if (replica == -1 || r == replica)
+                for (int a = 0; a < 3; ++a) ctx->h_B[3 * r + a] = B[a];
+        if (ctx->is_setup) {
+            ctx->upload_field();
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+int pyro_set_T_ref(pyro_ctx* ctx, double T_ref) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(T_ref > 0 && std::isfinite(T_ref), PYRO_E_INVALID, "T_ref must be positive");
+        ctx->T_ref = T_ref;
+        ctx->invalidate_graph();
+    });
+}
+
+int pyro_set_temperatures(pyro_ctx* ctx, const double* T) {
+    return guarded([&] {
+        require(ctx && T, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        for (int r = 0; r < ctx->R; ++r)
+            require(T[r] > 0 && std::isfinite(T[r]), PYRO_E_INVALID, "temperatures must be positive");
+        ctx->h_T.assign(T, T + ctx->R);
+        ctx->upload_temperatures();
+        ctx->identity_permutation();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_set_spins(pyro_ctx* ctx, int replica, const double* S) {
+    return guarded([&] {
+        require(ctx && S, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        ctx->check_replica(replica, false);
+        const int64_t N = ctx->N();
+        for (int64_t i = 0; i < 3 * N; ++i)
+            require(std::isfinite(S[i]), PYRO_E_INVALID, "spin components must be finite");
+        const size_t off = size_t(replica) * 3 * N;
+        if (ctx->prec == 64) {
+            std::vector<double> soa(3 * N);
+            for (int64_t i = 0; i < N; ++i)
+                for (int a = 0; a < 3; ++a) soa[a * N + i] = S[3 * i + a];
+            ctx->h2d(ctx->S.as<double>() + off, soa.data(), soa.size() * 8);
+        } else {
+            std::vector<float> soa(3 * N);
+            for (int64_t i = 0; i < N; ++i)
+                for (int a = 0; a < 3; ++a) soa[a * N + i] = float(S[3 * i + a]);
+            ctx->h2d(ctx->S.as<float>() + off, soa.data(), soa.size() * 4);
+        }
+    });
+}
+
+int pyro_get_spins(pyro_ctx* ctx, int replica, double* S) {
+    return guarded([&] {
+        require(ctx && S, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        ctx->check_replica(replica, false);
+        const int64_t N = ctx->N();
+        const size_t off = size_t(replica) * 3 * N;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->prec == 64) {
+            std::vector<double> soa(3 * N);
+            ctx->d2h(soa.data(), ctx->S.as<double>() + off, soa.size() * 8);
+            for (int64_t i = 0; i < N; ++i)
+                for (int a = 0; a < 3; ++a) S[3 * i + a] = soa[a * N + i];
+        } else {
+            std::vector<float> soa(3 * N);
+            ctx->d2h(soa.data(), ctx->S.as<float>() + off, soa.size() * 4);
+            for (int64_t i = 0; i < N; ++i)
+                for (int a = 0; a < 3; ++a) S[3 * i + a] = soa[a * N + i];
+        }
+    });
+}
+
+int pyro_set_lifted_dir(pyro_ctx* ctx, int replica, const int8_t* dir) {
+    return guarded([&] {
+        require(ctx && dir, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        ctx->check_replica(replica, false);
+        for (int64_t i = 0; i < ctx->N(); ++i)
+            require(dir[i] == 1 || dir[i] == -1, PYRO_E_INVALID, "lifted directions must be +1 or -1");
+        ctx->h2d(ctx->dir.as<int8_t>() + size_t(replica) * ctx->N(), dir, ctx->N());
+    });
+}
+
+int pyro_get_lifted_dir(pyro_ctx* ctx, int replica, int8_t* dir) {
+    return guarded([&] {
+        require(ctx && dir, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        ctx->check_replica(replica, false);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->d2h(dir, ctx->dir.as<int8_t>() + size_t(replica) * ctx->N(), ctx->N());
+    });
+}
+
+int pyro_randomize_spins(pyro_ctx* ctx, uint64_t seed) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        const uint32_t k0 = uint32_t(seed) ^ pyro::kTagInit, k1 = uint32_t(seed >> 32);
+        if (ctx->prec == 64) HIPCHK(pyro::launch_randomize<double>(ctx->state<double>(nullptr), k0, k1, ctx->stream));
+        else HIPCHK(pyro::launch_randomize<float>(ctx->state<float>(nullptr), k0, k1, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_pass_count(pyro_ctx* ctx, uint64_t* passes) {
+    return guarded([&] {
+        require(ctx && passes, PYRO_E_INVALID, "NULL argument");
+        *passes = ctx->pass_count;
+    });
+}
+
+int pyro_sweep(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweeps, uint64_t* accepted) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(kind == PYRO_METROPOLIS || kind == PYRO_LIFTED || kind == PYRO_LIFTED_ROT, PYRO_E_INVALID,
+                "sweep kind must be METROPOLIS, LIFTED or LIFTED_ROT");
+        require(n_overrelax >= 0 && n_sweeps >= 0, PYRO_E_INVALID, "negative pass count");
+        if (n_sweeps == 0) {
+            if (accepted) std::fill(accepted, accepted + ctx->R, 0);
+            return;
+        }
+        const int strat = ctx->choose_strategy(kind, n_overrelax);
+        HIPCHK(pyro::launch_zero(ctx->acc.p, size_t(2) * ctx->R, ctx->stream));
+        ctx->run_sweeps(strat, kind, n_overrelax, n_sweeps, nullptr, ctx->pass_count);
+        ctx->pass_count += uint64_t(n_sweeps) * (1 + n_overrelax);
+        if (accepted) {
+            std::vector<unsigned long long> a(ctx->R);
+            HIPCHK(hipMemcpyAsync(a.data(), ctx->acc.p, sizeof(unsigned long long) * ctx->R,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            for (int r = 0; r < ctx->R; ++r) accepted[r] = a[r];
+        }
+    });
+}
+
+int pyro_overrelax(pyro_ctx* ctx, int n_passes) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(n_passes >= 0, PYRO_E_INVALID, "negative pass count");
+        for (int p = 0; p < n_passes; ++p) {
+            if (ctx->prec == 64) ctx->pass_colours<double>(3, nullptr, ctx->pass_count);
+            else ctx->pass_colours<float>(3, nullptr, ctx->pass_count);
+            ctx->pass_count += 1;
+        }
+    });
+}
+
+int pyro_local_fields(pyro_ctx* ctx, int replica, double* h) {
+    return guarded([&] {
+        require(ctx && h, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        ctx->check_replica(replica, false);
+        const int64_t N = ctx->N();
+        if (ctx->prec == 64) {
+            HIPCHK(pyro::launch_fields<double>(ctx->state<double>(nullptr), replica, ctx->fields.as<double>(), ctx->stream));
+            std::vector<double> f(3 * N);
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->d2h(f.data(), ctx->fields.p, f.size() * 8);
+            for (int64_t i = 0; i < N; ++i)
+                for (int a = 0; a < 3; ++a) h[3 * i + a] = f[a * N + i];
+        } else {
+            HIPCHK(pyro::launch_fields<float>(ctx->state<float>(nullptr), replica, ctx->fields.as<float>(), ctx->stream));
+            std::vector<float> f(3 * N);
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->d2h(f.data(), ctx->fields.p, f.size() * 4);
+            for (int64_t i = 0; i < N; ++i)
+                for (int a = 0; a < 3; ++a) h[3 * i + a] = f[a * N + i];
+        }
+    });
+}
+
+int pyro_energy_per_cell(pyro_ctx* ctx, double* e) {
+    return guarded([&] {
+        require(ctx && e, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        ctx->energy(false);
+        HIPCHK(hipMemcpyAsync(e, ctx->e.p, sizeof(double) * ctx->R, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_energy_sample(pyro_ctx* ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        ctx->energy(true);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_pt_init(pyro_ctx* ctx, const double* T_ladder, uint64_t swap_seed) {
+    return guarded([&] {
+        require(ctx && T_ladder, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        for (int r = 0; r < ctx->R; ++r)
+            require(T_ladder[r] > 0 && std::isfinite(T_ladder[r]), PYRO_E_INVALID, "temperatures must be positive");
+        ctx->h_T.assign(T_ladder, T_ladder + ctx->R);
+        ctx->upload_temperatures();
+        ctx->identity_permutation();
+        ctx->swap_k0 = uint32_t(swap_seed) ^ pyro::kTagSwap;
+        ctx->swap_k1 = uint32_t(swap_seed >> 32);
+        ctx->round_count = 0;
+        HIPCHK(pyro::launch_zero(ctx->Esum.p, size_t(2) * ctx->R, ctx->stream));
+        HIPCHK(pyro::launch_zero(ctx->E2sum.p, size_t(2) * ctx->R, ctx->stream));
+        HIPCHK(pyro::launch_zero(ctx->nE.p, size_t(2) * ctx->R, ctx->stream));
+        HIPCHK(pyro::launch_zero(ctx->att.p, size_t(2) * ctx->R, ctx->stream));
+        HIPCHK(pyro::launch_zero(ctx->accp.p, size_t(2) * ctx->R, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->pt_ready = true;
+        ctx->invalidate_graph();
+    });
+}
+
+int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sample_ssf,
+                   int n_rounds) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(ctx->pt_ready, PYRO_E_STATE, "pt_init() has not been called");
+        require(kind == PYRO_METROPOLIS || kind == PYRO_LIFTED || kind == PYRO_LIFTED_ROT, PYRO_E_INVALID,
+                "sweep kind must be METROPOLIS, LIFTED or LIFTED_ROT");
+        require(n_overrelax >= 0 && n_sweep >= 0 && n_rounds >= 0, PYRO_E_INVALID, "negative count");
+        require(!sample_ssf || ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        require(!sample_ssf || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
+                "SSF sampling in tempering rounds needs one accumulator slot per replica");
+        if (n_rounds == 0) return;
+        const int strat = n_sweep > 0 ? ctx->choose_strategy(kind, n_overrelax) : 1;
+        const unsigned long long ppr = (unsigned long long)n_sweep * (1 + n_overrelax);
+        auto* ctr = ctx->counters.as<unsigned long long>();
+        // device counters: pass base, round
+        unsigned long long hc[2] = {ctx->pass_count, ctx->round_count};
+        HIPCHK(hipMemcpyAsync(ctx->counters.p, hc, sizeof(hc), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+
+        auto round_body = [&]() {
+            if (n_sweep > 0) ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull);
+            if (sample_ssf) ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+            ctx->energy(true);
+            pyro::ExchangeArgs a;
+            a.e = ctx->e.as<double>();
+            a.rep_of = ctx->rep_of.as<int32_t>();
+            a.slot_of = ctx->slot_of.as<int32_t>();
+            a.T_slot = ctx->T_slot.as<double>();
+            a.att = ctx->att.as<unsigned long long>();
+            a.accp = ctx->accp.as<unsigned long long>();
+            a.round = ctr + 1;
+            a.pass_base = ctr;
+            a.passes_per_round = ppr;
+            a.n_cells = double(ctx->nc());
+            a.k0 = ctx->swap_k0;
+            a.k1 = ctx->swap_k1;
+            a.R = ctx->R;
+            ctx->timed(kFamExchange, [&] { return pyro::launch_exchange(a, ctx->stream); });
+        };
+
+        const bool use_graph = !ctx->prof;
+        if (use_graph) {
+            if (!ctx->gexec || ctx->g_kind != kind || ctx->g_nor != n_overrelax || ctx->g_nsw != n_sweep ||
+                ctx->g_ssf != sample_ssf || ctx->g_strategy != strat) {
+                ctx->invalidate_graph();
+                hipGraph_t graph;
+                HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    round_body();
+                } catch (...) {
+                    (void)hipStreamEndCapture(ctx->stream, &graph);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
+                hipError_t ie = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIPCHK(ie);
+                ctx->g_kind = kind;
+                ctx->g_nor = n_overrelax;
+                ctx->g_nsw = n_sweep;
+                ctx->g_ssf = sample_ssf;
+                ctx->g_strategy = strat;
+            }
+        }
+        for (int i = 0; i < n_rounds; ++i) {
+            if (use_graph) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+            else round_body();
+            ctx->pass_count += ppr;
+            ctx->round_count += 1;
+            if (sample_ssf && ctx->munu) {
+                ctx->munu_pending += 1;
+                if (ctx->munu_pending >= ctx->munu_B) ctx->munu_flush();
+            }
+        }
+    });
+}
+
+int pyro_pt_stats(pyro_ctx* ctx, uint64_t* attempts, uint64_t* accepts, double* E, double* E2,
+                  uint64_t* n) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int R = ctx->R;
+        auto get_u64 = [&](const DevMem& m, uint64_t* dst, int cnt) {
+            if (!dst || cnt <= 0) return;
+            std::vector<unsigned long long> v(cnt);
+            ctx->d2h(v.data(), m.p, sizeof(unsigned long long) * cnt);
+            for (int i = 0; i < cnt; ++i) dst[i] = v[i];
+        };
+        get_u64(ctx->att, attempts, R - 1);
+        get_u64(ctx->accp, accepts, R - 1);
+        get_u64(ctx->nE, n, R);
+        if (E) ctx->d2h(E, ctx->Esum.p, sizeof(double) * R);
+        if (E2) ctx->d2h(E2, ctx->E2sum.p, sizeof(double) * R);
+    });
+}
+
+int pyro_pt_permutation(pyro_ctx* ctx, int32_t* replica_of_slot) {
+    return guarded([&] {
+        require(ctx && replica_of_slot, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->d2h(replica_of_slot, ctx->rep_of.p, sizeof(int32_t) * ctx->R);
+    });
+}
+
+int pyro_pt_reset_stats(pyro_ctx* ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        for (DevMem* m : {&ctx->Esum, &ctx->E2sum, &ctx->nE, &ctx->att, &ctx->accp})
+            HIPCHK(pyro::launch_zero(m->p, m->bytes / 4, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha, const int32_t* beta,
+                       int n_slots, int store_error_term, int munu, int munu_buffer) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(n_corr > 0 && alpha && beta, PYRO_E_INVALID, "need at least one correlator");
+        require(n_slots > 0, PYRO_E_INVALID, "need at least one accumulator slot");
+        for (int c = 0; c < n_corr; ++c)
+            require(alpha[c] >= 0 && alpha[c] < 3 && beta[c] >= 0 && beta[c] < 3, PYRO_E_INVALID,
+                    "correlator components must be 0 (x), 1 (y) or 2 (z)");
+        const int n_f = 3 * ctx->geom.n_sl;
+        if (munu) {
+            require(munu_buffer > 0, PYRO_E_INVALID, "munu_buffer must be positive");
+            require(n_f <= 48, PYRO_E_UNSUPPORTED, "sublattice-resolved products support at most 16 sublattices");
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->invalidate_graph();
+        ctx->n_corr = n_corr;
+        ctx->h_alpha.assign(alpha, alpha + n_corr);
+        ctx->h_beta.assign(beta, beta + n_corr);
+        ctx->n_slots = n_slots;
+        ctx->store2 = store_error_term != 0;
+        ctx->munu = munu != 0;
+        ctx->munu_B = munu ? munu_buffer : 0;
+        const size_t rs = ctx->real_size();
+        const int64_t nc = ctx->nc();
+        const int n_sl = ctx->geom.n_sl;
+        // replicas per DFT batch: keep F + G under ~4 GiB
+        const size_t per_rep = size_t(3) * n_sl * nc * 2 * rs;
+        ctx->ssf_chunk = int(std::max<size_t>(1, std::min<size_t>(ctx->R, (size_t(2) << 30) / per_rep)));
+        ctx->F.alloc(per_rep * ctx->ssf_chunk);
+        ctx->G.alloc(per_rep * ctx->ssf_chunk);
+        // twiddles e^{-2 pi i j / L_a} and sublattice phases e^{-i q(K).r_mu}
+        auto upload_real = [&](DevMem& m, const std::vector<double>& v) {
+            m.alloc(v.size() * rs);
+            if (rs == 8) {
+                ctx->h2d(m.p, v.data(), v.size() * 8);
+            } else {
+                std::vector<float> f(v.begin(), v.end());
+                ctx->h2d(m.p, f.data(), f.size() * 4);
+            }
+        };
+        const auto& L = ctx->geom.L;
+        DevMem* tws[3] = {&ctx->tw0, &ctx->tw1, &ctx->tw2};
+        for (int a = 0; a < 3; ++a) {
+            std::vector<double> tw(2 * L[a]);
+            for (int j = 0; j < L[a]; ++j) {
+                const double th = 2.0 * M_PI * double(j) / double(L[a]);
+                tw[2 * j] = std::cos(th);
+                tw[2 * j + 1] = -std::sin(th);
+            }
+            upload_real(*tws[a], tw);
+        }
+        std::vector<double> ph(size_t(2) * n_sl * nc);
+        for (int mu = 0; mu < n_sl; ++mu) {
+            const auto& r = ctx->geom.sl_pos[mu];
+            for (int64_t K = 0; K < nc; ++K) {
+                const int K2 = int(K % L[2]);
+                const int K1 = int((K / L[2]) % L[1]);
+                const int K0 = int(K / (int64_t(L[1]) * L[2]));
+                const int Ks[3] = {K0, K1, K2};
+                double arg = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const int Kc = (2 * Ks[a] < L[a]) ? Ks[a] : Ks[a] - L[a];
+                    arg += 2.0 * M_PI * double(Kc) / double(ctx->geom.cell[a] * L[a]) * double(r[a]);
+                }
+                ph[2 * (mu * nc + K)] = std::cos(arg);
+                ph[2 * (mu * nc + K) + 1] = -std::sin(arg);
+            }
+        }
+        upload_real(ctx->phase, ph);
+        ctx->d_alpha.alloc(sizeof(int32_t) * n_corr);
+        ctx->d_beta.alloc(sizeof(int32_t) * n_corr);
+        ctx->h2d(ctx->d_alpha.p, alpha, sizeof(int32_t) * n_corr);
+        ctx->h2d(ctx->d_beta.p, beta, sizeof(int32_t) * n_corr);
+        const size_t acc_bytes = sizeof(double) * 2 * size_t(n_slots) * n_corr * nc;
+        ctx->sacc.alloc(acc_bytes);
+        HIPCHK(hipMemset(ctx->sacc.p, 0, acc_bytes));
+        if (ctx->store2) {
+            ctx->sacc2.alloc(acc_bytes);
+            HIPCHK(hipMemset(ctx->sacc2.p, 0, acc_bytes));
+        } else {
+            ctx->sacc2.release();
+        }
+        ctx->sn.alloc(sizeof(unsigned long long) * n_slots);
+        HIPCHK(hipMemset(ctx->sn.p, 0, ctx->sn.bytes));
+        ctx->ssf_slot.alloc(sizeof(int32_t) * ctx->R);
+        if (ctx->munu) {
+            ctx->munu_T = (n_f + 15) / 16;
+            const size_t fw = size_t(32) * ctx->munu_T;
+            ctx->mbuf.alloc(sizeof(float) * fw * size_t(n_slots) * nc * ctx->munu_B);
+            ctx->mcnt.alloc(sizeof(unsigned int) * n_slots);
+            HIPCHK(hipMemset(ctx->mcnt.p, 0, ctx->mcnt.bytes));
+            const size_t mbytes = sizeof(double) * 2 * size_t(n_slots) * nc * n_f * n_f;
+            ctx->macc.alloc(mbytes);
+            HIPCHK(hipMemset(ctx->macc.p, 0, mbytes));
+            ctx->host_munu_count.assign(n_slots, 0);
+        } else {
+            ctx->mbuf.release();
+            ctx->mcnt.release();
+            ctx->macc.release();
+        }
+        HIPCHK(hipDeviceSynchronize());
+        ctx->munu_pending = 0;
+        ctx->ssf_ready = true;
+    });
+}
+
+int pyro_ssf_sample(pyro_ctx* ctx, const int32_t* slot_of_replica) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        if (!slot_of_replica) {
+            require(ctx->n_slots >= ctx->R, PYRO_E_INVALID, "tempering slots exceed the accumulator slots");
+            ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+            if (ctx->munu) {
+                for (auto& c : ctx->host_munu_count) ++c;
+                if (++ctx->munu_pending >= ctx->munu_B) ctx->munu_flush();
+            }
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            return;
+        }
+        for (int r = 0; r < ctx->R; ++r)
+            require(slot_of_replica[r] < ctx->n_slots, PYRO_E_INVALID, "slot index out of range");
+        // Split into layers in which every slot appears at most once, so the
+        // accumulators see no concurrent updates.
+        std::vector<int> seen(ctx->n_slots, 0);
+        int n_layers = 0;
+        std::vector<int> layer(ctx->R, -1);
+        for (int r = 0; r < ctx->R; ++r) {
+            const int s = slot_of_replica[r];
+            if (s < 0) continue;
+            layer[r] = seen[s]++;
+            n_layers = std::max(n_layers, layer[r] + 1);
+        }
+        for (int l = 0; l < n_layers; ++l) {
+            std::vector<int32_t> sl(ctx->R, -1);
+            for (int r = 0; r < ctx->R; ++r)
+                if (layer[r] == l) sl[r] = slot_of_replica[r];
+            if (ctx->munu) {
+                // flush slots whose buffer is full before staging more
+                bool full = false;
+                for (int r = 0; r < ctx->R; ++r)
+                    if (sl[r] >= 0 && ctx->host_munu_count[sl[r]] >= ctx->munu_B) full = true;
+                if (full) ctx->munu_flush();
+            }
+            HIPCHK(hipMemcpyAsync(ctx->ssf_slot.p, sl.data(), sizeof(int32_t) * ctx->R, hipMemcpyHostToDevice,
+                                  ctx->stream));
+            ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>());
+            if (ctx->munu)
+                for (int r = 0; r < ctx->R; ++r)
+                    if (sl[r] >= 0) ctx->host_munu_count[sl[r]] += 1;
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+int pyro_ssf_read(pyro_ctx* ctx, int slot, double* S, double* S2, uint64_t* n_samples) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        require(slot >= 0 && slot < ctx->n_slots, PYRO_E_INVALID, "slot index out of range");
+        require(!S2 || ctx->store2, PYRO_E_STATE, "error term was not configured");
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const size_t per = size_t(2) * ctx->n_corr * ctx->nc();
+        if (S) ctx->d2h(S, ctx->sacc.as<double>() + slot * per, per * 8);
+        if (S2) ctx->d2h(S2, ctx->sacc2.as<double>() + slot * per, per * 8);
+        if (n_samples) {
+            unsigned long long n = 0;
+            ctx->d2h(&n, ctx->sn.as<unsigned long long>() + slot, 8);
+            *n_samples = n;
+        }
+    });
+}
+
+int pyro_ssf_read_munu(pyro_ctx* ctx, int slot, int c, double* C) {
+    return guarded([&] {
+        require(ctx && C, PYRO_E_INVALID, "NULL argument");
+        require(ctx->ssf_ready && ctx->munu, PYRO_E_STATE, "sublattice-resolved products were not configured");
+        require(slot >= 0 && slot < ctx->n_slots, PYRO_E_INVALID, "slot index out of range");
+        require(c >= 0 && c < ctx->n_corr, PYRO_E_INVALID, "correlator index out of range");
+        ctx->munu_flush();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int n_sl = ctx->geom.n_sl, n_f = 3 * n_sl;
+        const int64_t nc = ctx->nc();
+        std::vector<double> full(size_t(nc) * n_f * n_f * 2);
+        ctx->d2h(full.data(), ctx->macc.as<double>() + size_t(slot) * full.size(), full.size() * 8);
+        const int al = ctx->h_alpha[c], be = ctx->h_beta[c];
+        for (int64_t K = 0; K < nc; ++K)
+            for (int mu = 0; mu < n_sl; ++mu)
+                for (int nu = 0; nu < n_sl; ++nu) {
+                    const size_t src = ((size_t(K) * n_f + (al * n_sl + mu)) * n_f + (be * n_sl + nu)) * 2;
+                    const size_t dst = ((size_t(K) * n_sl + mu) * n_sl + nu) * 2;
+                    C[dst] = full[src];
+                    C[dst + 1] = full[src + 1];
+                }
+    });
+}
+
+int pyro_ssf_reset(pyro_ctx* ctx, int slot) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        require(slot >= -1 && slot < ctx->n_slots, PYRO_E_INVALID, "slot index out of range");
+        const size_t per = size_t(2) * ctx->n_corr * ctx->nc();
+        const int s0 = slot < 0 ? 0 : slot, s1 = slot < 0 ? ctx->n_slots : slot + 1;
+        ctx->munu_flush();
+        for (int s = s0; s < s1; ++s) {
+            HIPCHK(pyro::launch_zero(ctx->sacc.as<double>() + s * per, per * 2, ctx->stream));
+            if (ctx->store2) HIPCHK(pyro::launch_zero(ctx->sacc2.as<double>() + s * per, per * 2, ctx->stream));
+            HIPCHK(pyro::launch_zero(ctx->sn.as<unsigned long long>() + s, 2, ctx->stream));
+            if (ctx->munu) {
+                const size_t mper = size_t(2) * ctx->nc() * 9 * ctx->geom.n_sl * ctx->geom.n_sl;
+                HIPCHK(pyro::launch_zero(ctx->macc.as<double>() + s * mper, mper * 2, ctx->stream));
+            }
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft) {
+    return guarded([&] {
+        require(ctx && ft, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        ctx->check_replica(replica, false);
+        require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        const size_t n = size_t(3) * ctx->geom.n_sl * ctx->nc() * 2;
+        if (ctx->prec == 64) {
+            HIPCHK(pyro::launch_dft<double>(ctx->ssf_state<double>(), replica, 1, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->d2h(ft, ctx->F.p, n * 8);
+        } else {
+            HIPCHK(pyro::launch_dft<float>(ctx->ssf_state<float>(), replica, 1, ctx->stream));
+            std::vector<float> f(n);
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ctx->d2h(f.data(), ctx->F.p, n * 4);
+            for (size_t i = 0; i < n; ++i) ft[i] = f[i];
+        }
+    });
+}
+
+int pyro_synchronize(pyro_ctx* ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(strategy >= 0 && strategy <= 2, PYRO_E_INVALID, "strategy must be 0, 1 or 2");
+        ctx->strategy = strategy;
+        ctx->tuned_key = -1;
+        ctx->invalidate_graph();
+    });
+}
+
+int pyro_get_strategy(pyro_ctx* ctx, int* strategy) {
+    return guarded([&] {
+        require(ctx && strategy, PYRO_E_INVALID, "NULL argument");
+        *strategy = ctx->strategy != 0 ? ctx->strategy : (ctx->tuned_key >= 0 ? ctx->tuned_strategy : 0);
+    });
+}
+
+int pyro_profile_enable(pyro_ctx* ctx, int on) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->collect_timing();
+        ctx->prof = on != 0;
+        if (on) {
+            std::fill(ctx->prof_ms, ctx->prof_ms + kFamilies, 0.0);
+            std::fill(ctx->prof_n, ctx->prof_n + kFamilies, 0);
+        }
+    });
+}
+
+int pyro_profile_read(pyro_ctx* ctx, double* ms, uint64_t* launches) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->collect_timing();
+        for (int f = 0; f < kFamilies; ++f) {
+            if (ms) ms[f] = ctx->prof_ms[f];
+            if (launches) launches[f] = ctx->prof_n[f];
+        }
+    });
+}
+
+int pyro_mark(pyro_ctx* ctx, int which) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(which == 0 || which == 1, PYRO_E_INVALID, "mark must be 0 or 1");
+        if (!ctx->marks[which]) HIPCHK(hipEventCreate(&ctx->marks[which]));
+        HIPCHK(hipEventRecord(ctx->marks[which], ctx->stream));
+    });
+}
+
+int pyro_elapsed_ms(pyro_ctx* ctx, double* ms) {
+    return guarded([&] {
+        require(ctx && ms, PYRO_E_INVALID, "NULL argument");
+        require(ctx->marks[0] && ctx->marks[1], PYRO_E_STATE, "both marks must be recorded");
+        HIPCHK(hipEventSynchronize(ctx->marks[1]));
+        float f = 0.f;
+        HIPCHK(hipEventElapsedTime(&f, ctx->marks[0], ctx->marks[1]));
+        *ms = f;
+    });
+}
+
+}  // extern "C"

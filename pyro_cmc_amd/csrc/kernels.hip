@@ -1,0 +1,813 @@
+// HIP kernels for gfx950 (MI355X): graph-coloured single-site sweeps,
+// local fields, energy, replica exchange, per-sublattice DFT, contracted
+// structure-factor correlators and the sublattice-resolved MFMA products.
+//
+// Data layout in HBM (per context):
+//   spins      S[r][c][i]      Real, c = x,y,z; i = sl*n_cells + cell
+//   lifted dir dir[r][i]       int8
+//   neighbours nbr[k][i]       int32, slot-major so a wave's loads coalesce
+//   couplings  cpl[sl][k]      Real scalar (isotropic) or [sl][k][9]
+// A colour is a set of (sublattice, residue) classes with no bond among
+// them, so all its sites update concurrently with the same result as any
+// sequential order over them (MC.cpp:155-163 visits storage order).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "kernels.hpp"
+#include "rng.hpp"
+
+namespace pyro {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename Real> __device__ inline Real dsqrt(Real x);
+template <> __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
+template <> __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
+template <typename Real> __device__ inline Real dexp(Real x);
+template <> __device__ inline float dexp<float>(float x) { return expf(x); }
+template <> __device__ inline double dexp<double>(double x) { return exp(x); }
+template <typename Real> __device__ inline Real dlog(Real x);
+template <> __device__ inline float dlog<float>(float x) { return logf(x); }
+template <> __device__ inline double dlog<double>(double x) { return log(x); }
+template <typename Real> __device__ inline void dsincos(Real x, Real* s, Real* c);
+template <> __device__ inline void dsincos<float>(float x, float* s, float* c) { sincosf(x, s, c); }
+template <> __device__ inline void dsincos<double>(double x, double* s, double* c) { sincos(x, s, c); }
+template <typename Real> __device__ inline Real two_pi();
+template <> __device__ inline float two_pi<float>() { return 6.28318530717958647692f; }
+template <> __device__ inline double two_pi<double>() { return 6.28318530717958647692; }
+
+// FP32 spins are renormalised after every accepted change (drift control);
+// FP64 follows the reference exactly (no renormalisation after OR).
+template <typename Real> struct Renorm { static constexpr bool value = false; };
+template <> struct Renorm<float> { static constexpr bool value = true; };
+
+__device__ inline int decode_site(int desc, int m, int n_cells, int L1, int L2, int p0,
+                                  int p1, int p2) {
+    const int sl = desc & 0xFFFF;
+    if (p0 == 1 && p1 == 1 && p2 == 1) return sl * n_cells + m;
+    const int r0 = (desc >> 16) & 15, r1 = (desc >> 20) & 15, r2 = (desc >> 24) & 15;
+    const int q2 = L2 / p2, q1 = L1 / p1;
+    const int m2 = m % q2;
+    const int t = m / q2;
+    const int m1 = t % q1;
+    const int m0 = t / q1;
+    return sl * n_cells + ((r0 + p0 * m0) * L1 + (r1 + p1 * m1)) * L2 + (r2 + p2 * m2);
+}
+
+// Local field h_i = sum_k J_k S_{nbr(k)} of site i in replica r (MC.cpp:87-96).
+template <typename Real, int NS, bool ISO>
+__device__ inline void local_field(const DeviceState<Real>& st, const Real* __restrict__ Sx,
+                                   const Real* __restrict__ Sy, const Real* __restrict__ Sz,
+                                   int site, int sl, Real& hx, Real& hy, Real& hz) {
+    int nb[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) nb[k] = st.nbr[size_t(k) * st.N + site];
+    hx = Real(0);
+    hy = Real(0);
+    hz = Real(0);
+    if constexpr (ISO) {
+        const Real* __restrict__ c = st.cpl + sl * NS;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const Real ck = c[k];
+            hx += ck * Sx[nb[k]];
+            hy += ck * Sy[nb[k]];
+            hz += ck * Sz[nb[k]];
+        }
+    } else {
+        const Real* __restrict__ J = st.cpl + size_t(sl) * NS * 9;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const Real x = Sx[nb[k]], y = Sy[nb[k]], z = Sz[nb[k]];
+            const Real* m = J + 9 * k;
+            hx += m[0] * x + m[1] * y + m[2] * z;
+            hy += m[3] * x + m[4] * y + m[5] * z;
+            hz += m[6] * x + m[7] * y + m[8] * z;
+        }
+    }
+}
+
+// One single-site update of kind KIND at `site` of replica r, pass index
+// `pass`.  Returns whether the proposal was accepted (OR: always false).
+template <typename Real, int NS, bool ISO, int KIND>
+__device__ inline bool update_site(const DeviceState<Real>& st, int site, int sl, int r,
+                                   unsigned long long pass) {
+    const size_t N = size_t(st.N);
+    Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
+    Real* __restrict__ Sy = Sx + N;
+    Real* __restrict__ Sz = Sy + N;
+
+    Real hx, hy, hz;
+    local_field<Real, NS, ISO>(st, Sx, Sy, Sz, site, sl, hx, hy, hz);
+    hx -= st.B[3 * r + 0];  // h_loc = local_field - global_field
+    hy -= st.B[3 * r + 1];
+    hz -= st.B[3 * r + 2];
+
+    const Real sx = Sx[site], sy = Sy[site], sz = Sz[site];
+    const double Td = st.T_slot[st.slot_of[r]];
+    const Real T = Real(Td);
+
+    const Words4 w = philox4x32_10(
+        Words4{uint32_t(site), uint32_t(r), uint32_t(pass), uint32_t(pass >> 32)}, st.k0, st.k1);
+    const Uniform5 u = split_uniform5(w);
+
+    if constexpr (KIND == 0) {
+        // Gaussian-kick Metropolis (MC.cpp:111-130).
+        const Real sig = Real(sqrt(Td / st.T_ref));
+        Real s1, c1, s2, c2;
+        const Real r1 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[0])));
+        dsincos<Real>(two_pi<Real>() * u24<Real>(u.m[1]), &s1, &c1);
+        const Real r2 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[2])));
+        dsincos<Real>(two_pi<Real>() * u24<Real>(u.m[3]), &s2, &c2);
+        Real nx = sig * (r1 * c1) + sx;
+        Real ny = sig * (r1 * s1) + sy;
+        Real nz = sig * (r2 * c2) + sz;
+        const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
+        nx /= nrm;
+        ny /= nrm;
+        nz /= nrm;
+        const Real curE = sx * hx + sy * hy + sz * hz;
+        const Real newE = nx * hx + ny * hy + nz * hz;
+        const Real dE = newE - curE;
+        if (dE < Real(0) || u24<Real>(u.m[4]) < dexp<Real>(-dE / T)) {
+            Sx[site] = nx;
+            Sy[site] = ny;
+            Sz[site] = nz;
+            return true;
+        }
+        return false;
+    } else if constexpr (KIND == 3) {
+        // Overrelaxation: rotation by 2*pi*U about h_loc (MC.cpp:102-109, 145-153).
+        const Real n2 = hx * hx + hy * hy + hz * hz;
+        if (!(n2 > Real(0))) return false;  // h_loc = 0: no axis, spin kept
+        Real s, c;
+        dsincos<Real>(two_pi<Real>() * u24<Real>(u.m[0]), &s, &c);
+        const Real d = hx * sx + hy * sy + hz * sz;
+        const Real f = ((Real(1) - c) / n2) * d;
+        const Real g = s / dsqrt<Real>(n2);
+        const Real cx = hy * sz - hz * sy, cy = hz * sx - hx * sz, cz = hx * sy - hy * sx;
+        Real nx = c * sx + f * hx + g * cx;
+        Real ny = c * sy + f * hy + g * cy;
+        Real nz = c * sz + f * hz + g * cz;
+        if constexpr (Renorm<Real>::value) {
+            const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
+            nx /= nrm;
+            ny /= nrm;
+            nz /= nrm;
+        }
+        Sx[site] = nx;
+        Sy[site] = ny;
+        Sz[site] = nz;
+        return false;
+    } else if constexpr (KIND == 1) {
+        // Lifted Metropolis, displacement form (MC.cpp:249-281).
+        int8_t* dp = st.dir + size_t(r) * N + site;
+        const int8_t d = *dp;
+        const Real hn = dsqrt<Real>(hx * hx + hy * hy + hz * hz);
+        if (!(hn > Real(0))) {  // no field direction: bounce
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real ex = hx / hn, ey = hy / hn, ez = hz / hn;
+        const Real uu = ex * sx + ey * sy + ez * sz;
+        const Real du = (Real(d) * Real(sqrt(Td / st.T_ref))) * (-dlog<Real>(u24<Real>(u.m[0])));
+        const Real nu = uu + du;
+        if (nu > Real(1) || nu < Real(-1) || fabs(Real(1) - uu * uu) < Real(1e-16)) {
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real f = dsqrt<Real>((Real(1) - nu * nu) / (Real(1) - uu * uu));
+        Real nx = nu * ex + f * (sx - uu * ex);
+        Real ny = nu * ey + f * (sy - uu * ey);
+        Real nz = nu * ez + f * (sz - uu * ez);
+        const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
+        nx /= nrm;
+        ny /= nrm;
+        nz /= nrm;
+        const Real curE = sx * hx + sy * hy + sz * hz;
+        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
+        if (u24<Real>(u.m[1]) < dexp<Real>(-dE / T)) {
+            Sx[site] = nx;
+            Sy[site] = ny;
+            Sz[site] = nz;
+            return true;
+        }
+        *dp = int8_t(-d);
+        return false;
+    } else {
+        // Lifted rotation Metropolis with reflecting poles (MC.cpp:194-244).
+        int8_t* dp = st.dir + size_t(r) * N + site;
+        const int8_t d = *dp;
+        const Real curE = sx * hx + sy * hy + sz * hz;
+        const Real ax = sy * hz - sz * hy, ay = sz * hx - sx * hz, az = sx * hy - sy * hx;
+        const Real a2 = ax * ax + ay * ay + az * az;
+        if (a2 <= Real(1e-20)) {
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real th = Real(d) * Real(sqrt(Td / st.T_ref));
+        Real s, c;
+        dsincos<Real>(th, &s, &c);
+        const Real dd = ax * sx + ay * sy + az * sz;
+        const Real f = ((Real(1) - c) / a2) * dd;
+        const Real g = s / dsqrt<Real>(a2);
+        const Real cx = ay * sz - az * sy, cy = az * sx - ax * sz, cz = ax * sy - ay * sx;
+        Real nx = c * sx + f * ax + g * cx;
+        Real ny = c * sy + f * ay + g * cy;
+        Real nz = c * sz + f * az + g * cz;
+        const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
+        nx /= nrm;
+        ny /= nrm;
+        nz /= nrm;
+        const Real bx = ny * hz - nz * hy, by = nz * hx - nx * hz, bz = nx * hy - ny * hx;
+        const Real b2 = bx * bx + by * by + bz * bz;
+        if (ax * bx + ay * by + az * bz < Real(0)) {
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
+        const Real ratio = dsqrt<Real>(b2 / a2) * dexp<Real>(-dE / T);
+        if (u24<Real>(u.m[0]) < ratio) {
+            Sx[site] = nx;
+            Sy[site] = ny;
+            Sz[site] = nz;
+            return true;
+        }
+        *dp = int8_t(-d);
+        return false;
+    }
+}
+
+// Wave-level count of accepted updates added to acc[r].
+__device__ inline void count_accepts(unsigned long long* acc, int r, bool a) {
+    const unsigned long long mask = __ballot(a);
+    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(acc + r, (unsigned long long)__popcll(mask));
+}
+
+// Strategy 1: one launch per colour; grid.x = classes x blocks-per-class,
+// grid.y = replicas.
+template <typename Real, int NS, bool ISO, int KIND>
+__global__ __launch_bounds__(kBlock) void sweep_colour_kernel(DeviceState<Real> st,
+                                                              const int32_t* __restrict__ cls,
+                                                              int spc, int bpc,
+                                                              unsigned long long pass_index) {
+    const int ci = blockIdx.x / bpc;
+    const int m = (blockIdx.x % bpc) * kBlock + threadIdx.x;
+    const int r = blockIdx.y;
+    const int desc = __builtin_amdgcn_readfirstlane(cls[ci]);
+    const int sl = desc & 0xFFFF;
+    const unsigned long long pass =
+        pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    bool accepted = false;
+    if (m < spc) {
+        const int site = decode_site(desc, m, st.n_cells, st.L1, st.L2, st.p0, st.p1, st.p2);
+        accepted = update_site<Real, NS, ISO, KIND>(st, site, sl, r, pass);
+    }
+    if constexpr (KIND != 3) count_accepts(st.acc, r, accepted);
+}
+
+// Strategy 2: one workgroup per replica runs whole sweeps; colours are
+// separated by workgroup barriers (the workgroup owns the replica, so no
+// other workgroup touches its spins).
+template <typename Real, int NS, bool ISO, int FIRST, int BS>
+__global__ __launch_bounds__(BS) void sweep_resident_kernel(
+    DeviceState<Real> st, const int32_t* __restrict__ all_cls,
+    const int32_t* __restrict__ colour_begin, int n_colours, int n_or, int n_sweeps,
+    unsigned long long pass_index) {
+    const int r = blockIdx.x;
+    const int spc = st.n_cells / (st.p0 * st.p1 * st.p2);
+    unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    unsigned int my_acc = 0;
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+        for (int ps = 0; ps <= n_or; ++ps, ++pass) {
+            for (int col = 0; col < n_colours; ++col) {
+                const int cb = colour_begin[col], ce = colour_begin[col + 1];
+                for (int ci = cb; ci < ce; ++ci) {
+                    const int desc = __builtin_amdgcn_readfirstlane(all_cls[ci]);
+                    const int sl = desc & 0xFFFF;
+                    for (int m = threadIdx.x; m < spc; m += BS) {
+                        const int site =
+                            decode_site(desc, m, st.n_cells, st.L1, st.L2, st.p0, st.p1, st.p2);
+                        if (ps == 0) {
+                            my_acc += update_site<Real, NS, ISO, FIRST>(st, site, sl, r, pass) ? 1u : 0u;
+                        } else {
+                            update_site<Real, NS, ISO, 3>(st, site, sl, r, pass);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // block reduction of the accept count
+    __shared__ unsigned int red[BS / 64];
+    unsigned int v = my_acc;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < BS / 64; ++i) t += red[i];
+        atomicAdd(st.acc + r, t);
+    }
+}
+
+template <typename Real, int NS, bool ISO>
+__global__ __launch_bounds__(kBlock) void fields_kernel(DeviceState<Real> st, int r,
+                                                        Real* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= st.N) return;
+    const size_t N = size_t(st.N);
+    const Real* Sx = st.S + size_t(r) * 3 * N;
+    const int sl = i / st.n_cells;
+    Real hx, hy, hz;
+    local_field<Real, NS, ISO>(st, Sx, Sx + N, Sx + 2 * N, i, sl, hx, hy, hz);
+    out[i] = hx;
+    out[N + i] = hy;
+    out[2 * N + i] = hz;
+}
+
+__device__ inline double block_sum(double v) {
+    __shared__ double red[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+    return t;
+}
+
+// Energy terms 1/2 S.h - B.S per site (MC.cpp:305-312), block partial sums.
+template <typename Real, int NS, bool ISO>
+__global__ __launch_bounds__(kBlock) void energy_partial_kernel(DeviceState<Real> st,
+                                                                double* __restrict__ partial) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int r = blockIdx.y;
+    double e = 0.0;
+    if (i < st.N) {
+        const size_t N = size_t(st.N);
+        const Real* Sx = st.S + size_t(r) * 3 * N;
+        const int sl = i / st.n_cells;
+        Real hx, hy, hz;
+        local_field<Real, NS, ISO>(st, Sx, Sx + N, Sx + 2 * N, i, sl, hx, hy, hz);
+        const Real sx = Sx[i], sy = Sx[N + i], sz = Sx[2 * N + i];
+        const Real half = Real(0.5) * (sx * hx + sy * hy + sz * hz);
+        const Real bs = sx * st.B[3 * r] + sy * st.B[3 * r + 1] + sz * st.B[3 * r + 2];
+        e = double(half) - double(bs);
+    }
+    const double t = block_sum(e);
+    if (threadIdx.x == 0) partial[size_t(r) * gridDim.x + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void energy_finish_kernel(
+    const double* __restrict__ partial, int nblk, double n_cells, double* __restrict__ e,
+    const int32_t* __restrict__ slot_of, double* Esum, double* E2sum,
+    unsigned long long* nE) {
+    const int r = blockIdx.x;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) v += partial[size_t(r) * nblk + b];
+    const double t = block_sum(v);
+    if (threadIdx.x == 0) {
+        const double epc = t / n_cells;
+        e[r] = epc;
+        if (Esum) {
+            const int s = slot_of[r];
+            Esum[s] += epc;
+            E2sum[s] += epc * epc;
+            nE[s] += 1ull;
+        }
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void randomize_kernel(DeviceState<Real> st, uint32_t k0,
+                                                           uint32_t k1) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= st.N) return;
+    const Words4 w = philox4x32_10(Words4{uint32_t(i), uint32_t(r), 0u, 0u}, k0, k1);
+    const Uniform5 u = split_uniform5(w);
+    const double z = 2.0 * u24<double>(u.m[0]) - 1.0;
+    const double phi = 6.283185307179586 * u24<double>(u.m[1]);
+    const double s = sqrt(1.0 - z * z);
+    const size_t N = size_t(st.N);
+    Real* Sx = st.S + size_t(r) * 3 * N;
+    Sx[i] = Real(s * cos(phi));
+    Sx[N + i] = Real(s * sin(phi));
+    Sx[2 * N + i] = Real(z);
+    st.dir[size_t(r) * N + i] = 1;
+}
+
+// Replica exchange (temper.cpp:437-473): pairs (s, s+1), s = parity mod 2,
+// arg = (1/T_s - 1/T_{s+1}) (E_s - E_{s+1}) with total energies; accept if
+// arg >= 0 or U < exp(arg).  Pairs are disjoint, so one thread per pair.
+__global__ __launch_bounds__(kBlock) void exchange_kernel(ExchangeArgs a) {
+    const unsigned long long round = *a.round;
+    const int parity = int(round & 1ull);
+    for (int s = parity + 2 * int(threadIdx.x); s + 1 < a.R; s += 2 * kBlock) {
+        const int ra = a.rep_of[s], rb = a.rep_of[s + 1];
+        const double Ea = a.e[ra] * a.n_cells, Eb = a.e[rb] * a.n_cells;
+        const double arg = (1.0 / a.T_slot[s] - 1.0 / a.T_slot[s + 1]) * (Ea - Eb);
+        const Words4 w = philox4x32_10(
+            Words4{uint32_t(s), 0xFFFFFFFFu, uint32_t(round), uint32_t(round >> 32)}, a.k0, a.k1);
+        const bool acc = arg >= 0.0 || u53(w.x, w.y) < exp(arg);
+        a.att[s] += 1ull;
+        if (acc) {
+            a.accp[s] += 1ull;
+            a.rep_of[s] = rb;
+            a.rep_of[s + 1] = ra;
+            a.slot_of[rb] = s;
+            a.slot_of[ra] = s + 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *a.round = round + 1ull;
+        if (a.pass_base) *a.pass_base += a.passes_per_round;
+    }
+}
+
+__global__ void bump_kernel(unsigned long long* ctr, unsigned long long n) { *ctr += n; }
+
+__global__ __launch_bounds__(kBlock) void zero_kernel(uint32_t* p, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * kBlock)
+        p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Structure factor: separable DFT along one axis of every (r, a, mu) field.
+// out[e] = sum_n in[base + n*stride] * W^{K n},  W = exp(-2 pi i / La).
+// ---------------------------------------------------------------------------
+template <typename Real, bool REAL_IN>
+__global__ __launch_bounds__(kBlock) void dft_axis_kernel(const Real* __restrict__ in,
+                                                          Real* __restrict__ out,
+                                                          const Real* __restrict__ tw, int La,
+                                                          int stride, long long total) {
+    const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= total) return;
+    const int K = int((e / stride) % La);
+    const long long base = e - (long long)K * stride;
+    Real re = Real(0), im = Real(0);
+    int j = 0;
+    for (int n = 0; n < La; ++n) {
+        const Real wr = tw[2 * j], wi = tw[2 * j + 1];
+        if constexpr (REAL_IN) {
+            const Real x = in[base + (long long)n * stride];
+            re += x * wr;
+            im += x * wi;
+        } else {
+            const Real xr = in[2 * (base + (long long)n * stride)];
+            const Real xi = in[2 * (base + (long long)n * stride) + 1];
+            re += xr * wr - xi * wi;
+            im += xr * wi + xi * wr;
+        }
+        j += K;
+        if (j >= La) j -= La;
+    }
+    out[2 * e] = re;
+    out[2 * e + 1] = im;
+}
+
+// Contracted correlators (ssf_manager.hpp:260-293 with the separable phase
+// fold): b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K); S_c = conj(b^alpha) b^beta.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void contract_kernel(SsfState<Real> st, CorrArgs c,
+                                                          int r_begin) {
+    const int K = blockIdx.x * kBlock + threadIdx.x;
+    const int rl = blockIdx.y;
+    const int r = r_begin + rl;
+    const int slot = c.slot_of[r];
+    if (slot < 0) return;
+    if (K < st.n_cells) {
+        Real br[3], bi[3];
+        const size_t nc = size_t(st.n_cells);
+        for (int a = 0; a < 3; ++a) {
+            Real sr = Real(0), si = Real(0);
+            const Real* F = st.F + 2 * ((size_t(rl) * 3 + a) * st.n_sl) * nc;
+            for (int mu = 0; mu < st.n_sl; ++mu) {
+                const Real fr = F[2 * (mu * nc + K)], fi = F[2 * (mu * nc + K) + 1];
+                const Real pr = st.phase[2 * (mu * nc + K)], pi = st.phase[2 * (mu * nc + K) + 1];
+                sr += fr * pr - fi * pi;
+                si += fr * pi + fi * pr;
+            }
+            br[a] = sr;
+            bi[a] = si;
+        }
+        for (int cc = 0; cc < c.n_corr; ++cc) {
+            const int al = c.alpha[cc], be = c.beta[cc];
+            const double re = double(br[al] * br[be] + bi[al] * bi[be]);
+            const double im = double(br[al] * bi[be] - bi[al] * br[be]);
+            double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+            A[0] += re;
+            A[1] += im;
+            if (c.acc2) {
+                double* A2 = c.acc2 + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+                A2[0] += re * re;
+                A2[1] += im * im;
+            }
+        }
+    }
+    if (K == 0) c.n_samples[slot] += 1ull;
+}
+
+// ---------------------------------------------------------------------------
+// Sublattice-resolved correlators C^{ab}_{mu nu}(K) = sum conj(A^a_mu) A^b_nu,
+// i.e. the Hermitian 48x48 matrix M = sum_samples a^H a per K.  With
+// X = [Re a | Im a] (one row per sample), X^T X holds RR, RI, IR, II and
+// Re M = RR + II, Im M = RI - IR.  X^T X runs on FP32 MFMA
+// (v_mfma_f32_16x16x4_f32) over the buffered samples and is added to FP64.
+// ---------------------------------------------------------------------------
+
+// Stage the transformed fields of replica (r_begin + blockIdx.y) into the
+// buffer row cnt[slot] of its slot: buf[slot][k][b][2*16*T].
+template <typename Real, int T>
+__global__ __launch_bounds__(kBlock) void munu_stage_kernel(SsfState<Real> st,
+                                                            const int32_t* __restrict__ slot_of,
+                                                            int r_begin, float* __restrict__ buf,
+                                                            const unsigned int* __restrict__ cnt,
+                                                            int Bmax) {
+    constexpr int FW = 2 * 16 * T;  // padded row width
+    const int K = blockIdx.x * kBlock + threadIdx.x;
+    const int rl = blockIdx.y;
+    const int slot = slot_of[r_begin + rl];
+    if (slot < 0 || K >= st.n_cells) return;
+    const unsigned int b = cnt[slot];
+    if (b >= unsigned(Bmax)) return;
+    const size_t nc = size_t(st.n_cells);
+    float* row = buf + ((size_t(slot) * nc + K) * Bmax + b) * FW;
+    const int nf = 3 * st.n_sl;
+    for (int f = 0; f < 16 * T; ++f) {
+        float re = 0.f, im = 0.f;
+        if (f < nf) {
+            const int a = f / st.n_sl, mu = f % st.n_sl;
+            const Real* F = st.F + 2 * (((size_t(rl) * 3 + a) * st.n_sl + mu) * nc + K);
+            re = float(F[0]);
+            im = float(F[1]);
+        }
+        row[f] = re;
+        row[16 * T + f] = im;
+    }
+}
+
+__global__ void munu_count_kernel(const int32_t* __restrict__ slot_of, int r_begin, int r_count,
+                                  unsigned int* cnt, int Bmax) {
+    for (int i = threadIdx.x; i < r_count; i += blockDim.x) {
+        const int s = slot_of[r_begin + i];
+        if (s >= 0 && cnt[s] < unsigned(Bmax)) atomicAdd(cnt + s, 1u);
+    }
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// One wave per (slot, K): X^T X over cnt[slot] buffered rows.
+template <int T>
+__global__ __launch_bounds__(256) void munu_flush_kernel(const float* __restrict__ buf,
+                                                         const unsigned int* __restrict__ cnt,
+                                                         double* __restrict__ acc, int n_k,
+                                                         int Bmax, int n_f) {
+    constexpr int NT = 2 * T;       // 16-wide tiles across [Re | Im]
+    constexpr int FW = 16 * NT;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int K = blockIdx.x * 4 + wave;
+    const int slot = blockIdx.y;
+    if (K >= n_k) return;
+    const int nb = int(cnt[slot]);
+    if (nb == 0) return;
+    v4f D[NT][NT];
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int J = 0; J < NT; ++J) D[I][J] = v4f{0.f, 0.f, 0.f, 0.f};
+    const float* X = buf + (size_t(slot) * n_k + K) * size_t(Bmax) * FW;
+    const int kk = lane >> 4;   // which of the 4 rows of this step
+    const int col = lane & 15;  // feature within a tile
+    for (int b0 = 0; b0 < nb; b0 += 4) {
+        const int b = b0 + kk;
+        float x[NT];
+#pragma unroll
+        for (int I = 0; I < NT; ++I) x[I] = (b < nb) ? X[size_t(b) * FW + 16 * I + col] : 0.f;
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+            for (int J = 0; J < NT; ++J)
+                D[I][J] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[I], x[J], D[I][J], 0, 0, 0);
+    }
+    // D[I][J] element v of this lane: row 16I + 4*(lane>>4) + v, column 16J + (lane&15).
+    double* out = acc + (size_t(slot) * n_k + K) * size_t(n_f) * n_f * 2;
+#pragma unroll
+    for (int I = 0; I < T; ++I)
+#pragma unroll
+        for (int J = 0; J < T; ++J)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int f = 16 * I + 4 * (lane >> 4) + v;
+                const int g = 16 * J + (lane & 15);
+                if (f < n_f && g < n_f) {
+                    const double re = double(D[I][J][v]) + double(D[I + T][J + T][v]);
+                    const double im = double(D[I][J + T][v]) - double(D[I + T][J][v]);
+                    out[2 * (size_t(f) * n_f + g)] += re;
+                    out[2 * (size_t(f) * n_f + g) + 1] += im;
+                }
+            }
+}
+
+__global__ void munu_reset_kernel(unsigned int* cnt, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0u;
+}
+
+template <typename F>
+hipError_t dispatch_ns(int NS, F&& f) {
+    switch (NS) {
+        case 6: return f(std::integral_constant<int, 6>{});
+        case 8: return f(std::integral_constant<int, 8>{});
+        case 18: return f(std::integral_constant<int, 18>{});
+        case 30: return f(std::integral_constant<int, 30>{});
+        case 32: return f(std::integral_constant<int, 32>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+int energy_blocks(int N) { return (N + kBlock - 1) / kBlock; }
+
+template <typename Real>
+hipError_t launch_sweep_colour(const DeviceState<Real>& st, const ColourList& col, int kind,
+                               unsigned long long pass_index, hipStream_t s) {
+    const int spc = st.n_cells / (st.p0 * st.p1 * st.p2);
+    const int bpc = (spc + kBlock - 1) / kBlock;
+    const dim3 grid(unsigned(bpc * col.n_cls), unsigned(st.R));
+    return dispatch_ns(st.NS, [&](auto ns) -> hipError_t {
+        constexpr int NS = decltype(ns)::value;
+        auto go = [&](auto iso) -> hipError_t {
+            constexpr bool ISO = decltype(iso)::value;
+            switch (kind) {
+                case 0: hipLaunchKernelGGL((sweep_colour_kernel<Real, NS, ISO, 0>), grid, dim3(kBlock), 0, s, st, col.cls, spc, bpc, pass_index); break;
+                case 1: hipLaunchKernelGGL((sweep_colour_kernel<Real, NS, ISO, 1>), grid, dim3(kBlock), 0, s, st, col.cls, spc, bpc, pass_index); break;
+                case 2: hipLaunchKernelGGL((sweep_colour_kernel<Real, NS, ISO, 2>), grid, dim3(kBlock), 0, s, st, col.cls, spc, bpc, pass_index); break;
+                case 3: hipLaunchKernelGGL((sweep_colour_kernel<Real, NS, ISO, 3>), grid, dim3(kBlock), 0, s, st, col.cls, spc, bpc, pass_index); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        };
+        return st.iso ? go(std::true_type{}) : go(std::false_type{});
+    });
+}
+
+template <typename Real>
+hipError_t launch_sweep_resident(const DeviceState<Real>& st, const int32_t* all_cls,
+                                 const int32_t* colour_begin, int n_colours, int kind,
+                                 int n_or, int n_sweeps, unsigned long long pass_index,
+                                 hipStream_t s) {
+    constexpr int BS = 512;
+    const dim3 grid(unsigned(st.R));
+    return dispatch_ns(st.NS, [&](auto ns) -> hipError_t {
+        constexpr int NS = decltype(ns)::value;
+        auto go = [&](auto iso) -> hipError_t {
+            constexpr bool ISO = decltype(iso)::value;
+            switch (kind) {
+                case 0: hipLaunchKernelGGL((sweep_resident_kernel<Real, NS, ISO, 0, BS>), grid, dim3(BS), 0, s, st, all_cls, colour_begin, n_colours, n_or, n_sweeps, pass_index); break;
+                case 1: hipLaunchKernelGGL((sweep_resident_kernel<Real, NS, ISO, 1, BS>), grid, dim3(BS), 0, s, st, all_cls, colour_begin, n_colours, n_or, n_sweeps, pass_index); break;
+                case 2: hipLaunchKernelGGL((sweep_resident_kernel<Real, NS, ISO, 2, BS>), grid, dim3(BS), 0, s, st, all_cls, colour_begin, n_colours, n_or, n_sweeps, pass_index); break;
+                case 3: hipLaunchKernelGGL((sweep_resident_kernel<Real, NS, ISO, 3, BS>), grid, dim3(BS), 0, s, st, all_cls, colour_begin, n_colours, n_or, n_sweeps, pass_index); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        };
+        return st.iso ? go(std::true_type{}) : go(std::false_type{});
+    });
+}
+
+template <typename Real>
+hipError_t launch_fields(const DeviceState<Real>& st, int r, Real* out, hipStream_t s) {
+    const dim3 grid(unsigned((st.N + kBlock - 1) / kBlock));
+    return dispatch_ns(st.NS, [&](auto ns) -> hipError_t {
+        constexpr int NS = decltype(ns)::value;
+        if (st.iso) hipLaunchKernelGGL((fields_kernel<Real, NS, true>), grid, dim3(kBlock), 0, s, st, r, out);
+        else hipLaunchKernelGGL((fields_kernel<Real, NS, false>), grid, dim3(kBlock), 0, s, st, r, out);
+        return hipGetLastError();
+    });
+}
+
+template <typename Real>
+hipError_t launch_energy(const DeviceState<Real>& st, double* partial, double* e, double* Esum,
+                         double* E2sum, unsigned long long* nE, hipStream_t s) {
+    const int nblk = energy_blocks(st.N);
+    const dim3 grid(unsigned(nblk), unsigned(st.R));
+    hipError_t err = dispatch_ns(st.NS, [&](auto ns) -> hipError_t {
+        constexpr int NS = decltype(ns)::value;
+        if (st.iso) hipLaunchKernelGGL((energy_partial_kernel<Real, NS, true>), grid, dim3(kBlock), 0, s, st, partial);
+        else hipLaunchKernelGGL((energy_partial_kernel<Real, NS, false>), grid, dim3(kBlock), 0, s, st, partial);
+        return hipGetLastError();
+    });
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(energy_finish_kernel, dim3(unsigned(st.R)), dim3(kBlock), 0, s, partial,
+                       nblk, double(st.n_cells), e, st.slot_of, Esum, E2sum, nE);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_randomize(const DeviceState<Real>& st, uint32_t k0, uint32_t k1,
+                            hipStream_t s) {
+    const dim3 grid(unsigned((st.N + kBlock - 1) / kBlock), unsigned(st.R));
+    hipLaunchKernelGGL(randomize_kernel<Real>, grid, dim3(kBlock), 0, s, st, k0, k1);
+    return hipGetLastError();
+}
+
+hipError_t launch_exchange(const ExchangeArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(exchange_kernel, dim3(1), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bump(unsigned long long* ctr, unsigned long long n, hipStream_t s) {
+    hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(1), 0, s, ctr, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
+    if (n_words32 == 0) return hipSuccess;
+    size_t blocks = (n_words32 + kBlock - 1) / kBlock;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s,
+                       static_cast<uint32_t*>(p), n_words32);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStream_t s) {
+    const size_t nc = size_t(st.n_cells);
+    const long long total = (long long)r_count * 3 * st.n_sl * (long long)nc;
+    const unsigned blocks = unsigned((total + kBlock - 1) / kBlock);
+    const Real* spins = st.S + size_t(r_begin) * 3 * st.n_sl * nc;
+    // axis 2 (stride 1): spins -> F; axis 1 (stride L2): F -> G; axis 0: G -> F
+    hipLaunchKernelGGL((dft_axis_kernel<Real, true>), dim3(blocks), dim3(kBlock), 0, s, spins, st.F,
+                       st.tw2, st.L2, 1, total);
+    hipLaunchKernelGGL((dft_axis_kernel<Real, false>), dim3(blocks), dim3(kBlock), 0, s, st.F, st.G,
+                       st.tw1, st.L1, st.L2, total);
+    hipLaunchKernelGGL((dft_axis_kernel<Real, false>), dim3(blocks), dim3(kBlock), 0, s, st.G, st.F,
+                       st.tw0, st.L0, st.L1 * st.L2, total);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_begin, int r_count,
+                           hipStream_t s) {
+    const dim3 grid(unsigned((st.n_cells + kBlock - 1) / kBlock), unsigned(r_count));
+    hipLaunchKernelGGL(contract_kernel<Real>, grid, dim3(kBlock), 0, s, st, c, r_begin);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
+                             int r_count, const MunuArgs& m, hipStream_t s) {
+    const dim3 grid(unsigned((st.n_cells + kBlock - 1) / kBlock), unsigned(r_count));
+    switch (m.T) {
+        case 1: hipLaunchKernelGGL((munu_stage_kernel<Real, 1>), grid, dim3(kBlock), 0, s, st, slot_of, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 2: hipLaunchKernelGGL((munu_stage_kernel<Real, 2>), grid, dim3(kBlock), 0, s, st, slot_of, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 3: hipLaunchKernelGGL((munu_stage_kernel<Real, 3>), grid, dim3(kBlock), 0, s, st, slot_of, r_begin, m.buf, m.cnt, m.Bmax); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(munu_count_kernel, dim3(1), dim3(kBlock), 0, s, slot_of, r_begin, r_count,
+                       m.cnt, m.Bmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_munu_flush(const MunuArgs& m, hipStream_t s) {
+    const dim3 grid(unsigned((m.n_k + 3) / 4), unsigned(m.n_slots));
+    switch (m.T) {
+        case 1: hipLaunchKernelGGL((munu_flush_kernel<1>), grid, dim3(256), 0, s, m.buf, m.cnt, m.acc, m.n_k, m.Bmax, m.n_f); break;
+        case 2: hipLaunchKernelGGL((munu_flush_kernel<2>), grid, dim3(256), 0, s, m.buf, m.cnt, m.acc, m.n_k, m.Bmax, m.n_f); break;
+        case 3: hipLaunchKernelGGL((munu_flush_kernel<3>), grid, dim3(256), 0, s, m.buf, m.cnt, m.acc, m.n_k, m.Bmax, m.n_f); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(munu_reset_kernel, dim3(1), dim3(kBlock), 0, s, m.cnt, m.n_slots);
+    return hipGetLastError();
+}
+
+}  // namespace pyro
+
+// Explicit instantiations.
+namespace pyro {
+template hipError_t launch_sweep_colour<float>(const DeviceState<float>&, const ColourList&, int, unsigned long long, hipStream_t);
+template hipError_t launch_sweep_colour<double>(const DeviceState<double>&, const ColourList&, int, unsigned long long, hipStream_t);
+template hipError_t launch_sweep_resident<float>(const DeviceState<float>&, const int32_t*, const int32_t*, int, int, int, int, unsigned long long, hipStream_t);
+template hipError_t launch_sweep_resident<double>(const DeviceState<double>&, const int32_t*, const int32_t*, int, int, int, int, unsigned long long, hipStream_t);
+template hipError_t launch_fields<float>(const DeviceState<float>&, int, float*, hipStream_t);
+template hipError_t launch_fields<double>(const DeviceState<double>&, int, double*, hipStream_t);
+template hipError_t launch_energy<float>(const DeviceState<float>&, double*, double*, double*, double*, unsigned long long*, hipStream_t);
+template hipError_t launch_energy<double>(const DeviceState<double>&, double*, double*, double*, double*, unsigned long long*, hipStream_t);
+template hipError_t launch_randomize<float>(const DeviceState<float>&, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_randomize<double>(const DeviceState<double>&, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_dft<float>(const SsfState<float>&, int, int, hipStream_t);
+template hipError_t launch_dft<double>(const SsfState<double>&, int, int, hipStream_t);
+template hipError_t launch_contract<float>(const SsfState<float>&, const CorrArgs&, int, int, hipStream_t);
+template hipError_t launch_contract<double>(const SsfState<double>&, const CorrArgs&, int, int, hipStream_t);
+template hipError_t launch_munu_stage<float>(const SsfState<float>&, const int32_t*, int, int, const MunuArgs&, hipStream_t);
+template hipError_t launch_munu_stage<double>(const SsfState<double>&, const int32_t*, int, int, const MunuArgs&, hipStream_t);
+}  // namespace pyro

@@ -1,0 +1,144 @@
+// Launch interface between the context code (capi.cpp) and the HIP kernels
+// (kernels.hip).  Everything here is host-callable; the kernels themselves
+// stay private to kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pyro {
+
+// Device view of a context's sweep state, shared by the sweep, field and
+// energy kernels.  Spins are SoA per replica: S[r][c][i], c = x, y, z.
+template <typename Real>
+struct DeviceState {
+    Real* S = nullptr;                   // [R][3][N]
+    int8_t* dir = nullptr;               // [R][N]
+    const int32_t* nbr = nullptr;        // [NS][N] slot-major neighbour table
+    const Real* cpl = nullptr;           // isotropic [n_sl][NS] else [n_sl][NS][9]
+    const double* T_slot = nullptr;      // [R] temperature of each slot
+    const int32_t* slot_of = nullptr;    // [R] slot of each replica
+    const Real* B = nullptr;             // [R][3] global field per replica
+    unsigned long long* acc = nullptr;   // [R] accepted-update counters
+    const unsigned long long* pass_base = nullptr;  // device pass counter (graph mode)
+    uint32_t k0 = 0, k1 = 0;             // RNG key
+    double T_ref = 1.0;
+    int N = 0, n_cells = 0, R = 0, NS = 0;
+    int L0 = 0, L1 = 0, L2 = 0;
+    int p0 = 1, p1 = 1, p2 = 1;          // colour residue periods
+    bool iso = true;
+};
+
+// One colour = a list of classes (sublattice, residues), packed as
+// sl | r0 << 16 | r1 << 20 | r2 << 24.
+struct ColourList {
+    const int32_t* cls = nullptr;  // device array of class descriptors
+    int n_cls = 0;
+};
+
+// One pass of `kind` over one colour for every replica (one launch).
+// pass_index is added to *pass_base when pass_base is set.
+template <typename Real>
+hipError_t launch_sweep_colour(const DeviceState<Real>& st, const ColourList& col, int kind,
+                               unsigned long long pass_index, hipStream_t s);
+
+// A whole sequence of passes (first kind, then n_or overrelaxation passes,
+// repeated n_sweeps times) over all colours, one resident workgroup per
+// replica synchronising with workgroup barriers between colours.
+template <typename Real>
+hipError_t launch_sweep_resident(const DeviceState<Real>& st, const int32_t* all_cls,
+                                 const int32_t* colour_begin, int n_colours, int kind,
+                                 int n_or, int n_sweeps, unsigned long long pass_index,
+                                 hipStream_t s);
+
+// Local fields (no global field) of replica r into out[3][N].
+template <typename Real>
+hipError_t launch_fields(const DeviceState<Real>& st, int r, Real* out, hipStream_t s);
+
+// Energy per cell of every replica -> e[R]; partial[R][nblk] is scratch.
+// If slot sums are given, also adds e and e^2 to the replica's slot.
+template <typename Real>
+hipError_t launch_energy(const DeviceState<Real>& st, double* partial, double* e,
+                         double* Esum, double* E2sum, unsigned long long* nE, hipStream_t s);
+int energy_blocks(int N);
+
+// Uniform-on-sphere spins, lifted directions +1.
+template <typename Real>
+hipError_t launch_randomize(const DeviceState<Real>& st, uint32_t k0, uint32_t k1, hipStream_t s);
+
+// Replica exchange of slot pairs (s, s+1), s = round parity mod 2; uses the
+// per-replica energies e[R] (per cell).  Increments *round and adds
+// passes_per_round to *pass_base when pass_base is non-null.
+struct ExchangeArgs {
+    const double* e = nullptr;
+    int32_t* rep_of = nullptr;
+    int32_t* slot_of = nullptr;
+    const double* T_slot = nullptr;
+    unsigned long long* att = nullptr;
+    unsigned long long* accp = nullptr;
+    unsigned long long* round = nullptr;
+    unsigned long long* pass_base = nullptr;
+    unsigned long long passes_per_round = 0;
+    double n_cells = 1.0;
+    uint32_t k0 = 0, k1 = 0;
+    int R = 0;
+};
+hipError_t launch_exchange(const ExchangeArgs& a, hipStream_t s);
+
+// Adds n to *ctr on the device (graph-safe counter bump).
+hipError_t launch_bump(unsigned long long* ctr, unsigned long long n, hipStream_t s);
+// Sets n words to zero (graph-safe replacement for a memset).
+hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Static structure factor.
+// ---------------------------------------------------------------------------
+template <typename Real>
+struct SsfState {
+    const Real* S = nullptr;         // [R][3][N]
+    Real* F = nullptr;               // [R][3][n_sl][n_cells] complex (2 Real)
+    Real* G = nullptr;               // same, scratch
+    const Real* tw0 = nullptr;       // twiddles per axis, [L_a] complex
+    const Real* tw1 = nullptr;
+    const Real* tw2 = nullptr;
+    const Real* phase = nullptr;     // [n_sl][n_cells] complex e^{-i q(K).r_mu}
+    int R = 0, n_sl = 0, n_cells = 0, L0 = 0, L1 = 0, L2 = 0;
+};
+
+// Per-sublattice DFT of all replicas' spins: F[r][a][mu][K] (complex).
+template <typename Real>
+hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStream_t s);
+
+// Contracted correlators of the transformed replicas, accumulated into
+// acc[slot][c][K][2] (and acc2 if non-null); n_samples[slot] += 1.
+struct CorrArgs {
+    const int32_t* slot_of = nullptr;   // [R] slot per replica (<0: skip)
+    const int32_t* alpha = nullptr;     // [n_corr]
+    const int32_t* beta = nullptr;      // [n_corr]
+    double* acc = nullptr;              // [n_slots][n_corr][n_k][2]
+    double* acc2 = nullptr;             // same or null
+    unsigned long long* n_samples = nullptr;  // [n_slots]
+    int n_corr = 0;
+};
+template <typename Real>
+hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_begin,
+                           int r_count, hipStream_t s);
+
+// Sublattice-resolved correlators.  The transformed fields of each sampled
+// replica are staged as one FP32 row X = [Re a | Im a] (a_f, f = a*n_sl+mu,
+// each half padded to 16*T) into buf[slot][k][cnt[slot]][32*T]; a flush
+// accumulates X^T X per (slot, k) on FP32 MFMA and adds
+// Re M = RR + II, Im M = RI - IR into acc[slot][k][n_f][n_f][2] (FP64),
+// M = sum conj(a)^T a, then clears the counts.
+struct MunuArgs {
+    float* buf = nullptr;
+    unsigned int* cnt = nullptr;   // [n_slots]
+    double* acc = nullptr;
+    int Bmax = 0, n_k = 0, n_slots = 0, n_f = 0, T = 0;
+};
+template <typename Real>
+hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
+                             int r_count, const MunuArgs& m, hipStream_t s);
+hipError_t launch_munu_flush(const MunuArgs& m, hipStream_t s);
+
+}  // namespace pyro

@@ -1,0 +1,302 @@
+"""GPU parity: the HIP library against the CPU oracle on the same inputs.
+
+Fixed configurations (random, ferromagnetic, spiral_byhand) give local
+fields, energies and structure factors to 1e-12 (FP64) and 1e-5 (FP32),
+relative to the natural scale of each quantity.  Sweeps are compared draw for
+draw: the oracle's counter mode visits the product's colours in order with
+the same Philox counters, so FP64 trajectories agree to rounding and every
+accept/reject decision and lifted direction matches.
+"""
+import numpy as np
+import pytest
+
+from helpers import byhand_spiral, random_spins
+
+pytestmark = pytest.mark.gpu
+
+TOL = {64: 1e-12, 32: 1e-5}
+
+
+def make(gpu, oracle, L, J=(1.0, 0.0, 0.0), Delta=1.0, prec=64, R=1, seed=1234, B=None, T_ref=1.0):
+    ctx = gpu.Context.pyrochlore(L, n_replicas=R, precision=prec, seed=seed)
+    ctx.define_J1J2J3(*J, Delta)
+    ctx.setup()
+    if B is not None:
+        ctx.set_field(np.asarray(B, float))
+    ctx.set_T_ref(T_ref)
+    os_ = []
+    for r in range(R):
+        o = oracle.Oracle.pyrochlore(L, *J, Delta)
+        if B is not None:
+            o.set_field(np.asarray(B, float))
+        o.set_T_ref(T_ref)
+        os_.append(o)
+    return ctx, os_
+
+
+def field_scale(J, B=None):
+    J1, J2, J3 = J
+    return 6 * abs(J1) + 12 * abs(J2) + 12 * abs(J3) + (0 if B is None else np.linalg.norm(B))
+
+
+CONFIGS = [
+    ((1.0, 0.0, 0.0), 1.0, None),
+    ((1.0, -0.05, 0.05), 1.0, None),
+    ((1.0, 0.1, 0.1), 1.0, (0.0, 0.0, 0.2)),
+    ((-1.0, 0.2, -0.1), 0.3, (0.1, -0.4, 0.25)),  # XXZ anisotropy + field
+]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("J,Delta,B", CONFIGS)
+@pytest.mark.parametrize("state", ["random", "spiral", "ferro"])
+def test_fields_and_energy(gpu, oracle, prec, J, Delta, B, state):
+    L = 3
+    ctx, (o,) = make(gpu, oracle, L, J, Delta, prec, B=B)
+    pos = o.positions()
+    if state == "random":
+        S = random_spins(ctx.N, np.random.default_rng(42))
+    elif state == "spiral":
+        S = byhand_spiral(pos, [2 * np.pi / (8 * L), 0, 2 * 2 * np.pi / (8 * L)])
+    else:
+        S = np.tile(np.array([2.0, 1.0, -2.0]) / 3, (ctx.N, 1))
+    ctx.set_spins(S)
+    o.set_spins(S)
+    h = ctx.local_fields()
+    want = o.local_fields()
+    scale = field_scale(J)
+    assert np.max(np.abs(h - want)) <= TOL[prec] * scale
+    e = ctx.energy_per_cell()[0]
+    e_ref = o.energy_per_cell()
+    e_scale = 16 * 0.5 * field_scale(J, B)  # sum of |terms| bound per cell
+    assert abs(e - e_ref) <= TOL[prec] * e_scale
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_randomize(gpu, oracle, prec):
+    ctx, os_ = make(gpu, oracle, 2, prec=prec, R=3)
+    ctx.randomize(987654321)
+    for r, o in enumerate(os_):
+        o.randomize_ctr(987654321, r)
+        S = ctx.get_spins(r)
+        assert np.max(np.abs(S - o.get_spins())) <= (1e-15 if prec == 64 else 1e-7)
+        assert np.allclose(np.linalg.norm(S, axis=1), 1.0, atol=1e-6)
+        assert np.all(ctx.get_lifted_dir(r) == 1)
+
+
+def _oracle_sweeps(o, r, kind, n_or, n_sweeps, T, colours, n_col, seed, pass0):
+    acc = 0
+    p = pass0
+    for _ in range(n_sweeps):
+        acc += o.pass_ctr(kind, T, colours, n_col, seed, r, p)
+        p += 1
+        for _ in range(n_or):
+            o.pass_ctr(3, T, colours, n_col, seed, r, p)
+            p += 1
+    return acc
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+@pytest.mark.parametrize("kind,n_or", [(0, 1), (0, 2), (1, 1), (2, 0), (1, 0)])
+@pytest.mark.parametrize("J,Delta,B", [CONFIGS[1], CONFIGS[3]])
+def test_sweep_trajectory_fp64(gpu, oracle, strategy, kind, n_or, J, Delta, B):
+    L, R, seed = 2, 3, 0xC0FFEE1234
+    T = np.array([0.3, 0.8, 2.5])
+    ctx, os_ = make(gpu, oracle, L, J, Delta, 64, R=R, seed=seed, B=B, T_ref=0.7)
+    ctx.set_strategy(strategy)
+    ctx.set_temperatures(T)
+    rng = np.random.default_rng(5)
+    dirs = []
+    for r in range(R):
+        S = random_spins(ctx.N, rng)
+        d = rng.choice(np.array([-1, 1], np.int8), ctx.N)
+        ctx.set_spins(S, r)
+        ctx.set_lifted_dir(d, r)
+        os_[r].set_spins(S)
+        os_[r].set_dir(d)
+    colours = ctx.colouring()
+    n_col = ctx.info()["n_colours"]
+    pass0 = ctx.pass_count()
+    acc = ctx.sweep(kind, n_or, 3)
+    for r, o in enumerate(os_):
+        want_acc = _oracle_sweeps(o, r, kind, n_or, 3, T[r], colours, n_col, seed, pass0)
+        assert int(acc[r]) == want_acc
+        assert np.max(np.abs(ctx.get_spins(r) - o.get_spins())) < 1e-12
+        assert np.array_equal(ctx.get_lifted_dir(r), o.get_dir())
+    assert ctx.pass_count() == pass0 + 3 * (1 + n_or)
+
+
+@pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 1), (2, 1)])
+def test_sweep_fp32_tracks_fp64(gpu, oracle, kind, n_or):
+    L, seed, T = 3, 77, 0.6
+    J = (1.0, -0.05, 0.05)
+    ctx, (o,) = make(gpu, oracle, L, J, 1.0, 32, seed=seed)
+    ctx.set_temperatures([T])
+    S = random_spins(ctx.N, np.random.default_rng(9))
+    ctx.set_spins(S)
+    o.set_spins(S)
+    colours = ctx.colouring()
+    n_col = ctx.info()["n_colours"]
+    ctx.sweep(kind, n_or, 1)
+    _oracle_sweeps(o, 0, kind, n_or, 1, T, colours, n_col, seed, 0)
+    diff = np.abs(ctx.get_spins() - o.get_spins()).max(1)
+    # identical uniforms: only decisions within FP32 rounding of the
+    # threshold may differ, so almost every site agrees to FP32 precision
+    assert np.mean(diff > 1e-4) < 0.01
+    assert np.median(diff) < 1e-5
+    assert np.allclose(np.linalg.norm(ctx.get_spins(), axis=1), 1.0, atol=1e-6)
+
+
+def test_strategies_bit_identical(gpu):
+    res = []
+    for strategy in (1, 2):
+        ctx = gpu.Context.pyrochlore(3, n_replicas=4, precision=32, seed=99)
+        ctx.define_J1J2J3(1.0, 0.1, 0.1)
+        ctx.setup()
+        ctx.set_strategy(strategy)
+        ctx.set_temperatures([0.1, 0.3, 1.0, 3.0])
+        ctx.randomize(5)
+        acc = ctx.sweep(gpu.LIFTED, 1, 4)
+        res.append((acc, [ctx.get_spins(r) for r in range(4)]))
+    assert np.array_equal(res[0][0], res[1][0])
+    for a, b in zip(res[0][1], res[1][1]):
+        assert np.array_equal(a, b)
+
+
+def test_autotuner_keeps_results(gpu):
+    out = []
+    for strategy in (0, 1):
+        ctx = gpu.Context.pyrochlore(2, n_replicas=8, precision=32, seed=3)
+        ctx.define_J1J2J3(1.0)
+        ctx.setup()
+        ctx.set_strategy(strategy)
+        ctx.set_temperatures(np.linspace(0.2, 2, 8))
+        ctx.randomize(1)
+        acc = ctx.sweep(gpu.METROPOLIS, 2, 3)
+        out.append((acc, ctx.get_spins(7), ctx.pass_count()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_tempering_rounds(gpu, oracle, prec):
+    L, R, swap_seed = 2, 6, 4242
+    T = np.geomspace(3.0, 0.2, R)
+    ctx, os_ = make(gpu, oracle, L, (1.0, 0.1, 0.0), 1.0, prec, R=R)
+    rng = np.random.default_rng(1)
+    for r in range(R):
+        S = random_spins(ctx.N, rng) if r % 2 else np.tile([0, 0, 1.0], (ctx.N, 1))
+        ctx.set_spins(S, r)
+        os_[r].set_spins(S)
+    ctx.pt_init(T, swap_seed)
+    e = np.array([o.energy_per_cell() for o in os_])
+    rep = np.arange(R, dtype=np.int32)
+    att = np.zeros(R, np.uint64)
+    accp = np.zeros(R, np.uint64)
+    Es = np.zeros(R)
+    E2s = np.zeros(R)
+    n_rounds = 7
+    for rnd in range(n_rounds):
+        for s in range(R):
+            Es[s] += e[rep[s]]
+            E2s[s] += e[rep[s]] ** 2
+        rep = oracle.exchange(T, e, L ** 3, rep, rnd, swap_seed, att, accp)
+    ctx.pt_rounds(n_sweep=0, n_rounds=n_rounds)
+    st = ctx.pt_stats()
+    assert np.array_equal(ctx.pt_permutation(), rep)
+    assert np.array_equal(st["attempts"], att[: R - 1])
+    assert np.array_equal(st["accepts"], accp[: R - 1])
+    assert np.all(st["n"] == n_rounds)
+    tol = TOL[prec] * 16 * 8
+    assert np.allclose(st["E"], Es, rtol=0, atol=tol * n_rounds)
+    assert np.allclose(st["E2"], E2s, rtol=0, atol=tol * n_rounds * 100)
+
+
+def test_tempering_graph_equals_eager(gpu):
+    out = []
+    for prof in (False, True):
+        ctx = gpu.Context.pyrochlore(2, n_replicas=8, precision=32, seed=17)
+        ctx.define_J1J2J3(1.0, -0.05, 0.05)
+        ctx.setup()
+        ctx.randomize(3)
+        ctx.pt_init(np.geomspace(2.0, 0.05, 8), 9)
+        ctx.ssf_configure(("xx", "yy", "zz"), n_slots=8)
+        ctx.profile(prof)
+        ctx.pt_rounds(gpu.METROPOLIS, 2, 4, True, 5)
+        st = ctx.pt_stats()
+        out.append((ctx.pt_permutation(), st["E"], ctx.get_spins(3), ctx.ssf_read(2)[0]))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_ssf_contracted(gpu, oracle, prec):
+    L, R = 3, 2
+    corr = ("xx", "yy", "zz", "xy", "xz", "yz", "yx", "zx", "zy")
+    ctx, os_ = make(gpu, oracle, L, prec=prec, R=R)
+    ctx.ssf_configure(corr, n_slots=2, store_error_term=True)
+    rng = np.random.default_rng(8)
+    want = np.zeros((2, len(corr), L ** 3, 2))
+    want2 = np.zeros_like(want)
+    pos = os_[0].positions()
+    for step in range(3):
+        for r in range(R):
+            S = random_spins(ctx.N, rng) if step < 2 else byhand_spiral(pos, [0, 0, 2 * np.pi / (8 * L)])
+            ctx.set_spins(S, r)
+            os_[r].set_spins(S)
+            s = os_[r].ssf_sample(corr)
+            want[1 - r] += s
+            want2[1 - r] += s ** 2
+        ctx.ssf_sample([1, 0])  # replica 0 -> slot 1, replica 1 -> slot 0
+    N2 = ctx.N ** 2
+    for slot in range(2):
+        S, S2, n = ctx.ssf_read(slot)
+        assert n == 3
+        assert np.max(np.abs(S - want[slot])) <= TOL[prec] * N2 * 3
+        assert np.max(np.abs(S2 - want2[slot])) <= TOL[prec] * N2 * N2 * 3 * 4
+    ft = ctx.spin_ft(1)
+    assert np.max(np.abs(ft - os_[1].spin_ft())) <= TOL[prec] * L ** 3
+
+
+def test_ssf_shared_slot_layers(gpu, oracle):
+    """Several replicas feeding one slot are accumulated without races."""
+    L, R = 2, 4
+    ctx, os_ = make(gpu, oracle, L, R=R)
+    ctx.ssf_configure(("xx", "zz"), n_slots=1)
+    rng = np.random.default_rng(2)
+    want = 0
+    for r in range(R):
+        S = random_spins(ctx.N, rng)
+        ctx.set_spins(S, r)
+        os_[r].set_spins(S)
+        want = want + os_[r].ssf_sample(("xx", "zz"))
+    ctx.ssf_sample([0, 0, 0, 0])
+    S, _, n = ctx.ssf_read(0)
+    assert n == R
+    assert np.max(np.abs(S - want)) <= 1e-12 * ctx.N ** 2 * R
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_ssf_munu_mfma(gpu, oracle, prec):
+    L, R = 2, 2
+    corr = ("xx", "yz", "zz")
+    ctx, os_ = make(gpu, oracle, L, prec=prec, R=R)
+    ctx.ssf_configure(corr, n_slots=2, munu=True, munu_buffer=3)
+    rng = np.random.default_rng(11)
+    want = np.zeros((2, len(corr), L ** 3, 16, 16, 2))
+    n_samp = 5  # > buffer: one flush in the middle, one at read
+    for _ in range(n_samp):
+        for r in range(R):
+            S = random_spins(ctx.N, rng)
+            ctx.set_spins(S, r)
+            os_[r].set_spins(S)
+            _, m = os_[r].ssf_sample(corr, munu=True)
+            want[r] += m
+        ctx.ssf_sample([0, 1])
+    # FP32 products: error ~ 1e-7 relative to |A|^2 ~ L^3 per term
+    tol = 2e-5 * L ** 3 * n_samp
+    for slot in range(2):
+        for c in range(len(corr)):
+            got = ctx.ssf_read_munu(slot, c)
+            assert np.max(np.abs(got - want[slot, c])) <= tol
