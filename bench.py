@@ -110,7 +110,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--strategy", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 << 28}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the tempering "

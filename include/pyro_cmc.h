@@ -87,6 +87,13 @@ int pyro_define_coupling(pyro_ctx* ctx, const char* name, int n_kinds,
 int pyro_define_J1J2J3(pyro_ctx* ctx, double J1, double J2, double J3,
                        double Delta);
 
+/* Pyrochlore neighbour displacement tables by kind p = 0..3
+ * (pyrochlore_geometry.hpp:110-149): which = 1 nn1 (|d|^2 = 8), 2 nn2 (24),
+ * 3 nn3a (32, bond midpoint on a site), 4 nn3b (32, midpoint empty).
+ * counts[4] receives the list lengths, vecs up to cap int32 triples.
+ * Host-only; needs no device. */
+int pyro_pyrochlore_shell(int which, int32_t counts[4], int32_t* vecs, int cap);
+
 /* MC_runner::setup_lattice (MC.cpp:15-52): builds the int32 neighbour
  * table, per-slot coupling table and the graph colouring, uploads them. */
 int pyro_setup_lattice(pyro_ctx* ctx);

@@ -34,6 +34,7 @@ _SIGS = {
     "pyro_define_coupling": [_vp, C.c_char_p, C.c_int, _i32p, _i32p, _f64p],
     "pyro_define_J1J2J3": [_vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "pyro_setup_lattice": [_vp],
+    "pyro_pyrochlore_shell": [C.c_int, _i32p, _vp, C.c_int],
     "pyro_info": [_vp, _i64p],
     "pyro_site_positions": [_vp, _i32p],
     "pyro_neighbours": [_vp, _vp, _vp],

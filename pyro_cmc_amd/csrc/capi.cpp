@@ -506,6 +506,22 @@ int pyro_define_J1J2J3(pyro_ctx* ctx, double J1, double J2, double J3, double De
     });
 }
 
+int pyro_pyrochlore_shell(int which, int32_t counts[4], int32_t* vecs, int cap) {
+    return guarded([&] {
+        require(which >= 1 && which <= 4 && counts, PYRO_E_INVALID, "which must be 1..4");
+        const auto sh = pyro::pyrochlore_shell(which);
+        int off = 0;
+        for (int p = 0; p < 4; ++p) {
+            counts[p] = int32_t(sh[p].size());
+            for (const auto& d : sh[p]) {
+                if (vecs && off < cap)
+                    for (int a = 0; a < 3; ++a) vecs[3 * off + a] = d[a];
+                ++off;
+            }
+        }
+    });
+}
+
 int pyro_setup_lattice(pyro_ctx* ctx) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
