@@ -217,9 +217,15 @@ int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft);
 /* ------------------------------------------------------------------ */
 int pyro_synchronize(pyro_ctx* ctx);
 /* Sweep launch strategy: 0 = autotuned (default), 1 = one launch per
- * colour class, 2 = one resident workgroup per replica looping over all
- * colours of a sweep.  Returns PYRO_E_UNSUPPORTED where a strategy does
- * not apply. */
+ * colour (neighbour table), 2 = one resident workgroup per replica looping
+ * over all colours of a sweep (neighbour table), 3 = one launch per colour
+ * with the compile-time pyrochlore stencil, 4 = resident workgroup per
+ * replica with the stencil (at most 2048 cells).  3 and 4 apply when the
+ * couplings were defined by pyro_define_J1J2J3 alone.  Strategies 1 and 2
+ * give bit-identical results, as do 3 and 4 (same colour order, per-site
+ * counters and arithmetic); the autotuner only chooses within a pair, so it
+ * never changes results.  Sweeps report PYRO_E_UNSUPPORTED for a forced
+ * strategy that does not apply. */
 int pyro_set_strategy(pyro_ctx* ctx, int strategy);
 int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
 /* Kernel timing with HIP events on the context's stream.  When enabled,

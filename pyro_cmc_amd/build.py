@@ -26,7 +26,7 @@ HDF5_ROOT = Path(os.environ.get("PYRO_HDF5_ROOT", "/opt/conda"))
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-fPIC",
     # keep packed-FP32 instructions out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -34,7 +34,15 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["kernels.hip", "capi.cpp", "topology.cpp"]
+SOURCES = [
+    "kernels.hip",
+    "kernels_pyro_colour_f32.hip",
+    "kernels_pyro_colour_f64.hip",
+    "kernels_pyro_resident_f32.hip",
+    "kernels_pyro_resident_f64.hip",
+    "capi.cpp",
+    "topology.cpp",
+]
 
 
 def _run(cmd: list[str]) -> None:

@@ -114,7 +114,15 @@ struct pyro_ctx {
     bool pt_ready = false;
     uint32_t swap_k0 = 0, swap_k1 = 0;
 
-    int strategy = 0;          // requested: 0 auto, 1 per-colour, 2 resident
+    // pyrochlore J1-J2-J3 couplings defined through pyro_define_J1J2J3 alone:
+    // the compile-time-stencil kernels apply
+    bool spec = false;
+    double sJ1 = 0, sJ2 = 0, sJ3 = 0, sDelta = 1;
+    int spec_shells = 0;
+
+    // requested: 0 auto, 1 generic per-colour, 2 generic resident,
+    // 3 pyrochlore per-colour, 4 pyrochlore resident
+    int strategy = 0;
     int tuned_key = -1;        // (kind, n_or) the auto choice was made for
     int tuned_strategy = 1;
 
@@ -234,6 +242,19 @@ struct pyro_ctx {
         return st;
     }
     template <typename Real>
+    pyro::PyroCouplings<Real> pyro_couplings() const {
+        pyro::PyroCouplings<Real> J;
+        J.J1 = Real(sJ1);
+        J.J2 = Real(sJ2);
+        J.J3 = Real(sJ3);
+        J.Jx = Real(sJ1 * (sDelta - 1.0) / 3.0);
+        J.shells = spec_shells;
+        J.xxz = sDelta != 1.0;
+        return J;
+    }
+    bool resident_fits() const { return nc() <= pyro::kPyroResidentMaxCells; }
+
+    template <typename Real>
     pyro::SsfState<Real> ssf_state() const {
         pyro::SsfState<Real> st;
         st.S = S.as<Real>();
@@ -278,10 +299,18 @@ struct pyro_ctx {
         h2d(rep_of.p, id.data(), sizeof(int32_t) * R);
     }
 
-    // One pass of `kind` over all colours (strategy 1).
+    // One pass of `kind` over all colours, one launch per colour
+    // (strategy 1 generic, 3 pyrochlore stencil).
     template <typename Real>
-    void pass_colours(int kind, const unsigned long long* pass_base, unsigned long long pass_index) {
+    void pass_colours(int kind, const unsigned long long* pass_base, unsigned long long pass_index,
+                      bool use_spec) {
         const auto st = state<Real>(pass_base);
+        if (use_spec) {
+            const auto J = pyro_couplings<Real>();
+            for (int c = 0; c < topo.n_colours; ++c)
+                timed(kFamSweep, [&] { return pyro::launch_pyro_colour<Real>(st, J, c, kind, pass_index, stream); });
+            return;
+        }
         for (int c = 0; c < topo.n_colours; ++c) {
             pyro::ColourList col;
             col.cls = cls.as<int32_t>() + h_colour_begin[c];
@@ -302,10 +331,18 @@ struct pyro_ctx {
             });
             return;
         }
+        if (strat == 4) {
+            const auto st = state<Real>(pass_base);
+            const auto J = pyro_couplings<Real>();
+            timed(kFamSweep, [&] {
+                return pyro::launch_pyro_resident<Real>(st, J, kind, n_or, n_sweeps, pass_index, stream);
+            });
+            return;
+        }
         unsigned long long p = pass_index;
         for (int s = 0; s < n_sweeps; ++s) {
-            pass_colours<Real>(kind, pass_base, p++);
-            for (int o = 0; o < n_or; ++o) pass_colours<Real>(3, pass_base, p++);
+            pass_colours<Real>(kind, pass_base, p++, strat == 3);
+            for (int o = 0; o < n_or; ++o) pass_colours<Real>(3, pass_base, p++, strat == 3);
         }
     }
     void run_sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
@@ -318,9 +355,27 @@ struct pyro_ctx {
     // saved copy of the state, then restore it (results do not depend on the
     // strategy: both use the same per-site counters and colour order).
     int choose_strategy(int kind, int n_or) {
-        if (strategy != 0) return strategy;
+        if (strategy != 0) {
+            require(strategy <= 2 || spec, PYRO_E_UNSUPPORTED,
+                    "strategies 3 and 4 need couplings defined by define_J1J2J3 alone");
+            require(strategy != 4 || resident_fits(), PYRO_E_UNSUPPORTED,
+                    "the pyrochlore resident strategy needs at most 2048 cells");
+            return strategy;
+        }
         const int key = kind * 64 + n_or;
         if (tuned_key == key) return tuned_strategy;
+        std::vector<int> cand;
+        if (spec) {
+            cand.push_back(3);
+            if (resident_fits()) cand.push_back(4);
+        } else {
+            cand = {1, 2};
+        }
+        if (cand.size() == 1) {
+            tuned_key = key;
+            tuned_strategy = cand[0];
+            return cand[0];
+        }
         const size_t sbytes = size_t(R) * 3 * N() * real_size();
         const size_t dbytes = size_t(R) * N();
         if (!S_save.p) {
@@ -333,8 +388,8 @@ struct pyro_ctx {
         const bool was_prof = prof;
         prof = false;
         float best = 1e30f;
-        int best_s = 1;
-        for (int s : {1, 2}) {
+        int best_s = spec ? 3 : 1;
+        for (int s : cand) {
             run_sweeps(s, kind, n_or, 1, nullptr, pass_count);  // warm-up
             HIPCHK(hipEventRecord(a, stream));
             run_sweeps(s, kind, n_or, 3, nullptr, pass_count);
@@ -486,6 +541,7 @@ int pyro_define_coupling(pyro_ctx* ctx, const char* name, int n_kinds, const int
         for (int e = 0; e < 9; ++e) c.J[e] = J[e];
         ctx->names.insert(c.name);
         ctx->couplings.push_back(std::move(c));
+        ctx->spec = false;
     });
 }
 
@@ -498,11 +554,18 @@ int pyro_define_J1J2J3(pyro_ctx* ctx, double J1, double J2, double J3, double De
         for (int s = 0; pyro_cell && s < 16; ++s)
             pyro_cell = ctx->geom.sl_pos[s] == ref.sl_pos[s] && ctx->geom.sl_kind[s] == ref.sl_kind[s];
         require(pyro_cell, PYRO_E_INVALID, "define_J1J2J3 needs the 16-site pyrochlore cubic cell");
+        const bool first = ctx->couplings.empty();
         for (auto& c : pyro::pyrochlore_J1J2J3(J1, J2, J3, Delta)) {
             require(!ctx->names.count(c.name), PYRO_E_LOGIC, "Coupling names must be unique");
             ctx->names.insert(c.name);
             ctx->couplings.push_back(std::move(c));
         }
+        ctx->spec = first && J1 != 0.0;
+        ctx->sJ1 = J1;
+        ctx->sJ2 = J2;
+        ctx->sJ3 = J3;
+        ctx->sDelta = Delta;
+        ctx->spec_shells = J3 != 0.0 ? 3 : (J2 != 0.0 ? 2 : 1);
     });
 }
 
@@ -581,6 +644,13 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
 
         // colour classes, colour-major
         const int P = t.period[0] * t.period[1] * t.period[2];
+        if (ctx->spec) {
+            // the stencil kernels colour by pyro_sl (4) or by sublattice (16)
+            const int ncol = ctx->spec_shells == 3 ? 16 : 4;
+            bool ok = P == 1 && t.n_colours == ncol;
+            for (int sl = 0; ok && sl < 16; ++sl) ok = t.class_colour[sl] == (ncol == 16 ? sl : sl % 4);
+            ctx->spec = ok;
+        }
         ctx->h_cls.clear();
         ctx->h_colour_begin.assign(1, 0);
         for (int c = 0; c < t.n_colours; ++c) {
@@ -849,9 +919,10 @@ int pyro_overrelax(pyro_ctx* ctx, int n_passes) {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
         ctx->need_setup();
         require(n_passes >= 0, PYRO_E_INVALID, "negative pass count");
+        const bool use_spec = ctx->strategy == 0 ? ctx->spec : ctx->strategy >= 3;
         for (int p = 0; p < n_passes; ++p) {
-            if (ctx->prec == 64) ctx->pass_colours<double>(3, nullptr, ctx->pass_count);
-            else ctx->pass_colours<float>(3, nullptr, ctx->pass_count);
+            if (ctx->prec == 64) ctx->pass_colours<double>(3, nullptr, ctx->pass_count, use_spec);
+            else ctx->pass_colours<float>(3, nullptr, ctx->pass_count, use_spec);
             ctx->pass_count += 1;
         }
     });
@@ -1294,7 +1365,7 @@ int pyro_synchronize(pyro_ctx* ctx) {
 int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
-        require(strategy >= 0 && strategy <= 2, PYRO_E_INVALID, "strategy must be 0, 1 or 2");
+        require(strategy >= 0 && strategy <= 4, PYRO_E_INVALID, "strategy must be 0..4");
         ctx->strategy = strategy;
         ctx->tuned_key = -1;
         ctx->invalidate_graph();

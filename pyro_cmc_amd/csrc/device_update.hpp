@@ -1,0 +1,218 @@
+// Device-side single-site update math shared by the generic and the
+// pyrochlore-specialised sweep kernels.  Included by .hip sources only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "kernels.hpp"
+#include "rng.hpp"
+
+namespace pyro {
+
+// FP64 uses the correctly rounded library functions (the reference's
+// arithmetic).  FP32 uses the hardware approximations: v_log_f32 / v_exp_f32
+// (base 2, ~1 ulp), v_sin_f32 / v_cos_f32, which take their argument in
+// revolutions, so sin(2 pi u) is one instruction, and v_rsq_f32.
+template <typename Real> __device__ __forceinline__ Real dsqrt(Real x);
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
+template <typename Real> __device__ __forceinline__ Real dexp(Real x);
+template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
+template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+template <typename Real> __device__ __forceinline__ Real dlog(Real x);
+template <> __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+template <> __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
+// sin and cos of an angle in radians
+template <typename Real> __device__ __forceinline__ void dsincos(Real x, Real* s, Real* c);
+template <> __device__ __forceinline__ void dsincos<float>(float x, float* s, float* c) { __sincosf(x, s, c); }
+template <> __device__ __forceinline__ void dsincos<double>(double x, double* s, double* c) { sincos(x, s, c); }
+// sin and cos of 2 pi u for a uniform u in (0, 1)
+template <typename Real> __device__ __forceinline__ void dsincos_rev(Real u, Real* s, Real* c);
+template <> __device__ __forceinline__ void dsincos_rev<float>(float u, float* s, float* c) {
+    *s = __builtin_amdgcn_sinf(u);
+    *c = __builtin_amdgcn_cosf(u);
+}
+template <> __device__ __forceinline__ void dsincos_rev<double>(double u, double* s, double* c) {
+    sincos(6.28318530717958647692 * u, s, c);
+}
+// a / b: fast reciprocal-multiply in FP32, a true division in FP64
+template <typename Real> __device__ __forceinline__ Real ddiv(Real a, Real b);
+template <> __device__ __forceinline__ float ddiv<float>(float a, float b) { return __fdividef(a, b); }
+template <> __device__ __forceinline__ double ddiv<double>(double a, double b) { return a / b; }
+// v <- v / |v|: v * rsq(|v|^2) in FP32, division by the square root in FP64
+template <typename Real>
+__device__ __forceinline__ void normalize3(Real& x, Real& y, Real& z) {
+    if constexpr (sizeof(Real) == 4) {
+        const Real inv = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+        x *= inv;
+        y *= inv;
+        z *= inv;
+    } else {
+        const Real nrm = sqrt(x * x + y * y + z * z);
+        x /= nrm;
+        y /= nrm;
+        z /= nrm;
+    }
+}
+// 1 / sqrt(x)
+template <typename Real> __device__ __forceinline__ Real dinv_sqrt(Real x);
+template <> __device__ __forceinline__ float dinv_sqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> __device__ __forceinline__ double dinv_sqrt<double>(double x) { return 1.0 / sqrt(x); }
+
+// Load base[idx] through a 32-bit unsigned byte offset from a wave-uniform
+// base, so the compiler can use SGPR-base global loads instead of a 64-bit
+// address computation per load.
+template <typename Real>
+__device__ __forceinline__ Real ld32(const Real* __restrict__ base, unsigned idx) {
+    return *reinterpret_cast<const Real*>(reinterpret_cast<const char*>(base) +
+                                          static_cast<size_t>(idx * unsigned(sizeof(Real))));
+}
+
+// FP32 spins are renormalised after every accepted change (drift control);
+// FP64 follows the reference exactly (no renormalisation after OR).
+template <typename Real> struct Renorm { static constexpr bool value = false; };
+template <> struct Renorm<float> { static constexpr bool value = true; };
+
+// One single-site update of kind KIND at `site` of replica r, pass index
+// `pass`, given the local field (hx, hy, hz) without the global field.
+// Returns whether the proposal was accepted (OR: always false).
+template <typename Real, int KIND>
+__device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sx,
+                                    Real* __restrict__ Sy, Real* __restrict__ Sz, int site, int r,
+                                    unsigned long long pass, Real hx, Real hy, Real hz) {
+    const size_t N = size_t(st.N);
+    hx -= st.B[3 * r + 0];  // h_loc = local_field - global_field
+    hy -= st.B[3 * r + 1];
+    hz -= st.B[3 * r + 2];
+
+    const Real sx = Sx[site], sy = Sy[site], sz = Sz[site];
+    const double Td = st.T_slot[st.slot_of[r]];
+    const Real T = Real(Td);
+
+    const Words4 w = philox4x32_10(
+        Words4{uint32_t(site), uint32_t(r), uint32_t(pass), uint32_t(pass >> 32)}, st.k0, st.k1);
+    const Uniform5 u = split_uniform5(w);
+
+    if constexpr (KIND == 0) {
+        // Gaussian-kick Metropolis (MC.cpp:111-130).
+        const Real sig = Real(sqrt(Td / st.T_ref));
+        Real s1, c1, s2, c2;
+        const Real r1 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[0])));
+        dsincos_rev<Real>(u24<Real>(u.m[1]), &s1, &c1);
+        const Real r2 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[2])));
+        dsincos_rev<Real>(u24<Real>(u.m[3]), &s2, &c2);
+        Real nx = sig * (r1 * c1) + sx;
+        Real ny = sig * (r1 * s1) + sy;
+        Real nz = sig * (r2 * c2) + sz;
+        normalize3<Real>(nx, ny, nz);
+        const Real curE = sx * hx + sy * hy + sz * hz;
+        const Real newE = nx * hx + ny * hy + nz * hz;
+        const Real dE = newE - curE;
+        if (dE < Real(0) || u24<Real>(u.m[4]) < dexp<Real>(ddiv<Real>(-dE, T))) {
+            Sx[site] = nx;
+            Sy[site] = ny;
+            Sz[site] = nz;
+            return true;
+        }
+        return false;
+    } else if constexpr (KIND == 3) {
+        // Overrelaxation: rotation by 2*pi*U about h_loc (MC.cpp:102-109, 145-153).
+        const Real n2 = hx * hx + hy * hy + hz * hz;
+        if (!(n2 > Real(0))) return false;  // h_loc = 0: no axis, spin kept
+        Real s, c;
+        dsincos_rev<Real>(u24<Real>(u.m[0]), &s, &c);
+        const Real d = hx * sx + hy * sy + hz * sz;
+        const Real f = ddiv<Real>(Real(1) - c, n2) * d;
+        const Real g = ddiv<Real>(s, dsqrt<Real>(n2));
+        const Real cx = hy * sz - hz * sy, cy = hz * sx - hx * sz, cz = hx * sy - hy * sx;
+        Real nx = c * sx + f * hx + g * cx;
+        Real ny = c * sy + f * hy + g * cy;
+        Real nz = c * sz + f * hz + g * cz;
+        if constexpr (Renorm<Real>::value) {
+            normalize3<Real>(nx, ny, nz);
+        }
+        Sx[site] = nx;
+        Sy[site] = ny;
+        Sz[site] = nz;
+        return false;
+    } else if constexpr (KIND == 1) {
+        // Lifted Metropolis, displacement form (MC.cpp:249-281).
+        int8_t* dp = st.dir + size_t(r) * N + site;
+        const int8_t d = *dp;
+        const Real hn = dsqrt<Real>(hx * hx + hy * hy + hz * hz);
+        if (!(hn > Real(0))) {  // no field direction: bounce
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real ex = ddiv<Real>(hx, hn), ey = ddiv<Real>(hy, hn), ez = ddiv<Real>(hz, hn);
+        const Real uu = ex * sx + ey * sy + ez * sz;
+        const Real du = (Real(d) * Real(sqrt(Td / st.T_ref))) * (-dlog<Real>(u24<Real>(u.m[0])));
+        const Real nu = uu + du;
+        if (nu > Real(1) || nu < Real(-1) || fabs(Real(1) - uu * uu) < Real(1e-16)) {
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real f = dsqrt<Real>(ddiv<Real>(Real(1) - nu * nu, Real(1) - uu * uu));
+        Real nx = nu * ex + f * (sx - uu * ex);
+        Real ny = nu * ey + f * (sy - uu * ey);
+        Real nz = nu * ez + f * (sz - uu * ez);
+        normalize3<Real>(nx, ny, nz);
+        const Real curE = sx * hx + sy * hy + sz * hz;
+        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
+        if (u24<Real>(u.m[1]) < dexp<Real>(ddiv<Real>(-dE, T))) {
+            Sx[site] = nx;
+            Sy[site] = ny;
+            Sz[site] = nz;
+            return true;
+        }
+        *dp = int8_t(-d);
+        return false;
+    } else {
+        // Lifted rotation Metropolis with reflecting poles (MC.cpp:194-244).
+        int8_t* dp = st.dir + size_t(r) * N + site;
+        const int8_t d = *dp;
+        const Real curE = sx * hx + sy * hy + sz * hz;
+        const Real ax = sy * hz - sz * hy, ay = sz * hx - sx * hz, az = sx * hy - sy * hx;
+        const Real a2 = ax * ax + ay * ay + az * az;
+        if (a2 <= Real(1e-20)) {
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real th = Real(d) * Real(sqrt(Td / st.T_ref));
+        Real s, c;
+        dsincos<Real>(th, &s, &c);
+        const Real dd = ax * sx + ay * sy + az * sz;
+        const Real f = ddiv<Real>(Real(1) - c, a2) * dd;
+        const Real g = ddiv<Real>(s, dsqrt<Real>(a2));
+        const Real cx = ay * sz - az * sy, cy = az * sx - ax * sz, cz = ax * sy - ay * sx;
+        Real nx = c * sx + f * ax + g * cx;
+        Real ny = c * sy + f * ay + g * cy;
+        Real nz = c * sz + f * az + g * cz;
+        normalize3<Real>(nx, ny, nz);
+        const Real bx = ny * hz - nz * hy, by = nz * hx - nx * hz, bz = nx * hy - ny * hx;
+        const Real b2 = bx * bx + by * by + bz * bz;
+        if (ax * bx + ay * by + az * bz < Real(0)) {
+            *dp = int8_t(-d);
+            return false;
+        }
+        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
+        const Real ratio = dsqrt<Real>(ddiv<Real>(b2, a2)) * dexp<Real>(ddiv<Real>(-dE, T));
+        if (u24<Real>(u.m[0]) < ratio) {
+            Sx[site] = nx;
+            Sy[site] = ny;
+            Sz[site] = nz;
+            return true;
+        }
+        *dp = int8_t(-d);
+        return false;
+    }
+}
+
+// Wave-level count of accepted updates added to acc[r].
+__device__ __forceinline__ void count_accepts(unsigned long long* acc, int r, bool a) {
+    const unsigned long long mask = __ballot(a);
+    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(acc + r, (unsigned long long)__popcll(mask));
+}
+
+}  // namespace pyro

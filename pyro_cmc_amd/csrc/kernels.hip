@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "device_update.hpp"
 #include "kernels.hpp"
 #include "rng.hpp"
 
@@ -22,27 +23,6 @@ namespace pyro {
 namespace {
 
 constexpr int kBlock = 256;
-
-template <typename Real> __device__ inline Real dsqrt(Real x);
-template <> __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
-template <> __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
-template <typename Real> __device__ inline Real dexp(Real x);
-template <> __device__ inline float dexp<float>(float x) { return expf(x); }
-template <> __device__ inline double dexp<double>(double x) { return exp(x); }
-template <typename Real> __device__ inline Real dlog(Real x);
-template <> __device__ inline float dlog<float>(float x) { return logf(x); }
-template <> __device__ inline double dlog<double>(double x) { return log(x); }
-template <typename Real> __device__ inline void dsincos(Real x, Real* s, Real* c);
-template <> __device__ inline void dsincos<float>(float x, float* s, float* c) { sincosf(x, s, c); }
-template <> __device__ inline void dsincos<double>(double x, double* s, double* c) { sincos(x, s, c); }
-template <typename Real> __device__ inline Real two_pi();
-template <> __device__ inline float two_pi<float>() { return 6.28318530717958647692f; }
-template <> __device__ inline double two_pi<double>() { return 6.28318530717958647692; }
-
-// FP32 spins are renormalised after every accepted change (drift control);
-// FP64 follows the reference exactly (no renormalisation after OR).
-template <typename Real> struct Renorm { static constexpr bool value = false; };
-template <> struct Renorm<float> { static constexpr bool value = true; };
 
 __device__ inline int decode_site(int desc, int m, int n_cells, int L1, int L2, int p0,
                                   int p1, int p2) {
@@ -62,9 +42,9 @@ template <typename Real, int NS, bool ISO>
 __device__ inline void local_field(const DeviceState<Real>& st, const Real* __restrict__ Sx,
                                    const Real* __restrict__ Sy, const Real* __restrict__ Sz,
                                    int site, int sl, Real& hx, Real& hy, Real& hz) {
-    int nb[NS];
+    unsigned nb[NS];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) nb[k] = st.nbr[size_t(k) * st.N + site];
+    for (int k = 0; k < NS; ++k) nb[k] = unsigned(ld32(st.nbr, unsigned(k * st.N + site)));
     hx = Real(0);
     hy = Real(0);
     hz = Real(0);
@@ -73,15 +53,15 @@ __device__ inline void local_field(const DeviceState<Real>& st, const Real* __re
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const Real ck = c[k];
-            hx += ck * Sx[nb[k]];
-            hy += ck * Sy[nb[k]];
-            hz += ck * Sz[nb[k]];
+            hx += ck * ld32(Sx, nb[k]);
+            hy += ck * ld32(Sy, nb[k]);
+            hz += ck * ld32(Sz, nb[k]);
         }
     } else {
         const Real* __restrict__ J = st.cpl + size_t(sl) * NS * 9;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const Real x = Sx[nb[k]], y = Sy[nb[k]], z = Sz[nb[k]];
+            const Real x = ld32(Sx, nb[k]), y = ld32(Sy, nb[k]), z = ld32(Sz, nb[k]);
             const Real* m = J + 9 * k;
             hx += m[0] * x + m[1] * y + m[2] * z;
             hy += m[3] * x + m[4] * y + m[5] * z;
@@ -90,8 +70,6 @@ __device__ inline void local_field(const DeviceState<Real>& st, const Real* __re
     }
 }
 
-// One single-site update of kind KIND at `site` of replica r, pass index
-// `pass`.  Returns whether the proposal was accepted (OR: always false).
 template <typename Real, int NS, bool ISO, int KIND>
 __device__ inline bool update_site(const DeviceState<Real>& st, int site, int sl, int r,
                                    unsigned long long pass) {
@@ -99,152 +77,9 @@ __device__ inline bool update_site(const DeviceState<Real>& st, int site, int sl
     Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
     Real* __restrict__ Sy = Sx + N;
     Real* __restrict__ Sz = Sy + N;
-
     Real hx, hy, hz;
     local_field<Real, NS, ISO>(st, Sx, Sy, Sz, site, sl, hx, hy, hz);
-    hx -= st.B[3 * r + 0];  // h_loc = local_field - global_field
-    hy -= st.B[3 * r + 1];
-    hz -= st.B[3 * r + 2];
-
-    const Real sx = Sx[site], sy = Sy[site], sz = Sz[site];
-    const double Td = st.T_slot[st.slot_of[r]];
-    const Real T = Real(Td);
-
-    const Words4 w = philox4x32_10(
-        Words4{uint32_t(site), uint32_t(r), uint32_t(pass), uint32_t(pass >> 32)}, st.k0, st.k1);
-    const Uniform5 u = split_uniform5(w);
-
-    if constexpr (KIND == 0) {
-        // Gaussian-kick Metropolis (MC.cpp:111-130).
-        const Real sig = Real(sqrt(Td / st.T_ref));
-        Real s1, c1, s2, c2;
-        const Real r1 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[0])));
-        dsincos<Real>(two_pi<Real>() * u24<Real>(u.m[1]), &s1, &c1);
-        const Real r2 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[2])));
-        dsincos<Real>(two_pi<Real>() * u24<Real>(u.m[3]), &s2, &c2);
-        Real nx = sig * (r1 * c1) + sx;
-        Real ny = sig * (r1 * s1) + sy;
-        Real nz = sig * (r2 * c2) + sz;
-        const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
-        nx /= nrm;
-        ny /= nrm;
-        nz /= nrm;
-        const Real curE = sx * hx + sy * hy + sz * hz;
-        const Real newE = nx * hx + ny * hy + nz * hz;
-        const Real dE = newE - curE;
-        if (dE < Real(0) || u24<Real>(u.m[4]) < dexp<Real>(-dE / T)) {
-            Sx[site] = nx;
-            Sy[site] = ny;
-            Sz[site] = nz;
-            return true;
-        }
-        return false;
-    } else if constexpr (KIND == 3) {
-        // Overrelaxation: rotation by 2*pi*U about h_loc (MC.cpp:102-109, 145-153).
-        const Real n2 = hx * hx + hy * hy + hz * hz;
-        if (!(n2 > Real(0))) return false;  // h_loc = 0: no axis, spin kept
-        Real s, c;
-        dsincos<Real>(two_pi<Real>() * u24<Real>(u.m[0]), &s, &c);
-        const Real d = hx * sx + hy * sy + hz * sz;
-        const Real f = ((Real(1) - c) / n2) * d;
-        const Real g = s / dsqrt<Real>(n2);
-        const Real cx = hy * sz - hz * sy, cy = hz * sx - hx * sz, cz = hx * sy - hy * sx;
-        Real nx = c * sx + f * hx + g * cx;
-        Real ny = c * sy + f * hy + g * cy;
-        Real nz = c * sz + f * hz + g * cz;
-        if constexpr (Renorm<Real>::value) {
-            const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
-            nx /= nrm;
-            ny /= nrm;
-            nz /= nrm;
-        }
-        Sx[site] = nx;
-        Sy[site] = ny;
-        Sz[site] = nz;
-        return false;
-    } else if constexpr (KIND == 1) {
-        // Lifted Metropolis, displacement form (MC.cpp:249-281).
-        int8_t* dp = st.dir + size_t(r) * N + site;
-        const int8_t d = *dp;
-        const Real hn = dsqrt<Real>(hx * hx + hy * hy + hz * hz);
-        if (!(hn > Real(0))) {  // no field direction: bounce
-            *dp = int8_t(-d);
-            return false;
-        }
-        const Real ex = hx / hn, ey = hy / hn, ez = hz / hn;
-        const Real uu = ex * sx + ey * sy + ez * sz;
-        const Real du = (Real(d) * Real(sqrt(Td / st.T_ref))) * (-dlog<Real>(u24<Real>(u.m[0])));
-        const Real nu = uu + du;
-        if (nu > Real(1) || nu < Real(-1) || fabs(Real(1) - uu * uu) < Real(1e-16)) {
-            *dp = int8_t(-d);
-            return false;
-        }
-        const Real f = dsqrt<Real>((Real(1) - nu * nu) / (Real(1) - uu * uu));
-        Real nx = nu * ex + f * (sx - uu * ex);
-        Real ny = nu * ey + f * (sy - uu * ey);
-        Real nz = nu * ez + f * (sz - uu * ez);
-        const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
-        nx /= nrm;
-        ny /= nrm;
-        nz /= nrm;
-        const Real curE = sx * hx + sy * hy + sz * hz;
-        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
-        if (u24<Real>(u.m[1]) < dexp<Real>(-dE / T)) {
-            Sx[site] = nx;
-            Sy[site] = ny;
-            Sz[site] = nz;
-            return true;
-        }
-        *dp = int8_t(-d);
-        return false;
-    } else {
-        // Lifted rotation Metropolis with reflecting poles (MC.cpp:194-244).
-        int8_t* dp = st.dir + size_t(r) * N + site;
-        const int8_t d = *dp;
-        const Real curE = sx * hx + sy * hy + sz * hz;
-        const Real ax = sy * hz - sz * hy, ay = sz * hx - sx * hz, az = sx * hy - sy * hx;
-        const Real a2 = ax * ax + ay * ay + az * az;
-        if (a2 <= Real(1e-20)) {
-            *dp = int8_t(-d);
-            return false;
-        }
-        const Real th = Real(d) * Real(sqrt(Td / st.T_ref));
-        Real s, c;
-        dsincos<Real>(th, &s, &c);
-        const Real dd = ax * sx + ay * sy + az * sz;
-        const Real f = ((Real(1) - c) / a2) * dd;
-        const Real g = s / dsqrt<Real>(a2);
-        const Real cx = ay * sz - az * sy, cy = az * sx - ax * sz, cz = ax * sy - ay * sx;
-        Real nx = c * sx + f * ax + g * cx;
-        Real ny = c * sy + f * ay + g * cy;
-        Real nz = c * sz + f * az + g * cz;
-        const Real nrm = dsqrt<Real>(nx * nx + ny * ny + nz * nz);
-        nx /= nrm;
-        ny /= nrm;
-        nz /= nrm;
-        const Real bx = ny * hz - nz * hy, by = nz * hx - nx * hz, bz = nx * hy - ny * hx;
-        const Real b2 = bx * bx + by * by + bz * bz;
-        if (ax * bx + ay * by + az * bz < Real(0)) {
-            *dp = int8_t(-d);
-            return false;
-        }
-        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
-        const Real ratio = dsqrt<Real>(b2 / a2) * dexp<Real>(-dE / T);
-        if (u24<Real>(u.m[0]) < ratio) {
-            Sx[site] = nx;
-            Sy[site] = ny;
-            Sz[site] = nz;
-            return true;
-        }
-        *dp = int8_t(-d);
-        return false;
-    }
-}
-
-// Wave-level count of accepted updates added to acc[r].
-__device__ inline void count_accepts(unsigned long long* acc, int r, bool a) {
-    const unsigned long long mask = __ballot(a);
-    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(acc + r, (unsigned long long)__popcll(mask));
+    return apply_update<Real, KIND>(st, Sx, Sy, Sz, site, r, pass, hx, hy, hz);
 }
 
 // Strategy 1: one launch per colour; grid.x = classes x blocks-per-class,

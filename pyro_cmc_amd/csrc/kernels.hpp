@@ -51,6 +51,27 @@ hipError_t launch_sweep_resident(const DeviceState<Real>& st, const int32_t* all
                                  int n_or, int n_sweeps, unsigned long long pass_index,
                                  hipStream_t s);
 
+// Couplings of the pyrochlore-specialised kernels: shells = 1 (J1),
+// 2 (J1, J2) or 3 (J1, J2, J3); Jx = J1 (Delta - 1) / 3 when xxz.
+template <typename Real>
+struct PyroCouplings {
+    Real J1 = 0, J2 = 0, J3 = 0, Jx = 0;
+    int shells = 1;
+    bool xxz = false;
+};
+// One pass of `kind` over one colour (4 colours = pyro_sl when shells < 3,
+// else 16 colours = sublattice) for every replica.
+template <typename Real>
+hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
+                              int kind, unsigned long long pass_index, hipStream_t s);
+// Whole sweeps with one resident workgroup (1024 threads, two cells per
+// thread) per replica; lattices of at most 2048 cells.
+constexpr int kPyroResidentMaxCells = 2048;
+template <typename Real>
+hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,
+                                int n_or, int n_sweeps, unsigned long long pass_index,
+                                hipStream_t s);
+
 // Local fields (no global field) of replica r into out[3][N].
 template <typename Real>
 hipError_t launch_fields(const DeviceState<Real>& st, int r, Real* out, hipStream_t s);

@@ -1,0 +1,6 @@
+// Instantiation unit: pyrochlore-specialised colour sweep kernels, float.
+#include "kernels_pyro_impl.hpp"
+
+namespace pyro {
+template hipError_t launch_pyro_colour<float>(const DeviceState<float>&, const PyroCouplings<float>&, int, int, unsigned long long, hipStream_t);
+}  // namespace pyro

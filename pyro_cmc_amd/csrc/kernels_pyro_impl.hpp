@@ -1,0 +1,291 @@
+// Sweep kernels specialised to the pyrochlore cubic cell with the
+// J1-J2-J3 couplings of build_J1J2J3_h (cli_bits.hpp:144-207).
+//
+// The neighbour stencil is a compile-time table (pyro_stencil.hpp), so a
+// site's 30 neighbour addresses are its cell's wrapped coordinates plus
+// constants: no neighbour-index loads, no dependent load chain.  The field
+// is summed by shell,
+//   h_i = J1 sum_nn1 S_j + J2 sum_nn2 S_j + J3 sum_nn3 S_j
+//         + J1 (Delta - 1)/3 l_i sum_nn1 (l_j . S_j),
+// where l_p is the [111] link of pyro_sl p (z_p = l_p / sqrt 3), which is
+// h_i = sum_j J_ij S_j for J_ij = J1 [I + (Delta - 1) z_i z_j^T] on nn1 bonds
+// (cli_bits.hpp:168-197) and isotropic J2, J3.
+//
+// Two launch shapes visit sites in the same colour order with the same
+// per-site counters and the same arithmetic, so they give bit-identical
+// results:
+//   resident: one workgroup per replica runs whole sweeps, colours separated
+//             by workgroup barriers; a replica's spins (12 B x 16 L^3) stay
+//             in the L2 of the XCD its workgroup runs on;
+//   colour:   one launch per colour over all replicas (large lattices).
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "device_update.hpp"
+#include "kernels.hpp"
+#include "pyro_stencil.hpp"
+
+namespace pyro {
+
+namespace {
+
+using pyrostencil::kStencil;
+
+static_assert(kStencil.n[0][29].sl >= 0, "stencil must hold 30 neighbours");
+
+template <int... I, typename F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F&& f) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// Wrapped row offsets of the three cells along each axis around n.
+struct CellAxes {
+    int a0[3], a1[3], a2[3];
+};
+
+// n / d and n % d for 0 <= n < 2^24 through a float reciprocal and one
+// correction step (integer division is a long instruction sequence).
+__device__ __forceinline__ int fast_div(int n, int d, int& rem) {
+    int q = int(float(n) * __frcp_rn(float(d)));
+    int r = n - q * d;
+    if (r >= d) { ++q; r -= d; }
+    if (r < 0) { --q; r += d; }
+    rem = r;
+    return q;
+}
+
+__device__ __forceinline__ CellAxes cell_axes_of(int cell, int L0, int L1, int L2);
+
+__device__ __forceinline__ CellAxes cell_axes(int n0, int n1, int n2, int L0, int L1, int L2) {
+    CellAxes c;
+    const int L12 = L1 * L2;
+    c.a0[0] = (n0 == 0 ? L0 - 1 : n0 - 1) * L12;
+    c.a0[1] = n0 * L12;
+    c.a0[2] = (n0 == L0 - 1 ? 0 : n0 + 1) * L12;
+    c.a1[0] = (n1 == 0 ? L1 - 1 : n1 - 1) * L2;
+    c.a1[1] = n1 * L2;
+    c.a1[2] = (n1 == L1 - 1 ? 0 : n1 + 1) * L2;
+    c.a2[0] = (n2 == 0 ? L2 - 1 : n2 - 1);
+    c.a2[1] = n2;
+    c.a2[2] = (n2 == L2 - 1 ? 0 : n2 + 1);
+    return c;
+}
+
+__device__ __forceinline__ CellAxes cell_axes_of(int cell, int L0, int L1, int L2) {
+    int n2, n1;
+    const int t = fast_div(cell, L2, n2);
+    const int n0 = fast_div(t, L1, n1);
+    return cell_axes(n0, n1, n2, L0, L1, L2);
+}
+
+template <typename Real, int SHELLS, int SL>
+__device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Real* __restrict__ Sy,
+                                  const Real* __restrict__ Sz, int nc, const CellAxes& c,
+                                  const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
+    constexpr int NS = pyrostencil::slots_for(SHELLS);
+    Real s[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // per shell x, y, z
+    Real zs = Real(0);
+    static_for<NS>([&](auto kk) __attribute__((always_inline)) {
+        constexpr int k = decltype(kk)::value;
+        constexpr pyrostencil::Nbr n = kStencil.n[SL][k];
+        constexpr int shell = k < 6 ? 0 : (k < 18 ? 1 : 2);
+        const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
+        const Real x = ld32(Sx, j), y = ld32(Sy, j), z = ld32(Sz, j);
+        s[shell][0] += x;
+        s[shell][1] += y;
+        s[shell][2] += z;
+        if constexpr (shell == 0) {
+            constexpr int tp = n.sl % 4;
+            zs += Real(pyrostencil::kLink[tp][0]) * x + Real(pyrostencil::kLink[tp][1]) * y +
+                  Real(pyrostencil::kLink[tp][2]) * z;
+        }
+    });
+    hx = J.J1 * s[0][0];
+    hy = J.J1 * s[0][1];
+    hz = J.J1 * s[0][2];
+    if constexpr (SHELLS >= 2) {
+        hx += J.J2 * s[1][0];
+        hy += J.J2 * s[1][1];
+        hz += J.J2 * s[1][2];
+    }
+    if constexpr (SHELLS >= 3) {
+        hx += J.J3 * s[2][0];
+        hy += J.J3 * s[2][1];
+        hz += J.J3 * s[2][2];
+    }
+    if (J.xxz) {
+        constexpr int p = SL % 4;
+        const Real t = J.Jx * zs;
+        hx += Real(pyrostencil::kLink[p][0]) * t;
+        hy += Real(pyrostencil::kLink[p][1]) * t;
+        hz += Real(pyrostencil::kLink[p][2]) * t;
+    }
+}
+
+// Runtime sublattice -> compile-time stencil.
+template <typename Real, int SHELLS>
+__device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sx, const Real* Sy, const Real* Sz, int nc,
+                                     const CellAxes& c, const PyroCouplings<Real>& J, Real& hx,
+                                     Real& hy, Real& hz) {
+    switch (sl) {
+#define PYRO_SL_CASE(X) \
+    case X: pyro_field<Real, SHELLS, X>(Sx, Sy, Sz, nc, c, J, hx, hy, hz); break;
+        PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+        PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+        PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+        PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+        default: hx = hy = hz = Real(0);
+    }
+}
+
+// One launch per colour; grid.x = (sublattices of the colour) x (cell
+// blocks), grid.y = replicas.
+template <typename Real, int SHELLS, int KIND>
+__global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
+                                                          PyroCouplings<Real> J, int colour,
+                                                          unsigned long long pass_index) {
+    constexpr int NCOL = pyrostencil::colours_for(SHELLS);
+    const int nc = st.n_cells;
+    const int bps = (nc + 255) / 256;
+    const int q = blockIdx.x / bps;
+    const int cell = (blockIdx.x % bps) * 256 + threadIdx.x;
+    const int sl = NCOL == 16 ? colour : colour + 4 * q;
+    const int r = blockIdx.y;
+    const unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    bool accepted = false;
+    if (cell < nc) {
+        const size_t N = size_t(st.N);
+        Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
+        Real* __restrict__ Sy = Sx + N;
+        Real* __restrict__ Sz = Sy + N;
+        const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
+        Real hx, hy, hz;
+        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J, hx, hy, hz);
+        accepted = apply_update<Real, KIND>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
+    }
+    if constexpr (KIND != 3) count_accepts(st.acc, r, accepted);
+}
+
+// One workgroup per replica; thread t owns cells t, t + BS, ... (CPT cells).
+template <typename Real, int SHELLS, int FIRST, int BS, int CPT>
+__global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
+                                                           PyroCouplings<Real> J, int n_or,
+                                                           int n_sweeps,
+                                                           unsigned long long pass_index) {
+    constexpr int NCOL = pyrostencil::colours_for(SHELLS);
+    constexpr int SPC = 16 / NCOL;
+    const int r = blockIdx.x;
+    const int nc = st.n_cells;
+    const size_t N = size_t(st.N);
+    Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
+    Real* __restrict__ Sy = Sx + N;
+    Real* __restrict__ Sz = Sy + N;
+    unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    unsigned int my_acc = 0;
+    static_assert(CPT == 2, "the resident kernel owns two cells per thread");
+    const int cell_b = threadIdx.x + BS < nc ? threadIdx.x + BS : 0;
+    const CellAxes ax_a = cell_axes_of(threadIdx.x < nc ? threadIdx.x : 0, st.L0, st.L1, st.L2);
+    const CellAxes ax_b = cell_axes_of(cell_b, st.L0, st.L1, st.L2);
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+        for (int ps = 0; ps <= n_or; ++ps, ++pass) {
+            for (int col = 0; col < NCOL; ++col) {
+                for (int q = 0; q < SPC; ++q) {
+                    const int sl = NCOL == 16 ? col : col + 4 * q;
+                    // one cell at a time: the 16 waves of the workgroup hide
+                    // the gather latency, and registers stay within 128
+#pragma unroll 1
+                    for (int j = 0; j < CPT; ++j) {
+                        const int cell = threadIdx.x + j * BS;
+                        if (cell >= nc) break;
+                        CellAxes c;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            c.a0[a] = j ? ax_b.a0[a] : ax_a.a0[a];
+                            c.a1[a] = j ? ax_b.a1[a] : ax_a.a1[a];
+                            c.a2[a] = j ? ax_b.a2[a] : ax_a.a2[a];
+                        }
+                        Real hx, hy, hz;
+                        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J, hx, hy, hz);
+                        if (ps == 0)
+                            my_acc += apply_update<Real, FIRST>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy,
+                                                                hz) ? 1u : 0u;
+                        else
+                            apply_update<Real, 3>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    __shared__ unsigned int red[BS / 64];
+    unsigned int v = my_acc;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < BS / 64; ++i) t += red[i];
+        atomicAdd(st.acc + r, t);
+    }
+}
+
+template <typename Real, typename F>
+hipError_t dispatch_pyro(const PyroCouplings<Real>& J, F&& f) {
+    switch (J.shells) {
+        case 1: return f(std::integral_constant<int, 1>{});
+        case 2: return f(std::integral_constant<int, 2>{});
+        case 3: return f(std::integral_constant<int, 3>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+template <typename Real>
+hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
+                              int kind, unsigned long long pass_index, hipStream_t s) {
+    const int ncol = J.shells == 3 ? 16 : 4;
+    const int bps = (st.n_cells + 255) / 256;
+    const dim3 grid(unsigned(bps * (16 / ncol)), unsigned(st.R));
+    return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+        constexpr int SH = decltype(shells)::value;
+        switch (kind) {
+            case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+            case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+            case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+            case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    });
+}
+
+template <typename Real>
+hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,
+                                int n_or, int n_sweeps, unsigned long long pass_index,
+                                hipStream_t s) {
+    constexpr int BS = 1024, CPT = 2;
+    if (st.n_cells > BS * CPT) return hipErrorInvalidValue;
+    const dim3 grid(unsigned(st.R));
+    return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+        constexpr int SH = decltype(shells)::value;
+        switch (kind) {
+            case 0: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 0, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
+            case 1: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 1, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
+            case 2: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 2, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    });
+}
+
+}  // namespace pyro

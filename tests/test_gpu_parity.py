@@ -96,7 +96,7 @@ def _oracle_sweeps(o, r, kind, n_or, n_sweeps, T, colours, n_col, seed, pass0):
     return acc
 
 
-@pytest.mark.parametrize("strategy", [1, 2])
+@pytest.mark.parametrize("strategy", [1, 2, 3, 4])
 @pytest.mark.parametrize("kind,n_or", [(0, 1), (0, 2), (1, 1), (2, 0), (1, 0)])
 @pytest.mark.parametrize("J,Delta,B", [CONFIGS[1], CONFIGS[3]])
 def test_sweep_trajectory_fp64(gpu, oracle, strategy, kind, n_or, J, Delta, B):
@@ -147,11 +147,13 @@ def test_sweep_fp32_tracks_fp64(gpu, oracle, kind, n_or):
     assert np.allclose(np.linalg.norm(ctx.get_spins(), axis=1), 1.0, atol=1e-6)
 
 
-def test_strategies_bit_identical(gpu):
+@pytest.mark.parametrize("pair,J", [((1, 2), (1.0, 0.1, 0.1)), ((3, 4), (1.0, 0.1, 0.1)),
+                                    ((3, 4), (1.0, -0.2, 0.0)), ((3, 4), (-1.0, 0.0, 0.0))])
+def test_strategies_bit_identical(gpu, pair, J):
     res = []
-    for strategy in (1, 2):
+    for strategy in pair:
         ctx = gpu.Context.pyrochlore(3, n_replicas=4, precision=32, seed=99)
-        ctx.define_J1J2J3(1.0, 0.1, 0.1)
+        ctx.define_J1J2J3(*J)
         ctx.setup()
         ctx.set_strategy(strategy)
         ctx.set_temperatures([0.1, 0.3, 1.0, 3.0])
@@ -163,11 +165,36 @@ def test_strategies_bit_identical(gpu):
         assert np.array_equal(a, b)
 
 
-def test_autotuner_keeps_results(gpu):
+@pytest.mark.parametrize("prec", [64, 32])
+def test_stencil_and_table_kernels_agree(gpu, prec):
+    """The compile-time-stencil kernels sum the field by shell, the table
+    kernels slot by slot: same visit order and draws, equal to rounding."""
     out = []
-    for strategy in (0, 1):
+    for strategy in (1, 3):
+        ctx = gpu.Context.pyrochlore(3, n_replicas=2, precision=prec, seed=5)
+        ctx.define_J1J2J3(1.0, -0.05, 0.05, 0.7)
+        ctx.setup()
+        ctx.set_strategy(strategy)
+        ctx.set_temperatures([0.4, 1.5])
+        ctx.randomize(8)
+        ctx.sweep(gpu.METROPOLIS, 1, 1)
+        out.append(ctx.get_spins(1))
+    tol = 1e-12 if prec == 64 else 1e-4
+    assert np.mean(np.abs(out[0] - out[1]).max(1) > tol) < 0.01
+
+
+@pytest.mark.parametrize("stencil", [True, False])
+def test_autotuner_keeps_results(gpu, oracle, stencil):
+    """Auto-tuning picks within a bit-identical pair (3/4 for the stencil
+    kernels, 1/2 for the table kernels), so results do not depend on it."""
+    out = []
+    for strategy in (0, 3 if stencil else 1):
         ctx = gpu.Context.pyrochlore(2, n_replicas=8, precision=32, seed=3)
-        ctx.define_J1J2J3(1.0)
+        if stencil:
+            ctx.define_J1J2J3(1.0)
+        else:
+            for name, rel, J in oracle.j1j2j3_couplings(1.0, 0.0, 0.0):
+                ctx.define_coupling(name, rel, J)
         ctx.setup()
         ctx.set_strategy(strategy)
         ctx.set_temperatures(np.linspace(0.2, 2, 8))

@@ -66,6 +66,9 @@ def test_tables_and_colouring(pyro, oracle, J, n_slot, n_col):
     b = o2.bonds()
     assert np.all(col[b[:, 0]] != col[b[:, 1]])
     assert len(set(col.tolist())) == n_col
+    # canonical colours used by the stencil kernels: pyro_sl (4) or sublattice (16)
+    sl = np.arange(ctx.N) // L ** 3
+    assert np.array_equal(col, sl % 4 if n_col == 4 else sl)
 
 
 def test_xxz_coupling_matrices(pyro, oracle):
