@@ -228,6 +228,13 @@ int pyro_synchronize(pyro_ctx* ctx);
  * strategy that does not apply. */
 int pyro_set_strategy(pyro_ctx* ctx, int strategy);
 int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
+/* The site visiting order of one sweep under the strategy the next sweep
+ * will use: site[k] and pass_offset[k] (0 = the sweep's first pass kind,
+ * 1..n_overrelax = OR passes) for k < N * (1 + n_overrelax).  Sites with
+ * no bond between them may run in any relative order, so this is one of
+ * the sequential orders the parallel sweep equals. */
+int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pass_offset);
+
 /* Kernel timing with HIP events on the context's stream.  When enabled,
  * every launch is bracketed by events and its duration is added to the
  * counter of its kernel family:

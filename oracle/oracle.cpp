@@ -515,6 +515,30 @@ uint64_t orc_pass_ctr(void* h, int kind, double T, const int32_t* colour_of_site
     return acc;
 }
 
+// Counter-mode updates in an explicit order: update k visits site[k] with
+// kind first_kind if pass_offset[k] == 0 and overrelaxation otherwise, pass
+// index pass0 + pass_offset[k].  Returns the accepted count of the first kind.
+uint64_t orc_sweep_order(void* h, int first_kind, double T, const int32_t* site,
+                         const int32_t* pass_offset, int64_t n, uint64_t seed, uint32_t replica,
+                         uint64_t pass0) {
+    auto* o = static_cast<Oracle*>(h);
+    uint64_t acc = 0;
+    const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = site[k];
+        const uint64_t pass = pass0 + uint64_t(pass_offset[k]);
+        const W4 w = philox10(W4{{uint32_t(i), replica, uint32_t(pass), uint32_t(pass >> 32)}}, k0, k1);
+        double u[5];
+        uniforms5(w, u);
+        Draws d;
+        d.u = u;
+        const int kind = pass_offset[k] == 0 ? first_kind : 3;
+        const int a = o->update(kind, T, &o->spins[i], d);
+        if (pass_offset[k] == 0) acc += a;
+    }
+    return acc;
+}
+
 // Uniform-on-sphere initial spins of replica r (z = 2u0 - 1, phi = 2 pi u1).
 void orc_randomize_ctr(void* h, uint64_t seed, uint32_t replica) {
     auto* o = static_cast<Oracle*>(h);

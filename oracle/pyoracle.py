@@ -41,6 +41,7 @@ _SIGS = {
     "orc_sweep_ref": (C.c_uint64, [_vp, C.c_int, C.c_double, C.c_int]),
     "orc_pass_ctr": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64]),
     "orc_randomize_ctr": (None, [_vp, C.c_uint64, C.c_uint32]),
+    "orc_sweep_order": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, _i32p, C.c_int64, C.c_uint64, C.c_uint32, C.c_uint64]),
     "orc_philox": (None, [_u32p, _u32p, _u32p]),
     "orc_xoroshiro": (None, [C.c_uint64, C.c_int, _u64p]),
     "orc_exchange": (None, [C.c_int, _f64p, _f64p, C.c_double, _i32p, C.c_uint64, C.c_uint64, _u64p, _u64p]),
@@ -223,6 +224,12 @@ class Oracle:
     def pass_ctr(self, kind, T, colour_of_site, n_colours, seed, replica, pass_index):
         return int(lib().orc_pass_ctr(self.h, kind, float(T), np.ascontiguousarray(colour_of_site, np.int32),
                                       int(n_colours), int(seed), int(replica), int(pass_index)))
+
+    def sweep_order(self, first_kind, T, site, pass_offset, seed, replica, pass0):
+        site = np.ascontiguousarray(site, np.int32)
+        koff = np.ascontiguousarray(pass_offset, np.int32)
+        return int(lib().orc_sweep_order(self.h, int(first_kind), float(T), site, koff, len(site),
+                                         int(seed), int(replica), int(pass0)))
 
     def randomize_ctr(self, seed, replica=0):
         lib().orc_randomize_ctr(self.h, int(seed), int(replica))

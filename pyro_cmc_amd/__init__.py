@@ -64,6 +64,7 @@ _SIGS = {
     "pyro_ssf_read_munu": [_vp, C.c_int, C.c_int, _f64p],
     "pyro_ssf_reset": [_vp, C.c_int],
     "pyro_spin_ft": [_vp, C.c_int, _f64p],
+    "pyro_sweep_order": [_vp, C.c_int, _i32p, _i32p],
     "pyro_synchronize": [_vp],
     "pyro_set_strategy": [_vp, C.c_int],
     "pyro_get_strategy": [_vp, C.POINTER(C.c_int)],
@@ -326,6 +327,13 @@ class Context:
         a = np.zeros((3, self.n_sl, self.n_cells, 2))
         _check(lib().pyro_spin_ft(self._h, replica, a.ravel()))
         return a
+
+    def sweep_order(self, n_overrelax=1):
+        n = self.N * (1 + n_overrelax)
+        site = np.zeros(n, np.int32)
+        koff = np.zeros(n, np.int32)
+        _check(lib().pyro_sweep_order(self._h, n_overrelax, site, koff))
+        return site, koff
 
     # execution control
     def synchronize(self):

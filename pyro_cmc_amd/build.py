@@ -40,6 +40,8 @@ SOURCES = [
     "kernels_pyro_colour_f64.hip",
     "kernels_pyro_resident_f32.hip",
     "kernels_pyro_resident_f64.hip",
+    "kernels_pyro_tiled_f32.hip",
+    "kernels_pyro_tiled_f64.hip",
     "capi.cpp",
     "topology.cpp",
 ]

@@ -121,8 +121,10 @@ struct pyro_ctx {
     int spec_shells = 0;
 
     // requested: 0 auto, 1 generic per-colour, 2 generic resident,
-    // 3 pyrochlore per-colour, 4 pyrochlore resident
+    // 3 pyrochlore per-colour, 4 pyrochlore resident, 5 pyrochlore tiled
     int strategy = 0;
+    int tile_kind = 0;          // tiled strategy tile shape (0: not applicable)
+    size_t max_lds = 65536;     // per-workgroup LDS limit of the device
     int tuned_key = -1;        // (kind, n_or) the auto choice was made for
     int tuned_strategy = 1;
 
@@ -254,6 +256,22 @@ struct pyro_ctx {
     }
     bool resident_fits() const { return nc() <= pyro::kPyroResidentMaxCells; }
 
+    // Tile shape for the tiled strategy: L_a divisible by twice the tile
+    // edge (checkerboard of tiles) and the tile + halo within the LDS limit.
+    int pick_tile_kind() const {
+        if (!spec) return 0;
+        const auto& L = geom.L;
+        for (int kind : {2, 1}) {
+            int d[3];
+            pyro::pyro_tile_dims(kind, d);
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) ok = ok && L[a] % (2 * d[a]) == 0;
+            const size_t lds = size_t(3) * 16 * (d[0] + 2) * (d[1] + 2) * (d[2] + 2) * real_size();
+            if (ok && lds <= max_lds) return kind;
+        }
+        return 0;
+    }
+
     template <typename Real>
     pyro::SsfState<Real> ssf_state() const {
         pyro::SsfState<Real> st;
@@ -331,6 +349,17 @@ struct pyro_ctx {
             });
             return;
         }
+        if (strat == 5) {
+            const auto st = state<Real>(pass_base);
+            const auto J = pyro_couplings<Real>();
+            unsigned long long p = pass_index;
+            for (int sw = 0; sw < n_sweeps; ++sw, p += 1 + n_or)
+                for (int phase = 0; phase < 8; ++phase)
+                    timed(kFamSweep, [&] {
+                        return pyro::launch_pyro_tiled<Real>(st, J, tile_kind, phase, kind, n_or, p, stream);
+                    });
+            return;
+        }
         if (strat == 4) {
             const auto st = state<Real>(pass_base);
             const auto J = pyro_couplings<Real>();
@@ -360,7 +389,16 @@ struct pyro_ctx {
                     "strategies 3 and 4 need couplings defined by define_J1J2J3 alone");
             require(strategy != 4 || resident_fits(), PYRO_E_UNSUPPORTED,
                     "the pyrochlore resident strategy needs at most 2048 cells");
+            require(strategy != 5 || tile_kind > 0, PYRO_E_UNSUPPORTED,
+                    "the tiled strategy needs each L divisible by twice the tile edge (8 or 12)");
             return strategy;
+        }
+        // Large lattices take the tiled schedule (a fixed rule, so results do
+        // not depend on timing).
+        if (tile_kind > 0 && nc() >= 4096) {
+            tuned_key = kind * 64 + n_or;
+            tuned_strategy = 5;
+            return 5;
         }
         const int key = kind * 64 + n_or;
         if (tuned_key == key) return tuned_strategy;
@@ -607,6 +645,12 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
         require(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, PYRO_E_HIP, "no HIP device available");
         HIPCHK(hipGetDevice(&ctx->device));
         if (!ctx->stream) HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        {
+            int lds = 0;
+            if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess &&
+                lds > 0)
+                ctx->max_lds = size_t(lds);
+        }
         const int64_t N = ctx->N();
         const int R = ctx->R;
         const int NS = ctx->NS;
@@ -700,6 +744,7 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
         ctx->upload_temperatures();
         ctx->upload_field();
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->tile_kind = ctx->pick_tile_kind();
         ctx->is_setup = true;
     });
 }
@@ -1355,6 +1400,64 @@ int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft) {
     });
 }
 
+int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pass_offset) {
+    return guarded([&] {
+        require(ctx && site && pass_offset, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        require(n_overrelax >= 0, PYRO_E_INVALID, "negative pass count");
+        const int strat = ctx->strategy != 0 ? ctx->strategy
+                          : (ctx->tile_kind > 0 && ctx->nc() >= 4096) ? 5
+                          : (ctx->spec ? 3 : 1);
+        const auto& L = ctx->geom.L;
+        const int64_t nc = ctx->nc();
+        const auto& t = ctx->topo;
+        const int P = t.period[0] * t.period[1] * t.period[2];
+        int64_t o = 0;
+        auto class_of = [&](int64_t i) {
+            const int s = int(i / nc);
+            int64_t c = i % nc;
+            const int n2 = int(c % L[2]);
+            c /= L[2];
+            const int n1 = int(c % L[1]);
+            const int n0 = int(c / L[1]);
+            return s * P + ((n0 % t.period[0]) * t.period[1] + (n1 % t.period[1])) * t.period[2] + (n2 % t.period[2]);
+        };
+        if (strat != 5) {
+            for (int k = 0; k <= n_overrelax; ++k)
+                for (int c = 0; c < t.n_colours; ++c)
+                    for (int64_t i = 0; i < ctx->N(); ++i)
+                        if (t.class_colour[class_of(i)] == c) {
+                            site[o] = int32_t(i);
+                            pass_offset[o++] = k;
+                        }
+            return;
+        }
+        require(ctx->tile_kind > 0, PYRO_E_UNSUPPORTED, "tiled strategy does not apply");
+        int T[3];
+        pyro::pyro_tile_dims(ctx->tile_kind, T);
+        const int ncol = ctx->spec_shells == 3 ? 16 : 4;
+        for (int phase = 0; phase < 8; ++phase)
+            for (int m0 = 0; m0 < L[0] / (2 * T[0]); ++m0)
+                for (int m1 = 0; m1 < L[1] / (2 * T[1]); ++m1)
+                    for (int m2 = 0; m2 < L[2] / (2 * T[2]); ++m2) {
+                        const int o0 = ((phase & 1) + 2 * m0) * T[0];
+                        const int o1 = (((phase >> 1) & 1) + 2 * m1) * T[1];
+                        const int o2 = (((phase >> 2) & 1) + 2 * m2) * T[2];
+                        for (int k = 0; k <= n_overrelax; ++k)
+                            for (int col = 0; col < ncol; ++col)
+                                for (int q = 0; q < 16 / ncol; ++q) {
+                                    const int sl = ncol == 16 ? col : col + 4 * q;
+                                    for (int i0 = 0; i0 < T[0]; ++i0)
+                                        for (int i1 = 0; i1 < T[1]; ++i1)
+                                            for (int i2 = 0; i2 < T[2]; ++i2) {
+                                                site[o] = int32_t(sl * nc + ctx->geom.cell_index(o0 + i0, o1 + i1, o2 + i2));
+                                                pass_offset[o++] = k;
+                                            }
+                                }
+                    }
+    });
+}
+
 int pyro_synchronize(pyro_ctx* ctx) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
@@ -1365,7 +1468,7 @@ int pyro_synchronize(pyro_ctx* ctx) {
 int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
-        require(strategy >= 0 && strategy <= 4, PYRO_E_INVALID, "strategy must be 0..4");
+        require(strategy >= 0 && strategy <= 5, PYRO_E_INVALID, "strategy must be 0..5");
         ctx->strategy = strategy;
         ctx->tuned_key = -1;
         ctx->invalidate_graph();

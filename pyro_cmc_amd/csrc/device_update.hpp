@@ -74,19 +74,21 @@ __device__ __forceinline__ Real ld32(const Real* __restrict__ base, unsigned idx
 template <typename Real> struct Renorm { static constexpr bool value = false; };
 template <> struct Renorm<float> { static constexpr bool value = true; };
 
-// One single-site update of kind KIND at `site` of replica r, pass index
-// `pass`, given the local field (hx, hy, hz) without the global field.
-// Returns whether the proposal was accepted (OR: always false).
+// One single-site update of kind KIND of the spin (px, py, pz) at `site` of
+// replica r, pass index `pass`, given the local field (hx, hy, hz) without
+// the global field.  Returns whether the proposal was accepted (OR: always
+// false); `changed` tells whether the spin was modified.
 template <typename Real, int KIND>
-__device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sx,
-                                    Real* __restrict__ Sy, Real* __restrict__ Sz, int site, int r,
-                                    unsigned long long pass, Real hx, Real hy, Real hz) {
+__device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& px, Real& py, Real& pz,
+                                            bool& changed, int site, int r, unsigned long long pass,
+                                            Real hx, Real hy, Real hz) {
+    changed = false;
     const size_t N = size_t(st.N);
     hx -= st.B[3 * r + 0];  // h_loc = local_field - global_field
     hy -= st.B[3 * r + 1];
     hz -= st.B[3 * r + 2];
 
-    const Real sx = Sx[site], sy = Sy[site], sz = Sz[site];
+    const Real sx = px, sy = py, sz = pz;
     const double Td = st.T_slot[st.slot_of[r]];
     const Real T = Real(Td);
 
@@ -110,9 +112,10 @@ __device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* 
         const Real newE = nx * hx + ny * hy + nz * hz;
         const Real dE = newE - curE;
         if (dE < Real(0) || u24<Real>(u.m[4]) < dexp<Real>(ddiv<Real>(-dE, T))) {
-            Sx[site] = nx;
-            Sy[site] = ny;
-            Sz[site] = nz;
+            px = nx;
+            py = ny;
+            pz = nz;
+            changed = true;
             return true;
         }
         return false;
@@ -132,9 +135,10 @@ __device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* 
         if constexpr (Renorm<Real>::value) {
             normalize3<Real>(nx, ny, nz);
         }
-        Sx[site] = nx;
-        Sy[site] = ny;
-        Sz[site] = nz;
+        px = nx;
+        py = ny;
+        pz = nz;
+        changed = true;
         return false;
     } else if constexpr (KIND == 1) {
         // Lifted Metropolis, displacement form (MC.cpp:249-281).
@@ -161,9 +165,10 @@ __device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* 
         const Real curE = sx * hx + sy * hy + sz * hz;
         const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
         if (u24<Real>(u.m[1]) < dexp<Real>(ddiv<Real>(-dE, T))) {
-            Sx[site] = nx;
-            Sy[site] = ny;
-            Sz[site] = nz;
+            px = nx;
+            py = ny;
+            pz = nz;
+            changed = true;
             return true;
         }
         *dp = int8_t(-d);
@@ -199,14 +204,32 @@ __device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* 
         const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
         const Real ratio = dsqrt<Real>(ddiv<Real>(b2, a2)) * dexp<Real>(ddiv<Real>(-dE, T));
         if (u24<Real>(u.m[0]) < ratio) {
-            Sx[site] = nx;
-            Sy[site] = ny;
-            Sz[site] = nz;
+            px = nx;
+            py = ny;
+            pz = nz;
+            changed = true;
             return true;
         }
         *dp = int8_t(-d);
         return false;
     }
+}
+
+
+// update_spin on the spin stored in global memory (S[r][c][site]).
+template <typename Real, int KIND>
+__device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sx,
+                                             Real* __restrict__ Sy, Real* __restrict__ Sz, int site,
+                                             int r, unsigned long long pass, Real hx, Real hy, Real hz) {
+    Real x = Sx[site], y = Sy[site], z = Sz[site];
+    bool changed;
+    const bool acc = update_spin<Real, KIND>(st, x, y, z, changed, site, r, pass, hx, hy, hz);
+    if (changed) {
+        Sx[site] = x;
+        Sy[site] = y;
+        Sz[site] = z;
+    }
+    return acc;
 }
 
 // Wave-level count of accepted updates added to acc[r].

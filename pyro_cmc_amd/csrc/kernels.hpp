@@ -72,6 +72,20 @@ hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings
                                 int n_or, int n_sweeps, unsigned long long pass_index,
                                 hipStream_t s);
 
+// Tiled sweeps: one call launches one of the 8 checkerboard phases of a
+// sweep; its workgroups hold a tile + halo in LDS and run every pass.
+// tile_kind 1 = 4x4x4 cells, 2 = 6x6x4 cells; returns the tile edge per
+// axis through tile_dims.  LDS bytes per workgroup: 3*16*(T0+2)(T1+2)(T2+2)
+// * sizeof(Real).
+inline void pyro_tile_dims(int tile_kind, int dims[3]) {
+    dims[0] = tile_kind == 2 ? 6 : 4;
+    dims[1] = tile_kind == 2 ? 6 : 4;
+    dims[2] = 4;
+}
+template <typename Real>
+hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
+                             int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s);
+
 // Local fields (no global field) of replica r into out[3][N].
 template <typename Real>
 hipError_t launch_fields(const DeviceState<Real>& st, int r, Real* out, hipStream_t s);

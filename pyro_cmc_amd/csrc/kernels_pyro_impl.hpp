@@ -86,10 +86,11 @@ __device__ __forceinline__ CellAxes cell_axes_of(int cell, int L0, int L1, int L
     return cell_axes(n0, n1, n2, L0, L1, L2);
 }
 
-template <typename Real, int SHELLS, int SL>
-__device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Real* __restrict__ Sy,
-                                  const Real* __restrict__ Sz, int nc, const CellAxes& c,
-                                  const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
+// Field of a site of sublattice SL; fetch(n, x, y, z) loads the spin of
+// stencil neighbour n (compile-time target sublattice and cell offset).
+template <typename Real, int SHELLS, int SL, typename Fetch>
+__device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplings<Real>& J, Real& hx,
+                                                Real& hy, Real& hz) {
     constexpr int NS = pyrostencil::slots_for(SHELLS);
     Real s[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // per shell x, y, z
     Real zs = Real(0);
@@ -97,8 +98,8 @@ __device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Re
         constexpr int k = decltype(kk)::value;
         constexpr pyrostencil::Nbr n = kStencil.n[SL][k];
         constexpr int shell = k < 6 ? 0 : (k < 18 ? 1 : 2);
-        const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
-        const Real x = ld32(Sx, j), y = ld32(Sy, j), z = ld32(Sz, j);
+        Real x, y, z;
+        fetch(n, x, y, z);
         s[shell][0] += x;
         s[shell][1] += y;
         s[shell][2] += z;
@@ -128,6 +129,35 @@ __device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Re
         hy += Real(pyrostencil::kLink[p][1]) * t;
         hz += Real(pyrostencil::kLink[p][2]) * t;
     }
+}
+
+// Field from spins in global memory, neighbour cells from the wrapped axes.
+template <typename Real, int SHELLS, int SL>
+__device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Real* __restrict__ Sy,
+                                           const Real* __restrict__ Sz, int nc, const CellAxes& c,
+                                           const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
+    pyro_field_with<Real, SHELLS, SL>(
+        [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
+            const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
+            x = ld32(Sx, j);
+            y = ld32(Sy, j);
+            z = ld32(Sz, j);
+        },
+        J, hx, hy, hz);
+}
+
+// Field from a tile in LDS laid out [sl][halo cell], halo dims H0 x H1 x H2.
+template <typename Real, int SHELLS, int SL, int H1, int H2, int HC>
+__device__ __forceinline__ void pyro_field_lds(const Real* lx, const Real* ly, const Real* lz, int lc,
+                                               const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
+    pyro_field_with<Real, SHELLS, SL>(
+        [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
+            const int j = lc + (n.sl * HC + n.d[0] * H1 * H2 + n.d[1] * H2 + n.d[2]);
+            x = lx[j];
+            y = ly[j];
+            z = lz[j];
+        },
+        J, hx, hy, hz);
 }
 
 // Runtime sublattice -> compile-time stencil.
@@ -238,6 +268,117 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     }
 }
 
+// Tiled sweep: one workgroup per (tile, replica) of one checkerboard phase.
+// The T0 x T1 x T2 tile and a one-cell halo (all 16 sublattices) are loaded
+// into LDS once; every pass of the sweep (FIRST, then n_or OR passes) and
+// every colour runs on the LDS copy; the tile interior is written back once.
+// Tiles of one phase are two tiles apart along each axis, so no two of them
+// share a bond or write a cell another reads: the phase equals the tiles
+// processed one after another.  A sweep is the 8 phases in order, and within
+// a tile the passes in order, each pass visiting the colours in order, so
+// every site gets each pass of the sweep exactly once (with the counter of
+// that pass).
+template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2>
+__global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
+                                                         int phase, int n_or,
+                                                         unsigned long long pass_index) {
+    constexpr int H0 = T0 + 2, H1 = T1 + 2, H2 = T2 + 2, HC = H0 * H1 * H2;
+    constexpr int NI = T0 * T1 * T2;
+    constexpr int NCOL = pyrostencil::colours_for(SHELLS);
+    constexpr int SPC = 16 / NCOL;
+    extern __shared__ __align__(16) unsigned char pyro_smem[];
+    Real* lx = reinterpret_cast<Real*>(pyro_smem);
+    Real* ly = lx + 16 * HC;
+    Real* lz = ly + 16 * HC;
+
+    const int r = blockIdx.y;
+    const int L0 = st.L0, L1 = st.L1, L2 = st.L2, nc = st.n_cells;
+    const int q1n = L1 / (2 * T1), q2n = L2 / (2 * T2);
+    const int m2 = int(blockIdx.x) % q2n;
+    const int t12 = int(blockIdx.x) / q2n;
+    const int m1 = t12 % q1n, m0 = t12 / q1n;
+    const int o0 = ((phase & 1) + 2 * m0) * T0;
+    const int o1 = (((phase >> 1) & 1) + 2 * m1) * T1;
+    const int o2 = (((phase >> 2) & 1) + 2 * m2) * T2;
+    const size_t N = size_t(st.N);
+    Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
+    Real* __restrict__ Sy = Sx + N;
+    Real* __restrict__ Sz = Sy + N;
+
+    for (int e = threadIdx.x; e < 16 * HC; e += 256) {
+        const int sl = e / HC, hc = e % HC;
+        const int a2 = hc % H2, a1 = (hc / H2) % H1, a0 = hc / (H1 * H2);
+        int g0 = o0 + a0 - 1, g1 = o1 + a1 - 1, g2 = o2 + a2 - 1;
+        g0 += g0 < 0 ? L0 : 0;
+        g0 -= g0 >= L0 ? L0 : 0;
+        g1 += g1 < 0 ? L1 : 0;
+        g1 -= g1 >= L1 ? L1 : 0;
+        g2 += g2 < 0 ? L2 : 0;
+        g2 -= g2 >= L2 ? L2 : 0;
+        const unsigned gi = unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2);
+        lx[e] = ld32(Sx, gi);
+        ly[e] = ld32(Sy, gi);
+        lz[e] = ld32(Sz, gi);
+    }
+    __syncthreads();
+
+    const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    unsigned int my_acc = 0;
+    for (int k = 0; k <= n_or; ++k) {
+        for (int col = 0; col < NCOL; ++col) {
+            for (int q = 0; q < SPC; ++q) {
+                const int sl = NCOL == 16 ? col : col + 4 * q;
+                for (int m = threadIdx.x; m < NI; m += 256) {
+                    const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
+                    const int lc = ((i0 + 1) * H1 + (i1 + 1)) * H2 + (i2 + 1);
+                    Real hx, hy, hz;
+                    switch (sl) {
+#define PYRO_SL_CASE(X) \
+    case X: pyro_field_lds<Real, SHELLS, X, H1, H2, HC>(lx, ly, lz, lc, J, hx, hy, hz); break;
+                        PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+                        PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+                        PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+                        PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+                        default: hx = hy = hz = Real(0);
+                    }
+                    const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
+                    const int li = sl * HC + lc;
+                    Real x = lx[li], y = ly[li], z = lz[li];
+                    bool ch;
+                    if (k == 0)
+                        my_acc += update_spin<Real, FIRST>(st, x, y, z, ch, site, r, pass0, hx, hy, hz) ? 1u : 0u;
+                    else
+                        update_spin<Real, 3>(st, x, y, z, ch, site, r, pass0 + k, hx, hy, hz);
+                    if (ch) {
+                        lx[li] = x;
+                        ly[li] = y;
+                        lz[li] = z;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    for (int e = threadIdx.x; e < 16 * NI; e += 256) {
+        const int sl = e / NI, m = e % NI;
+        const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
+        const int lc = ((i0 + 1) * H1 + (i1 + 1)) * H2 + (i2 + 1);
+        const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
+        Sx[site] = lx[sl * HC + lc];
+        Sy[site] = ly[sl * HC + lc];
+        Sz[site] = lz[sl * HC + lc];
+    }
+
+    __shared__ unsigned int red[4];
+    unsigned int v = my_acc;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(st.acc + r, (unsigned long long)(red[0] + red[1] + red[2] + red[3]));
+}
+
 template <typename Real, typename F>
 hipError_t dispatch_pyro(const PyroCouplings<Real>& J, F&& f) {
     switch (J.shells) {
@@ -286,6 +427,43 @@ hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings
         }
         return hipGetLastError();
     });
+}
+
+template <typename Real>
+hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
+                             int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s) {
+    auto go = [&](auto t0, auto t1, auto t2) -> hipError_t {
+        constexpr int T0 = decltype(t0)::value, T1 = decltype(t1)::value, T2 = decltype(t2)::value;
+        const size_t lds = size_t(3) * 16 * (T0 + 2) * (T1 + 2) * (T2 + 2) * sizeof(Real);
+        const unsigned tiles = unsigned((st.L0 / (2 * T0)) * (st.L1 / (2 * T1)) * (st.L2 / (2 * T2)));
+        const dim3 grid(tiles, unsigned(st.R));
+        return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+            constexpr int SH = decltype(shells)::value;
+            {
+                switch (kind) {
+#define PYRO_TILED_LAUNCH(K)                                                                      \
+    case K: {                                                                                   \
+        auto* fn = pyro_tiled_kernel<Real, SH, K, T0, T1, T2>;                                  \
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),                            \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));        \
+        hipLaunchKernelGGL(fn, grid, dim3(256), lds, s, st, J, phase, n_or, pass_index);        \
+        break;                                                                                  \
+    }
+                    PYRO_TILED_LAUNCH(0) PYRO_TILED_LAUNCH(1) PYRO_TILED_LAUNCH(2)
+#undef PYRO_TILED_LAUNCH
+                    default: return hipErrorInvalidValue;
+                }
+                const hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        });
+    };
+    switch (tile_kind) {
+        case 1: return go(std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{});
+        case 2: return go(std::integral_constant<int, 6>{}, std::integral_constant<int, 6>{}, std::integral_constant<int, 4>{});
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace pyro
