@@ -196,6 +196,11 @@ int pyro_energy_sample(pyro_ctx* ctx);
 int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha,
                        const int32_t* beta, int n_slots, int store_error_term,
                        int munu, int munu_buffer);
+/* Accumulator slot of each tempering slot for the SSF samples taken in
+ * pyro_pt_rounds: map[s] in [-1, n_slots), -1 = that rung is not sampled
+ * (the reference samples only its T_sample rungs, temper.cpp:426-429).
+ * NULL clears the map (then tempering slot s accumulates into slot s). */
+int pyro_pt_ssf_map(pyro_ctx* ctx, const int32_t* map);
 /* One SSF sample of every replica r with slot_of_replica[r] >= 0 into that
  * slot.  slot_of_replica == NULL: use each replica's tempering slot. */
 int pyro_ssf_sample(pyro_ctx* ctx, const int32_t* slot_of_replica);
@@ -209,7 +214,7 @@ int pyro_ssf_read_munu(pyro_ctx* ctx, int slot, int c, double* C);
 /* Zero one slot's accumulators (slot -1 = all). */
 int pyro_ssf_reset(pyro_ctx* ctx, int slot);
 /* Raw per-sublattice DFT of one replica's spins, ft[3][n_sl][n_k][2]
- * (save_ft_spin_state layout, MC.cpp:320-368). */
+ * (save_ft_spin_state layout, MC.cpp:320-368); needs no SSF configuration. */
 int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft);
 
 /* ------------------------------------------------------------------ */

@@ -60,6 +60,7 @@ _SIGS = {
     "pyro_pt_reset_stats": [_vp],
     "pyro_ssf_configure": [_vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int],
     "pyro_ssf_sample": [_vp, _vp],
+    "pyro_pt_ssf_map": [_vp, _vp],
     "pyro_ssf_read": [_vp, C.c_int, _vp, _vp, C.POINTER(C.c_uint64)],
     "pyro_ssf_read_munu": [_vp, C.c_int, C.c_int, _f64p],
     "pyro_ssf_reset": [_vp, C.c_int],
@@ -123,6 +124,24 @@ def _ptr(a: Optional[np.ndarray]):
 # Pyrochlore cubic cell (16 sites, sl = 4 f + p), edge 8.
 _FCC = np.array([[0, 0, 0], [0, 4, 4], [4, 0, 4], [4, 4, 0]], np.int32)
 _LINK = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], np.int32)
+
+
+def seed_key(seed_index: int) -> int:
+    """64-bit RNG key of a seed index: the SplitMix64 output for state
+    seed_index (the host drivers' --seed), used as context seed and
+    randomize key."""
+    m = (1 << 64) - 1
+    z = (int(seed_index) + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def host_tool(name: str):
+    """Path of a built host driver (anneal, temper, spiral_byhand)."""
+    from pathlib import Path
+
+    return Path(__file__).resolve().parent / name
 
 
 def pyrochlore_cell():
@@ -303,6 +322,10 @@ class Context:
         self.n_slots = n_slots
         _check(lib().pyro_ssf_configure(self._h, len(al), al, be, n_slots, int(store_error_term),
                                         int(bool(munu)), int(munu_buffer)))
+
+    def pt_ssf_map(self, mapping=None):
+        arr = None if mapping is None else np.ascontiguousarray(mapping, np.int32)
+        _check(lib().pyro_pt_ssf_map(self._h, _ptr(arr)))
 
     def ssf_sample(self, slot_of_replica=None):
         arr = None if slot_of_replica is None else np.ascontiguousarray(slot_of_replica, np.int32)

@@ -80,9 +80,13 @@ def build_host_tools() -> list[Path]:
         "g++", "-O2", "-std=c++20", f"-I{ROOT / 'include'}", f"-I{HOST}",
         f"-I{HDF5_ROOT / 'include'}",
     ]
+    # libhdf5 by full path, never -L on its directory: that directory also
+    # holds an older libstdc++ which must not win at link or load time, so
+    # the system library directory precedes it in the run path.
+    hdf5 = sorted((HDF5_ROOT / "lib").glob("libhdf5.so.*"))[0]
     link = [
-        f"-L{PKG}", "-lpyro_cmc", f"-Wl,-rpath,$ORIGIN",
-        f"-L{HDF5_ROOT / 'lib'}", "-lhdf5", f"-Wl,-rpath,{HDF5_ROOT / 'lib'}",
+        f"-L{PKG}", "-lpyro_cmc", str(hdf5),
+        f"-Wl,-rpath,$ORIGIN:/usr/lib/x86_64-linux-gnu:{HDF5_ROOT / 'lib'}",
     ]
     tools = sorted(p for p in HOST.glob("*.cpp") if p.name not in ("host_lib.cpp",))
     lib_src = HOST / "host_lib.cpp"

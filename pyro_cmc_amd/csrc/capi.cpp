@@ -135,6 +135,9 @@ struct pyro_ctx {
     std::vector<int32_t> h_alpha, h_beta;
     DevMem F, G, tw0, tw1, tw2, phase, d_alpha, d_beta, sacc, sacc2, sn, ssf_slot;
     DevMem mbuf, mcnt, macc;
+    DevMem ssf_map;            // tempering slot -> accumulator slot (-1 skip)
+    bool has_ssf_map = false;
+    bool dft_ready = false;
     int munu_pending = 0;      // samples staged since the last flush (PT rounds)
     std::vector<int> host_munu_count;
 
@@ -505,6 +508,55 @@ struct pyro_ctx {
         timed(kFamMunu, [&] { return pyro::launch_munu_flush(munu_args(), stream); });
         munu_pending = 0;
         std::fill(host_munu_count.begin(), host_munu_count.end(), 0);
+    }
+
+    // DFT work buffers, per-axis twiddles e^{-2 pi i j / L_a} and the
+    // sublattice phases e^{-i q(K).r_mu} (q with the centred K').
+    void ensure_dft() {
+        if (dft_ready) return;
+        const size_t rs = real_size();
+        const int n_sl = geom.n_sl;
+        const size_t per_rep = size_t(3) * n_sl * nc() * 2 * rs;
+        ssf_chunk = int(std::max<size_t>(1, std::min<size_t>(R, (size_t(2) << 30) / per_rep)));
+        F.alloc(per_rep * ssf_chunk);
+        G.alloc(per_rep * ssf_chunk);
+        auto upload_real = [&](DevMem& m, const std::vector<double>& v) {
+            m.alloc(v.size() * rs);
+            if (rs == 8) {
+                h2d(m.p, v.data(), v.size() * 8);
+            } else {
+                std::vector<float> f(v.begin(), v.end());
+                h2d(m.p, f.data(), f.size() * 4);
+            }
+        };
+        const auto& L = geom.L;
+        DevMem* tws[3] = {&tw0, &tw1, &tw2};
+        for (int a = 0; a < 3; ++a) {
+            std::vector<double> tw(2 * L[a]);
+            for (int j = 0; j < L[a]; ++j) {
+                const double th = 2.0 * M_PI * double(j) / double(L[a]);
+                tw[2 * j] = std::cos(th);
+                tw[2 * j + 1] = -std::sin(th);
+            }
+            upload_real(*tws[a], tw);
+        }
+        const int64_t n_k = nc();
+        std::vector<double> ph(size_t(2) * n_sl * n_k);
+        for (int mu = 0; mu < n_sl; ++mu) {
+            const auto& r = geom.sl_pos[mu];
+            for (int64_t K = 0; K < n_k; ++K) {
+                const int Ks[3] = {int(K / (int64_t(L[1]) * L[2])), int((K / L[2]) % L[1]), int(K % L[2])};
+                double arg = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const int Kc = (2 * Ks[a] < L[a]) ? Ks[a] : Ks[a] - L[a];
+                    arg += 2.0 * M_PI * double(Kc) / double(geom.cell[a] * L[a]) * double(r[a]);
+                }
+                ph[2 * (mu * n_k + K)] = std::cos(arg);
+                ph[2 * (mu * n_k + K) + 1] = -std::sin(arg);
+            }
+        }
+        upload_real(phase, ph);
+        dft_ready = true;
     }
 
     void invalidate_graph() {
@@ -1049,8 +1101,8 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
                 "sweep kind must be METROPOLIS, LIFTED or LIFTED_ROT");
         require(n_overrelax >= 0 && n_sweep >= 0 && n_rounds >= 0, PYRO_E_INVALID, "negative count");
         require(!sample_ssf || ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
-        require(!sample_ssf || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
-                "SSF sampling in tempering rounds needs one accumulator slot per replica");
+        require(!sample_ssf || ctx->has_ssf_map || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
+                "SSF sampling in tempering rounds needs a slot map or one accumulator slot per replica");
         if (n_rounds == 0) return;
         const int strat = n_sweep > 0 ? ctx->choose_strategy(kind, n_overrelax) : 1;
         const unsigned long long ppr = (unsigned long long)n_sweep * (1 + n_overrelax);
@@ -1062,7 +1114,15 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
 
         auto round_body = [&]() {
             if (n_sweep > 0) ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull);
-            if (sample_ssf) ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+            if (sample_ssf) {
+                if (ctx->has_ssf_map) {
+                    HIPCHK(pyro::launch_map_slots(ctx->slot_of.as<int32_t>(), ctx->ssf_map.as<int32_t>(),
+                                                  ctx->ssf_slot.as<int32_t>(), ctx->R, ctx->stream));
+                    ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>());
+                } else {
+                    ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+                }
+            }
             ctx->energy(true);
             pyro::ExchangeArgs a;
             a.e = ctx->e.as<double>();
@@ -1185,50 +1245,7 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha, const in
         const size_t rs = ctx->real_size();
         const int64_t nc = ctx->nc();
         const int n_sl = ctx->geom.n_sl;
-        // replicas per DFT batch: keep F + G under ~4 GiB
-        const size_t per_rep = size_t(3) * n_sl * nc * 2 * rs;
-        ctx->ssf_chunk = int(std::max<size_t>(1, std::min<size_t>(ctx->R, (size_t(2) << 30) / per_rep)));
-        ctx->F.alloc(per_rep * ctx->ssf_chunk);
-        ctx->G.alloc(per_rep * ctx->ssf_chunk);
-        // twiddles e^{-2 pi i j / L_a} and sublattice phases e^{-i q(K).r_mu}
-        auto upload_real = [&](DevMem& m, const std::vector<double>& v) {
-            m.alloc(v.size() * rs);
-            if (rs == 8) {
-                ctx->h2d(m.p, v.data(), v.size() * 8);
-            } else {
-                std::vector<float> f(v.begin(), v.end());
-                ctx->h2d(m.p, f.data(), f.size() * 4);
-            }
-        };
-        const auto& L = ctx->geom.L;
-        DevMem* tws[3] = {&ctx->tw0, &ctx->tw1, &ctx->tw2};
-        for (int a = 0; a < 3; ++a) {
-            std::vector<double> tw(2 * L[a]);
-            for (int j = 0; j < L[a]; ++j) {
-                const double th = 2.0 * M_PI * double(j) / double(L[a]);
-                tw[2 * j] = std::cos(th);
-                tw[2 * j + 1] = -std::sin(th);
-            }
-            upload_real(*tws[a], tw);
-        }
-        std::vector<double> ph(size_t(2) * n_sl * nc);
-        for (int mu = 0; mu < n_sl; ++mu) {
-            const auto& r = ctx->geom.sl_pos[mu];
-            for (int64_t K = 0; K < nc; ++K) {
-                const int K2 = int(K % L[2]);
-                const int K1 = int((K / L[2]) % L[1]);
-                const int K0 = int(K / (int64_t(L[1]) * L[2]));
-                const int Ks[3] = {K0, K1, K2};
-                double arg = 0.0;
-                for (int a = 0; a < 3; ++a) {
-                    const int Kc = (2 * Ks[a] < L[a]) ? Ks[a] : Ks[a] - L[a];
-                    arg += 2.0 * M_PI * double(Kc) / double(ctx->geom.cell[a] * L[a]) * double(r[a]);
-                }
-                ph[2 * (mu * nc + K)] = std::cos(arg);
-                ph[2 * (mu * nc + K) + 1] = -std::sin(arg);
-            }
-        }
-        upload_real(ctx->phase, ph);
+        ctx->ensure_dft();
         ctx->d_alpha.alloc(sizeof(int32_t) * n_corr);
         ctx->d_beta.alloc(sizeof(int32_t) * n_corr);
         ctx->h2d(ctx->d_alpha.p, alpha, sizeof(int32_t) * n_corr);
@@ -1263,6 +1280,23 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha, const in
         HIPCHK(hipDeviceSynchronize());
         ctx->munu_pending = 0;
         ctx->ssf_ready = true;
+    });
+}
+
+int pyro_pt_ssf_map(pyro_ctx* ctx, const int32_t* map) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        ctx->invalidate_graph();
+        if (!map) {
+            ctx->has_ssf_map = false;
+            return;
+        }
+        for (int s = 0; s < ctx->R; ++s)
+            require(map[s] >= -1 && map[s] < ctx->n_slots, PYRO_E_INVALID, "slot map entry out of range");
+        ctx->ssf_map.alloc(sizeof(int32_t) * ctx->R);
+        ctx->h2d(ctx->ssf_map.p, map, sizeof(int32_t) * ctx->R);
+        ctx->has_ssf_map = true;
     });
 }
 
@@ -1384,7 +1418,7 @@ int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft) {
         require(ctx && ft, PYRO_E_INVALID, "NULL argument");
         ctx->need_setup();
         ctx->check_replica(replica, false);
-        require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        ctx->ensure_dft();
         const size_t n = size_t(3) * ctx->geom.n_sl * ctx->nc() * 2;
         if (ctx->prec == 64) {
             HIPCHK(pyro::launch_dft<double>(ctx->ssf_state<double>(), replica, 1, ctx->stream));

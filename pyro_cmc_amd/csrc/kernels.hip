@@ -268,6 +268,12 @@ __global__ __launch_bounds__(kBlock) void exchange_kernel(ExchangeArgs a) {
 
 __global__ void bump_kernel(unsigned long long* ctr, unsigned long long n) { *ctr += n; }
 
+// out[r] = map[slot_of[r]] (tempering slot -> accumulator slot, -1 = skip).
+__global__ void map_slots_kernel(const int32_t* __restrict__ slot_of, const int32_t* __restrict__ map,
+                                 int32_t* __restrict__ out, int R) {
+    for (int r = threadIdx.x; r < R; r += blockDim.x) out[r] = map[slot_of[r]];
+}
+
 __global__ __launch_bounds__(kBlock) void zero_kernel(uint32_t* p, size_t n) {
     for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += size_t(gridDim.x) * kBlock)
@@ -562,6 +568,12 @@ hipError_t launch_exchange(const ExchangeArgs& a, hipStream_t s) {
 
 hipError_t launch_bump(unsigned long long* ctr, unsigned long long n, hipStream_t s) {
     hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(1), 0, s, ctr, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_slots(const int32_t* slot_of, const int32_t* map, int32_t* out, int R,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(map_slots_kernel, dim3(1), dim3(kBlock), 0, s, slot_of, map, out, R);
     return hipGetLastError();
 }
 

@@ -122,6 +122,9 @@ hipError_t launch_exchange(const ExchangeArgs& a, hipStream_t s);
 
 // Adds n to *ctr on the device (graph-safe counter bump).
 hipError_t launch_bump(unsigned long long* ctr, unsigned long long n, hipStream_t s);
+// out[r] = map[slot_of[r]] for r < R.
+hipError_t launch_map_slots(const int32_t* slot_of, const int32_t* map, int32_t* out, int R,
+                            hipStream_t s);
 // Sets n words to zero (graph-safe replacement for a memset).
 hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s);
 

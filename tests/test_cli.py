@@ -1,0 +1,287 @@
+"""Host drivers (anneal, temper, spiral_byhand) and their HDF5 output.
+
+CPU tests cover argument handling and the seed derivation.  GPU tests run
+each driver on a small lattice and check:
+  * the HDF5 layout of anneal.cpp:229-245 / ssf_manager.hpp:413-641 /
+    MC.cpp:416-454 (datasets, shapes, dtypes, attributes, T order);
+  * that the file holds exactly what the same sequence of library calls
+    gives (a replay through the Python bindings), so the drivers add no
+    arithmetic of their own;
+  * spiral_byhand's closed forms (S^yy = S^zz = N^2/4 at +-Q, S^yz = -+iN^2/4,
+    S^xx = 0) and brute-force pair energies of the saved spins;
+  * the saved spin DFT against the oracle's direct DFT.
+"""
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import brute_energy_per_cell, pyro_positions
+
+TOOLS = ("anneal", "temper", "spiral_byhand")
+
+
+def _tool(pyro, name):
+    exe = pyro.host_tool(name)
+    if not exe.exists():
+        pytest.skip(f"{name} not built (needs the HDF5 headers at build time)")
+    return str(exe)
+
+
+def _run(cmd, timeout=120):
+    print("RUN", " ".join(cmd), flush=True)
+    return subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+
+
+# ---------------------------------------------------------------------------
+# CPU
+# ---------------------------------------------------------------------------
+def test_seed_key_is_splitmix64(pyro):
+    # SplitMix64 published first outputs for states 0 and 1234567
+    assert pyro.seed_key(0) == 0xE220A8397B1DCDAF
+    assert pyro.seed_key(1234567) == 0x599ED017FB08FC85
+
+
+@pytest.mark.parametrize("name", TOOLS)
+def test_missing_required_arguments(pyro, name):
+    r = _run([_tool(pyro, name)])
+    assert r.returncode == 1
+    assert "missing required argument" in r.stderr
+    assert "usage:" in r.stderr
+
+
+def test_unknown_option_and_bad_outdir(pyro, tmp_path):
+    exe = _tool(pyro, "anneal")
+    r = _run([exe, "-o", str(tmp_path), "-s", "0", "--T_sample", "1", "--J1", "1", "4", "--bogus", "1"])
+    assert r.returncode == 1 and "unknown option --bogus" in r.stderr
+    r = _run([exe, "-o", str(tmp_path / "nope"), "-s", "0", "--T_sample", "1", "--J1", "1", "4"])
+    assert r.returncode == 1 and "Cannot open outdir" in r.stderr
+
+
+def test_temper_rejects_short_ladder(pyro, tmp_path):
+    exe = _tool(pyro, "temper")
+    r = _run([exe, "-o", str(tmp_path), "-s", "0", "--T_sample", "1", "0.5", "-R", "2", "--J1", "1", "2"])
+    assert r.returncode == 1 and "must exceed T_sample count" in r.stderr
+    r = _run([exe, "-o", str(tmp_path), "-s", "0", "--T_sample", "1", "--J1", "1", "--J3", "0.1", "--Q", "0.5", "2"])
+    assert r.returncode == 1 and "mutually exclusive" in r.stderr
+
+
+def test_gpu_test_arguments_parse(pyro, tmp_path):
+    """The argument lists of the GPU tests below parse; without a device
+    the drivers then stop at the first device operation."""
+    if pyro.device_count() > 0:
+        pytest.skip("a device is present: the GPU tests run these")
+    cases = [
+        ("spiral_byhand", ["-o", str(tmp_path), "4", "0", "0", "1", "--J2", "0.3", "--J3", "-0.2"]),
+        ("anneal", _anneal_args(tmp_path, True)),
+        ("temper", _temper_args(tmp_path, 6, 2, 5, True)),
+    ]
+    for name, args in cases:
+        r = _run([_tool(pyro, name), *args])
+        assert r.returncode == 1
+        assert "no HIP device available" in r.stderr, r.stderr
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _h5(path):
+    from pyro_cmc_amd.h5lite import H5File
+
+    return H5File(path)
+
+
+def _check_ssf_metadata(f, L, labels, T_list):
+    assert f.read("/ssf/corr_lookup") == list(labels)
+    np.testing.assert_array_equal(f.read("/ssf/T_list"), np.sort(T_list))
+    assert f.read("/ssf/n_samples").dtype == np.uint64
+    pos = f.read("/ssf/sl_positions")
+    assert pos.dtype == np.int64 and pos.shape == (16, 3)
+    assert int(f.attr("/ssf", "n_spins")) == 16 * L ** 3
+    np.testing.assert_array_equal(f.attr("/ssf", "k_dims"), [L, L, L])
+    np.testing.assert_array_equal(f.read("/geometry/index_cell"), 8 * np.eye(3, dtype=np.int64))
+    np.testing.assert_allclose(f.read("/geometry/recip_vectors"), 2 * np.pi / 8 * np.eye(3), rtol=0, atol=1e-15)
+
+
+@pytest.mark.gpu
+def test_spiral_byhand_closed_forms(gpu, oracle, tmp_path):
+    L, Q = 4, (0, 0, 1)
+    r = _run([_tool(gpu, "spiral_byhand"), "-o", str(tmp_path), str(L), *map(str, Q), "--J2", "0.3", "--J3", "-0.2"])
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / f"L={L}_Q={Q[0]},{Q[1]},{Q[2]}_.out.h5"
+    spins = tmp_path / f"L={L}_Q={Q[0]},{Q[1]},{Q[2]}_.spins.h5"
+    N = 16 * L ** 3
+    with _h5(spins) as f:
+        pos = f.read("spin_pos")
+        S = f.read("spin_orientation")
+    np.testing.assert_array_equal(pos, pyro_positions(L))
+    q = 2 * np.pi * np.array(Q) / (8 * L)
+    ph = pos @ q
+    np.testing.assert_allclose(S, np.stack([0 * ph, np.cos(ph), np.sin(ph)], 1), atol=1e-15)
+    with _h5(out) as f:
+        _check_ssf_metadata(f, L, ["xx", "yy", "zz"], [0.0])
+        Sq = f.read("/ssf/static_corr")
+        assert Sq.shape == (3, 1, L ** 3, 2)
+        E = f.read("/energy/E")
+        np.testing.assert_array_equal(f.read("/energy/n_samples"), [1])
+    K = (Q[0] * L + Q[1]) * L + Q[2]
+    Km = ((-Q[0]) % L * L + (-Q[1]) % L) * L + (-Q[2]) % L
+    expect = np.zeros((3, L ** 3, 2))
+    expect[1, K, 0] = expect[1, Km, 0] = N * N / 4
+    expect[2, K, 0] = expect[2, Km, 0] = N * N / 4
+    np.testing.assert_allclose(Sq[:, 0], expect, rtol=0, atol=1e-8 * N * N)
+    e_ref = brute_energy_per_cell(S, L, -1.0, 0.3, -0.2)
+    np.testing.assert_allclose(E[0], e_ref, rtol=1e-12, atol=1e-12)
+
+
+def _anneal_args(tmp_path, lifted=False):
+    args = ["-o", str(tmp_path), "-s", "3", "--J1", "1", "--J2", "-0.05", "--J3", "0.05", "--Delta", "1.2",
+            "-B", "0", "0.1", "0.2", "--T_sample", "0.5", "0.2", "--T_hot", "3", "--n_steps", "4",
+            "-w", "2", "--n_burn_in", "3", "--n_sample", "3", "--save_state", "--save_state_ft", "2"]
+    if lifted:
+        args.append("--lifted")
+    return args
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lifted", [False, True])
+def test_anneal_file_matches_library_replay(gpu, oracle, tmp_path, lifted):
+    pyro = gpu
+    r = _run([_tool(pyro, "anneal"), *_anneal_args(tmp_path, lifted)])
+    assert r.returncode == 0, r.stderr
+    stem = "run_L=2_J1=1_J2=-0.05_J3=0.05_Delta=1.2_B=0,0.1,0.2_seed=3_Tc=0.2_sw=2_"
+    out = tmp_path / (stem + ".out.h5")
+    labels = ["xx", "yy", "zz", "xy", "xz", "yz"]
+    with _h5(out) as f:
+        assert sorted(f.keys("/")) == ["energy", "geometry", "ssf"]
+        T_grid = f.read("/energy/T_list")
+        E = f.read("/energy/E")
+        E2 = f.read("/energy/E2")
+        nE = f.read("/energy/n_samples")
+        _check_ssf_metadata(f, 2, labels, [0.5, 0.2])
+        Sf = f.read("/ssf/static_corr")
+        S2f = f.read("/ssf/static_corr_2")
+        nS = f.read("/ssf/n_samples")
+    assert len(T_grid) == 4 + 2 and np.all(np.diff(T_grid) < 0) and T_grid[0] == 3.0
+    assert Sf.shape == S2f.shape == (6, 2, 8, 2)
+    np.testing.assert_array_equal(nE, np.ones(6, np.uint64))
+    np.testing.assert_array_equal(nS, [3, 3])
+
+    # replay the driver's library calls
+    kind = pyro.LIFTED if lifted else pyro.METROPOLIS
+    key = pyro.seed_key(3)
+    ctx = pyro.Context.pyrochlore(2, 1, pyro.FP64, seed=key)
+    ctx.define_J1J2J3(1.0, -0.05, 0.05, 1.2)
+    ctx.setup()
+    ctx.set_T_ref(1.0)
+    ctx.set_field([0, 0.1, 0.2])
+    ctx.randomize(key)
+    ctx.ssf_configure(labels, 1, True)
+    ctx.set_temperatures([3.0])
+    ctx.sweep(kind, 1, 3)
+    E_rep, S_rep, S2_rep = [], {}, {}
+    for T in T_grid:
+        ctx.set_temperatures([T])
+        ctx.sweep(kind, 1, 2)
+        E_rep.append(ctx.energy_per_cell()[0])
+        if T in (0.5, 0.2):
+            ctx.ssf_reset(0)
+            for _ in range(3):
+                ctx.sweep(kind, 1, 2)
+                ctx.ssf_sample([0])
+            S_rep[T], S2_rep[T], n = ctx.ssf_read(0)
+            assert n == 3
+    np.testing.assert_array_equal(E, E_rep)
+    np.testing.assert_array_equal(E2, np.square(E_rep))
+    for j, T in enumerate([0.2, 0.5]):
+        np.testing.assert_array_equal(Sf[:, j], S_rep[T])
+        np.testing.assert_array_equal(S2f[:, j], S2_rep[T])
+
+    # final state dumps
+    S_fin = ctx.get_spins(0)
+    with _h5(tmp_path / (stem + ".spins.h5")) as f:
+        np.testing.assert_array_equal(f.read("spin_orientation"), S_fin)
+        np.testing.assert_array_equal(f.read("spin_pos"), pyro_positions(2))
+    with _h5(tmp_path / (stem + ".spindft.h5")) as f:
+        ft = f.read("spin_ft")
+    assert ft.shape == (3, 16, 8, 2)
+    orc = oracle.Oracle.pyrochlore(2)
+    orc.set_spins(S_fin)
+    np.testing.assert_allclose(ft, orc.spin_ft(), rtol=0, atol=1e-12)
+
+
+def _temper_args(tmp_path, R, L, seed, munu):
+    args = ["-o", str(tmp_path), "-s", str(seed), "--J1", "1", "--J2", "-0.05", "--J3", "0.05",
+            "--T_sample", "0.4", "0.25", "--T_hot", "2", "-R", str(R), "-w", "2", "--n_burn_in", "4",
+            "--n_sample", "7", "--save_state", str(L)]
+    if munu:
+        args.append("--munu")
+    return args
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("munu", [False, True])
+def test_temper_file_matches_library_replay(gpu, tmp_path, munu):
+    pyro = gpu
+    R, L, seed = 6, 2, 5
+    r = _run([_tool(pyro, "temper"), *_temper_args(tmp_path, R, L, seed, munu)])
+    assert r.returncode == 0, r.stderr
+    stem = f"run_L={L}_J1=1_J2=-0.05_J3=0.05_Delta=1_B=0,0,0_seed={seed}_Tc=0.25_sw=2_"
+    with _h5(tmp_path / (stem + ".out.h5")) as f:
+        T_asc = f.read("/energy/T_list")
+        E = f.read("/energy/E")
+        E2 = f.read("/energy/E2")
+        nE = f.read("/energy/n_samples")
+        _check_ssf_metadata(f, L, ["xx", "yy", "zz"], [0.4, 0.25])
+        Sf = f.read("/ssf/static_corr")
+        S2f = f.read("/ssf/static_corr_2")
+        nS = f.read("/ssf/n_samples")
+        Mf = f.read("/ssf/static_corr_munu") if munu else None
+        assert f.exists("/ssf/static_corr_munu") == munu
+    assert len(T_asc) == R and np.all(np.diff(T_asc) > 0) and T_asc[-1] == 2.0
+    T_grid = T_asc[::-1].copy()
+    assert Sf.shape == (3, 2, L ** 3, 2)
+    np.testing.assert_array_equal(nE, np.full(R, 7, np.uint64))
+    np.testing.assert_array_equal(nS, [7, 7])
+
+    key = pyro.seed_key(seed)
+    ctx = pyro.Context.pyrochlore(L, R, pyro.FP64, seed=key)
+    ctx.define_J1J2J3(1.0, -0.05, 0.05, 1.0)
+    ctx.setup()
+    ctx.set_T_ref(1.0)
+    ctx.set_field([0, 0, 0])
+    ctx.randomize(key)
+    ctx.pt_init(T_grid, pyro.seed_key(seed + R))
+    sampled = sorted([int(np.flatnonzero(T_grid == T)[0]) for T in (0.4, 0.25)], key=lambda s: T_grid[s])
+    ctx.ssf_configure(("xx", "yy", "zz"), len(sampled), True, munu=munu, munu_buffer=16)
+    m = np.full(R, -1, np.int32)
+    for j, s in enumerate(sampled):
+        m[s] = j
+    ctx.pt_ssf_map(m)
+    ctx.sweep(pyro.METROPOLIS, 1, 4)
+    ctx.pt_rounds(pyro.METROPOLIS, 1, 2, True, 7)
+    st = ctx.pt_stats()
+    np.testing.assert_array_equal(E, st["E"][::-1])
+    np.testing.assert_array_equal(E2, st["E2"][::-1])
+    for j in range(len(sampled)):
+        S, S2, n = ctx.ssf_read(j)
+        assert n == 7
+        np.testing.assert_array_equal(Sf[:, j], S)
+        np.testing.assert_array_equal(S2f[:, j], S2)
+        if munu:
+            for c in range(3):
+                np.testing.assert_array_equal(Mf[c, j], ctx.ssf_read_munu(j, c))
+    # per-pair swap summary printed by the driver
+    pairs = re.findall(r"pair\s+(\d+)<->\s*(\d+).*\((\d+)/(\d+)\)", r.stdout)
+    assert len(pairs) == R - 1
+    for a, b, acc, att in pairs:
+        a = int(a)
+        assert int(b) == a + 1
+        assert int(acc) == st["accepts"][a] and int(att) == st["attempts"][a]
+        # checkerboard: pair a is attempted in the rounds of parity a % 2
+        assert int(att) == len([k for k in range(7) if k % 2 == a % 2])
+    # coldest replica's spins
+    rep = ctx.pt_permutation()[R - 1]
+    with _h5(tmp_path / (stem + ".spins.h5")) as f:
+        np.testing.assert_array_equal(f.read("spin_orientation"), ctx.get_spins(int(rep)))

@@ -240,6 +240,47 @@ def test_tempering_rounds(gpu, oracle, prec):
     assert np.allclose(st["E2"], E2s, rtol=0, atol=tol * n_rounds * 100)
 
 
+def test_tempering_ssf_slot_map(gpu, oracle):
+    """SSF samples taken inside tempering rounds land in the accumulator the
+    map names for the rung a replica occupies at that round (temper.cpp:426-429:
+    only the T_sample rungs sample)."""
+    L, R, swap_seed = 2, 5, 77
+    T = np.geomspace(2.0, 0.1, R)
+    ctx, os_ = make(gpu, oracle, L, (1.0, 0.1, 0.0), 1.0, 64, R=R)
+    rng = np.random.default_rng(4)
+    for r in range(R):
+        S = random_spins(ctx.N, rng)
+        ctx.set_spins(S, r)
+        os_[r].set_spins(S)
+    corr = ("xx", "yz")
+    ssf = [o.ssf_sample(corr) for o in os_]
+    e = np.array([o.energy_per_cell() for o in os_])
+    ctx.pt_init(T, swap_seed)
+    ctx.ssf_configure(corr, n_slots=2)
+    mapping = np.array([-1, 1, -1, 0, -1], np.int32)
+    ctx.pt_ssf_map(mapping)
+    rep = np.arange(R, dtype=np.int32)
+    att = np.zeros(R, np.uint64)
+    accp = np.zeros(R, np.uint64)
+    want = np.zeros((2, len(corr), L ** 3, 2))
+    n_rounds = 6
+    for rnd in range(n_rounds):
+        for s in range(R):
+            if mapping[s] >= 0:
+                want[mapping[s]] += ssf[rep[s]]
+        rep = oracle.exchange(T, e, L ** 3, rep, rnd, swap_seed, att, accp)
+    ctx.pt_rounds(n_sweep=0, sample_ssf=True, n_rounds=n_rounds)
+    assert np.array_equal(ctx.pt_permutation(), rep)
+    assert accp.sum() > 0  # the test must see swaps
+    for j in range(2):
+        S, _, n = ctx.ssf_read(j)
+        assert n == n_rounds
+        assert np.max(np.abs(S - want[j])) <= 1e-12 * ctx.N ** 2 * n_rounds
+    ctx.pt_ssf_map(None)
+    with pytest.raises(gpu.PyroError):
+        ctx.pt_rounds(n_sweep=0, sample_ssf=True, n_rounds=1)  # 2 slots < 5 rungs, no map
+
+
 def test_tempering_graph_equals_eager(gpu):
     out = []
     for prof in (False, True):
