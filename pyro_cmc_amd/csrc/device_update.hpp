@@ -233,6 +233,23 @@ __device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* 
 }
 
 // Wave-level count of accepted updates added to acc[r].
+// Workgroup g of a 1-D grid of gx * R workgroups -> (block bx within the
+// replica, replica r).  The dispatcher hands workgroups to the 8 XCDs
+// round-robin by linear id, so when R is a multiple of 8 this places replica
+// r wholly on XCD r % 8 and each XCD's L2 caches only its own replicas'
+// spins (the neighbour gathers of a colour pass then hit in L2).
+__device__ __forceinline__ void xcd_split(unsigned g, unsigned gx, unsigned R, unsigned& bx,
+                                          unsigned& r) {
+    if ((R & 7u) == 0u) {
+        const unsigned s = g >> 3, j = s / gx;
+        bx = s - j * gx;
+        r = (g & 7u) + 8u * j;
+    } else {
+        r = g / gx;
+        bx = g - r * gx;
+    }
+}
+
 __device__ __forceinline__ void count_accepts(unsigned long long* acc, int r, bool a) {
     const unsigned long long mask = __ballot(a);
     if ((threadIdx.x & 63) == 0 && mask) atomicAdd(acc + r, (unsigned long long)__popcll(mask));

@@ -89,9 +89,11 @@ __global__ __launch_bounds__(kBlock) void sweep_colour_kernel(DeviceState<Real> 
                                                               const int32_t* __restrict__ cls,
                                                               int spc, int bpc,
                                                               unsigned long long pass_index) {
-    const int ci = blockIdx.x / bpc;
-    const int m = (blockIdx.x % bpc) * kBlock + threadIdx.x;
-    const int r = blockIdx.y;
+    unsigned bx, ru;
+    xcd_split(blockIdx.x, gridDim.x / unsigned(st.R), unsigned(st.R), bx, ru);
+    const int ci = int(bx) / bpc;
+    const int m = (int(bx) % bpc) * kBlock + threadIdx.x;
+    const int r = int(ru);
     const int desc = __builtin_amdgcn_readfirstlane(cls[ci]);
     const int sl = desc & 0xFFFF;
     const unsigned long long pass =
@@ -483,7 +485,7 @@ hipError_t launch_sweep_colour(const DeviceState<Real>& st, const ColourList& co
                                unsigned long long pass_index, hipStream_t s) {
     const int spc = st.n_cells / (st.p0 * st.p1 * st.p2);
     const int bpc = (spc + kBlock - 1) / kBlock;
-    const dim3 grid(unsigned(bpc * col.n_cls), unsigned(st.R));
+    const dim3 grid(unsigned(bpc * col.n_cls) * unsigned(st.R));  // replica-major, XCD-aware (xcd_split)
     return dispatch_ns(st.NS, [&](auto ns) -> hipError_t {
         constexpr int NS = decltype(ns)::value;
         auto go = [&](auto iso) -> hipError_t {

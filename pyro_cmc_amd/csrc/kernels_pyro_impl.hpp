@@ -186,10 +186,12 @@ __global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     const int nc = st.n_cells;
     const int bps = (nc + 255) / 256;
-    const int q = blockIdx.x / bps;
-    const int cell = (blockIdx.x % bps) * 256 + threadIdx.x;
+    unsigned bx, ru;
+    xcd_split(blockIdx.x, gridDim.x / unsigned(st.R), unsigned(st.R), bx, ru);
+    const int q = int(bx) / bps;
+    const int cell = (int(bx) % bps) * 256 + threadIdx.x;
     const int sl = NCOL == 16 ? colour : colour + 4 * q;
-    const int r = blockIdx.y;
+    const int r = int(ru);
     const unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
     bool accepted = false;
     if (cell < nc) {
@@ -396,7 +398,7 @@ hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<R
                               int kind, unsigned long long pass_index, hipStream_t s) {
     const int ncol = J.shells == 3 ? 16 : 4;
     const int bps = (st.n_cells + 255) / 256;
-    const dim3 grid(unsigned(bps * (16 / ncol)), unsigned(st.R));
+    const dim3 grid(unsigned(bps * (16 / ncol)) * unsigned(st.R));  // replica-major, XCD-aware (xcd_split)
     return dispatch_pyro(J, [&](auto shells) -> hipError_t {
         constexpr int SH = decltype(shells)::value;
         switch (kind) {
