@@ -100,6 +100,9 @@ struct pyro_ctx {
     hipStream_t stream = nullptr;
 
     DevMem S, dir, nbr, cpl, T_slot, slot_of, rep_of, B, acc, cls, counters;
+    DevMem acc_wave;          // [R][acc_waves] per-wave accept counters
+    int acc_waves = 0;        // wave slots per replica (largest per-colour grid)
+    bool count_acc = false;   // sweeps count accepted updates (pyro_sweep with a buffer)
     DevMem epart, e, Esum, E2sum, nE, att, accp;
     DevMem S_save, dir_save, fields;
     std::vector<int32_t> h_cls;          // all class descriptors, colour-major
@@ -228,7 +231,9 @@ struct pyro_ctx {
         st.T_slot = T_slot.as<double>();
         st.slot_of = slot_of.as<int32_t>();
         st.B = B.as<Real>();
-        st.acc = acc.as<unsigned long long>();
+        st.acc = count_acc ? acc.as<unsigned long long>() : nullptr;
+        st.acc_wave = count_acc ? acc_wave.as<unsigned int>() : nullptr;
+        st.acc_waves = acc_waves;
         st.pass_base = pass_base;
         st.k0 = uint32_t(seed) ^ pyro::kTagSweep;
         st.k1 = uint32_t(seed >> 32);
@@ -774,6 +779,19 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
         ctx->rep_of.alloc(sizeof(int32_t) * R);
         ctx->B.alloc(rs * 3 * R);
         ctx->acc.alloc(sizeof(unsigned long long) * R);
+        {
+            // wave slots of the largest per-colour grid: 256-thread blocks of
+            // 4 waves; generic kernels spc-site classes, stencil kernels
+            // whole sublattices (4 per launch with 4 colours)
+            const int64_t nc = ctx->nc();
+            const int64_t spc = nc / (ctx->topo.period[0] * ctx->topo.period[1] * ctx->topo.period[2]);
+            int64_t blocks = 4 * ((nc + 255) / 256);
+            for (int c = 0; c < ctx->topo.n_colours; ++c)
+                blocks = std::max<int64_t>(blocks, ((spc + 255) / 256) *
+                                                       (ctx->h_colour_begin[c + 1] - ctx->h_colour_begin[c]));
+            ctx->acc_waves = int(4 * blocks);
+            ctx->acc_wave.alloc(sizeof(unsigned int) * size_t(R) * size_t(ctx->acc_waves));
+        }
         ctx->counters.alloc(sizeof(unsigned long long) * 2);
         ctx->epart.alloc(sizeof(double) * R * pyro::energy_blocks(int(N)));
         ctx->e.alloc(sizeof(double) * R);
@@ -998,10 +1016,20 @@ int pyro_sweep(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweeps, uint64_t*
             return;
         }
         const int strat = ctx->choose_strategy(kind, n_overrelax);
-        HIPCHK(pyro::launch_zero(ctx->acc.p, size_t(2) * ctx->R, ctx->stream));
+        struct CountScope {  // counting on only for this call, also on error
+            bool& f;
+            CountScope(bool& flag, bool on) : f(flag) { f = on; }
+            ~CountScope() { f = false; }
+        } counting(ctx->count_acc, accepted != nullptr);
+        if (accepted) {
+            HIPCHK(pyro::launch_zero(ctx->acc.p, size_t(2) * ctx->R, ctx->stream));
+            HIPCHK(pyro::launch_zero(ctx->acc_wave.p, size_t(ctx->R) * ctx->acc_waves, ctx->stream));
+        }
         ctx->run_sweeps(strat, kind, n_overrelax, n_sweeps, nullptr, ctx->pass_count);
         ctx->pass_count += uint64_t(n_sweeps) * (1 + n_overrelax);
         if (accepted) {
+            HIPCHK(pyro::launch_reduce_accepts(ctx->acc_wave.as<unsigned int>(), ctx->acc_waves,
+                                               ctx->acc.as<unsigned long long>(), ctx->R, ctx->stream));
             std::vector<unsigned long long> a(ctx->R);
             HIPCHK(hipMemcpyAsync(a.data(), ctx->acc.p, sizeof(unsigned long long) * ctx->R,
                                   hipMemcpyDeviceToHost, ctx->stream));

@@ -250,9 +250,16 @@ __device__ __forceinline__ void xcd_split(unsigned g, unsigned gx, unsigned R, u
     }
 }
 
-__device__ __forceinline__ void count_accepts(unsigned long long* acc, int r, bool a) {
+// Per-colour kernels: wave w of block bx owns counter slot bx * (block
+// size / 64) + w of its replica.  Must be reached by every lane of the wave.
+template <typename Real>
+__device__ __forceinline__ void count_accepts(const DeviceState<Real>& st, int r, unsigned bx, bool a) {
+    if (st.acc_wave == nullptr) return;
     const unsigned long long mask = __ballot(a);
-    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(acc + r, (unsigned long long)__popcll(mask));
+    if ((threadIdx.x & 63) == 0 && mask) {
+        unsigned int* p = st.acc_wave + size_t(r) * st.acc_waves + bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        *p += unsigned(__popcll(mask));
+    }
 }
 
 }  // namespace pyro

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void sweep_colour_kernel(DeviceState<Real> 
         const int site = decode_site(desc, m, st.n_cells, st.L1, st.L2, st.p0, st.p1, st.p2);
         accepted = update_site<Real, NS, ISO, KIND>(st, site, sl, r, pass);
     }
-    if constexpr (KIND != 3) count_accepts(st.acc, r, accepted);
+    if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
 }
 
 // Strategy 2: one workgroup per replica runs whole sweeps; colours are
@@ -139,6 +139,7 @@ __global__ __launch_bounds__(BS) void sweep_resident_kernel(
             }
         }
     }
+    if (st.acc == nullptr) return;  // uniform: counts not requested
     // block reduction of the accept count
     __shared__ unsigned int red[BS / 64];
     unsigned int v = my_acc;
@@ -280,6 +281,22 @@ __global__ __launch_bounds__(kBlock) void zero_kernel(uint32_t* p, size_t n) {
     for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += size_t(gridDim.x) * kBlock)
         p[i] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void reduce_accepts_kernel(const unsigned int* __restrict__ acc_wave,
+                                                                int n, unsigned long long* __restrict__ acc) {
+    const int r = blockIdx.x;
+    unsigned long long t = 0;
+    for (int i = threadIdx.x; i < n; i += kBlock) t += acc_wave[size_t(r) * n + i];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    __shared__ unsigned long long red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long v = 0;
+        for (int i = 0; i < kBlock / 64; ++i) v += red[i];
+        acc[r] += v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -479,6 +496,13 @@ hipError_t dispatch_ns(int NS, F&& f) {
 }  // namespace
 
 int energy_blocks(int N) { return (N + kBlock - 1) / kBlock; }
+
+hipError_t launch_reduce_accepts(const unsigned int* acc_wave, int n, unsigned long long* acc, int R,
+                                 hipStream_t s) {
+    if (R <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_accepts_kernel, dim3(unsigned(R)), dim3(kBlock), 0, s, acc_wave, n, acc);
+    return hipGetLastError();
+}
 
 template <typename Real>
 hipError_t launch_sweep_colour(const DeviceState<Real>& st, const ColourList& col, int kind,

@@ -19,7 +19,15 @@ struct DeviceState {
     const double* T_slot = nullptr;      // [R] temperature of each slot
     const int32_t* slot_of = nullptr;    // [R] slot of each replica
     const Real* B = nullptr;             // [R][3] global field per replica
-    unsigned long long* acc = nullptr;   // [R] accepted-update counters
+    // Accepted-update counting, only when a caller asks for the counts
+    // (both NULL otherwise).  Whole-replica kernels add once per workgroup
+    // to acc[r]; per-colour kernels keep one counter per wave slot,
+    // acc_wave[r * acc_waves + slot], updated by its owning wave alone, so
+    // no two waves contend on an address.  launch_reduce_accepts folds the
+    // slots into acc.
+    unsigned long long* acc = nullptr;   // [R]
+    unsigned int* acc_wave = nullptr;    // [R][acc_waves]
+    int acc_waves = 0;
     const unsigned long long* pass_base = nullptr;  // device pass counter (graph mode)
     uint32_t k0 = 0, k1 = 0;             // RNG key
     double T_ref = 1.0;
@@ -127,6 +135,9 @@ hipError_t launch_map_slots(const int32_t* slot_of, const int32_t* map, int32_t*
                             hipStream_t s);
 // Sets n words to zero (graph-safe replacement for a memset).
 hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s);
+// acc[r] += sum of the n per-wave counters acc_wave[r][0..n), r < R.
+hipError_t launch_reduce_accepts(const unsigned int* acc_wave, int n, unsigned long long* acc, int R,
+                                 hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Static structure factor.

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
         pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J, hx, hy, hz);
         accepted = apply_update<Real, KIND>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
     }
-    if constexpr (KIND != 3) count_accepts(st.acc, r, accepted);
+    if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
 }
 
 // One workgroup per replica; thread t owns cells t, t + BS, ... (CPT cells).
@@ -258,6 +258,7 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
             }
         }
     }
+    if (st.acc == nullptr) return;  // uniform: counts not requested
     __shared__ unsigned int red[BS / 64];
     unsigned int v = my_acc;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -373,6 +374,7 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
         Sz[site] = lz[sl * HC + lc];
     }
 
+    if (st.acc == nullptr) return;  // uniform: counts not requested
     __shared__ unsigned int red[4];
     unsigned int v = my_acc;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
