@@ -11,10 +11,16 @@ rung, the energy sample and the neighbour-pair exchange.
 Metric: site-updates/s = N * R * 3 passes * 4 sweeps * steps / wall time.
 
 Also reported: the plain FP64 one-replica GPU path on the same lattice and
-sweep (the north-star comparison), the sweep kernel's HBM roofline and the
-CPU restatement of the reference sweep timed on this host.
+sweep, the sweep kernel's HBM roofline, the sweep kernel of the large
+tempering lattice (L = 48 x 16 replicas, lifted + OR), the sublattice-resolved
+correlator products on FP32 MFMA at that size, and the CPU restatement of the
+reference sweep timed on this host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+HBM traffic per launch ("traffic") comes from the rocprofv3 FETCH_SIZE and
+WRITE_SIZE summaries committed under profiles/ for the same kernel and
+workload (lines "traffic tag=<tag> bytes_per_launch=<B>"), or is null.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs c3,fp64,large,munu,cpu]
 """
 from __future__ import annotations
 
@@ -24,6 +30,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import re
 
 import numpy as np
 
@@ -38,6 +46,20 @@ N_OR = 2
 N_SWEEP = 4
 CORR = ("xx", "yy", "zz", "xy", "xz", "yz", "yx", "zx", "zy")
 PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E spec
+PEAK_FP32_MATRIX = 157.3e12  # flop/s, MI355X dense FP32 MFMA spec
+ALL_LEGS = ("c3", "fp64", "large", "munu", "cpu")
+
+
+def profiled_traffic(tag: str):
+    """HBM bytes per launch measured for `tag` by the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/*.txt, or None."""
+    pat = re.compile(r"traffic tag=(\S+) bytes_per_launch=([0-9.eE+]+)")
+    for f in sorted((ROOT / "profiles").glob("*.txt")):
+        for line in f.read_text().splitlines():
+            m = pat.search(line)
+            if m and m.group(1) == tag:
+                return float(m.group(2))
+    return None
 
 
 def bytes_per_update(prec_bytes: int, n_nbr: int, r_share: int, lifted_frac: float = 0.0) -> float:
@@ -100,6 +122,54 @@ def large_lattice_sweep(pyro, sweeps=6):
     return updates / (ms / 1e3), sweep_ms / launches, upd_per_launch, strategy
 
 
+def munu_correlators(pyro, samples=128, buffer=64):
+    """Sublattice-resolved correlators of the large tempering lattice:
+    L = 48 x 16 replicas, FP32 spins, all 9 (alpha, beta) pairs over the 16
+    sublattices (the 48 x 48 Hermitian matrix per k), one accumulator slot
+    per replica.  Samples are staged as FP32 rows X = [Re | Im] of the 48
+    transformed fields, `buffer` deep, and flushed as X^T X on FP32 MFMA
+    into FP64 sums (tile pairs I <= J of the Hermitian matrix).  Returns the
+    flush kernel's timing and work."""
+    R, L = 16, 48
+    ctx = pyro.Context.pyrochlore(L, n_replicas=R, precision=pyro.FP32, seed=48)
+    ctx.define_J1J2J3(1.0, 0.1, 0.1)
+    ctx.setup()
+    ctx.randomize(49)
+    ctx.ssf_configure(CORR, n_slots=R, store_error_term=True, munu=True, munu_buffer=buffer)
+    ctx.profile(True)
+    for _ in range(samples):
+        ctx.ssf_sample()
+    ctx.synchronize()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.close()
+    flush_ms, n_flush = prof["munu_flush"]
+    n_k, T = L ** 3, 3  # 48 features = 3 tiles of 16
+    pairs = T * (T + 1) // 2  # tile pairs I <= J of the Hermitian matrix
+    # per row and tile pair: RR, II, RI, IR products, 16 x 16 x 2 flops each
+    flops = R * n_k * buffer * pairs * 4 * 16 * 16 * 2
+    acc_bytes = R * n_k * pairs * 256 * 16 * 2  # FP64 complex accumulators read + written
+    row_bytes = R * n_k * buffer * 2 * 16 * T * 4  # staged FP32 rows read
+    return {
+        "workload": f"L={L} (1769472 spins) x {R} replicas, FP32, 9 correlators over 16 sublattices, "
+                    f"one slot per replica, {buffer} samples buffered per flush",
+        "flush_avg_ms": flush_ms / max(n_flush, 1),
+        "flushes": n_flush,
+        "stage_ms_per_sample": prof["munu"][0] / max(samples, 1),
+        "dft_ms_per_sample": prof["dft"][0] / max(samples, 1),
+        "roofline": {
+            "bound": "mfma",
+            "achieved": flops / (flush_ms / max(n_flush, 1) / 1e3) / 1e12,
+            "peak": PEAK_FP32_MATRIX / 1e12,
+            "unit": "TFLOP/s",
+            "frac": flops / (flush_ms / max(n_flush, 1) / 1e3) / PEAK_FP32_MATRIX,
+            "flops_per_launch": flops,
+            "bytes_per_launch": acc_bytes + row_bytes,
+            "traffic": profiled_traffic("munu_flush_L48x16"),
+        },
+    }
+
+
 def cpu_baseline(seconds: float = 12.0):
     """The oracle's restatement of the reference sweep (sequential, AoS
     spins with bond-shell pointer lists, Xoroshiro128++ + libstdc++
@@ -132,6 +202,29 @@ def cpu_baseline(seconds: float = 12.0):
     }
 
 
+def large_lattice_sweep_line(pyro):
+    rate, launch_ms, upl, strat_big = large_lattice_sweep(pyro)
+    # 1 lifted + 1 OR pass: the lifted direction byte (read + write) on half
+    # of the passes; the int32 table term is zero for the stencil kernels (no
+    # table is read), kept at its formula value for the comparison with the
+    # table kernels
+    b_big = bytes_per_update(4, 30, 16, lifted_frac=0.5)
+    return {
+        "workload": "L=48 (1769472 spins) x 16 replicas, FP32, J1=1 J2=J3=0.1, B=(0,0,0.2), "
+                    "sweep = 1 lifted Metropolis + 1 OR",
+        "site_updates_per_s": rate,
+        "strategy": strat_big,
+        "roofline": {
+            "bound": "hbm", "achieved": b_big * upl / (launch_ms / 1e3) / 1e9,
+            "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": b_big * upl / (launch_ms / 1e3) / PEAK_HBM,
+            "traffic": profiled_traffic("sweep_L48x16_lifted"),
+            "bytes_per_update": b_big, "updates_per_launch": upl,
+            "avg_launch_us": launch_ms * 1e3,
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +234,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the L=48 sweep-kernel line")
+    ap.add_argument("--no-munu", action="store_true", help="skip the L=48 MFMA correlator line")
+    ap.add_argument("--legs", default=",".join(ALL_LEGS),
+                    help="comma list of parts to run: " + ",".join(ALL_LEGS))
     ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (counter calibration)")
@@ -167,6 +263,20 @@ def main():
     if pyro.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+
+    legs = set(args.legs.split(","))
+    for flag, leg in ((args.no_cpu, "cpu"), (args.no_fp64, "fp64"), (args.no_large, "large"),
+                      (args.no_munu, "munu")):
+        if flag:
+            legs.discard(leg)
+    if "c3" not in legs:  # profiling runs of a single leg
+        out = {}
+        if "large" in legs:
+            out["large_lattice_sweep"] = large_lattice_sweep_line(pyro)
+        if "munu" in legs:
+            out["munu_correlators"] = munu_correlators(pyro)
+        print(json.dumps(out))
+        return
 
     # ---- headline: batched FP32 tempering ----
     ctx = make_c3(pyro, pyro.FP32, R_REP)
@@ -196,36 +306,16 @@ def main():
 
     # ---- FP64 one-replica GPU path, same lattice and sweep ----
     fp64 = None
-    if not args.no_fp64:
+    if "fp64" in legs:
         c1 = make_c3(pyro, pyro.FP64, 1)
         steps64 = max(4, args.steps // 16)
         w64, _ = time_rounds(c1, pyro, steps64, 4, sample_ssf=False)
         fp64 = c1.N * (1 + N_OR) * N_SWEEP * steps64 / w64
         c1.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
-
-    big = None
-    if not args.no_large:
-        rate, launch_ms, upl, strat_big = large_lattice_sweep(pyro)
-        # 1 lifted + 1 OR pass: the lifted direction byte (read + write) on
-        # half of the passes; the int32 table term is zero for the stencil
-        # kernels (no table is read), kept at its formula value for the
-        # comparison with the table kernels
-        b_big = bytes_per_update(4, 30, 16, lifted_frac=0.5)
-        big = {
-            "workload": "L=48 (1769472 spins) x 16 replicas, FP32, J1=1 J2=J3=0.1, B=(0,0,0.2), "
-                        "sweep = 1 lifted Metropolis + 1 OR",
-            "site_updates_per_s": rate,
-            "strategy": strat_big,
-            "roofline": {
-                "bound": "hbm", "achieved": b_big * upl / (launch_ms / 1e3) / 1e9,
-                "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": b_big * upl / (launch_ms / 1e3) / PEAK_HBM,
-                "bytes_per_update": b_big, "updates_per_launch": upl,
-                "avg_launch_us": launch_ms * 1e3,
-            },
-        }
+    cpu = cpu_baseline(args.cpu_seconds) if "cpu" in legs else None
+    big = large_lattice_sweep_line(pyro) if "large" in legs else None
+    munu = munu_correlators(pyro) if "munu" in legs else None
 
     line = {
         "metric": "site-updates/s (Metropolis+OR) at L=12x64 replicas",
@@ -256,8 +346,8 @@ def main():
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM,
-            "traffic": None,
-            "kernel": "sweep (" + ("resident workgroup per replica" if strategy == 2 else "one launch per colour") + ")",
+            "traffic": profiled_traffic("sweep_c3_L12x64"),
+            "kernel": "sweep (" + ("resident workgroup per replica" if strategy in (2, 4) else "one launch per colour") + ")",
             "bytes_per_update": b_alg,
             "updates_per_launch": upd_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,
@@ -266,6 +356,7 @@ def main():
         "kernel_launches": {k: v[1] for k, v in prof.items()},
         "swap_acceptance_min": float(min(acc_rate)) if acc_rate else None,
         "large_lattice_sweep": big,
+        "munu_correlators": munu,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

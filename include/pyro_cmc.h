@@ -199,7 +199,8 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha,
 /* Accumulator slot of each tempering slot for the SSF samples taken in
  * pyro_pt_rounds: map[s] in [-1, n_slots), -1 = that rung is not sampled
  * (the reference samples only its T_sample rungs, temper.cpp:426-429).
- * NULL clears the map (then tempering slot s accumulates into slot s). */
+ * No two rungs may share a slot.  NULL clears the map (then tempering slot
+ * s accumulates into slot s). */
 int pyro_pt_ssf_map(pyro_ctx* ctx, const int32_t* map);
 /* One SSF sample of every replica r with slot_of_replica[r] >= 0 into that
  * slot.  slot_of_replica == NULL: use each replica's tempering slot. */
@@ -244,8 +245,9 @@ int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pas
  * every launch is bracketed by events and its duration is added to the
  * counter of its kernel family:
  *   0 = sweep passes, 1 = energy, 2 = exchange, 3 = SSF transforms,
- *   4 = SSF contraction, 5 = sublattice-resolved products.
- * ms[6], launches[6]. */
+ *   4 = SSF contraction, 5 = sublattice-resolved staging of samples,
+ *   6 = sublattice-resolved MFMA products flushed into the accumulators.
+ * ms[7], launches[7]. */
 int pyro_profile_enable(pyro_ctx* ctx, int on);
 int pyro_profile_read(pyro_ctx* ctx, double* ms, uint64_t* launches);
 /* Elapsed milliseconds between two marks recorded on the context stream. */

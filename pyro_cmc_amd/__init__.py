@@ -18,7 +18,7 @@ LIB_PATH = _HERE / "libpyro_cmc.so"
 FP32, FP64 = 32, 64
 METROPOLIS, LIFTED, LIFTED_ROT, OVERRELAX = 0, 1, 2, 3
 STATUS = {0: "OK", 1: "INVALID", 2: "STATE", 3: "LOGIC", 4: "HIP", 5: "UNSUPPORTED"}
-FAMILIES = ("sweep", "energy", "exchange", "dft", "contract", "munu")
+FAMILIES = ("sweep", "energy", "exchange", "dft", "contract", "munu", "munu_flush")
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -374,8 +374,8 @@ class Context:
         _check(lib().pyro_profile_enable(self._h, int(on)))
 
     def profile_read(self):
-        ms = np.zeros(6)
-        n = np.zeros(6, np.uint64)
+        ms = np.zeros(len(FAMILIES))
+        n = np.zeros(len(FAMILIES), np.uint64)
         _check(lib().pyro_profile_read(self._h, ms, n))
         return {f: (float(ms[i]), int(n[i])) for i, f in enumerate(FAMILIES)}
 

@@ -81,8 +81,8 @@ struct DevMem {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-constexpr int kFamilies = 6;
-enum Family { kFamSweep = 0, kFamEnergy, kFamExchange, kFamDft, kFamContract, kFamMunu };
+constexpr int kFamilies = 7;
+enum Family { kFamSweep = 0, kFamEnergy, kFamExchange, kFamDft, kFamContract, kFamMunu, kFamMunuFlush };
 
 }  // namespace
 
@@ -510,7 +510,7 @@ struct pyro_ctx {
     }
     void munu_flush() {
         if (!munu) return;
-        timed(kFamMunu, [&] { return pyro::launch_munu_flush(munu_args(), stream); });
+        timed(kFamMunuFlush, [&] { return pyro::launch_munu_flush(munu_args(), stream); });
         munu_pending = 0;
         std::fill(host_munu_count.begin(), host_munu_count.end(), 0);
     }
@@ -1320,8 +1320,13 @@ int pyro_pt_ssf_map(pyro_ctx* ctx, const int32_t* map) {
             ctx->has_ssf_map = false;
             return;
         }
-        for (int s = 0; s < ctx->R; ++s)
+        std::vector<int> used(ctx->n_slots, 0);
+        for (int s = 0; s < ctx->R; ++s) {
             require(map[s] >= -1 && map[s] < ctx->n_slots, PYRO_E_INVALID, "slot map entry out of range");
+            // one rung per accumulator: the accumulation kernels of one
+            // round update each slot from a single replica
+            if (map[s] >= 0) require(used[map[s]]++ == 0, PYRO_E_INVALID, "two rungs map to one SSF slot");
+        }
         ctx->ssf_map.alloc(sizeof(int32_t) * ctx->R);
         ctx->h2d(ctx->ssf_map.p, map, sizeof(int32_t) * ctx->R);
         ctx->has_ssf_map = true;
@@ -1409,13 +1414,19 @@ int pyro_ssf_read_munu(pyro_ctx* ctx, int slot, int c, double* C) {
         std::vector<double> full(size_t(nc) * n_f * n_f * 2);
         ctx->d2h(full.data(), ctx->macc.as<double>() + size_t(slot) * full.size(), full.size() * 8);
         const int al = ctx->h_alpha[c], be = ctx->h_beta[c];
+        // The flush accumulates 16-feature tile pairs (I, J) with I <= J
+        // only; an entry of a lower tile is the conjugate of its transpose
+        // (the matrix sum_samples conj(A_f) A_g is Hermitian).
         for (int64_t K = 0; K < nc; ++K)
             for (int mu = 0; mu < n_sl; ++mu)
                 for (int nu = 0; nu < n_sl; ++nu) {
-                    const size_t src = ((size_t(K) * n_f + (al * n_sl + mu)) * n_f + (be * n_sl + nu)) * 2;
+                    const int f = al * n_sl + mu, g = be * n_sl + nu;
+                    const bool lower = f / 16 > g / 16;
+                    const size_t src = lower ? ((size_t(K) * n_f + g) * n_f + f) * 2
+                                             : ((size_t(K) * n_f + f) * n_f + g) * 2;
                     const size_t dst = ((size_t(K) * n_sl + mu) * n_sl + nu) * 2;
                     C[dst] = full[src];
-                    C[dst + 1] = full[src + 1];
+                    C[dst + 1] = lower ? -full[src + 1] : full[src + 1];
                 }
     });
 }

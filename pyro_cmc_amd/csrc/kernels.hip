@@ -333,6 +333,223 @@ __global__ __launch_bounds__(kBlock) void dft_axis_kernel(const Real* __restrict
     out[2 * e + 1] = im;
 }
 
+// Line DFT staged through LDS.  The fields are viewed as n_cols columns of
+// La points: column c = (o, i) with o = c / s, i = c % s holds the points
+// o*La*s + n*s + i, n < La (s = stride of the transformed axis).  A
+// workgroup loads 64 columns (coalesced: along i when s > 1, along the
+// contiguous lines when s == 1) and the twiddle table into LDS; wave w then
+// computes outputs K = w, w + 4, ... of all 64 columns, lane = column, with
+// the twiddle index K*n mod La advanced incrementally (exact table values,
+// no recurrence), and the results leave through LDS in the load's order.
+template <typename Real, bool REAL_IN, int KPT>
+__global__ __launch_bounds__(256) void dft_lds_kernel(const Real* __restrict__ in, Real* __restrict__ out,
+                                                      const Real* __restrict__ tw, int La, int s,
+                                                      unsigned n_cols) {
+    extern __shared__ __align__(16) unsigned char dft_smem[];
+    Real* xr = reinterpret_cast<Real*>(dft_smem);  // [La][64]
+    Real* xi = xr + 64 * La;                       // [La][64]
+    Real* wr = xi + 64 * La;                       // [La]
+    Real* wi = wr + La;                            // [La]
+    const unsigned c0 = blockIdx.x * 64u;  // launcher keeps 2 * La * n_cols < 2^31
+    const int P = 64 * La;
+    for (int n = threadIdx.x; n < La; n += 256) {
+        wr[n] = tw[2 * n];
+        wi[n] = tw[2 * n + 1];
+    }
+    for (int e = threadIdx.x; e < P; e += 256) {
+        int j, n;
+        unsigned a;
+        if (s == 1) {  // contiguous lines: element e of the 64-line block
+            j = e / La;
+            n = e - j * La;
+            a = (c0 + j) * La + n;
+        } else {       // strided lines: consecutive lanes take consecutive i
+            n = e >> 6;
+            j = e & 63;
+            const unsigned c = c0 + j, o = c / unsigned(s);
+            a = (o * La + n) * unsigned(s) + (c - o * unsigned(s));
+        }
+        Real re = Real(0), im = Real(0);
+        if (c0 + j < n_cols) {
+            if constexpr (REAL_IN) {
+                re = in[a];
+            } else {
+                re = in[2 * a];
+                im = in[2 * a + 1];
+            }
+        }
+        xr[n * 64 + j] = re;
+        xi[n * 64 + j] = im;
+    }
+    __syncthreads();
+
+    const int j = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    Real ar[KPT], ai[KPT];
+    int idx[KPT];
+#pragma unroll
+    for (int t = 0; t < KPT; ++t) {
+        ar[t] = Real(0);
+        ai[t] = Real(0);
+        idx[t] = 0;
+    }
+    for (int n = 0; n < La; ++n) {
+        const Real x = xr[n * 64 + j], y = xi[n * 64 + j];
+#pragma unroll
+        for (int t = 0; t < KPT; ++t) {
+            const int K = w + 4 * t;
+            if (K < La) {
+                const Real c = wr[idx[t]], d = wi[idx[t]];
+                ar[t] += x * c - y * d;
+                ai[t] += x * d + y * c;
+                idx[t] += K;
+                if (idx[t] >= La) idx[t] -= La;
+            }
+        }
+    }
+    __syncthreads();  // every read of xr/xi is done: reuse them for the outputs
+#pragma unroll
+    for (int t = 0; t < KPT; ++t) {
+        const int K = w + 4 * t;
+        if (K < La) {
+            xr[K * 64 + j] = ar[t];
+            xi[K * 64 + j] = ai[t];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += 256) {
+        int jj, K;
+        unsigned a;
+        if (s == 1) {
+            jj = e / La;
+            K = e - jj * La;
+            a = (c0 + jj) * La + K;
+        } else {
+            K = e >> 6;
+            jj = e & 63;
+            const unsigned c = c0 + jj, o = c / unsigned(s);
+            a = (o * La + K) * unsigned(s) + (c - o * unsigned(s));
+        }
+        if (c0 + jj < n_cols) {
+            out[2 * a] = xr[K * 64 + jj];
+            out[2 * a + 1] = xi[K * 64 + jj];
+        }
+    }
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// FP32 line DFT as a complex GEMM on MFMA: for 64 columns of La points
+// (same column view as dft_lds_kernel), Y = W X with W[K][n] = e^{-2 pi i K n
+// / La}, as real products Yr = Wr Xr - Wi Xi, Yi = Wr Xi + Wi Xr on
+// v_mfma_f32_16x16x4f32.  M (= K) is padded to 16 * MT, the reduction (= n)
+// to a multiple of 4, both with zeros.  Wave w owns output columns
+// 16w .. 16w + 15; X and W^T sit in LDS with bank-conflict-free strides.
+template <bool REAL_IN, int MT>
+__global__ __launch_bounds__(256) void dft_mfma_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                       const float* __restrict__ tw, int La, int s,
+                                                       unsigned n_cols) {
+    constexpr int XS = 80;       // X row stride (floats): rows 4 apart land 16 banks apart
+    constexpr int P = 16 * MT;   // padded output rows
+    extern __shared__ __align__(16) unsigned char dft_smem[];
+    const int Kp = (La + 3) & ~3;  // padded reduction length
+    float* xr = reinterpret_cast<float*>(dft_smem);  // [Kp][XS]
+    float* xi = xr + Kp * XS;                        // [Kp][XS]
+    float* wtr = xi + Kp * XS;                       // [Kp][P]  W^T, real part
+    float* wti = wtr + Kp * P;                       // [Kp][P]  W^T, imaginary part
+    const unsigned c0 = blockIdx.x * 64u;  // launcher keeps 2 * La * n_cols < 2^31
+    for (int e = threadIdx.x; e < Kp * P; e += 256) {
+        const int n = e / P, K = e - n * P;
+        float c = 0.f, d = 0.f;
+        if (n < La && K < La) {
+            const int j = (K * n) % La;
+            c = tw[2 * j];
+            d = tw[2 * j + 1];
+        }
+        wtr[e] = c;
+        wti[e] = d;
+    }
+    for (int e = threadIdx.x; e < Kp * 64; e += 256) {
+        int j, n;
+        unsigned a;
+        if (s == 1) {
+            j = e / Kp;
+            n = e - j * Kp;
+            a = (c0 + j) * La + n;
+        } else {
+            n = e >> 6;
+            j = e & 63;
+            const unsigned c = c0 + j, o = c / unsigned(s);
+            a = (o * La + n) * unsigned(s) + (c - o * unsigned(s));
+        }
+        float re = 0.f, im = 0.f;
+        if (n < La && c0 + j < n_cols) {
+            if constexpr (REAL_IN) {
+                re = in[a];
+            } else {
+                re = in[2 * a];
+                im = in[2 * a + 1];
+            }
+        }
+        xr[n * XS + j] = re;
+        xi[n * XS + j] = im;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int kq = lane >> 4, ci = lane & 15;
+    v4f dr[MT], di[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) dr[m] = di[m] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < Kp; k0 += 4) {
+        const int n = k0 + kq;
+        const float br = xr[n * XS + 16 * w + ci];
+        const float bi = REAL_IN ? 0.f : xi[n * XS + 16 * w + ci];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const float ar = wtr[n * P + 16 * m + ci];
+            const float ai = wti[n * P + 16 * m + ci];
+            dr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, dr[m], 0, 0, 0);
+            di[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, di[m], 0, 0, 0);
+            if constexpr (!REAL_IN) {
+                dr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, dr[m], 0, 0, 0);
+                di[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, di[m], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();  // every read of xr/xi is done: reuse them for the outputs
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int K = 16 * m + 4 * kq + v;
+            if (K < Kp) {
+                xr[K * XS + 16 * w + ci] = dr[m][v];
+                xi[K * XS + 16 * w + ci] = di[m][v];
+            }
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < La * 64; e += 256) {
+        int jj, K;
+        unsigned a;
+        if (s == 1) {
+            jj = e / La;
+            K = e - jj * La;
+            a = (c0 + jj) * La + K;
+        } else {
+            K = e >> 6;
+            jj = e & 63;
+            const unsigned c = c0 + jj, o = c / unsigned(s);
+            a = (o * La + K) * unsigned(s) + (c - o * unsigned(s));
+        }
+        if (c0 + jj < n_cols) {
+            out[2 * a] = xr[K * XS + jj];
+            out[2 * a + 1] = xi[K * XS + jj];
+        }
+    }
+}
+
 // Contracted correlators (ssf_manager.hpp:260-293 with the separable phase
 // fold): b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K); S_c = conj(b^alpha) b^beta.
 template <typename Real>
@@ -383,34 +600,46 @@ __global__ __launch_bounds__(kBlock) void contract_kernel(SsfState<Real> st, Cor
 // (v_mfma_f32_16x16x4_f32) over the buffered samples and is added to FP64.
 // ---------------------------------------------------------------------------
 
-// Stage the transformed fields of replica (r_begin + blockIdx.y) into the
-// buffer row cnt[slot] of its slot: buf[slot][k][b][2*16*T].
+// Stage the transformed fields of replica (r_begin + blockIdx.y) into row
+// cnt[slot] of its slot: buf[slot][b][k][2*16*T] = [Re A_f | Im A_f] for the
+// features f = a * n_sl + mu (zero-padded to 16*T).  A workgroup takes 64
+// consecutive k: reads along k (coalesced), transposes through LDS, and
+// writes the 64 rows as one contiguous block.
 template <typename Real, int T>
 __global__ __launch_bounds__(kBlock) void munu_stage_kernel(SsfState<Real> st,
                                                             const int32_t* __restrict__ slot_of,
                                                             int r_begin, float* __restrict__ buf,
                                                             const unsigned int* __restrict__ cnt,
                                                             int Bmax) {
-    constexpr int FW = 2 * 16 * T;  // padded row width
-    const int K = blockIdx.x * kBlock + threadIdx.x;
+    constexpr int FW = 2 * 16 * T;
+    __shared__ float xs[FW][65];  // [feature][k], padded against bank conflicts
     const int rl = blockIdx.y;
     const int slot = slot_of[r_begin + rl];
-    if (slot < 0 || K >= st.n_cells) return;
+    if (slot < 0) return;
     const unsigned int b = cnt[slot];
     if (b >= unsigned(Bmax)) return;
-    const size_t nc = size_t(st.n_cells);
-    float* row = buf + ((size_t(slot) * nc + K) * Bmax + b) * FW;
+    const int nc = st.n_cells;
+    const int K0 = int(blockIdx.x) * 64;
     const int nf = 3 * st.n_sl;
-    for (int f = 0; f < 16 * T; ++f) {
+    const int kl = threadIdx.x & 63;
+    for (int f = int(threadIdx.x >> 6); f < 16 * T; f += kBlock / 64) {
         float re = 0.f, im = 0.f;
-        if (f < nf) {
+        const int K = K0 + kl;
+        if (f < nf && K < nc) {
             const int a = f / st.n_sl, mu = f % st.n_sl;
-            const Real* F = st.F + 2 * (((size_t(rl) * 3 + a) * st.n_sl + mu) * nc + K);
+            const Real* F = st.F + 2 * (((size_t(rl) * 3 + a) * st.n_sl + mu) * size_t(nc) + K);
             re = float(F[0]);
             im = float(F[1]);
         }
-        row[f] = re;
-        row[16 * T + f] = im;
+        xs[f][kl] = re;
+        xs[16 * T + f][kl] = im;
+    }
+    __syncthreads();
+    const int nk = min(64, nc - K0);
+    float* dst = buf + ((size_t(slot) * Bmax + b) * size_t(nc) + K0) * FW;
+    for (int e = threadIdx.x; e < nk * FW; e += kBlock) {
+        const int k = e / FW, f = e - k * FW;
+        dst[e] = xs[f][k];
     }
 }
 
@@ -422,16 +651,18 @@ __global__ void munu_count_kernel(const int32_t* __restrict__ slot_of, int r_beg
     }
 }
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-// One wave per (slot, K): X^T X over cnt[slot] buffered rows.
+// One wave per (slot, K): the Hermitian 16T x 16T matrix
+// M_fg += sum_b conj(A_f) A_g over the cnt[slot] buffered rows, as real
+// products of the rows X = [Re | Im]: Re M = RR + II, Im M = RI - IR.  Only
+// the 16-feature tile pairs (I, J) with I <= J are formed and accumulated;
+// the lower tiles follow by conjugation when read.
 template <int T>
 __global__ __launch_bounds__(256) void munu_flush_kernel(const float* __restrict__ buf,
                                                          const unsigned int* __restrict__ cnt,
                                                          double* __restrict__ acc, int n_k,
                                                          int Bmax, int n_f) {
-    constexpr int NT = 2 * T;       // 16-wide tiles across [Re | Im]
-    constexpr int FW = 16 * NT;
+    constexpr int FW = 32 * T;
+    constexpr int NP = T * (T + 1) / 2;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int K = blockIdx.x * 4 + wave;
@@ -439,40 +670,49 @@ __global__ __launch_bounds__(256) void munu_flush_kernel(const float* __restrict
     if (K >= n_k) return;
     const int nb = int(cnt[slot]);
     if (nb == 0) return;
-    v4f D[NT][NT];
+    v4f RR[NP], II[NP], RI[NP], IR[NP];
 #pragma unroll
-    for (int I = 0; I < NT; ++I)
-#pragma unroll
-        for (int J = 0; J < NT; ++J) D[I][J] = v4f{0.f, 0.f, 0.f, 0.f};
-    const float* X = buf + (size_t(slot) * n_k + K) * size_t(Bmax) * FW;
+    for (int p = 0; p < NP; ++p) RR[p] = II[p] = RI[p] = IR[p] = v4f{0.f, 0.f, 0.f, 0.f};
     const int kk = lane >> 4;   // which of the 4 rows of this step
     const int col = lane & 15;  // feature within a tile
+    const size_t row_stride = size_t(n_k) * FW;  // from row b to row b + 1
+    const float* X = buf + (size_t(slot) * Bmax * n_k + K) * FW + col;
     for (int b0 = 0; b0 < nb; b0 += 4) {
         const int b = b0 + kk;
-        float x[NT];
+        float xr[T], xi[T];
 #pragma unroll
-        for (int I = 0; I < NT; ++I) x[I] = (b < nb) ? X[size_t(b) * FW + 16 * I + col] : 0.f;
+        for (int I = 0; I < T; ++I) {
+            xr[I] = (b < nb) ? X[size_t(b) * row_stride + 16 * I] : 0.f;
+            xi[I] = (b < nb) ? X[size_t(b) * row_stride + 16 * (T + I)] : 0.f;
+        }
+        int p = 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I)
+        for (int I = 0; I < T; ++I)
 #pragma unroll
-            for (int J = 0; J < NT; ++J)
-                D[I][J] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[I], x[J], D[I][J], 0, 0, 0);
+            for (int J = I; J < T; ++J, ++p) {
+                RR[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[I], xr[J], RR[p], 0, 0, 0);
+                II[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[I], xi[J], II[p], 0, 0, 0);
+                RI[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[I], xi[J], RI[p], 0, 0, 0);
+                IR[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[I], xr[J], IR[p], 0, 0, 0);
+            }
     }
-    // D[I][J] element v of this lane: row 16I + 4*(lane>>4) + v, column 16J + (lane&15).
-    double* out = acc + (size_t(slot) * n_k + K) * size_t(n_f) * n_f * 2;
+    // element v of a tile accumulator on this lane: row 16I + 4*(lane>>4) + v,
+    // column 16J + (lane&15)
+    double2* out = reinterpret_cast<double2*>(acc) + (size_t(slot) * n_k + K) * size_t(n_f) * n_f;
+    int p = 0;
 #pragma unroll
     for (int I = 0; I < T; ++I)
 #pragma unroll
-        for (int J = 0; J < T; ++J)
+        for (int J = I; J < T; ++J, ++p)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int f = 16 * I + 4 * (lane >> 4) + v;
                 const int g = 16 * J + (lane & 15);
                 if (f < n_f && g < n_f) {
-                    const double re = double(D[I][J][v]) + double(D[I + T][J + T][v]);
-                    const double im = double(D[I][J + T][v]) - double(D[I + T][J][v]);
-                    out[2 * (size_t(f) * n_f + g)] += re;
-                    out[2 * (size_t(f) * n_f + g) + 1] += im;
+                    double2 m = out[size_t(f) * n_f + g];
+                    m.x += double(RR[p][v]) + double(II[p][v]);
+                    m.y += double(RI[p][v]) - double(IR[p][v]);
+                    out[size_t(f) * n_f + g] = m;
                 }
             }
 }
@@ -612,20 +852,64 @@ hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
     return hipGetLastError();
 }
 
+// One axis of the per-sublattice DFT: the LDS line kernel for La <= 64,
+// the direct kernel otherwise.
+template <typename Real, bool REAL_IN>
+hipError_t dft_axis(const Real* in, Real* out, const Real* tw, int La, int stride, long long total,
+                    hipStream_t s) {
+    if (La > 64 || total >= (1ll << 30)) {
+        const unsigned blocks = unsigned((total + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL((dft_axis_kernel<Real, REAL_IN>), dim3(blocks), dim3(kBlock), 0, s, in, out, tw, La,
+                           stride, total);
+        return hipGetLastError();
+    }
+    const unsigned n_cols = unsigned(total / La);
+    const dim3 grid((n_cols + 63u) / 64u);
+    if constexpr (std::is_same_v<Real, float>) {
+        // FP32: the DFT as a complex GEMM on MFMA
+        const int Kp = (La + 3) & ~3, MT = (La + 15) / 16;
+        const size_t lds = (size_t(2) * Kp * 80 + size_t(2) * Kp * 16 * MT) * sizeof(float);
+        auto go = [&](auto mt) -> hipError_t {
+            constexpr int M = decltype(mt)::value;
+            auto* fn = dft_mfma_kernel<REAL_IN, M>;
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(fn, grid, dim3(256), lds, s, in, out, tw, La, stride, n_cols);
+            return hipGetLastError();
+        };
+        switch (MT) {
+            case 1: return go(std::integral_constant<int, 1>{});
+            case 2: return go(std::integral_constant<int, 2>{});
+            case 3: return go(std::integral_constant<int, 3>{});
+            default: return go(std::integral_constant<int, 4>{});
+        }
+    }
+    const size_t lds = (size_t(2) * 64 * La + 2 * La) * sizeof(Real);
+    auto go = [&](auto kpt) -> hipError_t {
+        constexpr int KPT = decltype(kpt)::value;
+        auto* fn = dft_lds_kernel<Real, REAL_IN, KPT>;
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(fn, grid, dim3(256), lds, s, in, out, tw, La, stride, n_cols);
+        return hipGetLastError();
+    };
+    if (La <= 16) return go(std::integral_constant<int, 4>{});
+    if (La <= 32) return go(std::integral_constant<int, 8>{});
+    return go(std::integral_constant<int, 16>{});
+}
+
 template <typename Real>
 hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStream_t s) {
     const size_t nc = size_t(st.n_cells);
     const long long total = (long long)r_count * 3 * st.n_sl * (long long)nc;
-    const unsigned blocks = unsigned((total + kBlock - 1) / kBlock);
     const Real* spins = st.S + size_t(r_begin) * 3 * st.n_sl * nc;
     // axis 2 (stride 1): spins -> F; axis 1 (stride L2): F -> G; axis 0: G -> F
-    hipLaunchKernelGGL((dft_axis_kernel<Real, true>), dim3(blocks), dim3(kBlock), 0, s, spins, st.F,
-                       st.tw2, st.L2, 1, total);
-    hipLaunchKernelGGL((dft_axis_kernel<Real, false>), dim3(blocks), dim3(kBlock), 0, s, st.F, st.G,
-                       st.tw1, st.L1, st.L2, total);
-    hipLaunchKernelGGL((dft_axis_kernel<Real, false>), dim3(blocks), dim3(kBlock), 0, s, st.G, st.F,
-                       st.tw0, st.L0, st.L1 * st.L2, total);
-    return hipGetLastError();
+    hipError_t e = dft_axis<Real, true>(spins, st.F, st.tw2, st.L2, 1, total, s);
+    if (e == hipSuccess) e = dft_axis<Real, false>(st.F, st.G, st.tw1, st.L1, st.L2, total, s);
+    if (e == hipSuccess) e = dft_axis<Real, false>(st.G, st.F, st.tw0, st.L0, st.L1 * st.L2, total, s);
+    return e;
 }
 
 template <typename Real>
@@ -639,7 +923,7 @@ hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_be
 template <typename Real>
 hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
                              int r_count, const MunuArgs& m, hipStream_t s) {
-    const dim3 grid(unsigned((st.n_cells + kBlock - 1) / kBlock), unsigned(r_count));
+    const dim3 grid(unsigned((st.n_cells + 63) / 64), unsigned(r_count));
     switch (m.T) {
         case 1: hipLaunchKernelGGL((munu_stage_kernel<Real, 1>), grid, dim3(kBlock), 0, s, st, slot_of, r_begin, m.buf, m.cnt, m.Bmax); break;
         case 2: hipLaunchKernelGGL((munu_stage_kernel<Real, 2>), grid, dim3(kBlock), 0, s, st, slot_of, r_begin, m.buf, m.cnt, m.Bmax); break;

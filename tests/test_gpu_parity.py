@@ -327,6 +327,40 @@ def test_ssf_contracted(gpu, oracle, prec):
     assert np.max(np.abs(ft - os_[1].spin_ft())) <= TOL[prec] * L ** 3
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("L", [5, 12])
+def test_spin_ft_against_direct_dft(gpu, oracle, prec, L):
+    """Per-sublattice DFT (save_ft_spin_state layout) at line lengths that
+    pad the FP32 GEMM tiles (5) and at the tempering lattice (12)."""
+    ctx, os_ = make(gpu, oracle, L, prec=prec)
+    S = random_spins(ctx.N, np.random.default_rng(L))
+    ctx.set_spins(S, 0)
+    os_[0].set_spins(S)
+    ft = ctx.spin_ft(0)
+    want = os_[0].spin_ft()
+    assert np.max(np.abs(ft - want)) <= TOL[prec] * L ** 3
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_spin_ft_parseval_full_size(gpu, prec):
+    """L = 48 (the large tempering lattice): sum_K |A_mu(K)|^2 = L^3 sum_n |S_mu(n)|^2
+    per component and sublattice, and A(0) = sum_n S(n)."""
+    L = 48
+    ctx = gpu.Context.pyrochlore(L, precision=prec, seed=3)
+    ctx.define_J1J2J3(1.0)
+    ctx.setup()
+    ctx.randomize(5)
+    S = ctx.get_spins(0)  # [N][3], site = sl * L^3 + cell
+    ft = ctx.spin_ft(0)   # [3][16][L^3][2]
+    Sf = S.T.reshape(3, 16, L ** 3)
+    power = (ft ** 2).sum(axis=(2, 3))
+    want = L ** 3 * (Sf ** 2).sum(axis=2)
+    tol = 1e-12 if prec == 64 else 2e-5
+    assert np.max(np.abs(power / want - 1)) <= tol
+    assert np.max(np.abs(ft[:, :, 0, 0] - Sf.sum(axis=2))) <= tol * L ** 3
+    assert np.max(np.abs(ft[:, :, 0, 1])) <= tol * L ** 3
+
+
 def test_ssf_shared_slot_layers(gpu, oracle):
     """Several replicas feeding one slot are accumulated without races."""
     L, R = 2, 4
@@ -348,7 +382,7 @@ def test_ssf_shared_slot_layers(gpu, oracle):
 @pytest.mark.parametrize("prec", [64, 32])
 def test_ssf_munu_mfma(gpu, oracle, prec):
     L, R = 2, 2
-    corr = ("xx", "yz", "zz")
+    corr = ("xx", "yz", "zz", "zx")  # zx: a lower tile, read back by conjugation
     ctx, os_ = make(gpu, oracle, L, prec=prec, R=R)
     ctx.ssf_configure(corr, n_slots=2, munu=True, munu_buffer=3)
     rng = np.random.default_rng(11)
