@@ -77,13 +77,13 @@ template <> struct Renorm<float> { static constexpr bool value = true; };
 // One single-site update of kind KIND of the spin (px, py, pz) at `site` of
 // replica r, pass index `pass`, given the local field (hx, hy, hz) without
 // the global field.  Returns whether the proposal was accepted (OR: always
-// false); `changed` tells whether the spin was modified.
+// false); `changed` tells whether the spin was modified.  dp points at the
+// site's lifted direction (read and flipped by the lifted kinds only).
 template <typename Real, int KIND>
 __device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& px, Real& py, Real& pz,
                                             bool& changed, int site, int r, unsigned long long pass,
-                                            Real hx, Real hy, Real hz) {
+                                            Real hx, Real hy, Real hz, int8_t* dp) {
     changed = false;
-    const size_t N = size_t(st.N);
     hx -= st.B[3 * r + 0];  // h_loc = local_field - global_field
     hy -= st.B[3 * r + 1];
     hz -= st.B[3 * r + 2];
@@ -142,7 +142,6 @@ __device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& p
         return false;
     } else if constexpr (KIND == 1) {
         // Lifted Metropolis, displacement form (MC.cpp:249-281).
-        int8_t* dp = st.dir + size_t(r) * N + site;
         const int8_t d = *dp;
         const Real hn = dsqrt<Real>(hx * hx + hy * hy + hz * hz);
         if (!(hn > Real(0))) {  // no field direction: bounce
@@ -175,7 +174,6 @@ __device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& p
         return false;
     } else {
         // Lifted rotation Metropolis with reflecting poles (MC.cpp:194-244).
-        int8_t* dp = st.dir + size_t(r) * N + site;
         const int8_t d = *dp;
         const Real curE = sx * hx + sy * hy + sz * hz;
         const Real ax = sy * hz - sz * hy, ay = sz * hx - sx * hz, az = sx * hy - sy * hx;
@@ -223,7 +221,8 @@ __device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* 
                                              int r, unsigned long long pass, Real hx, Real hy, Real hz) {
     Real x = Sx[site], y = Sy[site], z = Sz[site];
     bool changed;
-    const bool acc = update_spin<Real, KIND>(st, x, y, z, changed, site, r, pass, hx, hy, hz);
+    const bool acc = update_spin<Real, KIND>(st, x, y, z, changed, site, r, pass, hx, hy, hz,
+                                             st.dir + size_t(r) * size_t(st.N) + site);
     if (changed) {
         Sx[site] = x;
         Sy[site] = y;

@@ -280,7 +280,10 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
 // processed one after another.  A sweep is the 8 phases in order, and within
 // a tile the passes in order, each pass visiting the colours in order, so
 // every site gets each pass of the sweep exactly once (with the counter of
-// that pass).
+// that pass).  The lifted kinds also stage the interior's directions in LDS.
+// Workgroups map to tiles XCD by XCD: the 8 XCDs take contiguous chunks of
+// the (replica, tile) order, so consecutive tiles along n2, whose halo rows
+// share cache lines, are fetched through the same L2.
 template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2>
 __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                          int phase, int n_or,
@@ -289,16 +292,22 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
     constexpr int NI = T0 * T1 * T2;
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     constexpr int SPC = 16 / NCOL;
+    constexpr bool LIFTED = FIRST == 1 || FIRST == 2;
     extern __shared__ __align__(16) unsigned char pyro_smem[];
     Real* lx = reinterpret_cast<Real*>(pyro_smem);
     Real* ly = lx + 16 * HC;
     Real* lz = ly + 16 * HC;
+    int8_t* ld = reinterpret_cast<int8_t*>(lz + 16 * HC);  // [16][NI] lifted directions
 
-    const int r = blockIdx.y;
+    const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica and phase
+    unsigned g = blockIdx.x;
+    if ((gridDim.x & 7u) == 0u) g = (g & 7u) * (gridDim.x >> 3) + (g >> 3);
+    const int r = int(g / tiles);
+    const int tile = int(g - unsigned(r) * tiles);
     const int L0 = st.L0, L1 = st.L1, L2 = st.L2, nc = st.n_cells;
     const int q1n = L1 / (2 * T1), q2n = L2 / (2 * T2);
-    const int m2 = int(blockIdx.x) % q2n;
-    const int t12 = int(blockIdx.x) / q2n;
+    const int m2 = tile % q2n;
+    const int t12 = tile / q2n;
     const int m1 = t12 % q1n, m0 = t12 / q1n;
     const int o0 = ((phase & 1) + 2 * m0) * T0;
     const int o1 = (((phase >> 1) & 1) + 2 * m1) * T1;
@@ -322,6 +331,14 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
         lx[e] = ld32(Sx, gi);
         ly[e] = ld32(Sy, gi);
         lz[e] = ld32(Sz, gi);
+    }
+    int8_t* const dir_r = st.dir + size_t(r) * N;
+    if constexpr (LIFTED) {
+        for (int e = threadIdx.x; e < 16 * NI; e += 256) {
+            const int sl = e / NI, m = e % NI;
+            const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
+            ld[e] = dir_r[sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2)];
+        }
     }
     __syncthreads();
 
@@ -349,10 +366,11 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
                     const int li = sl * HC + lc;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool ch;
+                    int8_t* dp = LIFTED ? ld + sl * NI + m : nullptr;
                     if (k == 0)
-                        my_acc += update_spin<Real, FIRST>(st, x, y, z, ch, site, r, pass0, hx, hy, hz) ? 1u : 0u;
+                        my_acc += update_spin<Real, FIRST>(st, x, y, z, ch, site, r, pass0, hx, hy, hz, dp) ? 1u : 0u;
                     else
-                        update_spin<Real, 3>(st, x, y, z, ch, site, r, pass0 + k, hx, hy, hz);
+                        update_spin<Real, 3>(st, x, y, z, ch, site, r, pass0 + k, hx, hy, hz, dp);
                     if (ch) {
                         lx[li] = x;
                         ly[li] = y;
@@ -372,6 +390,7 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
         Sx[site] = lx[sl * HC + lc];
         Sy[site] = ly[sl * HC + lc];
         Sz[site] = lz[sl * HC + lc];
+        if constexpr (LIFTED) dir_r[site] = ld[e];
     }
 
     if (st.acc == nullptr) return;  // uniform: counts not requested
@@ -438,9 +457,10 @@ hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Re
                              int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s) {
     auto go = [&](auto t0, auto t1, auto t2) -> hipError_t {
         constexpr int T0 = decltype(t0)::value, T1 = decltype(t1)::value, T2 = decltype(t2)::value;
-        const size_t lds = size_t(3) * 16 * (T0 + 2) * (T1 + 2) * (T2 + 2) * sizeof(Real);
+        const size_t lds = size_t(3) * 16 * (T0 + 2) * (T1 + 2) * (T2 + 2) * sizeof(Real) +
+                           ((kind == 1 || kind == 2) ? size_t(16) * T0 * T1 * T2 : 0);
         const unsigned tiles = unsigned((st.L0 / (2 * T0)) * (st.L1 / (2 * T1)) * (st.L2 / (2 * T2)));
-        const dim3 grid(tiles, unsigned(st.R));
+        const dim3 grid(tiles * unsigned(st.R));  // (replica, tile), XCD-chunked in the kernel
         return dispatch_pyro(J, [&](auto shells) -> hipError_t {
             constexpr int SH = decltype(shells)::value;
             {
