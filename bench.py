@@ -295,9 +295,18 @@ def main():
     ctx.pt_rounds(pyro.METROPOLIS, N_OR, N_SWEEP, True, prof_steps)
     prof = ctx.profile_read()
     ctx.profile(False)
-    sweep_ms, sweep_launches = prof["sweep"]
-    prof_updates = N * R_REP * (1 + N_OR) * N_SWEEP * prof_steps
-    upd_per_launch = prof_updates / sweep_launches
+    # Sweep-kernel launch duration: HIP events on the context stream around
+    # back-to-back launches of whole sweeps (no per-launch events, no
+    # accept counting), as the tempering graph issues them.
+    n_sw = max(16, min(args.steps, 256))
+    ctx.synchronize()
+    ctx.mark(0)
+    ctx.sweep(pyro.METROPOLIS, N_OR, n_sw, count=False)
+    ctx.mark(1)
+    sweep_ms = ctx.elapsed_ms()
+    sweep_launches = {1: n_sw * (1 + N_OR) * info["n_colours"], 3: n_sw * (1 + N_OR) * info["n_colours"],
+                      2: 1, 4: 1, 5: 8 * n_sw}[strategy]
+    upd_per_launch = N * R_REP * (1 + N_OR) * n_sw / sweep_launches
     avg_launch_s = sweep_ms / 1e3 / sweep_launches
     b_alg = bytes_per_update(4, info["n_slot"], R_REP)
     achieved = b_alg * upd_per_launch / avg_launch_s
@@ -347,7 +356,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM,
             "traffic": profiled_traffic("sweep_c3_L12x64"),
-            "kernel": "sweep (" + ("resident workgroup per replica" if strategy in (2, 4) else "one launch per colour") + ")",
+            "kernel": {1: "sweep_colour_kernel", 2: "sweep_resident_kernel", 3: "pyro_colour_kernel",
+                       4: "pyro_resident_kernel", 5: "pyro_tiled_kernel"}[strategy],
             "bytes_per_update": b_alg,
             "updates_per_launch": upd_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,

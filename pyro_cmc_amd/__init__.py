@@ -265,8 +265,10 @@ class Context:
         return int(v.value)
 
     # sweeps
-    def sweep(self, kind=METROPOLIS, n_overrelax=1, n_sweeps=1):
-        acc = np.zeros(self.R, np.uint64)
+    def sweep(self, kind=METROPOLIS, n_overrelax=1, n_sweeps=1, count=True):
+        """Sweeps; returns the accepted counts per replica (None when
+        count=False, which skips the counting)."""
+        acc = np.zeros(self.R, np.uint64) if count else None
         _check(lib().pyro_sweep(self._h, kind, n_overrelax, n_sweeps, _ptr(acc)))
         return acc
 

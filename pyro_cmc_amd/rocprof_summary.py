@@ -2,10 +2,16 @@
 
     python -m pyro_cmc_amd.rocprof_summary trace  <results.db> [--top N]
     python -m pyro_cmc_amd.rocprof_summary counter <results.db> <COUNTER>
+    python -m pyro_cmc_amd.rocprof_summary traffic <fetch.db> <write.db> --kernel REGEX
+           --tag TAG [--fetch-scale F] [--write-scale W]
 
 `trace` prints per-kernel call count, total and average duration (us) from
 a `rocprofv3 --kernel-trace --stats` run; `counter` prints the per-kernel
 mean of one PMC counter per dispatch from a `rocprofv3 --pmc` run.
+`traffic` combines a FETCH_SIZE run and a WRITE_SIZE run (KiB per dispatch)
+into HBM bytes per launch of the kernels matching REGEX (dispatch-weighted
+mean), each counter scaled by what it reads for a plain copy of known size,
+and prints "traffic tag=TAG bytes_per_launch=B".
 """
 from __future__ import annotations
 
@@ -42,14 +48,42 @@ def counter(db: str, cname: str) -> str:
     return "\n".join(out)
 
 
+def _mean_kib(db: str, cname: str, kernel: str) -> tuple[float, int]:
+    cur = sqlite3.connect(db).cursor()
+    rows = list(cur.execute("select name, counter_value from pmc_events where counter_name = ?", (cname,)))
+    vals = [v for name, v in rows if re.search(kernel, _short(name))]
+    if not vals:
+        raise SystemExit(f"no {cname} dispatches of kernels matching {kernel!r} in {db}")
+    return sum(vals) / len(vals), len(vals)
+
+
+def traffic(fetch_db: str, write_db: str, kernel: str, tag: str, fetch_scale: float,
+            write_scale: float) -> str:
+    f, nf = _mean_kib(fetch_db, "FETCH_SIZE", kernel)
+    w, nw = _mean_kib(write_db, "WRITE_SIZE", kernel)
+    total = (f * fetch_scale + w * write_scale) * 1024.0
+    return (f"FETCH_SIZE {f:.1f} KiB x {fetch_scale} ({nf} dispatches) + WRITE_SIZE {w:.1f} KiB x "
+            f"{write_scale} ({nw} dispatches) per launch of /{kernel}/\n"
+            f"traffic tag={tag} bytes_per_launch={total:.0f}")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counter"])
+    ap.add_argument("mode", choices=["trace", "counter", "traffic"])
     ap.add_argument("db")
-    ap.add_argument("counter", nargs="?")
+    ap.add_argument("arg", nargs="?", help="COUNTER (counter mode) or the WRITE_SIZE db (traffic mode)")
     ap.add_argument("--top", type=int, default=20)
+    ap.add_argument("--kernel", default=".")
+    ap.add_argument("--tag", default="kernel")
+    ap.add_argument("--fetch-scale", type=float, default=2.0)
+    ap.add_argument("--write-scale", type=float, default=1.0)
     a = ap.parse_args()
-    print(trace(a.db, a.top) if a.mode == "trace" else counter(a.db, a.counter))
+    if a.mode == "trace":
+        print(trace(a.db, a.top))
+    elif a.mode == "counter":
+        print(counter(a.db, a.arg))
+    else:
+        print(traffic(a.db, a.arg, a.kernel, a.tag, a.fetch_scale, a.write_scale))
 
 
 if __name__ == "__main__":
