@@ -465,6 +465,23 @@ struct pyro_ctx {
         double* es = accumulate ? Esum.as<double>() : nullptr;
         double* e2 = accumulate ? E2sum.as<double>() : nullptr;
         unsigned long long* ne = accumulate ? nE.as<unsigned long long>() : nullptr;
+        // pyrochlore couplings take the stencil field (no neighbour table read)
+        // unless a table strategy (1, 2) is forced
+        const bool stencil = spec && (strategy == 0 || strategy >= 3);
+        if (stencil) {
+            timed(kFamEnergy, [&] {
+                return prec == 64
+                           ? pyro::launch_pyro_energy<double>(state<double>(nullptr), pyro_couplings<double>(),
+                                                              epart.as<double>(), stream)
+                           : pyro::launch_pyro_energy<float>(state<float>(nullptr), pyro_couplings<float>(),
+                                                             epart.as<double>(), stream);
+            });
+            timed(kFamEnergy, [&] {
+                return pyro::launch_energy_finish(epart.as<double>(), pyro::energy_blocks(int(N())), int(nc()),
+                                                  e.as<double>(), slot_of.as<int32_t>(), es, e2, ne, R, stream);
+            });
+            return;
+        }
         timed(kFamEnergy, [&] {
             return prec == 64 ? pyro::launch_energy<double>(state<double>(nullptr), epart.as<double>(),
                                                             e.as<double>(), es, e2, ne, stream)

@@ -737,6 +737,14 @@ hipError_t dispatch_ns(int NS, F&& f) {
 
 int energy_blocks(int N) { return (N + kBlock - 1) / kBlock; }
 
+hipError_t launch_energy_finish(const double* partial, int nblk, int n_cells, double* e,
+                                const int32_t* slot_of, double* Esum, double* E2sum,
+                                unsigned long long* nE, int R, hipStream_t s) {
+    hipLaunchKernelGGL(energy_finish_kernel, dim3(unsigned(R)), dim3(kBlock), 0, s, partial, nblk,
+                       double(n_cells), e, slot_of, Esum, E2sum, nE);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce_accepts(const unsigned int* acc_wave, int n, unsigned long long* acc, int R,
                                  hipStream_t s) {
     if (R <= 0 || n <= 0) return hipSuccess;

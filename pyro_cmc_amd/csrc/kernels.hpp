@@ -104,6 +104,15 @@ template <typename Real>
 hipError_t launch_energy(const DeviceState<Real>& st, double* partial, double* e,
                          double* Esum, double* E2sum, unsigned long long* nE, hipStream_t s);
 int energy_blocks(int N);
+// Stencil form of the partial sums for contexts with pyrochlore J1-J2-J3
+// couplings: the same partial[R][energy_blocks(N)] layout, no table read.
+template <typename Real>
+hipError_t launch_pyro_energy(const DeviceState<Real>& st, const PyroCouplings<Real>& J, double* partial,
+                              hipStream_t s);
+// Second stage of launch_energy (sums the partials of every replica).
+hipError_t launch_energy_finish(const double* partial, int nblk, int n_cells, double* e,
+                                const int32_t* slot_of, double* Esum, double* E2sum,
+                                unsigned long long* nE, int R, hipStream_t s);
 
 // Uniform-on-sphere spins, lifted directions +1.
 template <typename Real>

@@ -207,6 +207,36 @@ __global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
 }
 
+// Energy terms 1/2 S.h - B.S per site (MC.cpp:305-312) with the stencil
+// field; one partial sum per 256-site block and replica.
+template <typename Real, int SHELLS>
+__global__ __launch_bounds__(256) void pyro_energy_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
+                                                          double* __restrict__ partial) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int r = blockIdx.y;
+    double e = 0.0;
+    if (i < st.N) {
+        const int nc = st.n_cells;
+        const size_t N = size_t(st.N);
+        const Real* Sx = st.S + size_t(r) * 3 * N;
+        const Real* Sy = Sx + N;
+        const Real* Sz = Sy + N;
+        const int sl = i / nc;
+        const CellAxes c = cell_axes_of(i - sl * nc, st.L0, st.L1, st.L2);
+        Real hx, hy, hz;
+        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J, hx, hy, hz);
+        const Real sx = Sx[i], sy = Sy[i], sz = Sz[i];
+        const Real half = Real(0.5) * (sx * hx + sy * hy + sz * hz);
+        const Real bs = sx * st.B[3 * r] + sy * st.B[3 * r + 1] + sz * st.B[3 * r + 2];
+        e = double(half) - double(bs);
+    }
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[size_t(r) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
 // One workgroup per replica; thread t owns cells t, t + BS, ... (CPT cells).
 template <typename Real, int SHELLS, int FIRST, int BS, int CPT>
 __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
@@ -429,6 +459,17 @@ hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<R
             case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
             default: return hipErrorInvalidValue;
         }
+        return hipGetLastError();
+    });
+}
+
+template <typename Real>
+hipError_t launch_pyro_energy(const DeviceState<Real>& st, const PyroCouplings<Real>& J, double* partial,
+                              hipStream_t s) {
+    const dim3 grid(unsigned(energy_blocks(st.N)), unsigned(st.R));
+    return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+        constexpr int SH = decltype(shells)::value;
+        hipLaunchKernelGGL((pyro_energy_kernel<Real, SH>), grid, dim3(256), 0, s, st, J, partial);
         return hipGetLastError();
     });
 }
