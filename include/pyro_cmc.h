@@ -186,6 +186,36 @@ int pyro_pt_reset_stats(pyro_ctx* ctx);
 int pyro_energy_sample(pyro_ctx* ctx);
 
 /* ------------------------------------------------------------------ */
+/* Batches of independent anneals.  No counterpart in the reference,    */
+/* which runs one anneal per process (seeds merged afterwards,          */
+/* acc_runs.py:13-57); here every replica of a context can be its own   */
+/* model and follow a device-side temperature schedule.                 */
+/* ------------------------------------------------------------------ */
+/* Per-replica pyrochlore couplings J1J2J3Delta[R][4] = (J1, J2, J3,
+ * Delta) for a context whose couplings come from pyro_define_J1J2J3; a
+ * replica may not switch on a shell the context defined as zero (the
+ * colouring is the context's).  Fields B stay per replica
+ * (pyro_set_field).  NULL returns to the context's couplings.  Needs the
+ * pyrochlore strategies (0 = automatic, 3, 4, 5). */
+int pyro_set_replica_couplings(pyro_ctx* ctx, const double* J1J2J3Delta);
+/* Temperature schedule T[n_steps] shared by all replicas (the anneal
+ * grid of anneal.cpp:164-227 / cli_bits.hpp:250-289); resets the step
+ * counter, the energy history and the permutation (slot r = replica r). */
+int pyro_anneal_schedule(pyro_ctx* ctx, const double* T, int n_steps);
+/* n_steps anneal steps, each: every replica at the schedule's next
+ * temperature, n_sweep sweeps (kind + n_overrelax OR passes), if
+ * sample_ssf one SSF sample of every replica into slot r, then its energy
+ * per cell recorded in the history (energy_manager::sample per step,
+ * anneal.cpp:203-206).  Captured once as a HIP graph and only enqueued:
+ * contexts (each with its own stream) run concurrently until synchronised
+ * by a reading call. */
+int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep,
+                      int sample_ssf, int n_steps);
+/* Energy per cell of every replica at every schedule step, E[R][n_steps]
+ * (zero for steps not yet run). */
+int pyro_anneal_history(pyro_ctx* ctx, double* E);
+
+/* ------------------------------------------------------------------ */
 /* Static structure factor (ssf_manager.hpp:42-57, 260-293).            */
 /* ------------------------------------------------------------------ */
 /* n_corr correlators (alpha[c], beta[c] in {0,1,2}); n_slots accumulator

@@ -58,6 +58,10 @@ _SIGS = {
     "pyro_pt_stats": [_vp, _vp, _vp, _vp, _vp, _vp],
     "pyro_pt_permutation": [_vp, _i32p],
     "pyro_pt_reset_stats": [_vp],
+    "pyro_set_replica_couplings": [_vp, _vp],
+    "pyro_anneal_schedule": [_vp, _f64p, C.c_int],
+    "pyro_anneal_steps": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "pyro_anneal_history": [_vp, _f64p],
     "pyro_ssf_configure": [_vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int],
     "pyro_ssf_sample": [_vp, _vp],
     "pyro_pt_ssf_map": [_vp, _vp],
@@ -313,6 +317,26 @@ class Context:
 
     def pt_reset_stats(self):
         _check(lib().pyro_pt_reset_stats(self._h))
+
+    # batches of independent anneals
+    def set_replica_couplings(self, J1J2J3Delta):
+        """Per-replica (J1, J2, J3, Delta), shape [R][4]; None restores the
+        context's couplings."""
+        arr = None if J1J2J3Delta is None else np.ascontiguousarray(J1J2J3Delta, np.float64).reshape(self.R, 4)
+        _check(lib().pyro_set_replica_couplings(self._h, _ptr(arr)))
+
+    def anneal_schedule(self, T):
+        T = np.ascontiguousarray(T, np.float64)
+        self.n_sched = len(T)
+        _check(lib().pyro_anneal_schedule(self._h, T, len(T)))
+
+    def anneal_steps(self, kind=METROPOLIS, n_overrelax=1, n_sweep=1, sample_ssf=False, n_steps=1):
+        _check(lib().pyro_anneal_steps(self._h, kind, n_overrelax, n_sweep, int(bool(sample_ssf)), n_steps))
+
+    def anneal_history(self):
+        E = np.zeros((self.R, self.n_sched))
+        _check(lib().pyro_anneal_history(self._h, E.ravel()))
+        return E
 
     # structure factor
     def ssf_configure(self, correlators=("xx", "yy", "zz"), n_slots=1, store_error_term=True,

@@ -122,6 +122,15 @@ struct pyro_ctx {
     bool spec = false;
     double sJ1 = 0, sJ2 = 0, sJ3 = 0, sDelta = 1;
     int spec_shells = 0;
+    // per-replica (J1, J2, J3, Jx) of a batch of models in one context
+    DevMem Jrep;
+    bool has_Jrep = false, xxz_rep = false;
+    // device anneal schedule: sched[n_sched], energy history [R][n_sched]
+    DevMem sched, ehist;
+    std::vector<double> h_sched;
+    int n_sched = 0, anneal_step = 0;
+    hipGraphExec_t agexec = nullptr;
+    int a_kind = -1, a_nor = -1, a_nsw = -1, a_ssf = -1, a_strategy = -1;
 
     // requested: 0 auto, 1 generic per-colour, 2 generic resident,
     // 3 pyrochlore per-colour, 4 pyrochlore resident, 5 pyrochlore tiled
@@ -159,6 +168,7 @@ struct pyro_ctx {
 
     ~pyro_ctx() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (agexec) (void)hipGraphExecDestroy(agexec);
         for (auto& p : pending) {
             (void)hipEventDestroy(p.a);
             (void)hipEventDestroy(p.b);
@@ -260,6 +270,11 @@ struct pyro_ctx {
         J.Jx = Real(sJ1 * (sDelta - 1.0) / 3.0);
         J.shells = spec_shells;
         J.xxz = sDelta != 1.0;
+        if (has_Jrep) {
+            // the replicas' own couplings replace the context's, Delta included
+            J.per_replica = Jrep.as<Real>();
+            J.xxz = xxz_rep;
+        }
         return J;
     }
     bool resident_fits() const { return nc() <= pyro::kPyroResidentMaxCells; }
@@ -397,6 +412,8 @@ struct pyro_ctx {
         if (strategy != 0) {
             require(strategy <= 2 || spec, PYRO_E_UNSUPPORTED,
                     "strategies 3 and 4 need couplings defined by define_J1J2J3 alone");
+            require(strategy >= 3 || !has_Jrep, PYRO_E_UNSUPPORTED,
+                    "per-replica couplings need the pyrochlore strategies 3, 4 or 5");
             require(strategy != 4 || resident_fits(), PYRO_E_UNSUPPORTED,
                     "the pyrochlore resident strategy needs at most 2048 cells");
             require(strategy != 5 || tile_kind > 0, PYRO_E_UNSUPPORTED,
@@ -465,10 +482,7 @@ struct pyro_ctx {
         double* es = accumulate ? Esum.as<double>() : nullptr;
         double* e2 = accumulate ? E2sum.as<double>() : nullptr;
         unsigned long long* ne = accumulate ? nE.as<unsigned long long>() : nullptr;
-        // pyrochlore couplings take the stencil field (no neighbour table read)
-        // unless a table strategy (1, 2) is forced
-        const bool stencil = spec && (strategy == 0 || strategy >= 3);
-        if (stencil) {
+        if (stencil_observables()) {
             timed(kFamEnergy, [&] {
                 return prec == 64
                            ? pyro::launch_pyro_energy<double>(state<double>(nullptr), pyro_couplings<double>(),
@@ -586,7 +600,12 @@ struct pyro_ctx {
     void invalidate_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         gexec = nullptr;
+        if (agexec) (void)hipGraphExecDestroy(agexec);
+        agexec = nullptr;
     }
+    // Stencil kernels for field and energy: pyrochlore couplings, unless a
+    // table strategy (1, 2) is forced without per-replica couplings.
+    bool stencil_observables() const { return spec && (has_Jrep || strategy == 0 || strategy >= 3); }
 };
 
 // ---------------------------------------------------------------------------
@@ -811,7 +830,7 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
             ctx->acc_waves = int(4 * blocks);
             ctx->acc_wave.alloc(sizeof(unsigned int) * size_t(R) * size_t(ctx->acc_waves));
         }
-        ctx->counters.alloc(sizeof(unsigned long long) * 2);
+        ctx->counters.alloc(sizeof(unsigned long long) * 3);  // pass base, round, anneal step
         ctx->epart.alloc(sizeof(double) * R * pyro::energy_blocks(int(N)));
         ctx->e.alloc(sizeof(double) * R);
         ctx->Esum.alloc(sizeof(double) * R);
@@ -1078,15 +1097,26 @@ int pyro_local_fields(pyro_ctx* ctx, int replica, double* h) {
         ctx->need_setup();
         ctx->check_replica(replica, false);
         const int64_t N = ctx->N();
+        const bool sten = ctx->stencil_observables();
         if (ctx->prec == 64) {
-            HIPCHK(pyro::launch_fields<double>(ctx->state<double>(nullptr), replica, ctx->fields.as<double>(), ctx->stream));
+            if (sten)
+                HIPCHK(pyro::launch_pyro_fields<double>(ctx->state<double>(nullptr), ctx->pyro_couplings<double>(),
+                                                        replica, ctx->fields.as<double>(), ctx->stream));
+            else
+                HIPCHK(pyro::launch_fields<double>(ctx->state<double>(nullptr), replica, ctx->fields.as<double>(),
+                                                   ctx->stream));
             std::vector<double> f(3 * N);
             HIPCHK(hipStreamSynchronize(ctx->stream));
             ctx->d2h(f.data(), ctx->fields.p, f.size() * 8);
             for (int64_t i = 0; i < N; ++i)
                 for (int a = 0; a < 3; ++a) h[3 * i + a] = f[a * N + i];
         } else {
-            HIPCHK(pyro::launch_fields<float>(ctx->state<float>(nullptr), replica, ctx->fields.as<float>(), ctx->stream));
+            if (sten)
+                HIPCHK(pyro::launch_pyro_fields<float>(ctx->state<float>(nullptr), ctx->pyro_couplings<float>(),
+                                                       replica, ctx->fields.as<float>(), ctx->stream));
+            else
+                HIPCHK(pyro::launch_fields<float>(ctx->state<float>(nullptr), replica, ctx->fields.as<float>(),
+                                                  ctx->stream));
             std::vector<float> f(3 * N);
             HIPCHK(hipStreamSynchronize(ctx->stream));
             ctx->d2h(f.data(), ctx->fields.p, f.size() * 4);
@@ -1222,6 +1252,150 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
                 if (ctx->munu_pending >= ctx->munu_B) ctx->munu_flush();
             }
         }
+    });
+}
+
+int pyro_set_replica_couplings(pyro_ctx* ctx, const double* J1J2J3Delta) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(ctx->spec, PYRO_E_UNSUPPORTED, "per-replica couplings need couplings defined by define_J1J2J3");
+        ctx->invalidate_graph();
+        if (!J1J2J3Delta) {
+            ctx->has_Jrep = false;
+            ctx->xxz_rep = false;
+            ctx->Jrep.release();
+            return;
+        }
+        require(ctx->strategy == 0 || ctx->strategy >= 3, PYRO_E_UNSUPPORTED,
+                "per-replica couplings need the pyrochlore strategies 3, 4 or 5");
+        const int R = ctx->R;
+        std::vector<double> v(size_t(4) * R);
+        bool xxz = false;
+        for (int r = 0; r < R; ++r) {
+            const double* q = J1J2J3Delta + 4 * r;
+            for (int k = 0; k < 4; ++k) require(std::isfinite(q[k]), PYRO_E_INVALID, "couplings must be finite");
+            // the colouring was built for the context's shells: a replica may
+            // not add a shell the context left out
+            require(ctx->spec_shells >= 2 || q[1] == 0.0, PYRO_E_INVALID,
+                    "replica J2 != 0 but the context was defined with J2 = 0");
+            require(ctx->spec_shells >= 3 || q[2] == 0.0, PYRO_E_INVALID,
+                    "replica J3 != 0 but the context was defined with J3 = 0");
+            v[4 * r] = q[0];
+            v[4 * r + 1] = q[1];
+            v[4 * r + 2] = q[2];
+            v[4 * r + 3] = q[0] * (q[3] - 1.0) / 3.0;
+            xxz = xxz || q[3] != 1.0;
+        }
+        const size_t rs = ctx->real_size();
+        ctx->Jrep.alloc(rs * 4 * R);
+        if (rs == 8) {
+            ctx->h2d(ctx->Jrep.p, v.data(), 8 * v.size());
+        } else {
+            std::vector<float> f(v.begin(), v.end());
+            ctx->h2d(ctx->Jrep.p, f.data(), 4 * f.size());
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->has_Jrep = true;
+        ctx->xxz_rep = xxz;
+    });
+}
+
+int pyro_anneal_schedule(pyro_ctx* ctx, const double* T, int n_steps) {
+    return guarded([&] {
+        require(ctx && T, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        require(n_steps > 0, PYRO_E_INVALID, "the schedule needs at least one step");
+        for (int k = 0; k < n_steps; ++k)
+            require(T[k] > 0 && std::isfinite(T[k]), PYRO_E_INVALID, "temperatures must be positive");
+        ctx->invalidate_graph();
+        ctx->sched.alloc(sizeof(double) * n_steps);
+        ctx->ehist.alloc(sizeof(double) * size_t(n_steps) * ctx->R);
+        ctx->h2d(ctx->sched.p, T, sizeof(double) * n_steps);
+        HIPCHK(pyro::launch_zero(ctx->ehist.p, size_t(2) * n_steps * ctx->R, ctx->stream));
+        ctx->n_sched = n_steps;
+        ctx->h_sched.assign(T, T + n_steps);
+        ctx->anneal_step = 0;
+        // every replica is its own model: identity permutation, slot r = replica r
+        ctx->h_T.assign(ctx->R, T[0]);
+        ctx->upload_temperatures();
+        ctx->identity_permutation();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sample_ssf, int n_steps) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(ctx->n_sched > 0, PYRO_E_STATE, "anneal_schedule() has not been called");
+        require(kind == PYRO_METROPOLIS || kind == PYRO_LIFTED || kind == PYRO_LIFTED_ROT, PYRO_E_INVALID,
+                "sweep kind must be METROPOLIS, LIFTED or LIFTED_ROT");
+        require(n_overrelax >= 0 && n_sweep >= 0 && n_steps >= 0, PYRO_E_INVALID, "negative count");
+        require(ctx->anneal_step + n_steps <= ctx->n_sched, PYRO_E_INVALID, "steps beyond the end of the schedule");
+        require(!sample_ssf || ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        require(!sample_ssf || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
+                "SSF sampling in anneal steps needs one accumulator slot per replica");
+        if (n_steps == 0) return;
+        const int strat = n_sweep > 0 ? ctx->choose_strategy(kind, n_overrelax) : 1;
+        const unsigned long long ppr = (unsigned long long)n_sweep * (1 + n_overrelax);
+        auto* ctr = ctx->counters.as<unsigned long long>();
+        const unsigned long long hc[3] = {ctx->pass_count, ctx->round_count, (unsigned long long)ctx->anneal_step};
+        ctx->h2d(ctx->counters.p, hc, sizeof(hc));
+        auto step_body = [&]() {
+            HIPCHK(pyro::launch_anneal_set_T(ctx->T_slot.as<double>(), ctx->R, ctx->sched.as<double>(), ctx->n_sched,
+                                             ctr + 2, ctx->stream));
+            if (n_sweep > 0) ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull);
+            if (sample_ssf) ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+            ctx->energy(false);
+            HIPCHK(pyro::launch_anneal_record(ctx->e.as<double>(), ctx->R, ctx->ehist.as<double>(), ctx->n_sched,
+                                              ctr + 2, ctr, ppr, ctx->stream));
+        };
+        const bool use_graph = !ctx->prof;
+        if (use_graph && (!ctx->agexec || ctx->a_kind != kind || ctx->a_nor != n_overrelax ||
+                          ctx->a_nsw != n_sweep || ctx->a_ssf != sample_ssf || ctx->a_strategy != strat)) {
+            if (ctx->agexec) (void)hipGraphExecDestroy(ctx->agexec);
+            ctx->agexec = nullptr;
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                step_body();
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx->stream, &graph);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
+            hipError_t ie = hipGraphInstantiate(&ctx->agexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(ie);
+            ctx->a_kind = kind;
+            ctx->a_nor = n_overrelax;
+            ctx->a_nsw = n_sweep;
+            ctx->a_ssf = sample_ssf;
+            ctx->a_strategy = strat;
+        }
+        // enqueue only: contexts on their own streams run concurrently
+        for (int i = 0; i < n_steps; ++i) {
+            if (use_graph) HIPCHK(hipGraphLaunch(ctx->agexec, ctx->stream));
+            else step_body();
+            ctx->pass_count += ppr;
+            ctx->anneal_step += 1;
+            if (sample_ssf && ctx->munu) {
+                ctx->munu_pending += 1;
+                if (ctx->munu_pending >= ctx->munu_B) ctx->munu_flush();
+            }
+        }
+        ctx->h_T.assign(ctx->R, ctx->h_sched[ctx->anneal_step - 1]);
+    });
+}
+
+int pyro_anneal_history(pyro_ctx* ctx, double* E) {
+    return guarded([&] {
+        require(ctx && E, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        require(ctx->n_sched > 0, PYRO_E_STATE, "anneal_schedule() has not been called");
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->d2h(E, ctx->ehist.p, sizeof(double) * size_t(ctx->n_sched) * ctx->R);
     });
 }
 

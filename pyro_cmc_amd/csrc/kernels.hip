@@ -737,6 +737,40 @@ hipError_t dispatch_ns(int NS, F&& f) {
 
 int energy_blocks(int N) { return (N + kBlock - 1) / kBlock; }
 
+__global__ void anneal_set_T_kernel(double* __restrict__ T_slot, int R, const double* __restrict__ sched,
+                                    int n_sched, const unsigned long long* __restrict__ step) {
+    const unsigned long long k = *step;
+    if (k >= (unsigned long long)n_sched) return;
+    const double T = sched[k];
+    for (int s = threadIdx.x; s < R; s += blockDim.x) T_slot[s] = T;
+}
+
+__global__ void anneal_record_kernel(const double* __restrict__ e, int R, double* __restrict__ hist, int n_sched,
+                                     unsigned long long* step, unsigned long long* pass_base,
+                                     unsigned long long passes) {
+    const unsigned long long k = *step;
+    if (k < (unsigned long long)n_sched)
+        for (int r = threadIdx.x; r < R; r += blockDim.x) hist[size_t(r) * n_sched + k] = e[r];
+    __syncthreads();  // every thread has read *step
+    if (threadIdx.x == 0) {
+        *step = k + 1ull;
+        *pass_base += passes;
+    }
+}
+
+hipError_t launch_anneal_set_T(double* T_slot, int R, const double* sched, int n_sched,
+                               const unsigned long long* step, hipStream_t s) {
+    hipLaunchKernelGGL(anneal_set_T_kernel, dim3(1), dim3(kBlock), 0, s, T_slot, R, sched, n_sched, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_anneal_record(const double* e, int R, double* hist, int n_sched, unsigned long long* step,
+                                unsigned long long* pass_base, unsigned long long passes, hipStream_t s) {
+    hipLaunchKernelGGL(anneal_record_kernel, dim3(1), dim3(kBlock), 0, s, e, R, hist, n_sched, step, pass_base,
+                       passes);
+    return hipGetLastError();
+}
+
 hipError_t launch_energy_finish(const double* partial, int nblk, int n_cells, double* e,
                                 const int32_t* slot_of, double* Esum, double* E2sum,
                                 unsigned long long* nE, int R, hipStream_t s) {

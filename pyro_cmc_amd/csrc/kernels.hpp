@@ -61,11 +61,24 @@ hipError_t launch_sweep_resident(const DeviceState<Real>& st, const int32_t* all
 
 // Couplings of the pyrochlore-specialised kernels: shells = 1 (J1),
 // 2 (J1, J2) or 3 (J1, J2, J3); Jx = J1 (Delta - 1) / 3 when xxz.
+// per_replica, when set, holds (J1, J2, J3, Jx) for every replica and
+// overrides the scalars (a batch of independent models in one context).
 template <typename Real>
 struct PyroCouplings {
     Real J1 = 0, J2 = 0, J3 = 0, Jx = 0;
     int shells = 1;
     bool xxz = false;
+    const Real* per_replica = nullptr;  // [R][4] or null
+    __device__ __forceinline__ PyroCouplings at(int r) const {
+        PyroCouplings c = *this;
+        if (per_replica) {
+            c.J1 = per_replica[4 * r];
+            c.J2 = per_replica[4 * r + 1];
+            c.J3 = per_replica[4 * r + 2];
+            c.Jx = per_replica[4 * r + 3];
+        }
+        return c;
+    }
 };
 // One pass of `kind` over one colour (4 colours = pyro_sl when shells < 3,
 // else 16 colours = sublattice) for every replica.
@@ -109,6 +122,18 @@ int energy_blocks(int N);
 template <typename Real>
 hipError_t launch_pyro_energy(const DeviceState<Real>& st, const PyroCouplings<Real>& J, double* partial,
                               hipStream_t s);
+// Local field (without the global field) of one replica, out[3][N], with
+// the stencil field.
+template <typename Real>
+hipError_t launch_pyro_fields(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int replica, Real* out,
+                              hipStream_t s);
+// Anneal bookkeeping on the device, replayable from a graph:
+// T_slot[s] = sched[*step] for s < R ...
+hipError_t launch_anneal_set_T(double* T_slot, int R, const double* sched, int n_sched,
+                               const unsigned long long* step, hipStream_t s);
+// ... and hist[r][*step] = e[r], *step += 1, *pass_base += passes.
+hipError_t launch_anneal_record(const double* e, int R, double* hist, int n_sched, unsigned long long* step,
+                                unsigned long long* pass_base, unsigned long long passes, hipStream_t s);
 // Second stage of launch_energy (sums the partials of every replica).
 hipError_t launch_energy_finish(const double* partial, int nblk, int n_cells, double* e,
                                 const int32_t* slot_of, double* Esum, double* E2sum,

@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
         Real* __restrict__ Sz = Sy + N;
         const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
         Real hx, hy, hz;
-        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J, hx, hy, hz);
+        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J.at(r), hx, hy, hz);
         accepted = apply_update<Real, KIND>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
     }
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void pyro_energy_kernel(DeviceState<Real> st, 
         const int sl = i / nc;
         const CellAxes c = cell_axes_of(i - sl * nc, st.L0, st.L1, st.L2);
         Real hx, hy, hz;
-        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J, hx, hy, hz);
+        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J.at(r), hx, hy, hz);
         const Real sx = Sx[i], sy = Sy[i], sz = Sz[i];
         const Real half = Real(0.5) * (sx * hx + sy * hy + sz * hz);
         const Real bs = sx * st.B[3 * r] + sy * st.B[3 * r + 1] + sz * st.B[3 * r + 2];
@@ -237,6 +237,24 @@ __global__ __launch_bounds__(256) void pyro_energy_kernel(DeviceState<Real> st, 
     if (threadIdx.x == 0) partial[size_t(r) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Local field of every site of one replica (MC.cpp:87-96), out[3][N].
+template <typename Real, int SHELLS>
+__global__ __launch_bounds__(256) void pyro_fields_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
+                                                          int r, Real* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= st.N) return;
+    const int nc = st.n_cells;
+    const size_t N = size_t(st.N);
+    const Real* Sx = st.S + size_t(r) * 3 * N;
+    const int sl = i / nc;
+    const CellAxes c = cell_axes_of(i - sl * nc, st.L0, st.L1, st.L2);
+    Real hx, hy, hz;
+    pyro_field_rt<Real, SHELLS>(sl, Sx, Sx + N, Sx + 2 * N, nc, c, J.at(r), hx, hy, hz);
+    out[i] = hx;
+    out[N + i] = hy;
+    out[2 * N + i] = hz;
+}
+
 // One workgroup per replica; thread t owns cells t, t + BS, ... (CPT cells).
 template <typename Real, int SHELLS, int FIRST, int BS, int CPT>
 __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
@@ -246,6 +264,7 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     constexpr int SPC = 16 / NCOL;
     const int r = blockIdx.x;
+    const PyroCouplings<Real> Jr = J.at(r);
     const int nc = st.n_cells;
     const size_t N = size_t(st.N);
     Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
@@ -276,7 +295,7 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                             c.a2[a] = j ? ax_b.a2[a] : ax_a.a2[a];
                         }
                         Real hx, hy, hz;
-                        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J, hx, hy, hz);
+                        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, Jr, hx, hy, hz);
                         if (ps == 0)
                             my_acc += apply_update<Real, FIRST>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy,
                                                                 hz) ? 1u : 0u;
@@ -334,6 +353,7 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
     if ((gridDim.x & 7u) == 0u) g = (g & 7u) * (gridDim.x >> 3) + (g >> 3);
     const int r = int(g / tiles);
     const int tile = int(g - unsigned(r) * tiles);
+    const PyroCouplings<Real> Jr = J.at(r);
     const int L0 = st.L0, L1 = st.L1, L2 = st.L2, nc = st.n_cells;
     const int q1n = L1 / (2 * T1), q2n = L2 / (2 * T2);
     const int m2 = tile % q2n;
@@ -384,7 +404,7 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
                     Real hx, hy, hz;
                     switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field_lds<Real, SHELLS, X, H1, H2, HC>(lx, ly, lz, lc, J, hx, hy, hz); break;
+    case X: pyro_field_lds<Real, SHELLS, X, H1, H2, HC>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
                         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
                         PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
                         PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -470,6 +490,17 @@ hipError_t launch_pyro_energy(const DeviceState<Real>& st, const PyroCouplings<R
     return dispatch_pyro(J, [&](auto shells) -> hipError_t {
         constexpr int SH = decltype(shells)::value;
         hipLaunchKernelGGL((pyro_energy_kernel<Real, SH>), grid, dim3(256), 0, s, st, J, partial);
+        return hipGetLastError();
+    });
+}
+
+template <typename Real>
+hipError_t launch_pyro_fields(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int replica, Real* out,
+                              hipStream_t s) {
+    const dim3 grid(unsigned((st.N + 255) / 256));
+    return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+        constexpr int SH = decltype(shells)::value;
+        hipLaunchKernelGGL((pyro_fields_kernel<Real, SH>), grid, dim3(256), 0, s, st, J, replica, out);
         return hipGetLastError();
     });
 }

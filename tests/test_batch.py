@@ -1,0 +1,206 @@
+"""Batches of independent models in one context, and device-side anneal
+schedules.
+
+Every replica may carry its own pyrochlore couplings (J1, J2, J3, Delta) and
+field; each replica is checked against an oracle instance built with that
+replica's model (fields and energy to 1e-12 / 1e-5, FP64 trajectories draw
+for draw in colour order and in the tiled order).  The anneal schedule
+(temperature steps run as a replayed HIP graph) must equal the same steps
+issued one call at a time, and contexts of different sizes run concurrently
+on their own streams must give what each gives alone.
+"""
+import numpy as np
+import pytest
+
+from helpers import random_spins
+
+pytestmark = pytest.mark.gpu
+
+MODELS = [  # (J1, J2, J3, Delta), B
+    ((1.0, -0.2, 0.15, 1.0), (0.0, 0.0, 0.0)),
+    ((-1.0, 0.1, -0.05, 0.7), (0.1, -0.3, 0.2)),
+    ((0.5, 0.0, 0.0, 1.3), (0.0, 0.0, 0.4)),
+    ((1.0, 0.2, 0.2, 1.0), (-0.2, 0.1, 0.0)),
+]
+TOL = {64: 1e-12, 32: 1e-5}
+
+
+def _batch(pyro, oracle, L, prec, seed=4321, context_J=(1.0, 0.1, 0.1)):
+    R = len(MODELS)
+    ctx = pyro.Context.pyrochlore(L, n_replicas=R, precision=prec, seed=seed)
+    ctx.define_J1J2J3(*context_J)
+    ctx.setup()
+    ctx.set_replica_couplings(np.array([m[0] for m in MODELS]))
+    os_ = []
+    for r, (J, B) in enumerate(MODELS):
+        ctx.set_field(np.array(B), r)
+        o = oracle.Oracle.pyrochlore(L, *J)
+        o.set_field(np.array(B))
+        os_.append(o)
+    return ctx, os_
+
+
+def _scale(J, B):
+    return 6 * abs(J[0]) * max(1.0, abs(J[3])) + 12 * abs(J[1]) + 12 * abs(J[2]) + np.linalg.norm(B)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_replica_couplings_fields_and_energy(gpu, oracle, prec):
+    L = 3
+    ctx, os_ = _batch(gpu, oracle, L, prec)
+    rng = np.random.default_rng(7)
+    for r, o in enumerate(os_):
+        S = random_spins(ctx.N, rng)
+        ctx.set_spins(S, r)
+        o.set_spins(S)
+    e = ctx.energy_per_cell()
+    for r, ((J, B), o) in enumerate(zip(MODELS, os_)):
+        s = _scale(J, B)
+        assert np.max(np.abs(ctx.local_fields(r) - o.local_fields())) <= TOL[prec] * s
+        assert abs(e[r] - o.energy_per_cell()) <= TOL[prec] * 16 * s
+
+
+def _oracle_sweeps(o, r, kind, n_or, n_sweeps, T, colours, n_col, seed, pass0):
+    acc, p = 0, pass0
+    for _ in range(n_sweeps):
+        acc += o.pass_ctr(kind, T, colours, n_col, seed, r, p)
+        p += 1
+        for _ in range(n_or):
+            o.pass_ctr(3, T, colours, n_col, seed, r, p)
+            p += 1
+    return acc
+
+
+@pytest.mark.parametrize("strategy", [3, 4])
+@pytest.mark.parametrize("kind,n_or", [(0, 1), (1, 1), (2, 0)])
+def test_replica_couplings_trajectory(gpu, oracle, strategy, kind, n_or):
+    L, seed = 2, 0xABCDEF
+    T = np.array([0.3, 0.9, 2.0, 0.5])
+    ctx, os_ = _batch(gpu, oracle, L, 64, seed=seed)
+    ctx.set_strategy(strategy)
+    ctx.set_temperatures(T)
+    rng = np.random.default_rng(11)
+    for r, o in enumerate(os_):
+        S = random_spins(ctx.N, rng)
+        ctx.set_spins(S, r)
+        o.set_spins(S)
+    colours = ctx.colouring()
+    n_col = ctx.info()["n_colours"]
+    acc = ctx.sweep(kind, n_or, 3)
+    for r, o in enumerate(os_):
+        assert int(acc[r]) == _oracle_sweeps(o, r, kind, n_or, 3, T[r], colours, n_col, seed, 0)
+        assert np.max(np.abs(ctx.get_spins(r) - o.get_spins())) < 1e-12
+        assert np.array_equal(ctx.get_lifted_dir(r), o.get_dir())
+
+
+@pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 1)])
+def test_replica_couplings_tiled(gpu, oracle, kind, n_or):
+    L, seed = 8, 0x5EED
+    T = [0.4, 1.1, 0.7, 2.5]
+    ctx, os_ = _batch(gpu, oracle, L, 64, seed=seed)
+    ctx.set_strategy(5)
+    ctx.set_temperatures(T)
+    rng = np.random.default_rng(13)
+    for r, o in enumerate(os_):
+        S = random_spins(ctx.N, rng)
+        ctx.set_spins(S, r)
+        o.set_spins(S)
+    site, koff = ctx.sweep_order(n_or)
+    try:
+        acc = ctx.sweep(kind, n_or, 1)
+    except gpu.PyroError as e:
+        assert e.code == 5
+        pytest.skip(str(e))
+    for r, o in enumerate(os_):
+        assert int(acc[r]) == o.sweep_order(kind, T[r], site, koff, seed, r, 0)
+        assert np.abs(ctx.get_spins(r) - o.get_spins()).max() < 1e-10
+
+
+def test_replica_couplings_validation(gpu, oracle):
+    ctx = gpu.Context.pyrochlore(2, n_replicas=2, precision=32, seed=1)
+    ctx.define_J1J2J3(1.0, 0.1, 0.0)  # no J3 shell: 4-colour schedule
+    ctx.setup()
+    with pytest.raises(gpu.PyroError) as e:
+        ctx.set_replica_couplings([[1, 0.1, 0.05, 1], [1, 0, 0, 1]])
+    assert e.value.code == 1
+    ctx.set_replica_couplings([[1, 0.1, 0.0, 1], [-1, 0.0, 0.0, 0.5]])
+    ctx.set_strategy(1)
+    ctx.set_temperatures([1.0, 1.0])
+    with pytest.raises(gpu.PyroError) as e:
+        ctx.sweep()
+    assert e.value.code == 5
+    g = gpu.Context.pyrochlore(2, n_replicas=2, precision=32, seed=1)
+    for name, rel, J in oracle.j1j2j3_couplings(1.0, 0.0, 0.0):
+        g.define_coupling(name, rel, J)
+    g.setup()
+    with pytest.raises(gpu.PyroError) as e:
+        g.set_replica_couplings([[1, 0, 0, 1], [1, 0, 0, 1]])
+    assert e.value.code == 5
+
+
+def _anneal_eager(pyro, ctx, sched, kind, n_or, n_sweep, ssf_from):
+    E = []
+    for k, T in enumerate(sched):
+        ctx.set_temperatures(np.full(ctx.R, T))
+        ctx.sweep(kind, n_or, n_sweep, count=False)
+        if k >= ssf_from:
+            ctx.ssf_sample()
+        E.append(ctx.energy_per_cell())
+    return np.array(E).T
+
+
+@pytest.mark.parametrize("prec,strategy", [(32, 3), (64, 4), (32, 5)])
+def test_anneal_steps_equal_eager_steps(gpu, oracle, prec, strategy):
+    L = 8 if strategy == 5 else 3
+    sched = np.geomspace(3.0, 0.1, 9)
+    out = []
+    for graph in (True, False):
+        ctx, _ = _batch(gpu, oracle, L, prec, seed=77)
+        ctx.set_strategy(strategy)
+        ctx.randomize(5)
+        ctx.ssf_configure(("xx", "zz", "xy"), n_slots=ctx.R)
+        if graph:
+            ctx.anneal_schedule(sched)
+            ctx.anneal_steps(gpu.LIFTED, 1, 2, False, 6)
+            ctx.anneal_steps(gpu.LIFTED, 1, 2, True, 3)
+            E = ctx.anneal_history()
+        else:
+            E = _anneal_eager(gpu, ctx, sched, gpu.LIFTED, 1, 2, 6)
+        out.append((E, [ctx.get_spins(r) for r in range(ctx.R)], [ctx.ssf_read(r)[0] for r in range(ctx.R)],
+                    ctx.pass_count()))
+    (Eg, Sg, Qg, pg), (Ee, Se, Qe, pe) = out
+    assert pg == pe
+    assert np.array_equal(Eg, Ee)
+    for a, b in zip(Sg, Se):
+        assert np.array_equal(a, b)
+    for a, b in zip(Qg, Qe):
+        assert np.array_equal(a, b)
+
+
+def test_concurrent_contexts_match_sequential(gpu, oracle):
+    """Contexts of different L enqueued in turn run concurrently on their
+    own streams; each ends where it ends when run alone."""
+    sched = np.geomspace(5.0, 0.2, 12)
+    sizes = (2, 4, 8)
+
+    def run(interleave):
+        ctxs = []
+        for i, L in enumerate(sizes):
+            c, _ = _batch(gpu, oracle, L, 32, seed=100 + i)
+            c.randomize(200 + i)
+            c.anneal_schedule(sched)
+            ctxs.append(c)
+        if interleave:
+            for _ in range(4):
+                for c in ctxs:
+                    c.anneal_steps(gpu.METROPOLIS, 2, 3, False, 3)
+        else:
+            for c in ctxs:
+                c.anneal_steps(gpu.METROPOLIS, 2, 3, False, 12)
+                c.synchronize()
+        return [(c.anneal_history(), c.get_spins(1)) for c in ctxs]
+
+    a, b = run(True), run(False)
+    for (Ea, Sa), (Eb, Sb) in zip(a, b):
+        assert np.array_equal(Ea, Eb)
+        assert np.array_equal(Sa, Sb)
