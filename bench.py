@@ -47,7 +47,7 @@ N_SWEEP = 4
 CORR = ("xx", "yy", "zz", "xy", "xz", "yz", "yx", "zx", "zy")
 PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E spec
 PEAK_FP32_MATRIX = 157.3e12  # flop/s, MI355X dense FP32 MFMA spec
-ALL_LEGS = ("c3", "fp64", "large", "munu", "cpu")
+ALL_LEGS = ("c3", "fp64", "large", "munu", "batch", "cpu")
 
 
 def profiled_traffic(tag: str):
@@ -170,6 +170,27 @@ def munu_correlators(pyro, samples=128, buffer=64):
     }
 
 
+def uneven_batch(pyro):
+    """256 independent anneals with L drawn from {4, 8, 12, 16, 24}, J1 = 1,
+    J2, J3 uniform in [-0.2, 0.2], |B| uniform in [0, 0.5] along a random
+    direction; T 10 -> 0.05 over 100 log steps, 16 burn-in sweeps, 16 sweeps
+    per step, 64 SSF samples (xx, yy, zz) at T_cold; FP32 spins, FP64 sums;
+    Metropolis + OR.  Items of equal L share a context; the five contexts run
+    concurrently on their own streams."""
+    from pyro_cmc_amd import batch
+
+    items = batch.uneven_items(256, seed=4)
+    _, st = batch.run_anneal_batch(items, seed=4)
+    return {
+        "workload": "256 anneals, L in {4,8,12,16,24}, random J2/J3 in [-0.2,0.2], |B| in [0,0.5], "
+                    "T 10->0.05 over 100 steps, 16 sweeps/step, 64 SSF samples at T_cold, FP32",
+        "site_updates_per_s": st["site_updates_per_s"],
+        "wall_s": st["wall_s"],
+        "site_updates": st["site_updates"],
+        "items_per_L": st["contexts"],
+    }
+
+
 def cpu_baseline(seconds: float = 12.0):
     """The oracle's restatement of the reference sweep (sequential, AoS
     spins with bond-shell pointer lists, Xoroshiro128++ + libstdc++
@@ -275,6 +296,8 @@ def main():
             out["large_lattice_sweep"] = large_lattice_sweep_line(pyro)
         if "munu" in legs:
             out["munu_correlators"] = munu_correlators(pyro)
+        if "batch" in legs:
+            out["uneven_batch"] = uneven_batch(pyro)
         print(json.dumps(out))
         return
 
@@ -325,6 +348,7 @@ def main():
     cpu = cpu_baseline(args.cpu_seconds) if "cpu" in legs else None
     big = large_lattice_sweep_line(pyro) if "large" in legs else None
     munu = munu_correlators(pyro) if "munu" in legs else None
+    unev = uneven_batch(pyro) if "batch" in legs else None
 
     line = {
         "metric": "site-updates/s (Metropolis+OR) at L=12x64 replicas",
@@ -367,6 +391,7 @@ def main():
         "swap_acceptance_min": float(min(acc_rate)) if acc_rate else None,
         "large_lattice_sweep": big,
         "munu_correlators": munu,
+        "uneven_batch": unev,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -1,0 +1,130 @@
+"""Uneven batches of independent anneals on one GPU.
+
+The reference runs one anneal per process (anneal.cpp:33-248) and merges
+seeds afterwards (acc_runs.py:13-57).  Here a batch of models with different
+lattice sizes, couplings and fields runs as one job: items of equal L share a
+context (one replica per item, per-replica couplings and field), every
+context follows the same device-side temperature schedule as a replayed HIP
+graph on its own stream, and the contexts' steps are enqueued in turn so the
+small lattices fill the GPU beside the large ones.
+
+Schedule per item, as in anneal.cpp:193-221 with T_sample = {T_cold}:
+  n_burn_in sweeps at T_hot; then for every T of generate_T_profile(T_hot,
+  T_cold, [T_cold], n_steps) n_sweep sweeps and one energy sample; then at
+  T_cold n_sample x (n_sweep sweeps + one SSF sample).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import FP32, METROPOLIS, Context
+
+
+@dataclass
+class AnnealItem:
+    L: int
+    J: tuple  # (J1, J2, J3, Delta)
+    B: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class AnnealResult:
+    item: AnnealItem
+    T_list: np.ndarray              # the anneal grid, descending
+    E: np.ndarray                   # energy per cell after each grid step
+    S: np.ndarray                   # SSF sums at T_cold, [n_corr][n_k][2]
+    S2: np.ndarray                  # (Re S)^2, (Im S)^2 sums
+    n_samples: int
+    E_final: float                  # energy per cell of the final configuration
+    spins: np.ndarray | None = field(default=None, repr=False)  # final [N][3] when kept
+
+
+def generate_T_profile(T_hot, T_cold, T_sample, total_steps):
+    """Log grid T_hot * f^i, i < total_steps, f = (T_cold/T_hot)^(1/total_steps),
+    merged in descending order with T_sample (cli_bits.hpp:250-289); returns
+    the grid and the indices of the T_sample entries."""
+    f = (T_cold / T_hot) ** (1.0 / total_steps)
+    sweep = sorted((T_hot * f ** i for i in range(total_steps)), reverse=True)
+    samp = sorted(T_sample, reverse=True)
+    out, idx, i, j = [], [], 0, 0
+    while i < len(sweep) and j < len(samp):
+        if sweep[i] > samp[j]:
+            out.append(sweep[i])
+            i += 1
+        else:
+            idx.append(len(out))
+            out.append(samp[j])
+            j += 1
+    for t in samp[j:]:
+        idx.append(len(out))
+        out.append(t)
+    out.extend(sweep[i:])
+    return np.array(out), idx
+
+
+def uneven_items(n=256, sizes=(4, 8, 12, 16, 24), seed=0):
+    """The uneven batch: L uniform over `sizes`, J1 = 1, J2 and J3 uniform in
+    [-0.2, 0.2], |B| uniform in [0, 0.5] along a uniform random direction."""
+    rng = np.random.default_rng(seed)
+    items = []
+    for _ in range(n):
+        L = int(rng.choice(sizes))
+        J2, J3 = rng.uniform(-0.2, 0.2, 2)
+        v = rng.normal(size=3)
+        B = v / np.linalg.norm(v) * rng.uniform(0.0, 0.5)
+        items.append(AnnealItem(L, (1.0, float(J2), float(J3), 1.0), tuple(float(b) for b in B)))
+    return items
+
+
+def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_burn_in=16, n_sample=64,
+                     n_overrelax=1, kind=METROPOLIS, precision=FP32, correlators=("xx", "yy", "zz"),
+                     seed=0, keep_spins=False, strategy=0):
+    """Run every item's anneal; returns (results in item order, stats)."""
+    grid, _ = generate_T_profile(T_hot, T_cold, [T_cold], n_steps)
+    sched = np.concatenate([[T_hot], grid, np.full(n_sample, T_cold)])
+    groups = {}
+    for i, it in enumerate(items):
+        groups.setdefault(it.L, []).append(i)
+    ctxs = []
+    for L, idx in sorted(groups.items()):
+        c = Context.pyrochlore(L, n_replicas=len(idx), precision=precision, seed=seed * 1000003 + L)
+        # the context's shells must cover every item's: all three, with zeros allowed per replica
+        c.define_J1J2J3(1.0, 1.0, 1.0)
+        c.setup()
+        c.set_strategy(strategy)
+        c.set_replica_couplings(np.array([items[i].J for i in idx], np.float64))
+        for r, i in enumerate(idx):
+            c.set_field(np.array(items[i].B, np.float64), r)
+        c.randomize(seed * 7919 + L)
+        c.ssf_configure(correlators, n_slots=len(idx), store_error_term=True)
+        c.anneal_schedule(sched)
+        ctxs.append((c, idx))
+    for c, _ in ctxs:
+        c.synchronize()
+    # timed: everything from the first step on, including each context's
+    # strategy choice and graph capture on its first call
+    t0 = time.perf_counter()
+    phases = [(n_burn_in, False, 1), (n_sweep, False, len(grid)), (n_sweep, True, n_sample)]
+    for n_sw, ssf, steps in phases:
+        for c, _ in ctxs:  # enqueue only: the contexts' streams run side by side
+            c.anneal_steps(kind, n_overrelax, n_sw, ssf, steps)
+    for c, _ in ctxs:
+        c.synchronize()
+    wall = time.perf_counter() - t0
+    results = [None] * len(items)
+    updates = 0
+    for c, idx in ctxs:
+        hist = c.anneal_history()
+        for r, i in enumerate(idx):
+            S, S2, n = c.ssf_read(r)
+            results[i] = AnnealResult(items[i], grid.copy(), hist[r, 1:1 + len(grid)].copy(), S, S2, n,
+                                      float(hist[r, -1]), c.get_spins(r) if keep_spins else None)
+        passes = (1 + n_overrelax) * (n_burn_in + n_sweep * (len(grid) + n_sample))
+        updates += c.N * c.R * passes
+        c.close()
+    stats = {"wall_s": wall, "site_updates": updates, "site_updates_per_s": updates / wall,
+             "contexts": {int(L): len(idx) for L, idx in sorted(groups.items())}, "schedule_steps": len(sched)}
+    return results, stats

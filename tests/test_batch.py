@@ -204,3 +204,26 @@ def test_concurrent_contexts_match_sequential(gpu, oracle):
     for (Ea, Sa), (Eb, Sb) in zip(a, b):
         assert np.array_equal(Ea, Eb)
         assert np.array_equal(Sa, Sb)
+
+
+def test_uneven_batch_driver(gpu, oracle):
+    """Items of different L, couplings and fields in one job: each result
+    belongs to its item (final energy of the returned spins under the item's
+    own model equals the last recorded energy), on the anneal grid."""
+    from pyro_cmc_amd import batch
+
+    items = batch.uneven_items(n=9, sizes=(2, 4, 8), seed=3)
+    res, stats = batch.run_anneal_batch(items, T_hot=5.0, T_cold=0.1, n_steps=6, n_sweep=2, n_burn_in=2,
+                                        n_sample=3, keep_spins=True, seed=1)
+    grid, idx = batch.generate_T_profile(5.0, 0.1, [0.1], 6)
+    assert len(grid) == 7 and idx == [6] and np.all(np.diff(grid) < 0)
+    assert stats["contexts"] == {L: sum(it.L == L for it in items) for L in (2, 4, 8) if any(it.L == L for it in items)}
+    for it, r in zip(items, res):
+        assert r.item is it
+        assert np.array_equal(r.T_list, grid) and r.E.shape == (7,) and r.n_samples == 3
+        o = oracle.Oracle.pyrochlore(it.L, *it.J)
+        o.set_field(np.array(it.B))
+        o.set_spins(r.spins)
+        e = o.energy_per_cell()
+        assert abs(r.E_final - e) <= 1e-5 * 16 * _scale(it.J, it.B)
+        assert np.all(np.isfinite(r.S)) and r.S.shape == (3, it.L ** 3, 2)
