@@ -19,7 +19,7 @@ import pytest
 
 from helpers import brute_energy_per_cell, pyro_positions
 
-TOOLS = ("anneal", "temper", "spiral_byhand")
+TOOLS = ("anneal", "temper", "spiral_byhand", "acc_runs")
 
 
 def _tool(pyro, name):
@@ -285,3 +285,31 @@ def test_temper_file_matches_library_replay(gpu, tmp_path, munu):
     rep = ctx.pt_permutation()[R - 1]
     with _h5(tmp_path / (stem + ".spins.h5")) as f:
         np.testing.assert_array_equal(f.read("spin_orientation"), ctx.get_spins(int(rep)))
+
+
+@pytest.mark.gpu
+def test_anneal_seeds_merge(gpu, tmp_path):
+    """Two seeds of one anneal merged by acc_runs: one group, summed energy
+    and SSF, cross-seed variance of the energy per cell."""
+    from pyro_cmc_amd.h5lite import H5File
+
+    base = ["-o", str(tmp_path), "--J1", "1", "--T_sample", "0.3", "--T_hot", "2", "--n_steps", "3",
+            "-w", "1", "--n_burn_in", "1", "--n_sample", "2", "--fp32", "2"]
+    for seed in ("4", "5"):
+        r = _run([_tool(gpu, "anneal"), "-s", seed, *base])
+        assert r.returncode == 0, r.stderr
+    r = _run([_tool(gpu, "acc_runs"), str(tmp_path)])
+    assert r.returncode == 0, r.stderr
+    runs = sorted(tmp_path.glob("*.out.h5"))
+    merged = sorted(tmp_path.glob("*merge2_*.avg.h5"))
+    assert len(runs) == 2 and len(merged) == 1
+    E, S = [], []
+    for p in runs:
+        with _h5(p) as f:
+            E.append(f.read("/energy/E"))
+            S.append(f.read("/ssf/static_corr"))
+    with H5File(merged[0]) as f:
+        np.testing.assert_allclose(f.read("/energy/E"), E[0] + E[1], rtol=1e-14)
+        np.testing.assert_allclose(f.read("/energy/var"), np.var(np.array(E), axis=0), rtol=1e-9, atol=1e-14)
+        np.testing.assert_allclose(f.read("/ssf/static_corr"), S[0] + S[1], rtol=1e-14)
+        assert int(f.read("/energy/n_seeds")) == 2
