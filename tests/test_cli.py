@@ -19,7 +19,7 @@ import pytest
 
 from helpers import brute_energy_per_cell, pyro_positions
 
-TOOLS = ("anneal", "temper", "spiral_byhand", "acc_runs")
+TOOLS = ("anneal", "temper", "spiral_byhand", "acc_runs", "fieldcool")
 
 
 def _tool(pyro, name):
@@ -313,3 +313,49 @@ def test_anneal_seeds_merge(gpu, tmp_path):
         np.testing.assert_allclose(f.read("/energy/var"), np.var(np.array(E), axis=0), rtol=1e-9, atol=1e-14)
         np.testing.assert_allclose(f.read("/ssf/static_corr"), S[0] + S[1], rtol=1e-14)
         assert int(f.read("/energy/n_seeds")) == 2
+
+
+@pytest.mark.gpu
+def test_fieldcool_file_matches_library_replay(gpu, tmp_path):
+    """fieldcool.cpp:31-214: anneal in a field, release it, re-burn, sample."""
+    pyro = gpu
+    args = ["-o", str(tmp_path), "-s", "6", "--J1", "1", "--J2", "0.1", "-B", "0", "0", "0.3",
+            "--T_hot", "2", "--T_cold", "0.2", "--n_steps", "4", "--n_sweep", "2", "--n_burn_in", "3",
+            "--n_sample", "2", "2"]
+    r = _run([_tool(pyro, "fieldcool"), *args])
+    assert r.returncode == 0, r.stderr
+    stem = "run_L=2_J1=1_J2=0.1_J3=0_Delta=1_B=0,0,0.3_seed=6_Tc=0.2_"
+    with _h5(tmp_path / (stem + ".out.h5")) as f:
+        T_steps = f.read("/energy/T_list")
+        E = f.read("/energy/E")
+        T_final = f.read("/ssf/T_list")
+        S = f.read("/ssf/static_corr")
+        assert f.read("/ssf/corr_lookup") == ["xx", "yy", "zz"]
+        np.testing.assert_array_equal(f.read("/ssf/n_samples"), [2])
+    f = (0.2 / 2.0) ** (1 / 4)
+    np.testing.assert_allclose(T_steps, [2.0 * f ** i for i in range(4)], rtol=1e-14)
+    np.testing.assert_allclose(T_final, [2.0 * f ** 4], rtol=1e-14)
+
+    key = pyro.seed_key(6)
+    ctx = pyro.Context.pyrochlore(2, 1, pyro.FP64, seed=key)
+    ctx.define_J1J2J3(1.0, 0.1, 0.0, 1.0)
+    ctx.setup()
+    ctx.set_T_ref(1.0)
+    ctx.set_field([0, 0, 0.3])
+    ctx.randomize(key)
+    ctx.set_temperatures([2.0])
+    ctx.sweep(pyro.METROPOLIS, 1, 3)
+    E_rep = []
+    for T in T_steps:
+        ctx.set_temperatures([T])
+        ctx.sweep(pyro.METROPOLIS, 1, 2)
+        E_rep.append(ctx.energy_per_cell()[0])
+    np.testing.assert_array_equal(E, E_rep)
+    ctx.ssf_configure(("xx", "yy", "zz"), 1, True)
+    ctx.set_field([0, 0, 0])
+    ctx.set_temperatures([T_final[0]])
+    ctx.sweep(pyro.METROPOLIS, 1, 3)
+    for _ in range(2):
+        ctx.sweep(pyro.METROPOLIS, 1, 2)
+        ctx.ssf_sample([0])
+    np.testing.assert_array_equal(S[:, 0], ctx.ssf_read(0)[0])
