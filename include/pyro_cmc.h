@@ -284,6 +284,23 @@ int pyro_profile_read(pyro_ctx* ctx, double* ms, uint64_t* launches);
 int pyro_mark(pyro_ctx* ctx, int which);
 int pyro_elapsed_ms(pyro_ctx* ctx, double* ms);
 
+/* ------------------------------------------------------------------ */
+/* Luttinger-Tisza spectrum (luttinger_tisza.cpp:31-181).               */
+/* ------------------------------------------------------------------ */
+/* For the pyrochlore J1-J2-J3 XXZ model on the L[0] x L[1] x L[2] grid of
+ * k (cubic cell of edge 8, q_a = 2 pi K'_a / (8 L_a) with the centred
+ * K' = K < L/2 ? K : K - L) builds the 48 x 48 Hermitian
+ * M(k) = sum over directed bonds alpha -> beta of J e^{i q.d} in block
+ * (alpha, beta) and returns its lowest eigenvalue lambda_min[k] for every
+ * k (flat order (K0 L1 + K1) L2 + K2).  k_star[3] receives the first k
+ * (in that order) of the strict minimum and eigvec[48][2] the (re, im)
+ * eigenvector of the lowest eigenvalue there, rows ordered spin x, y, z of
+ * sublattice 0, 1, ...  The LT energy per spin is lambda_min / 2.  Uses
+ * the current device; no context is needed.  Replaces the Eigen loop of
+ * luttinger_tisza.cpp:147-181. */
+int pyro_lt_spectrum(const int32_t L[3], double J1, double J2, double J3, double Delta,
+                     double* lambda_min, int32_t k_star[3], double* eigvec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,75 @@ def j1j2j3_couplings(J1, J2, J3, Delta=1.0):
     return out
 
 
+# ---------------------------------------------------------------------------
+# Luttinger-Tisza matrix: bonds as in luttinger_tisza.cpp:31-81, the k sweep
+# and first-strict-minimum search as in :147-181.  The q representative is
+# the centred index K' (K < L/2 ? K : K - L); eigenvalues do not depend on
+# it, eigenvectors change by a sublattice-diagonal phase.
+# ---------------------------------------------------------------------------
+def lt_bonds(J1, J2=0.0, J3=0.0, Delta=1.0):
+    """Directed bonds (alpha, beta, d, J) of the 16-sublattice cubic cell:
+    for each sublattice alpha and displacement v of its kind's shell, beta =
+    the sublattice at r_alpha + v modulo the edge 8; nn1 carries
+    J1 [I + (Delta - 1) z_alpha z_beta^T], the other shells J I; zero
+    couplings are skipped (luttinger_tisza.cpp:46-47)."""
+    pos, _, _ = pyro_cell()
+    where = {tuple(int(c) % 8 for c in p): s for s, p in enumerate(pos)}
+    out = []
+    for j, table, xxz in ((J1, NN1, True), (J2, NN2, False), (J3, NN3A, False), (J3, NN3B, False)):
+        if j == 0.0:
+            continue
+        for a in range(16):
+            for v in table[a % 4]:
+                b = where[tuple(int(pos[a][c] + v[c]) % 8 for c in range(3))]
+                J = np.eye(3)
+                if xxz and Delta != 1.0:
+                    J = J + (Delta - 1.0) * np.outer(local_z(a % 4), local_z(b % 4))
+                out.append((a, b, np.array(v, np.float64), j * J))
+    return out
+
+
+def lt_q(K, L):
+    """q = 2 pi K' / (8 L) per axis for the integer index K."""
+    K = np.asarray(K)
+    L = np.asarray(L)
+    Kc = np.where(2 * K < L, K, K - L)
+    return 2 * np.pi * Kc / (8.0 * L)
+
+
+def lt_k_grid(L):
+    """All k indices of an L0 x L1 x L2 grid in flat order (K0 L1 + K1) L2 + K2."""
+    L = [L] * 3 if np.isscalar(L) else list(L)
+    K = np.stack(np.meshgrid(*[np.arange(n) for n in L], indexing="ij"), -1).reshape(-1, 3)
+    return K, np.array(L)
+
+
+def lt_matrices(q, bonds):
+    """M(k)[48][48] for q[n_k][3]: block (alpha, beta) += J e^{i q.d}."""
+    q = np.atleast_2d(q)
+    M = np.zeros((len(q), 48, 48), complex)
+    for a, b, d, J in bonds:
+        M[:, 3 * a:3 * a + 3, 3 * b:3 * b + 3] += J[None] * np.exp(1j * (q @ d))[:, None, None]
+    return M
+
+
+def lt_spectrum(L, J1, J2=0.0, J3=0.0, Delta=1.0):
+    """(lambda_min[n_k], k_star[3], eigvec[48]) with k_star the first k of
+    the strict minimum (luttinger_tisza.cpp:150-181)."""
+    K, Ls = lt_k_grid(L)
+    bonds = lt_bonds(J1, J2, J3, Delta)
+    lam = np.empty(len(K))
+    best, vec, k_star = np.inf, None, None
+    for k0 in range(0, len(K), 4096):
+        M = lt_matrices(lt_q(K[k0:k0 + 4096], Ls), bonds)
+        w, v = np.linalg.eigh(M)
+        lam[k0:k0 + len(M)] = w[:, 0]
+        for i in range(len(M)):
+            if w[i, 0] < best:
+                best, vec, k_star = w[i, 0], v[i, :, 0], K[k0 + i]
+    return lam, np.asarray(k_star), vec
+
+
 class Oracle:
     """One replica of the CPU restatement."""
 

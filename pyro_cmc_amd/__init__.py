@@ -77,6 +77,7 @@ _SIGS = {
     "pyro_profile_read": [_vp, _f64p, _u64p],
     "pyro_mark": [_vp, C.c_int],
     "pyro_elapsed_ms": [_vp, C.POINTER(C.c_double)],
+    "pyro_lt_spectrum": [_i32p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _i32p, _f64p],
 }
 
 _lib: Optional[C.CDLL] = None
@@ -146,6 +147,22 @@ def host_tool(name: str):
     from pathlib import Path
 
     return Path(__file__).resolve().parent / name
+
+
+def lt_spectrum(L, J1, J2=0.0, J3=0.0, Delta=1.0):
+    """Luttinger-Tisza spectrum of the pyrochlore J1-J2-J3 XXZ model on the
+    L x L x L (or L[0] x L[1] x L[2]) grid of k (luttinger_tisza.cpp:147-181).
+
+    Returns (lambda_min[n_k], k_star[3], eigvec[48] complex): the lowest
+    eigenvalue of M(k) per k (flat order (K0 L1 + K1) L2 + K2), the first k
+    of the strict minimum and the eigenvector there.  The LT energy per
+    spin is lambda_min.min() / 2."""
+    Ls = np.array([L] * 3 if np.isscalar(L) else list(L), np.int32)
+    lam = np.empty(int(np.prod(Ls.astype(np.int64))), np.float64)
+    k_star = np.zeros(3, np.int32)
+    ev = np.zeros((48, 2), np.float64)
+    _check(lib().pyro_lt_spectrum(Ls, float(J1), float(J2), float(J3), float(Delta), lam, k_star, ev))
+    return lam, k_star, ev[:, 0] + 1j * ev[:, 1]
 
 
 def pyrochlore_cell():

@@ -42,9 +42,15 @@ SOURCES = [
     "kernels_pyro_resident_f64.hip",
     "kernels_pyro_tiled_f32.hip",
     "kernels_pyro_tiled_f64.hip",
+    "lt.hip",
     "capi.cpp",
     "topology.cpp",
 ]
+
+# rocSOLVER's batched Hermitian eigensolver (and the rocBLAS handle it runs
+# on) for the Luttinger-Tisza spectrum.
+ROCM_LIB = Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib"
+LINK_LIBS = [f"-L{ROCM_LIB}", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM_LIB}"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -66,7 +72,7 @@ def build_library() -> Path:
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), *LINK_LIBS])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1791,4 +1791,18 @@ int pyro_elapsed_ms(pyro_ctx* ctx, double* ms) {
     });
 }
 
+int pyro_lt_spectrum(const int32_t L[3], double J1, double J2, double J3, double Delta, double* lambda_min,
+                     int32_t k_star[3], double* eigvec) {
+    return guarded([&] {
+        require(L && lambda_min && k_star && eigvec, PYRO_E_INVALID, "NULL argument");
+        require(L[0] >= 1 && L[1] >= 1 && L[2] >= 1, PYRO_E_INVALID, "L must be positive");
+        require(int64_t(L[0]) * L[1] * L[2] <= (int64_t(1) << 24), PYRO_E_INVALID, "at most 2^24 k points");
+        for (double j : {J1, J2, J3, Delta}) require(std::isfinite(j), PYRO_E_INVALID, "couplings must be finite");
+        const pyro::LtBondTable table = pyro::pyrochlore_lt_bonds(J1, J2, J3, Delta);
+        const int Li[3] = {L[0], L[1], L[2]}, cell[3] = {8, 8, 8};
+        const std::string err = pyro::lt_spectrum(table, 16, Li, cell, lambda_min, k_star, eigvec);
+        require(err.empty(), PYRO_E_HIP, err);
+    });
+}
+
 }  // extern "C"

@@ -6,6 +6,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
+#include "topology.hpp"
+
 namespace pyro {
 
 // Device view of a context's sweep state, shared by the sweep, field and
@@ -223,5 +227,15 @@ template <typename Real>
 hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
                              int r_count, const MunuArgs& m, hipStream_t s);
 hipError_t launch_munu_flush(const MunuArgs& m, hipStream_t s);
+
+// Luttinger-Tisza spectrum over the L[0] x L[1] x L[2] grid of k with
+// q_a = 2 pi K'_a / (cell_a L_a), K' the centred index: builds the
+// (3 n_sl)^2 Hermitian M(k) = sum_bonds J e^{i q.d} on the device and
+// solves it with rocSOLVER.  Writes lambda_min[k] (flat order
+// (K0 L1 + K1) L2 + K2), the first k of the strict minimum into k_star and
+// the lowest eigenvector there into eigvec[3 n_sl][2].  Returns an empty
+// string or the error.
+std::string lt_spectrum(const LtBondTable& table, int n_sl, const int L[3], const int cell[3],
+                        double* lambda_min, int32_t* k_star, double* eigvec);
 
 }  // namespace pyro

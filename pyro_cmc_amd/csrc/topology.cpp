@@ -331,4 +331,43 @@ std::vector<CouplingDef> pyrochlore_J1J2J3(double J1, double J2, double J3, doub
     return cs;
 }
 
+LtBondTable pyrochlore_lt_bonds(double J1, double J2, double J3, double Delta) {
+    const LatticeGeom g = pyrochlore_cell(1, 1, 1);
+    const int n_sl = g.n_sl;
+    const double inv_s3 = 1.0 / std::sqrt(3.0);
+    const double Js[4] = {J1, J2, J3, J3};
+    std::vector<std::vector<LtBond>> by_block(size_t(n_sl) * n_sl);
+    for (int shell = 1; shell <= 4; ++shell) {
+        const double j = Js[shell - 1];
+        if (j == 0.0) continue;
+        const auto rel = pyrochlore_shell(shell);
+        for (int a = 0; a < n_sl; ++a) {
+            const int mu = g.sl_kind[a];
+            for (const auto& v : rel[mu]) {
+                const ivec3& r = g.sl_pos[a];
+                const int64_t b = g.site_at({r[0] + v[0], r[1] + v[1], r[2] + v[2]});
+                if (b < 0) throw std::invalid_argument("pyrochlore shell displacement leaves the lattice");
+                const int nu = g.sl_kind[b];
+                LtBond bd;
+                for (int e = 0; e < 3; ++e) bd.d[e] = double(v[e]);
+                for (int x = 0; x < 3; ++x)
+                    for (int y = 0; y < 3; ++y) {
+                        double m = x == y ? 1.0 : 0.0;
+                        if (shell == 1 && Delta != 1.0)
+                            m += (Delta - 1.0) * (inv_s3 * kLink[mu][x]) * (inv_s3 * kLink[nu][y]);
+                        bd.J[3 * x + y] = j * m;
+                    }
+                by_block[size_t(a) * n_sl + size_t(b)].push_back(bd);
+            }
+        }
+    }
+    LtBondTable t;
+    t.begin.push_back(0);
+    for (const auto& blk : by_block) {
+        t.bonds.insert(t.bonds.end(), blk.begin(), blk.end());
+        t.begin.push_back(int32_t(t.bonds.size()));
+    }
+    return t;
+}
+
 }  // namespace pyro

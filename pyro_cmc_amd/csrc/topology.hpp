@@ -79,4 +79,25 @@ std::vector<std::vector<ivec3>> pyrochlore_shell(int which);
 std::vector<CouplingDef> pyrochlore_J1J2J3(double J1, double J2, double J3,
                                            double Delta);
 
+// Directed bond alpha -> beta of the Luttinger-Tisza matrix: displacement d
+// (integer coordinates) and 3x3 coupling J (row-major).
+struct LtBond {
+    double d[3];
+    double J[9];
+};
+
+// Bonds of the pyrochlore J1-J2-J3 XXZ model for the Luttinger-Tisza matrix
+// (luttinger_tisza.cpp:31-81): for every sublattice alpha of the cubic cell
+// and every displacement v of the nn1, nn2, nn3a, nn3b shells of its kind,
+// the bond alpha -> beta (beta = the sublattice at r_alpha + v modulo the
+// cell) with J = J1 [I + (Delta - 1) z_alpha z_beta^T] for nn1 (z_p the
+// local [111] axis of kind p) and J = J2 I, J3 I, J3 I for the others;
+// shells with zero coupling are left out.  Bonds are grouped by block
+// alpha * 16 + beta: block b holds bonds[begin[b] .. begin[b + 1]).
+struct LtBondTable {
+    std::vector<LtBond> bonds;
+    std::vector<int32_t> begin;  // [16 * 16 + 1]
+};
+LtBondTable pyrochlore_lt_bonds(double J1, double J2, double J3, double Delta);
+
 }  // namespace pyro

@@ -131,3 +131,23 @@ def direct_ssf(S, pos, L, a, b):
         bb = (ph * S[:, b]).sum()
         out[K] = np.conj(ba) * bb
     return out
+
+
+def brute_coupling_matrix(L, J1, J2, J3, Delta=1.0):
+    """Real-space coupling matrix H[3N][3N] of the supercell from the
+    distance-search bond lists, with the energy sum_bonds S_i^T J S_j =
+    S^T H S / 2: block (i, j) holds the coupling of bond i -> j and block
+    (j, i) its transpose.  Needs L >= 2 (no bond reaches its own image)."""
+    bonds, _ = pyro_bond_lists(L)
+    kinds = pyro_kinds(L)
+    z = LINK / np.sqrt(3.0)
+    N = len(kinds)
+    H = np.zeros((3 * N, 3 * N))
+    for cls, J in ((1, J1), (2, J2), (3, J3), (4, J3)):
+        for i, j in bonds[cls]:
+            Jm = J * np.eye(3)
+            if cls == 1:
+                Jm = Jm + J * (Delta - 1) * np.outer(z[kinds[i]], z[kinds[j]])
+            H[3 * i:3 * i + 3, 3 * j:3 * j + 3] += Jm
+            H[3 * j:3 * j + 3, 3 * i:3 * i + 3] += Jm.T
+    return H

@@ -19,7 +19,7 @@ import pytest
 
 from helpers import brute_energy_per_cell, pyro_positions
 
-TOOLS = ("anneal", "temper", "spiral_byhand", "acc_runs", "fieldcool")
+TOOLS = ("anneal", "temper", "spiral_byhand", "acc_runs", "fieldcool", "ltgs")
 
 
 def _tool(pyro, name):
