@@ -96,7 +96,7 @@ def _oracle_sweeps(o, r, kind, n_or, n_sweeps, T, colours, n_col, seed, pass0):
     return acc
 
 
-@pytest.mark.parametrize("strategy", [1, 2, 3, 4])
+@pytest.mark.parametrize("strategy", [1, 2, 3, 4, 6])
 @pytest.mark.parametrize("kind,n_or", [(0, 1), (0, 2), (1, 1), (2, 0), (1, 0)])
 @pytest.mark.parametrize("J,Delta,B", [CONFIGS[1], CONFIGS[3]])
 def test_sweep_trajectory_fp64(gpu, oracle, strategy, kind, n_or, J, Delta, B):
@@ -147,21 +147,32 @@ def test_sweep_fp32_tracks_fp64(gpu, oracle, kind, n_or):
     assert np.allclose(np.linalg.norm(ctx.get_spins(), axis=1), 1.0, atol=1e-6)
 
 
-@pytest.mark.parametrize("pair,J", [((1, 2), (1.0, 0.1, 0.1)), ((3, 4), (1.0, 0.1, 0.1)),
-                                    ((3, 4), (1.0, -0.2, 0.0)), ((3, 4), (-1.0, 0.0, 0.0))])
-def test_strategies_bit_identical(gpu, pair, J):
+@pytest.mark.parametrize("pair,J,R,prec", [((1, 2), (1.0, 0.1, 0.1), 4, 32), ((3, 4), (1.0, 0.1, 0.1), 4, 32),
+                                            ((3, 4), (1.0, -0.2, 0.0), 4, 32), ((3, 4), (-1.0, 0.0, 0.0), 4, 32),
+                                            ((3, 6), (1.0, 0.1, 0.1), 4, 32), ((3, 6), (1.0, -0.2, 0.0), 16, 32),
+                                            ((3, 6), (-1.0, 0.0, 0.0), 8, 64), ((4, 6), (1.0, 0.1, 0.1), 24, 64),
+                                            ((3, 6), (1.0, -0.05, 0.05), 1, 32)])
+def test_strategies_bit_identical(gpu, pair, J, R, prec):
+    """Per-colour launches (1, 3), resident workgroups (2, 4) and the
+    persistent XCD-owned sweep (6), with R a multiple of 8 (replicas owned
+    by their XCD) or not (any XCD may own a replica)."""
     res = []
+    T = np.geomspace(0.1, 3.0, R)
     for strategy in pair:
-        ctx = gpu.Context.pyrochlore(3, n_replicas=4, precision=32, seed=99)
+        ctx = gpu.Context.pyrochlore(3, n_replicas=R, precision=prec, seed=99)
         ctx.define_J1J2J3(*J)
         ctx.setup()
         ctx.set_strategy(strategy)
-        ctx.set_temperatures([0.1, 0.3, 1.0, 3.0])
+        ctx.set_temperatures(T)
         ctx.randomize(5)
         acc = ctx.sweep(gpu.LIFTED, 1, 4)
-        res.append((acc, [ctx.get_spins(r) for r in range(4)]))
+        acc2 = ctx.sweep(gpu.METROPOLIS, 2, 3)
+        res.append((acc, acc2, [ctx.get_spins(r) for r in range(R)], [ctx.get_lifted_dir(r) for r in range(R)]))
     assert np.array_equal(res[0][0], res[1][0])
-    for a, b in zip(res[0][1], res[1][1]):
+    assert np.array_equal(res[0][1], res[1][1])
+    for a, b in zip(res[0][2], res[1][2]):
+        assert np.array_equal(a, b)
+    for a, b in zip(res[0][3], res[1][3]):
         assert np.array_equal(a, b)
 
 
