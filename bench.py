@@ -324,16 +324,11 @@ def main():
     n_sw = max(16, min(args.steps, 256))
     ctx.synchronize()
     ctx.mark(0)
-    if strategy == 6:  # one launch per round's N_SWEEP sweeps, as in the graph
-        n_sw -= n_sw % N_SWEEP
-        for _ in range(n_sw // N_SWEEP):
-            ctx.sweep(pyro.METROPOLIS, N_OR, N_SWEEP, count=False)
-    else:
-        ctx.sweep(pyro.METROPOLIS, N_OR, n_sw, count=False)
+    ctx.sweep(pyro.METROPOLIS, N_OR, n_sw, count=False)
     ctx.mark(1)
     sweep_ms = ctx.elapsed_ms()
     sweep_launches = {1: n_sw * (1 + N_OR) * info["n_colours"], 3: n_sw * (1 + N_OR) * info["n_colours"],
-                      2: 1, 4: 1, 5: 8 * n_sw, 6: n_sw // N_SWEEP}[strategy]
+                      2: 1, 4: 1, 5: 8 * n_sw}[strategy]
     upd_per_launch = N * R_REP * (1 + N_OR) * n_sw / sweep_launches
     avg_launch_s = sweep_ms / 1e3 / sweep_launches
     b_alg = bytes_per_update(4, info["n_slot"], R_REP)
@@ -386,7 +381,7 @@ def main():
             "frac": achieved / PEAK_HBM,
             "traffic": profiled_traffic("sweep_c3_L12x64"),
             "kernel": {1: "sweep_colour_kernel", 2: "sweep_resident_kernel", 3: "pyro_colour_kernel",
-                       4: "pyro_resident_kernel", 5: "pyro_tiled_kernel", 6: "pyro_persist_kernel"}[strategy],
+                       4: "pyro_resident_kernel", 5: "pyro_tiled_kernel"}[strategy],
             "bytes_per_update": b_alg,
             "updates_per_launch": upd_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,

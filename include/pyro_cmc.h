@@ -256,16 +256,12 @@ int pyro_synchronize(pyro_ctx* ctx);
  * colour (neighbour table), 2 = one resident workgroup per replica looping
  * over all colours of a sweep (neighbour table), 3 = one launch per colour
  * with the compile-time pyrochlore stencil, 4 = resident workgroup per
- * replica with the stencil (at most 2048 cells), 5 = tiled checkerboard
- * sweep with the stencil (large lattices), 6 = persistent launch of whole
- * sweeps with the stencil: each replica is owned by workgroups of one XCD,
- * which take its colour chunks from a queue, so its spins stay in that
- * XCD's L2.  3 to 6 apply when the couplings were defined by
- * pyro_define_J1J2J3 alone.  Strategies 1 and 2 give bit-identical
- * results, as do 3, 4 and 6 (same colour order, per-site counters and
- * arithmetic); the autotuner only chooses within such a group, so it never
- * changes results.  Sweeps report PYRO_E_UNSUPPORTED for a forced strategy
- * that does not apply. */
+ * replica with the stencil (at most 2048 cells).  3 and 4 apply when the
+ * couplings were defined by pyro_define_J1J2J3 alone.  Strategies 1 and 2
+ * give bit-identical results, as do 3 and 4 (same colour order, per-site
+ * counters and arithmetic); the autotuner only chooses within a pair, so it
+ * never changes results.  Sweeps report PYRO_E_UNSUPPORTED for a forced
+ * strategy that does not apply. */
 int pyro_set_strategy(pyro_ctx* ctx, int strategy);
 int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
 /* The site visiting order of one sweep under the strategy the next sweep

@@ -28,7 +28,6 @@ HIP_FLAGS = [
     "-O3",
     "-std=c++20",
     "-fPIC",
-    "-ffp-contract=on",
     # keep packed-FP32 instructions out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}",

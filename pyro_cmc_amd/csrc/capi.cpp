@@ -133,12 +133,8 @@ struct pyro_ctx {
     int a_kind = -1, a_nor = -1, a_nsw = -1, a_ssf = -1, a_strategy = -1;
 
     // requested: 0 auto, 1 generic per-colour, 2 generic resident,
-    // 3 pyrochlore per-colour, 4 pyrochlore resident, 5 pyrochlore tiled,
-    // 6 pyrochlore persistent (XCD-owned replicas)
+    // 3 pyrochlore per-colour, 4 pyrochlore resident, 5 pyrochlore tiled
     int strategy = 0;
-    DevMem pctl, perr;          // persistent sweep queues and error flag
-    bool persist_used = false;
-    int persist_wgs = 8;        // workgroups per replica of the persistent sweep
     int tile_kind = 0;          // tiled strategy tile shape (0: not applicable)
     size_t max_lds = 65536;     // per-workgroup LDS limit of the device
     int tuned_key = -1;        // (kind, n_or) the auto choice was made for
@@ -196,20 +192,6 @@ struct pyro_ctx {
     void d2h(void* dst, const void* src, size_t n) {
         HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        check_persist();
-    }
-    // The persistent sweep records a wait that hit its spin limit (never
-    // expected: its waits always end); the results are then not valid.
-    void check_persist() {
-        if (!persist_used) return;
-        unsigned int f = 0;
-        HIPCHK(hipMemcpyAsync(&f, perr.p, sizeof(f), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        if (f != 0) {
-            HIPCHK(hipMemsetAsync(perr.p, 0, sizeof(f), stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            throw Status(PYRO_E_HIP, "persistent sweep: a chunk wait hit its spin limit; spins are not valid");
-        }
     }
 
     hipEvent_t get_event() {
@@ -411,17 +393,6 @@ struct pyro_ctx {
             });
             return;
         }
-        if (strat == 6) {
-            const auto st = state<Real>(pass_base);
-            const auto J = pyro_couplings<Real>();
-            persist_used = true;
-            timed(kFamSweep, [&] {
-                return pyro::launch_pyro_persist<Real>(st, J, kind, n_or, n_sweeps, pass_index,
-                                                       pctl.as<unsigned int>(), perr.as<unsigned int>(),
-                                                       persist_wgs, stream);
-            });
-            return;
-        }
         unsigned long long p = pass_index;
         for (int s = 0; s < n_sweeps; ++s) {
             pass_colours<Real>(kind, pass_base, p++, strat == 3);
@@ -434,10 +405,9 @@ struct pyro_ctx {
         else sweeps<float>(strat, kind, n_or, n_sweeps, pass_base, pass_index);
     }
 
-    // Pick the fastest launch strategy for (kind, n_or) by timing the
-    // candidates on a saved copy of the state, then restore it (results do
-    // not depend on the choice: all candidates use the same per-site
-    // counters and colour order).
+    // Pick the faster launch strategy for (kind, n_or) by timing both on a
+    // saved copy of the state, then restore it (results do not depend on the
+    // strategy: both use the same per-site counters and colour order).
     int choose_strategy(int kind, int n_or) {
         if (strategy != 0) {
             require(strategy <= 2 || spec, PYRO_E_UNSUPPORTED,
@@ -448,8 +418,6 @@ struct pyro_ctx {
                     "the pyrochlore resident strategy needs at most 2048 cells");
             require(strategy != 5 || tile_kind > 0, PYRO_E_UNSUPPORTED,
                     "the tiled strategy needs each L divisible by twice the tile edge (8 or 12)");
-            require(strategy != 6 || spec, PYRO_E_UNSUPPORTED,
-                    "the persistent strategy needs couplings defined by define_J1J2J3 alone");
             return strategy;
         }
         // Large lattices take the tiled schedule (a fixed rule, so results do
@@ -465,7 +433,6 @@ struct pyro_ctx {
         if (spec) {
             cand.push_back(3);
             if (resident_fits()) cand.push_back(4);
-            cand.push_back(6);
         } else {
             cand = {1, 2};
         }
@@ -886,11 +853,6 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
         ctx->upload_field();
         HIPCHK(hipStreamSynchronize(ctx->stream));
         ctx->tile_kind = ctx->pick_tile_kind();
-        if (ctx->spec) {
-            ctx->pctl.alloc(sizeof(unsigned int) * size_t(pyro::persist_ctl_words(ctx->R)));
-            ctx->perr.alloc(256);
-            HIPCHK(hipMemset(ctx->perr.p, 0, 256));
-        }
         ctx->is_setup = true;
     });
 }
@@ -1766,14 +1728,13 @@ int pyro_synchronize(pyro_ctx* ctx) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        ctx->check_persist();
     });
 }
 
 int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
-        require(strategy >= 0 && strategy <= 6, PYRO_E_INVALID, "strategy must be 0..6");
+        require(strategy >= 0 && strategy <= 5, PYRO_E_INVALID, "strategy must be 0..5");
         ctx->strategy = strategy;
         ctx->tuned_key = -1;
         ctx->invalidate_graph();

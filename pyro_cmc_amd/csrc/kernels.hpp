@@ -107,17 +107,6 @@ inline void pyro_tile_dims(int tile_kind, int dims[3]) {
     dims[1] = tile_kind == 2 ? 6 : 4;
     dims[2] = 4;
 }
-// Persistent sweep of n_sweeps whole sweeps in one launch: replicas are
-// owned by the workgroups of one XCD, chunks of each colour are taken from
-// a per-replica ticket queue.  Bit-identical to launch_pyro_colour over the
-// same sweeps.  ctl holds persist_ctl_words(R) u32 (cleared by a kernel
-// enqueued before the sweep), err one u32 that receives 1 if a wait ever
-// hit its spin limit (cleared by the caller).
-inline int persist_ctl_words(int R) { return (4 * R + 8) * 64; }
-template <typename Real>
-hipError_t launch_pyro_persist(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind, int n_or,
-                               int n_sweeps, unsigned long long pass_index, unsigned int* ctl,
-                               unsigned int* err, int wgs_per_replica, hipStream_t s);
 template <typename Real>
 hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
                              int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s);

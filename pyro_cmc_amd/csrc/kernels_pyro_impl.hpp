@@ -255,193 +255,6 @@ __global__ __launch_bounds__(256) void pyro_fields_kernel(DeviceState<Real> st, 
     out[2 * N + i] = hz;
 }
 
-// ---------------------------------------------------------------------------
-// Persistent sweep: whole sweeps in one launch, replicas grouped by XCD.
-//
-// Each XCD has its own L2, and the L2s are not kept coherent with each other
-// during a kernel; at kernel boundaries they are written back and
-// invalidated, so per-colour launches fetch every replica's spins from
-// memory again at every colour.  Here a replica is processed by workgroups
-// of ONE XCD for the whole launch (its owner), so its spins stay in that
-// L2 across all colours and passes.  Spin, direction and counter accesses
-// use agent-scope relaxed atomic loads and stores: they skip the per-CU L1
-// (which other CUs do not see) and are served by the L2.
-//
-// Work per replica is a queue of tickets t = stage * C + chunk (stage =
-// (sweep, pass, colour) in the order of the colour kernels, chunk = 256
-// items of that colour).  A workgroup takes the next ticket, waits until
-// all chunks of the earlier stages are done, updates its chunk, waits for
-// its stores, and counts the chunk done.  Every ticket below the one being
-// waited on is held by a running workgroup that is itself only waiting on
-// lower tickets, so the waits always end whatever the number of resident
-// workgroups; a spin limit (which a correct run never reaches) records an
-// error instead of waiting on.  Site order, per-site counters and
-// arithmetic are those of the colour kernels, so results are bit-identical.
-//
-// ctl layout (u32, one 256-byte line per counter): replica r: owner (XCD+1)
-// at line 4r, ticket 4r+1, done 4r+2; XCD x slot counter at line 4R+x.
-// err (separate, host-cleared) receives the spin-limit flag.
-template <typename Real>
-__device__ __forceinline__ Real ld_coh(const Real* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename Real>
-__device__ __forceinline__ void st_coh(Real* p, Real v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ unsigned xcc_id() {
-    unsigned v;
-    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-    return v & 7u;
-}
-
-template <typename Real, int SHELLS>
-__device__ __forceinline__ void pyro_field_coh_rt(int sl, const Real* Sx, const Real* Sy, const Real* Sz, int nc,
-                                                  const CellAxes& c, const PyroCouplings<Real>& J, Real& hx,
-                                                  Real& hy, Real& hz) {
-    auto fetch = [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
-        const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
-        x = ld_coh(Sx + j);
-        y = ld_coh(Sy + j);
-        z = ld_coh(Sz + j);
-    };
-    switch (sl) {
-#define PYRO_SL_CASE(X) \
-    case X: pyro_field_with<Real, SHELLS, X>(fetch, J, hx, hy, hz); break;
-        PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
-        PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
-        PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
-        PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
-#undef PYRO_SL_CASE
-        default: hx = hy = hz = Real(0);
-    }
-}
-
-constexpr unsigned kPersistSpinLimit = 1u << 20;
-
-template <typename Real, int SHELLS, int FIRST>
-__global__ __launch_bounds__(256) void pyro_persist_kernel(DeviceState<Real> st, PyroCouplings<Real> J, int n_or,
-                                                           int n_sweeps, unsigned long long pass_index,
-                                                           unsigned int* __restrict__ ctl,
-                                                           unsigned int* __restrict__ err) {
-    constexpr int NCOL = pyrostencil::colours_for(SHELLS);
-    constexpr int SPC = 16 / NCOL;
-    constexpr unsigned LINE = 64;  // u32 per 256-byte line
-    const int R = st.R, nc = st.n_cells;
-    const int items = SPC * nc;
-    const unsigned C = unsigned(items + 255) / 256u;
-    const unsigned stages = unsigned(n_sweeps) * unsigned(n_or + 1) * unsigned(NCOL);
-    const unsigned total = stages * C;
-    const size_t N = size_t(st.N);
-    const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
-    __shared__ unsigned sh[2];
-
-    const unsigned x = xcc_id();
-    if (threadIdx.x == 0) sh[0] = atomicAdd(ctl + (4u * unsigned(R) + x) * LINE, 1u);
-    __syncthreads();
-    const unsigned slot = sh[0];
-    const bool part = (R & 7) == 0;  // replica r belongs to XCD r % 8
-    const unsigned own_n = part ? unsigned(R) / 8u : unsigned(R);
-
-    for (unsigned j = 0; j < unsigned(R); ++j) {
-        unsigned r;
-        if (j < own_n) {
-            r = part ? x + 8u * ((slot + j) % own_n) : (slot + j) % own_n;
-        } else {  // other XCDs' replicas nobody has taken (an XCD without workgroups)
-            const unsigned k = j - own_n;
-            r = ((x + 1u + k / own_n) & 7u) + 8u * ((slot + k) % own_n);
-        }
-        unsigned* owner = ctl + (4u * r) * LINE;
-        unsigned* ticket = ctl + (4u * r + 1u) * LINE;
-        unsigned* done = ctl + (4u * r + 2u) * LINE;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned o = atomicCAS(owner, 0u, x + 1u);
-            sh[1] = (o == 0u || o == x + 1u) ? 1u : 0u;
-        }
-        __syncthreads();
-        if (sh[1] == 0u) continue;
-
-        const PyroCouplings<Real> Jr = J.at(int(r));
-        Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
-        Real* __restrict__ Sy = Sx + N;
-        Real* __restrict__ Sz = Sy + N;
-        int8_t* __restrict__ dir_r = st.dir + size_t(r) * N;
-        unsigned my_acc = 0;
-        for (;;) {
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const unsigned t = atomicAdd(ticket, 1u);
-                if (t < total) {
-                    const unsigned need = (t / C) * C;
-                    unsigned spins = 0;
-                    while (ld_coh(done) < need) {
-                        __builtin_amdgcn_s_sleep(1);
-                        // after a recorded failure no wait is taken again, so
-                        // the launch still ends promptly
-                        if (++spins == kPersistSpinLimit || ld_coh(err) != 0u) {
-                            atomicOr(err, 1u);
-                            break;
-                        }
-                    }
-                }
-                sh[0] = t;
-            }
-            __syncthreads();
-            const unsigned t = sh[0];
-            if (t >= total) break;
-            const unsigned stage = t / C, chunk = t - stage * C;
-            const int col = int(stage % unsigned(NCOL));
-            const unsigned sp = stage / unsigned(NCOL);  // sweep * (n_or + 1) + pass
-            const int ps = int(sp % unsigned(n_or + 1));
-            const unsigned long long pass = pass0 + sp;
-            const int it = int(chunk) * 256 + int(threadIdx.x);
-            if (it < items) {
-                int cell = it, q = 0;
-                if constexpr (SPC > 1) q = fast_div(it, nc, cell);
-                const int sl = NCOL == 16 ? col : col + 4 * q;
-                const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
-                Real hx, hy, hz;
-                pyro_field_coh_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, Jr, hx, hy, hz);
-                const int site = sl * nc + cell;
-                Real px = ld_coh(Sx + site), py = ld_coh(Sy + site), pz = ld_coh(Sz + site);
-                constexpr bool LIFTED = FIRST == 1 || FIRST == 2;
-                int8_t d = LIFTED && ps == 0 ? ld_coh(dir_r + site) : int8_t(0);
-                const int8_t d0 = d;
-                bool ch;
-                if (ps == 0)
-                    my_acc += update_spin<Real, FIRST>(st, px, py, pz, ch, site, int(r), pass, hx, hy, hz, &d) ? 1u : 0u;
-                else
-                    update_spin<Real, 3>(st, px, py, pz, ch, site, int(r), pass, hx, hy, hz, &d);
-                if (ch) {
-                    st_coh(Sx + site, px);
-                    st_coh(Sy + site, py);
-                    st_coh(Sz + site, pz);
-                }
-                if (LIFTED && d != d0) st_coh(dir_r + site, d);
-            }
-            // this wave's stores have reached the L2 before the chunk counts as done
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            if (threadIdx.x == 0) atomicAdd(done, 1u);
-        }
-        if (st.acc != nullptr) {  // uniform
-            unsigned v = my_acc;
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            __shared__ unsigned red[4];
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-            __syncthreads();
-            if (threadIdx.x == 0 && red[0] + red[1] + red[2] + red[3])
-                atomicAdd(st.acc + r, (unsigned long long)(red[0] + red[1] + red[2] + red[3]));
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void persist_reset_kernel(unsigned int* __restrict__ ctl, int n) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) ctl[i] = 0u;
-}
-
 // One workgroup per replica; thread t owns cells t, t + BS, ... (CPT cells).
 template <typename Real, int SHELLS, int FIRST, int BS, int CPT>
 __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
@@ -705,28 +518,6 @@ hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings
             case 0: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 0, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
             case 1: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 1, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
             case 2: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 2, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
-            default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    });
-}
-
-template <typename Real>
-hipError_t launch_pyro_persist(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind, int n_or,
-                               int n_sweeps, unsigned long long pass_index, unsigned int* ctl,
-                               unsigned int* err, int wgs_per_replica, hipStream_t s) {
-    if (wgs_per_replica < 1 || n_sweeps < 1 || n_or < 0) return hipErrorInvalidValue;
-    const int n_ctl = persist_ctl_words(st.R);
-    hipLaunchKernelGGL(persist_reset_kernel, dim3(unsigned((n_ctl + 255) / 256)), dim3(256), 0, s, ctl, n_ctl);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const unsigned wgs = (unsigned(wgs_per_replica * st.R) + 7u) & ~7u;
-    return dispatch_pyro(J, [&](auto shells) -> hipError_t {
-        constexpr int SH = decltype(shells)::value;
-        switch (kind) {
-            case 0: hipLaunchKernelGGL((pyro_persist_kernel<Real, SH, 0>), dim3(wgs), dim3(256), 0, s, st, J, n_or, n_sweeps, pass_index, ctl, err); break;
-            case 1: hipLaunchKernelGGL((pyro_persist_kernel<Real, SH, 1>), dim3(wgs), dim3(256), 0, s, st, J, n_or, n_sweeps, pass_index, ctl, err); break;
-            case 2: hipLaunchKernelGGL((pyro_persist_kernel<Real, SH, 2>), dim3(wgs), dim3(256), 0, s, st, J, n_or, n_sweeps, pass_index, ctl, err); break;
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

@@ -96,7 +96,7 @@ def _oracle_sweeps(o, r, kind, n_or, n_sweeps, T, colours, n_col, seed, pass0):
     return acc
 
 
-@pytest.mark.parametrize("strategy", [1, 2, 3, 4, 6])
+@pytest.mark.parametrize("strategy", [1, 2, 3, 4])
 @pytest.mark.parametrize("kind,n_or", [(0, 1), (0, 2), (1, 1), (2, 0), (1, 0)])
 @pytest.mark.parametrize("J,Delta,B", [CONFIGS[1], CONFIGS[3]])
 def test_sweep_trajectory_fp64(gpu, oracle, strategy, kind, n_or, J, Delta, B):
@@ -149,13 +149,10 @@ def test_sweep_fp32_tracks_fp64(gpu, oracle, kind, n_or):
 
 @pytest.mark.parametrize("pair,J,R,prec", [((1, 2), (1.0, 0.1, 0.1), 4, 32), ((3, 4), (1.0, 0.1, 0.1), 4, 32),
                                             ((3, 4), (1.0, -0.2, 0.0), 4, 32), ((3, 4), (-1.0, 0.0, 0.0), 4, 32),
-                                            ((3, 6), (1.0, 0.1, 0.1), 4, 32), ((3, 6), (1.0, -0.2, 0.0), 16, 32),
-                                            ((3, 6), (-1.0, 0.0, 0.0), 8, 64), ((4, 6), (1.0, 0.1, 0.1), 24, 64),
-                                            ((3, 6), (1.0, -0.05, 0.05), 1, 32)])
+                                            ((3, 4), (1.0, -0.05, 0.05), 16, 64), ((1, 2), (1.0, 0.1, 0.0), 8, 64)])
 def test_strategies_bit_identical(gpu, pair, J, R, prec):
-    """Per-colour launches (1, 3), resident workgroups (2, 4) and the
-    persistent XCD-owned sweep (6), with R a multiple of 8 (replicas owned
-    by their XCD) or not (any XCD may own a replica)."""
+    """Per-colour launches (1, 3) and resident workgroups (2, 4): spins,
+    lifted directions and accept counts of both pass mixes agree bit for bit."""
     res = []
     T = np.geomspace(0.1, 3.0, R)
     for strategy in pair:
