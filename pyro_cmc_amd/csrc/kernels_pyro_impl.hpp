@@ -178,8 +178,10 @@ __device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sx, const Real
 }
 
 // One launch per colour; grid.x = (sublattices of the colour) x (cell
-// blocks), grid.y = replicas.
-template <typename Real, int SHELLS, int KIND>
+// blocks), grid.y = replicas.  PER_REP: per-replica couplings (a batch of
+// models); the single-model instance carries no branch on them, which
+// would otherwise sit ahead of the neighbour gathers.
+template <typename Real, int SHELLS, int KIND, bool PER_REP>
 __global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
                                                           PyroCouplings<Real> J, int colour,
                                                           unsigned long long pass_index) {
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
         Real* __restrict__ Sz = Sy + N;
         const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
         Real hx, hy, hz;
-        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J.at(r), hx, hy, hz);
+        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, PER_REP ? J.at(r) : J, hx, hy, hz);
         accepted = apply_update<Real, KIND>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
     }
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
@@ -470,17 +472,21 @@ hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<R
     const int ncol = J.shells == 3 ? 16 : 4;
     const int bps = (st.n_cells + 255) / 256;
     const dim3 grid(unsigned(bps * (16 / ncol)) * unsigned(st.R));  // replica-major, XCD-aware (xcd_split)
-    return dispatch_pyro(J, [&](auto shells) -> hipError_t {
-        constexpr int SH = decltype(shells)::value;
-        switch (kind) {
-            case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
-            case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
-            case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
-            case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
-            default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    });
+    auto go = [&](auto per_rep) -> hipError_t {
+        constexpr bool PR = decltype(per_rep)::value;
+        return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+            constexpr int SH = decltype(shells)::value;
+            switch (kind) {
+                case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+                case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+                case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+                case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        });
+    };
+    return J.per_replica ? go(std::true_type{}) : go(std::false_type{});
 }
 
 template <typename Real>
