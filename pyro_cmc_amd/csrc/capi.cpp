@@ -289,8 +289,8 @@ struct pyro_ctx {
             pyro::pyro_tile_dims(kind, d);
             bool ok = true;
             for (int a = 0; a < 3; ++a) ok = ok && L[a] % (2 * d[a]) == 0;
-            // spins of tile + halo, plus the interior's lifted directions
-            const size_t lds = size_t(3) * 16 * (d[0] + 2) * (d[1] + 2) * (d[2] + 2) * real_size() +
+            // spins of tile + one-sided halo, plus the interior's lifted directions
+            const size_t lds = size_t(3) * 16 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1) * real_size() +
                                size_t(16) * d[0] * d[1] * d[2];
             if (ok && lds <= max_lds) return kind;
         }

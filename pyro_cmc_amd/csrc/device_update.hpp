@@ -79,10 +79,21 @@ template <> struct Renorm<float> { static constexpr bool value = true; };
 // the global field.  Returns whether the proposal was accepted (OR: always
 // false); `changed` tells whether the spin was modified.  dp points at the
 // site's lifted direction (read and flipped by the lifted kinds only).
+// The five 24-bit uniforms of (site, replica, pass): one Philox4x32-10
+// block.  They do not depend on the field, so kernels draw them next to the
+// neighbour gathers, where their arithmetic hides the load latency.
+template <typename Real>
+__device__ __forceinline__ Uniform5 site_uniforms(const DeviceState<Real>& st, int site, int r,
+                                                  unsigned long long pass) {
+    return split_uniform5(philox4x32_10(
+        Words4{uint32_t(site), uint32_t(r), uint32_t(pass), uint32_t(pass >> 32)}, st.k0, st.k1));
+}
+
+// update_spin with the site's uniforms already drawn (site_uniforms).
 template <typename Real, int KIND>
-__device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& px, Real& py, Real& pz,
-                                            bool& changed, int site, int r, unsigned long long pass,
-                                            Real hx, Real hy, Real hz, int8_t* dp) {
+__device__ __forceinline__ bool update_spin_u(const DeviceState<Real>& st, Real& px, Real& py, Real& pz,
+                                              bool& changed, int r, const Uniform5& u, Real hx, Real hy,
+                                              Real hz, int8_t* dp) {
     changed = false;
     hx -= st.B[3 * r + 0];  // h_loc = local_field - global_field
     hy -= st.B[3 * r + 1];
@@ -91,10 +102,6 @@ __device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& p
     const Real sx = px, sy = py, sz = pz;
     const double Td = st.T_slot[st.slot_of[r]];
     const Real T = Real(Td);
-
-    const Words4 w = philox4x32_10(
-        Words4{uint32_t(site), uint32_t(r), uint32_t(pass), uint32_t(pass >> 32)}, st.k0, st.k1);
-    const Uniform5 u = split_uniform5(w);
 
     if constexpr (KIND == 0) {
         // Gaussian-kick Metropolis (MC.cpp:111-130).
@@ -214,21 +221,35 @@ __device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& p
 }
 
 
-// update_spin on the spin stored in global memory (S[r][c][site]).
 template <typename Real, int KIND>
-__device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sx,
-                                             Real* __restrict__ Sy, Real* __restrict__ Sz, int site,
-                                             int r, unsigned long long pass, Real hx, Real hy, Real hz) {
+__device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& px, Real& py, Real& pz,
+                                            bool& changed, int site, int r, unsigned long long pass,
+                                            Real hx, Real hy, Real hz, int8_t* dp) {
+    return update_spin_u<Real, KIND>(st, px, py, pz, changed, r, site_uniforms(st, site, r, pass), hx, hy, hz,
+                                     dp);
+}
+
+// update_spin_u on the spin stored in global memory (S[r][c][site]).
+template <typename Real, int KIND>
+__device__ __forceinline__ bool apply_update_u(const DeviceState<Real>& st, Real* __restrict__ Sx,
+                                               Real* __restrict__ Sy, Real* __restrict__ Sz, int site, int r,
+                                               const Uniform5& u, Real hx, Real hy, Real hz) {
     Real x = Sx[site], y = Sy[site], z = Sz[site];
     bool changed;
-    const bool acc = update_spin<Real, KIND>(st, x, y, z, changed, site, r, pass, hx, hy, hz,
-                                             st.dir + size_t(r) * size_t(st.N) + site);
+    const bool acc = update_spin_u<Real, KIND>(st, x, y, z, changed, r, u, hx, hy, hz,
+                                               st.dir + size_t(r) * size_t(st.N) + site);
     if (changed) {
         Sx[site] = x;
         Sy[site] = y;
         Sz[site] = z;
     }
     return acc;
+}
+template <typename Real, int KIND>
+__device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sx,
+                                             Real* __restrict__ Sy, Real* __restrict__ Sz, int site,
+                                             int r, unsigned long long pass, Real hx, Real hy, Real hz) {
+    return apply_update_u<Real, KIND>(st, Sx, Sy, Sz, site, r, site_uniforms(st, site, r, pass), hx, hy, hz);
 }
 
 // Wave-level count of accepted updates added to acc[r].

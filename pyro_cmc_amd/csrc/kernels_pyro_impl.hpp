@@ -146,13 +146,18 @@ __device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Re
         J, hx, hy, hz);
 }
 
-// Field from a tile in LDS laid out [sl][halo cell], halo dims H0 x H1 x H2.
-template <typename Real, int SHELLS, int SL, int H1, int H2, int HC>
+// Field from a tile in LDS laid out [sl][box cell], every sublattice's box
+// B0 x B1 x B2 = (T0 + 1) x (T1 + 1) x (T2 + 1) cells shifted down one cell
+// on the axes where its halo is on the lower side (pyrostencil::halo_lo);
+// lc = (i0 * B1 + i1) * B2 + i2 for the site's tile cell (i0, i1, i2).
+template <typename Real, int SHELLS, int SL, int B1, int B2, int BC>
 __device__ __forceinline__ void pyro_field_lds(const Real* lx, const Real* ly, const Real* lz, int lc,
                                                const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
     pyro_field_with<Real, SHELLS, SL>(
         [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
-            const int j = lc + (n.sl * HC + n.d[0] * H1 * H2 + n.d[1] * H2 + n.d[2]);
+            using pyrostencil::halo_lo;
+            const int j = lc + (n.sl * BC + (n.d[0] + halo_lo(n.sl, 0)) * B1 * B2 +
+                                (n.d[1] + halo_lo(n.sl, 1)) * B2 + (n.d[2] + halo_lo(n.sl, 2)));
             x = lx[j];
             y = ly[j];
             z = lz[j];
@@ -323,8 +328,9 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
 }
 
 // Tiled sweep: one workgroup per (tile, replica) of one checkerboard phase.
-// The T0 x T1 x T2 tile and a one-cell halo (all 16 sublattices) are loaded
-// into LDS once; every pass of the sweep (FIRST, then n_or OR passes) and
+// The T0 x T1 x T2 tile and its one-sided one-cell halo (per sublattice the
+// side its bonds reach, so (T0 + 1)(T1 + 1)(T2 + 1) cells of each of the 16
+// sublattices) are loaded into LDS once; every pass of the sweep (FIRST, then n_or OR passes) and
 // every colour runs on the LDS copy; the tile interior is written back once.
 // Tiles of one phase are two tiles apart along each axis, so no two of them
 // share a bond or write a cell another reads: the phase equals the tiles
@@ -339,16 +345,16 @@ template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2>
 __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                          int phase, int n_or,
                                                          unsigned long long pass_index) {
-    constexpr int H0 = T0 + 2, H1 = T1 + 2, H2 = T2 + 2, HC = H0 * H1 * H2;
+    constexpr int B0 = T0 + 1, B1 = T1 + 1, B2 = T2 + 1, BC = B0 * B1 * B2;
     constexpr int NI = T0 * T1 * T2;
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     constexpr int SPC = 16 / NCOL;
     constexpr bool LIFTED = FIRST == 1 || FIRST == 2;
     extern __shared__ __align__(16) unsigned char pyro_smem[];
     Real* lx = reinterpret_cast<Real*>(pyro_smem);
-    Real* ly = lx + 16 * HC;
-    Real* lz = ly + 16 * HC;
-    int8_t* ld = reinterpret_cast<int8_t*>(lz + 16 * HC);  // [16][NI] lifted directions
+    Real* ly = lx + 16 * BC;
+    Real* lz = ly + 16 * BC;
+    int8_t* ld = reinterpret_cast<int8_t*>(lz + 16 * BC);  // [16][NI] lifted directions
 
     const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica and phase
     unsigned g = blockIdx.x;
@@ -369,10 +375,11 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
     Real* __restrict__ Sy = Sx + N;
     Real* __restrict__ Sz = Sy + N;
 
-    for (int e = threadIdx.x; e < 16 * HC; e += 256) {
-        const int sl = e / HC, hc = e % HC;
-        const int a2 = hc % H2, a1 = (hc / H2) % H1, a0 = hc / (H1 * H2);
-        int g0 = o0 + a0 - 1, g1 = o1 + a1 - 1, g2 = o2 + a2 - 1;
+    for (int e = threadIdx.x; e < 16 * BC; e += 256) {
+        const int sl = e / BC, bc = e % BC;
+        const int a2 = bc % B2, a1 = (bc / B2) % B1, a0 = bc / (B1 * B2);
+        const int f = sl >> 2;  // halo_lo(sl, a) = 1 iff R_fcc[f] has 4 on axis a
+        int g0 = o0 + a0 - (f == 2 || f == 3), g1 = o1 + a1 - (f == 1 || f == 3), g2 = o2 + a2 - (f == 1 || f == 2);
         g0 += g0 < 0 ? L0 : 0;
         g0 -= g0 >= L0 ? L0 : 0;
         g1 += g1 < 0 ? L1 : 0;
@@ -400,13 +407,18 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
         for (int col = 0; col < NCOL; ++col) {
             for (int q = 0; q < SPC; ++q) {
                 const int sl = NCOL == 16 ? col : col + 4 * q;
+                const int f = sl >> 2;
+                const int own = ((f == 2 || f == 3) * B1 + (f == 1 || f == 3)) * B2 + (f == 1 || f == 2);
                 for (int m = threadIdx.x; m < NI; m += 256) {
                     const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
-                    const int lc = ((i0 + 1) * H1 + (i1 + 1)) * H2 + (i2 + 1);
+                    const int lc = (i0 * B1 + i1) * B2 + i2;
+                    // the field in a block of its own: drawing the uniforms
+                    // next to the LDS gathers raises the register count past
+                    // what three workgroups per CU allow
                     Real hx, hy, hz;
                     switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field_lds<Real, SHELLS, X, H1, H2, HC>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
+    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, BC>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
                         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
                         PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
                         PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -415,7 +427,7 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
                         default: hx = hy = hz = Real(0);
                     }
                     const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
-                    const int li = sl * HC + lc;
+                    const int li = sl * BC + lc + own;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool ch;
                     int8_t* dp = LIFTED ? ld + sl * NI + m : nullptr;
@@ -437,11 +449,13 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
     for (int e = threadIdx.x; e < 16 * NI; e += 256) {
         const int sl = e / NI, m = e % NI;
         const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
-        const int lc = ((i0 + 1) * H1 + (i1 + 1)) * H2 + (i2 + 1);
+        const int f = sl >> 2;
+        const int li = sl * BC + ((i0 + (f == 2 || f == 3)) * B1 + (i1 + (f == 1 || f == 3))) * B2 +
+                       (i2 + (f == 1 || f == 2));
         const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
-        Sx[site] = lx[sl * HC + lc];
-        Sy[site] = ly[sl * HC + lc];
-        Sz[site] = lz[sl * HC + lc];
+        Sx[site] = lx[li];
+        Sy[site] = ly[li];
+        Sz[site] = lz[li];
         if constexpr (LIFTED) dir_r[site] = ld[e];
     }
 
@@ -535,7 +549,7 @@ hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Re
                              int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s) {
     auto go = [&](auto t0, auto t1, auto t2) -> hipError_t {
         constexpr int T0 = decltype(t0)::value, T1 = decltype(t1)::value, T2 = decltype(t2)::value;
-        const size_t lds = size_t(3) * 16 * (T0 + 2) * (T1 + 2) * (T2 + 2) * sizeof(Real) +
+        const size_t lds = size_t(3) * 16 * (T0 + 1) * (T1 + 1) * (T2 + 1) * sizeof(Real) +
                            ((kind == 1 || kind == 2) ? size_t(16) * T0 * T1 * T2 : 0);
         const unsigned tiles = unsigned((st.L0 / (2 * T0)) * (st.L1 / (2 * T1)) * (st.L2 / (2 * T2)));
         const dim3 grid(tiles * unsigned(st.R));  // (replica, tile), XCD-chunked in the kernel

@@ -65,5 +65,24 @@ constexpr int slots_for(int shells) { return shells == 1 ? 6 : shells == 2 ? 18 
 // colour (16 colours, colour = sl).
 constexpr int colours_for(int shells) { return shells == 3 ? 16 : 4; }
 
+// One-sided halo of a tile.  Inside its cell a site of sublattice sl sits
+// at R_fcc[f] + link[p], coordinate -1, 1 (R_fcc = 0) or 3, 5 (R_fcc = 4)
+// along each axis.  Bonds reach at most 4 along an axis, so sites of the
+// first kind are only reached across the tile's upper face and sites of
+// the second kind only across its lower face: a tile of T cells needs T + 1
+// cells of each sublattice per axis, starting one cell below the tile
+// (halo_lo = 1) or at the tile (halo_lo = 0).
+constexpr int halo_lo(int sl, int axis) { return kFcc[sl / 4][axis] == 4 ? 1 : 0; }
+constexpr bool one_sided_halo_covers() {
+    for (int sl = 0; sl < 16; ++sl)
+        for (int k = 0; k < 30; ++k)
+            for (int a = 0; a < 3; ++a) {
+                const int v = kStencil.n[sl][k].d[a] + halo_lo(kStencil.n[sl][k].sl, a);
+                if (v < 0 || v > 1) return false;
+            }
+    return true;
+}
+static_assert(one_sided_halo_covers(), "every neighbour must lie in its sublattice's one-sided halo box");
+
 }  // namespace pyrostencil
 }  // namespace pyro
