@@ -264,6 +264,16 @@ int pyro_synchronize(pyro_ctx* ctx);
  * strategy that does not apply. */
 int pyro_set_strategy(pyro_ctx* ctx, int strategy);
 int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
+/* Tile shape of the tiled sweep (strategy 5), in cells per axis: one of
+ * 4x4x4, 6x6x4, 8x4x4, 4x4x8, 8x8x4, applicable when each L_a is divisible
+ * by twice the edge.  T = NULL restores the automatic choice: the first of
+ * 6x6x4, 8x8x4, 4x4x4, 4x4x8, 8x4x4 that applies and gives at least 8
+ * tiles per checkerboard phase, else the first that applies.  The tile shape
+ * changes the tiled visiting order (pyro_sweep_order reports it), so it
+ * is a fixed rule, never a timing choice.  pyro_get_tile_shape returns the
+ * shape in use (0, 0, 0 when none applies). */
+int pyro_set_tile_shape(pyro_ctx* ctx, const int32_t T[3]);
+int pyro_get_tile_shape(pyro_ctx* ctx, int32_t T[3]);
 /* The site visiting order of one sweep under the strategy the next sweep
  * will use: site[k] and pass_offset[k] (0 = the sweep's first pass kind,
  * 1..n_overrelax = OR passes) for k < N * (1 + n_overrelax).  Sites with

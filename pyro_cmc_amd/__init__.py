@@ -73,6 +73,8 @@ _SIGS = {
     "pyro_synchronize": [_vp],
     "pyro_set_strategy": [_vp, C.c_int],
     "pyro_get_strategy": [_vp, C.POINTER(C.c_int)],
+    "pyro_set_tile_shape": [_vp, _vp],
+    "pyro_get_tile_shape": [_vp, _i32p],
     "pyro_profile_enable": [_vp, C.c_int],
     "pyro_profile_read": [_vp, _f64p, _u64p],
     "pyro_mark": [_vp, C.c_int],
@@ -407,6 +409,16 @@ class Context:
 
     def set_strategy(self, s):
         _check(lib().pyro_set_strategy(self._h, s))
+
+    def set_tile_shape(self, T=None):
+        """Tile shape (cells per axis) of the tiled sweep; None = automatic."""
+        t = None if T is None else np.ascontiguousarray(T, np.int32)
+        _check(lib().pyro_set_tile_shape(self._h, _ptr(t)))
+
+    def tile_shape(self):
+        t = np.zeros(3, np.int32)
+        _check(lib().pyro_get_tile_shape(self._h, t))
+        return tuple(int(v) for v in t)
 
     def strategy(self):
         v = C.c_int()

@@ -24,6 +24,14 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Tile kinds of the tiled strategy in order of preference: the first that
+// applies to the lattice and gives at least kMinTilesPerPhase tiles per
+// phase and replica, else the first that applies.  (Measured on L = 48, 32,
+// 24 and 16 lattices: 6x6x4 and 8x8x4 are fastest where they give enough
+// tiles; with fewer, 4x4x4 keeps more workgroups in flight.)
+constexpr int kTileOrder[] = {2, 5, 1, 4, 3};
+constexpr int kMinTilesPerPhase = 8;
+
 struct Status : std::runtime_error {
     int code;
     Status(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -136,6 +144,7 @@ struct pyro_ctx {
     // 3 pyrochlore per-colour, 4 pyrochlore resident, 5 pyrochlore tiled
     int strategy = 0;
     int tile_kind = 0;          // tiled strategy tile shape (0: not applicable)
+    int tile_forced = 0;        // tile shape requested by pyro_set_tile_shape (0: automatic)
     size_t max_lds = 65536;     // per-workgroup LDS limit of the device
     int tuned_key = -1;        // (kind, n_or) the auto choice was made for
     int tuned_strategy = 1;
@@ -281,19 +290,24 @@ struct pyro_ctx {
 
     // Tile shape for the tiled strategy: L_a divisible by twice the tile
     // edge (checkerboard of tiles) and the tile + halo within the LDS limit.
+    bool tile_applies(int kind) const {
+        if (!spec || kind < 1 || kind > pyro::kPyroTileKinds) return false;
+        int d[3];
+        pyro::pyro_tile_dims(kind, d);
+        for (int a = 0; a < 3; ++a)
+            if (geom.L[a] % (2 * d[a]) != 0) return false;
+        return pyro::pyro_tile_lds(d[0], d[1], d[2], int(real_size()), true) <= max_lds;
+    }
     int pick_tile_kind() const {
-        if (!spec) return 0;
-        const auto& L = geom.L;
-        for (int kind : {2, 1}) {
+        if (tile_forced) return tile_applies(tile_forced) ? tile_forced : 0;
+        for (int kind : kTileOrder) {
             int d[3];
             pyro::pyro_tile_dims(kind, d);
-            bool ok = true;
-            for (int a = 0; a < 3; ++a) ok = ok && L[a] % (2 * d[a]) == 0;
-            // spins of tile + one-sided halo, plus the interior's lifted directions
-            const size_t lds = size_t(3) * 16 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1) * real_size() +
-                               size_t(16) * d[0] * d[1] * d[2];
-            if (ok && lds <= max_lds) return kind;
+            const int64_t tiles = int64_t(geom.L[0] / (2 * d[0])) * (geom.L[1] / (2 * d[1])) * (geom.L[2] / (2 * d[2]));
+            if (tile_applies(kind) && tiles >= kMinTilesPerPhase) return kind;
         }
+        for (int kind : kTileOrder)
+            if (tile_applies(kind)) return kind;
         return 0;
     }
 
@@ -1738,6 +1752,38 @@ int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
         ctx->strategy = strategy;
         ctx->tuned_key = -1;
         ctx->invalidate_graph();
+    });
+}
+
+int pyro_set_tile_shape(pyro_ctx* ctx, const int32_t T[3]) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        int kind = 0;
+        if (T) {
+            for (int k = 1; k <= pyro::kPyroTileKinds && !kind; ++k) {
+                int d[3];
+                pyro::pyro_tile_dims(k, d);
+                if (d[0] == T[0] && d[1] == T[1] && d[2] == T[2]) kind = k;
+            }
+            require(kind > 0, PYRO_E_UNSUPPORTED, "tile shape not available (4x4x4, 6x6x4, 8x4x4, 4x4x8, 8x8x4)");
+            require(ctx->tile_applies(kind), PYRO_E_UNSUPPORTED,
+                    "tile shape does not divide the lattice (L_a divisible by twice the edge) or exceeds LDS");
+        }
+        ctx->tile_forced = kind;
+        ctx->tile_kind = ctx->pick_tile_kind();
+        ctx->tuned_key = -1;
+        ctx->invalidate_graph();
+    });
+}
+
+int pyro_get_tile_shape(pyro_ctx* ctx, int32_t T[3]) {
+    return guarded([&] {
+        require(ctx && T, PYRO_E_INVALID, "NULL argument");
+        ctx->need_setup();
+        int d[3];
+        pyro::pyro_tile_dims(ctx->tile_kind, d);
+        for (int a = 0; a < 3; ++a) T[a] = d[a];
     });
 }
 

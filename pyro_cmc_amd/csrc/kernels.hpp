@@ -98,14 +98,24 @@ hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings
                                 hipStream_t s);
 
 // Tiled sweeps: one call launches one of the 8 checkerboard phases of a
-// sweep; its workgroups hold a tile + halo in LDS and run every pass.
-// tile_kind 1 = 4x4x4 cells, 2 = 6x6x4 cells; returns the tile edge per
-// axis through tile_dims.  LDS bytes per workgroup: 3*16*(T0+2)(T1+2)(T2+2)
-// * sizeof(Real).
+// sweep; its workgroups hold a tile + one-sided halo in LDS and run every
+// pass.  Tile kinds (cells per axis): 1 = 4x4x4, 2 = 6x6x4, 3 = 8x4x4,
+// 4 = 4x4x8, 5 = 8x8x4.  A kind applies when every L_a is divisible by twice
+// its edge and its LDS fits.
+constexpr int kPyroTileKinds = 5;
 inline void pyro_tile_dims(int tile_kind, int dims[3]) {
-    dims[0] = tile_kind == 2 ? 6 : 4;
-    dims[1] = tile_kind == 2 ? 6 : 4;
-    dims[2] = 4;
+    static const int T[kPyroTileKinds + 1][3] = {{0, 0, 0}, {4, 4, 4}, {6, 6, 4}, {8, 4, 4}, {4, 4, 8}, {8, 8, 4}};
+    const int k = tile_kind >= 1 && tile_kind <= kPyroTileKinds ? tile_kind : 0;
+    for (int a = 0; a < 3; ++a) dims[a] = T[k][a];
+}
+// Threads per tiled workgroup: the tile's cells rounded up to whole waves.
+constexpr int pyro_tile_threads(int cells) { return cells >= 256 ? 256 : (cells + 63) / 64 * 64; }
+// LDS bytes of a tiled workgroup: spins of the tile + one-sided halo
+// (3 x 16 x (T0+1)(T1+1)(T2+1) reals) and, for the lifted kinds, the
+// interior's directions as bits (16 x ceil(T0 T1 T2 / 32) words).
+constexpr size_t pyro_tile_lds(int T0, int T1, int T2, int real_size, bool lifted) {
+    return size_t(3) * 16 * (T0 + 1) * (T1 + 1) * (T2 + 1) * size_t(real_size) +
+           (lifted ? size_t(16) * 4 * ((T0 * T1 * T2 + 31) / 32) : 0);
 }
 template <typename Real>
 hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,

@@ -327,6 +327,43 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     }
 }
 
+// Lane -> tile cell of the tiled sweep.  A colour stage reads, for every
+// cell of the tile, the neighbour at the same constant offset from the
+// cell's LDS word, so a wave's lanes conflict in an LDS bank when their
+// cells' words agree modulo the bank count.  The halo padding of the box
+// rows makes row-major lanes collide about 1.8-fold; here wave w's lane b
+// takes an unassigned cell whose word is b modulo 64, so each wave reads 64
+// distinct banks (and each half-wave 32 distinct banks of 32).  Cells left
+// over (tiles with more than BS/64 cells per bank) fill idle lanes; idle
+// lanes hold -1.
+template <int T0, int T1, int T2, int BS>
+struct TileLanes {
+    short cell[BS];
+    constexpr TileLanes() : cell{} {
+        constexpr int NI = T0 * T1 * T2, B1 = T1 + 1, B2 = T2 + 1;
+        bool used[NI] = {};
+        for (int l = 0; l < BS; ++l) cell[l] = -1;
+        for (int w = 0; w < BS / 64; ++w)
+            for (int b = 0; b < 64; ++b)
+                for (int m = 0; m < NI; ++m) {
+                    const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
+                    if (!used[m] && ((i0 * B1 + i1) * B2 + i2) % 64 == b) {
+                        cell[w * 64 + b] = short(m);
+                        used[m] = true;
+                        break;
+                    }
+                }
+        for (int m = 0; m < NI; ++m)
+            for (int l = 0; l < BS && !used[m]; ++l)
+                if (cell[l] < 0) {
+                    cell[l] = short(m);
+                    used[m] = true;
+                }
+    }
+};
+template <int T0, int T1, int T2, int BS>
+inline constexpr TileLanes<T0, T1, T2, BS> kTileLanes{};
+
 // Tiled sweep: one workgroup per (tile, replica) of one checkerboard phase.
 // The T0 x T1 x T2 tile and its one-sided one-cell halo (per sublattice the
 // side its bonds reach, so (T0 + 1)(T1 + 1)(T2 + 1) cells of each of the 16
@@ -341,8 +378,8 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
 // Workgroups map to tiles XCD by XCD: the 8 XCDs take contiguous chunks of
 // the (replica, tile) order, so consecutive tiles along n2, whose halo rows
 // share cache lines, are fetched through the same L2.
-template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2>
-__global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
+template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2, int BS>
+__global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                          int phase, int n_or,
                                                          unsigned long long pass_index) {
     constexpr int B0 = T0 + 1, B1 = T1 + 1, B2 = T2 + 1, BC = B0 * B1 * B2;
@@ -354,7 +391,11 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
     Real* lx = reinterpret_cast<Real*>(pyro_smem);
     Real* ly = lx + 16 * BC;
     Real* lz = ly + 16 * BC;
-    int8_t* ld = reinterpret_cast<int8_t*>(lz + 16 * BC);  // [16][NI] lifted directions
+    // lifted directions of the interior, one bit per site (set: -1), W words
+    // per sublattice; sites of one sublattice flip their own bits with LDS
+    // atomics, so lanes sharing a word never overwrite each other
+    constexpr int W = (NI + 31) / 32;
+    unsigned int* ldw = reinterpret_cast<unsigned int*>(lz + 16 * BC);  // [16][W]
 
     const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica and phase
     unsigned g = blockIdx.x;
@@ -375,7 +416,7 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
     Real* __restrict__ Sy = Sx + N;
     Real* __restrict__ Sz = Sy + N;
 
-    for (int e = threadIdx.x; e < 16 * BC; e += 256) {
+    for (int e = threadIdx.x; e < 16 * BC; e += BS) {
         const int sl = e / BC, bc = e % BC;
         const int a2 = bc % B2, a1 = (bc / B2) % B1, a0 = bc / (B1 * B2);
         const int f = sl >> 2;  // halo_lo(sl, a) = 1 iff R_fcc[f] has 4 on axis a
@@ -393,15 +434,20 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
     }
     int8_t* const dir_r = st.dir + size_t(r) * N;
     if constexpr (LIFTED) {
-        for (int e = threadIdx.x; e < 16 * NI; e += 256) {
+        for (int w = threadIdx.x; w < 16 * W; w += BS) ldw[w] = 0u;
+        __syncthreads();
+        for (int e = threadIdx.x; e < 16 * NI; e += BS) {
             const int sl = e / NI, m = e % NI;
             const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
-            ld[e] = dir_r[sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2)];
+            if (dir_r[sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2)] < 0)
+                atomicOr(ldw + sl * W + (m >> 5), 1u << (m & 31));
         }
     }
     __syncthreads();
 
     const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    static_assert(NI <= BS, "one tile cell per thread");
+    const int my_cell = kTileLanes<T0, T1, T2, BS>.cell[threadIdx.x];
     unsigned int my_acc = 0;
     for (int k = 0; k <= n_or; ++k) {
         for (int col = 0; col < NCOL; ++col) {
@@ -409,7 +455,8 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
                 const int sl = NCOL == 16 ? col : col + 4 * q;
                 const int f = sl >> 2;
                 const int own = ((f == 2 || f == 3) * B1 + (f == 1 || f == 3)) * B2 + (f == 1 || f == 2);
-                for (int m = threadIdx.x; m < NI; m += 256) {
+                if (my_cell >= 0) {
+                    const int m = my_cell;
                     const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
                     const int lc = (i0 * B1 + i1) * B2 + i2;
                     // the field in a block of its own: drawing the uniforms
@@ -430,11 +477,22 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
                     const int li = sl * BC + lc + own;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool ch;
-                    int8_t* dp = LIFTED ? ld + sl * NI + m : nullptr;
-                    if (k == 0)
-                        my_acc += update_spin<Real, FIRST>(st, x, y, z, ch, site, r, pass0, hx, hy, hz, dp) ? 1u : 0u;
-                    else
-                        update_spin<Real, 3>(st, x, y, z, ch, site, r, pass0 + k, hx, hy, hz, dp);
+                    if (k == 0) {
+                        int8_t d = 1, d0 = 1;
+                        unsigned int* wp = nullptr;
+                        const unsigned int bit = 1u << (m & 31);
+                        if constexpr (LIFTED) {
+                            wp = ldw + sl * W + (m >> 5);
+                            d0 = d = (*wp & bit) ? int8_t(-1) : int8_t(1);
+                        }
+                        my_acc += update_spin<Real, FIRST>(st, x, y, z, ch, site, r, pass0, hx, hy, hz,
+                                                           LIFTED ? &d : nullptr) ? 1u : 0u;
+                        if constexpr (LIFTED) {
+                            if (d != d0) atomicXor(wp, bit);
+                        }
+                    } else {
+                        update_spin<Real, 3>(st, x, y, z, ch, site, r, pass0 + k, hx, hy, hz, nullptr);
+                    }
                     if (ch) {
                         lx[li] = x;
                         ly[li] = y;
@@ -446,7 +504,7 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
         }
     }
 
-    for (int e = threadIdx.x; e < 16 * NI; e += 256) {
+    for (int e = threadIdx.x; e < 16 * NI; e += BS) {
         const int sl = e / NI, m = e % NI;
         const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
         const int f = sl >> 2;
@@ -456,16 +514,20 @@ __global__ __launch_bounds__(256) void pyro_tiled_kernel(DeviceState<Real> st, P
         Sx[site] = lx[li];
         Sy[site] = ly[li];
         Sz[site] = lz[li];
-        if constexpr (LIFTED) dir_r[site] = ld[e];
+        if constexpr (LIFTED) dir_r[site] = ((ldw[sl * W + (m >> 5)] >> (m & 31)) & 1u) ? int8_t(-1) : int8_t(1);
     }
 
     if (st.acc == nullptr) return;  // uniform: counts not requested
-    __shared__ unsigned int red[4];
+    __shared__ unsigned int red[BS / 64];
     unsigned int v = my_acc;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(st.acc + r, (unsigned long long)(red[0] + red[1] + red[2] + red[3]));
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < BS / 64; ++i) t += red[i];
+        atomicAdd(st.acc + r, t);
+    }
 }
 
 template <typename Real, typename F>
@@ -549,35 +611,37 @@ hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Re
                              int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s) {
     auto go = [&](auto t0, auto t1, auto t2) -> hipError_t {
         constexpr int T0 = decltype(t0)::value, T1 = decltype(t1)::value, T2 = decltype(t2)::value;
-        const size_t lds = size_t(3) * 16 * (T0 + 1) * (T1 + 1) * (T2 + 1) * sizeof(Real) +
-                           ((kind == 1 || kind == 2) ? size_t(16) * T0 * T1 * T2 : 0);
+        constexpr int BS = pyro_tile_threads(T0 * T1 * T2);
+        const size_t lds = pyro_tile_lds(T0, T1, T2, int(sizeof(Real)), kind == 1 || kind == 2);
         const unsigned tiles = unsigned((st.L0 / (2 * T0)) * (st.L1 / (2 * T1)) * (st.L2 / (2 * T2)));
         const dim3 grid(tiles * unsigned(st.R));  // (replica, tile), XCD-chunked in the kernel
         return dispatch_pyro(J, [&](auto shells) -> hipError_t {
             constexpr int SH = decltype(shells)::value;
-            {
-                switch (kind) {
+            switch (kind) {
 #define PYRO_TILED_LAUNCH(K)                                                                      \
     case K: {                                                                                   \
-        auto* fn = pyro_tiled_kernel<Real, SH, K, T0, T1, T2>;                                  \
+        auto* fn = pyro_tiled_kernel<Real, SH, K, T0, T1, T2, BS>;                              \
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),                            \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));        \
-        hipLaunchKernelGGL(fn, grid, dim3(256), lds, s, st, J, phase, n_or, pass_index);        \
+        hipLaunchKernelGGL(fn, grid, dim3(BS), lds, s, st, J, phase, n_or, pass_index);         \
         break;                                                                                  \
     }
-                    PYRO_TILED_LAUNCH(0) PYRO_TILED_LAUNCH(1) PYRO_TILED_LAUNCH(2)
+                PYRO_TILED_LAUNCH(0) PYRO_TILED_LAUNCH(1) PYRO_TILED_LAUNCH(2)
 #undef PYRO_TILED_LAUNCH
-                    default: return hipErrorInvalidValue;
-                }
-                const hipError_t e = hipGetLastError();
-                if (e != hipSuccess) return e;
+                default: return hipErrorInvalidValue;
             }
-            return hipSuccess;
+            return hipGetLastError();
         });
     };
+    using I4 = std::integral_constant<int, 4>;
+    using I6 = std::integral_constant<int, 6>;
+    using I8 = std::integral_constant<int, 8>;
     switch (tile_kind) {
-        case 1: return go(std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{});
-        case 2: return go(std::integral_constant<int, 6>{}, std::integral_constant<int, 6>{}, std::integral_constant<int, 4>{});
+        case 1: return go(I4{}, I4{}, I4{});
+        case 2: return go(I6{}, I6{}, I4{});
+        case 3: return go(I8{}, I4{}, I4{});
+        case 4: return go(I4{}, I4{}, I8{});
+        case 5: return go(I8{}, I8{}, I4{});
         default: return hipErrorInvalidValue;
     }
 }
