@@ -484,3 +484,50 @@ def test_tile_shape_rules(gpu):
     ctx.set_tile_shape(None)
     assert ctx.tile_shape() == (6, 6, 4)
     ctx.close()
+
+
+@pytest.mark.parametrize("L", [1, 2, (1, 2, 3), (2, 1, 4), (3, 2, 1)])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_small_and_rectangular_lattices(gpu, oracle, L, prec):
+    """Lattices of one or two cells along an axis: the bonds wrap onto the
+    same cells, so one neighbour can be reached by several displacements
+    (each counted, as the reference's shell lists count it).  Fields,
+    energy and one sweep of every kind against the oracle."""
+    J, Delta, B = (1.0, -0.3, 0.2), 0.6, (0.1, -0.2, 0.05)
+    ctx, (o,) = make(gpu, oracle, L, J, Delta, prec, B=B, seed=77)
+    S = random_spins(ctx.N, np.random.default_rng(11))
+    ctx.set_spins(S)
+    o.set_spins(S)
+    scale = field_scale(J)
+    assert np.max(np.abs(ctx.local_fields() - o.local_fields())) <= TOL[prec] * scale
+    assert abs(ctx.energy_per_cell()[0] - o.energy_per_cell()) <= TOL[prec] * 16 * 0.5 * field_scale(J, B)
+    if prec == 64:
+        ctx.set_strategy(3)
+        ctx.set_temperatures([0.7])
+        colours = ctx.colouring()
+        n_col = ctx.info()["n_colours"]
+        for kind, n_or in [(0, 1), (1, 1), (2, 0)]:
+            p0 = ctx.pass_count()
+            acc = ctx.sweep(kind, n_or, 1)
+            want = _oracle_sweeps(o, 0, kind, n_or, 1, 0.7, colours, n_col, 77, p0)
+            assert int(acc[0]) == want
+            assert np.max(np.abs(ctx.get_spins(0) - o.get_spins())) < 1e-12
+
+
+def test_invalid_contexts_rejected(gpu):
+    with pytest.raises(gpu.PyroError):
+        gpu.Context.pyrochlore(0)
+    with pytest.raises(gpu.PyroError):
+        gpu.Context.pyrochlore(2, n_replicas=0)
+    with pytest.raises(gpu.PyroError):
+        gpu.Context.pyrochlore(2, precision=16)
+    ctx = gpu.Context.pyrochlore(2)
+    with pytest.raises(gpu.PyroError):
+        ctx.sweep(gpu.METROPOLIS, 1, 1)  # before setup
+    ctx.define_J1J2J3(1.0)
+    ctx.setup()
+    with pytest.raises(gpu.PyroError):
+        ctx.sweep(7, 1, 1)
+    with pytest.raises(gpu.PyroError):
+        ctx.set_temperatures([-1.0])
+    ctx.close()
