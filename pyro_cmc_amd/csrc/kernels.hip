@@ -894,6 +894,65 @@ hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Whole 3-D DFT of one (replica, component, sublattice) field in LDS for
+// cubic lattices of edge L = 4, 8, 12 or 16.  The field (L^3 reals) is read
+// once, transformed along axis 2, 1, 0 in place and written once as
+// F[r][a][mu][K], the layout of the per-axis path.  A thread owns one line
+// per pass: it keeps the line's L values in registers and forms all L
+// outputs with compile-time twiddle indices (2 L^2 independent FMA chains),
+// then writes them back over its own line, so lines never overlap.
+template <typename Real, int L>
+__global__ __launch_bounds__(kBlock) void dft3_kernel(const Real* __restrict__ spins, Real* __restrict__ F,
+                                                      const Real* __restrict__ tw, int n_sl) {
+    constexpr int NC = L * L * L, LINES = L * L;
+    __shared__ Real x[2 * NC];
+    __shared__ Real w[2 * L];
+    const size_t f_off = (size_t(blockIdx.y) * 3 * n_sl + size_t(blockIdx.x)) * size_t(NC);
+    const Real* src = spins + f_off;
+    for (int i = threadIdx.x; i < NC; i += kBlock) {
+        x[2 * i] = src[i];
+        x[2 * i + 1] = Real(0);
+    }
+    if (threadIdx.x < 2 * L) w[threadIdx.x] = tw[threadIdx.x];
+    __syncthreads();
+#pragma unroll
+    for (int pass = 0; pass < 3; ++pass) {
+        const int sa = pass == 0 ? 1 : (pass == 1 ? L : L * L);
+        for (int line = threadIdx.x; line < LINES; line += kBlock) {
+            const int base = (line / sa) * (sa * L) + line % sa;
+            Real xr[L], xi[L], yr[L], yi[L];
+#pragma unroll
+            for (int j = 0; j < L; ++j) {
+                xr[j] = x[2 * (base + j * sa)];
+                xi[j] = x[2 * (base + j * sa) + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                yr[k] = xr[0];
+                yi[k] = xi[0];
+            }
+#pragma unroll
+            for (int j = 1; j < L; ++j) {
+#pragma unroll
+                for (int k = 0; k < L; ++k) {
+                    const int t = (j * k) % L;
+                    const Real cr = w[2 * t], ci = w[2 * t + 1];
+                    yr[k] += xr[j] * cr - xi[j] * ci;
+                    yi[k] += xr[j] * ci + xi[j] * cr;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                x[2 * (base + k * sa)] = yr[k];
+                x[2 * (base + k * sa) + 1] = yi[k];
+            }
+        }
+        __syncthreads();
+    }
+    Real* dst = F + 2 * f_off;
+    for (int i = threadIdx.x; i < 2 * NC; i += kBlock) dst[i] = x[i];
+}
+
 // One axis of the per-sublattice DFT: the LDS line kernel for La <= 64,
 // the direct kernel otherwise.
 template <typename Real, bool REAL_IN>
@@ -947,6 +1006,21 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
     const size_t nc = size_t(st.n_cells);
     const long long total = (long long)r_count * 3 * st.n_sl * (long long)nc;
     const Real* spins = st.S + size_t(r_begin) * 3 * st.n_sl * nc;
+    if (st.L0 == st.L1 && st.L1 == st.L2) {
+        const dim3 grid(unsigned(3 * st.n_sl), unsigned(r_count));
+        auto go = [&](auto l) -> hipError_t {
+            hipLaunchKernelGGL((dft3_kernel<Real, decltype(l)::value>), grid, dim3(kBlock), 0, s, spins, st.F,
+                               st.tw0, st.n_sl);
+            return hipGetLastError();
+        };
+        switch (st.L0) {
+            case 4: return go(std::integral_constant<int, 4>{});
+            case 8: return go(std::integral_constant<int, 8>{});
+            case 12: return go(std::integral_constant<int, 12>{});
+            case 16: return go(std::integral_constant<int, 16>{});
+            default: break;
+        }
+    }
     // axis 2 (stride 1): spins -> F; axis 1 (stride L2): F -> G; axis 0: G -> F
     hipError_t e = dft_axis<Real, true>(spins, st.F, st.tw2, st.L2, 1, total, s);
     if (e == hipSuccess) e = dft_axis<Real, false>(st.F, st.G, st.tw1, st.L1, st.L2, total, s);

@@ -336,10 +336,12 @@ def test_ssf_contracted(gpu, oracle, prec):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-@pytest.mark.parametrize("L", [5, 12])
+@pytest.mark.parametrize("L", [4, 5, 8, 12, 16])
 def test_spin_ft_against_direct_dft(gpu, oracle, prec, L):
-    """Per-sublattice DFT (save_ft_spin_state layout) at line lengths that
-    pad the FP32 GEMM tiles (5) and at the tempering lattice (12)."""
+    """Per-sublattice DFT (save_ft_spin_state layout): through the per-axis
+    path at line lengths that pad the FP32 GEMM tiles (5), and through the
+    fused whole-field kernel on the cubic edges it is built for (4, 8, 12,
+    16; 12 is the tempering lattice)."""
     ctx, os_ = make(gpu, oracle, L, prec=prec)
     S = random_spins(ctx.N, np.random.default_rng(L))
     ctx.set_spins(S, 0)
