@@ -88,7 +88,9 @@ __device__ __forceinline__ CellAxes cell_axes_of(int cell, int L0, int L1, int L
 
 // Field of a site of sublattice SL; fetch(n, x, y, z) loads the spin of
 // stencil neighbour n (compile-time target sublattice and cell offset).
-template <typename Real, int SHELLS, int SL, typename Fetch>
+// FWD: only the neighbours on higher sublattices (no bond joins equal
+// sublattices), so summing S_i . h_i over all sites counts every bond once.
+template <typename Real, int SHELLS, int SL, bool FWD = false, typename Fetch>
 __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplings<Real>& J, Real& hx,
                                                 Real& hy, Real& hz) {
     constexpr int NS = pyrostencil::slots_for(SHELLS);
@@ -98,15 +100,18 @@ __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplin
         constexpr int k = decltype(kk)::value;
         constexpr pyrostencil::Nbr n = kStencil.n[SL][k];
         constexpr int shell = k < 6 ? 0 : (k < 18 ? 1 : 2);
-        Real x, y, z;
-        fetch(n, x, y, z);
-        s[shell][0] += x;
-        s[shell][1] += y;
-        s[shell][2] += z;
-        if constexpr (shell == 0) {
-            constexpr int tp = n.sl % 4;
-            zs += Real(pyrostencil::kLink[tp][0]) * x + Real(pyrostencil::kLink[tp][1]) * y +
-                  Real(pyrostencil::kLink[tp][2]) * z;
+        static_assert(n.sl != SL, "no bond joins a sublattice to itself");
+        if constexpr (!FWD || n.sl > SL) {
+            Real x, y, z;
+            fetch(n, x, y, z);
+            s[shell][0] += x;
+            s[shell][1] += y;
+            s[shell][2] += z;
+            if constexpr (shell == 0) {
+                constexpr int tp = n.sl % 4;
+                zs += Real(pyrostencil::kLink[tp][0]) * x + Real(pyrostencil::kLink[tp][1]) * y +
+                      Real(pyrostencil::kLink[tp][2]) * z;
+            }
         }
     });
     hx = J.J1 * s[0][0];
@@ -214,8 +219,10 @@ __global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
 }
 
-// Energy terms 1/2 S.h - B.S per site (MC.cpp:305-312) with the stencil
-// field; one partial sum per 256-site block and replica.
+// Energy terms per site (MC.cpp:305-312) with the stencil: S_i . h_i over
+// the bonds to higher sublattices (each bond once, the same sum as
+// 1/2 sum_i S_i . h_i with half the gathers) minus B . S_i; one partial sum
+// per 256-site block and replica.
 template <typename Real, int SHELLS>
 __global__ __launch_bounds__(256) void pyro_energy_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                           double* __restrict__ partial) {
@@ -230,12 +237,28 @@ __global__ __launch_bounds__(256) void pyro_energy_kernel(DeviceState<Real> st, 
         const Real* Sz = Sy + N;
         const int sl = i / nc;
         const CellAxes c = cell_axes_of(i - sl * nc, st.L0, st.L1, st.L2);
-        Real hx, hy, hz;
-        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, J.at(r), hx, hy, hz);
+        const PyroCouplings<Real> Jr = J.at(r);
+        auto fetch = [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
+            const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
+            x = ld32(Sx, j);
+            y = ld32(Sy, j);
+            z = ld32(Sz, j);
+        };
+        Real hx = Real(0), hy = Real(0), hz = Real(0);
+        switch (sl) {
+#define PYRO_SL_CASE(X) \
+    case X: pyro_field_with<Real, SHELLS, X, true>(fetch, Jr, hx, hy, hz); break;
+            PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+            PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+            PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+            PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+            default: break;
+        }
         const Real sx = Sx[i], sy = Sy[i], sz = Sz[i];
-        const Real half = Real(0.5) * (sx * hx + sy * hy + sz * hz);
+        const Real bond = sx * hx + sy * hy + sz * hz;
         const Real bs = sx * st.B[3 * r] + sy * st.B[3 * r + 1] + sz * st.B[3 * r + 2];
-        e = double(half) - double(bs);
+        e = double(bond) - double(bs);
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
     __shared__ double red[4];
