@@ -174,7 +174,6 @@ struct pyro_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     hipEvent_t marks[2] = {nullptr, nullptr};
-
     ~pyro_ctx() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (agexec) (void)hipGraphExecDestroy(agexec);
@@ -519,7 +518,9 @@ struct pyro_ctx {
     }
 
     // SSF sample of replicas [0, R) using the device slot array `slots`.
-    void ssf_sample_device(const int32_t* slots) {
+    // SSF sample on stream s (kernel timing records on the context stream,
+    // so s is the context stream whenever timing is on).
+    void ssf_sample_device(const int32_t* slots, hipStream_t s) {
         for (int r0 = 0; r0 < R; r0 += ssf_chunk) {
             const int rc = std::min(ssf_chunk, R - r0);
             pyro::CorrArgs c;
@@ -532,14 +533,14 @@ struct pyro_ctx {
             c.n_corr = n_corr;
             if (prec == 64) {
                 const auto st = ssf_state<double>();
-                timed(kFamDft, [&] { return pyro::launch_dft<double>(st, r0, rc, stream); });
-                timed(kFamContract, [&] { return pyro::launch_contract<double>(st, c, r0, rc, stream); });
-                if (munu) timed(kFamMunu, [&] { return pyro::launch_munu_stage<double>(st, slots, r0, rc, munu_args(), stream); });
+                timed(kFamDft, [&] { return pyro::launch_dft<double>(st, r0, rc, s); });
+                timed(kFamContract, [&] { return pyro::launch_contract<double>(st, c, r0, rc, s); });
+                if (munu) timed(kFamMunu, [&] { return pyro::launch_munu_stage<double>(st, slots, r0, rc, munu_args(), s); });
             } else {
                 const auto st = ssf_state<float>();
-                timed(kFamDft, [&] { return pyro::launch_dft<float>(st, r0, rc, stream); });
-                timed(kFamContract, [&] { return pyro::launch_contract<float>(st, c, r0, rc, stream); });
-                if (munu) timed(kFamMunu, [&] { return pyro::launch_munu_stage<float>(st, slots, r0, rc, munu_args(), stream); });
+                timed(kFamDft, [&] { return pyro::launch_dft<float>(st, r0, rc, s); });
+                timed(kFamContract, [&] { return pyro::launch_contract<float>(st, c, r0, rc, s); });
+                if (munu) timed(kFamMunu, [&] { return pyro::launch_munu_stage<float>(st, slots, r0, rc, munu_args(), s); });
             }
         }
     }
@@ -1205,13 +1206,16 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
 
         auto round_body = [&]() {
             if (n_sweep > 0) ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull);
+            // One stream: running the SSF branch on a second stream beside the
+            // energy (fork and join by events inside the captured round) made
+            // each round 0.19 ms slower, more than the branch itself takes.
             if (sample_ssf) {
                 if (ctx->has_ssf_map) {
                     HIPCHK(pyro::launch_map_slots(ctx->slot_of.as<int32_t>(), ctx->ssf_map.as<int32_t>(),
                                                   ctx->ssf_slot.as<int32_t>(), ctx->R, ctx->stream));
-                    ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>());
+                    ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>(), ctx->stream);
                 } else {
-                    ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+                    ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
                 }
             }
             ctx->energy(true);
@@ -1360,7 +1364,7 @@ int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int
             HIPCHK(pyro::launch_anneal_set_T(ctx->T_slot.as<double>(), ctx->R, ctx->sched.as<double>(), ctx->n_sched,
                                              ctr + 2, ctx->stream));
             if (n_sweep > 0) ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull);
-            if (sample_ssf) ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+            if (sample_ssf) ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
             ctx->energy(false);
             HIPCHK(pyro::launch_anneal_record(ctx->e.as<double>(), ctx->R, ctx->ehist.as<double>(), ctx->n_sched,
                                               ctr + 2, ctr, ppr, ctx->stream));
@@ -1547,7 +1551,7 @@ int pyro_ssf_sample(pyro_ctx* ctx, const int32_t* slot_of_replica) {
         require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
         if (!slot_of_replica) {
             require(ctx->n_slots >= ctx->R, PYRO_E_INVALID, "tempering slots exceed the accumulator slots");
-            ctx->ssf_sample_device(ctx->slot_of.as<int32_t>());
+            ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
             if (ctx->munu) {
                 for (auto& c : ctx->host_munu_count) ++c;
                 if (++ctx->munu_pending >= ctx->munu_B) ctx->munu_flush();
@@ -1581,7 +1585,7 @@ int pyro_ssf_sample(pyro_ctx* ctx, const int32_t* slot_of_replica) {
             }
             HIPCHK(hipMemcpyAsync(ctx->ssf_slot.p, sl.data(), sizeof(int32_t) * ctx->R, hipMemcpyHostToDevice,
                                   ctx->stream));
-            ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>());
+            ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>(), ctx->stream);
             if (ctx->munu)
                 for (int r = 0; r < ctx->R; ++r)
                     if (sl[r] >= 0) ctx->host_munu_count[sl[r]] += 1;
