@@ -191,17 +191,19 @@ __device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sx, const Real
 // blocks), grid.y = replicas.  PER_REP: per-replica couplings (a batch of
 // models); the single-model instance carries no branch on them, which
 // would otherwise sit ahead of the neighbour gathers.
-template <typename Real, int SHELLS, int KIND, bool PER_REP>
-__global__ __launch_bounds__(256) void pyro_colour_kernel(DeviceState<Real> st,
+constexpr int kPyroColourThreads = 64;
+
+template <typename Real, int SHELLS, int KIND, bool PER_REP, int BS>
+__global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
                                                           PyroCouplings<Real> J, int colour,
                                                           unsigned long long pass_index) {
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     const int nc = st.n_cells;
-    const int bps = (nc + 255) / 256;
+    const int bps = (nc + BS - 1) / BS;
     unsigned bx, ru;
     xcd_split(blockIdx.x, gridDim.x / unsigned(st.R), unsigned(st.R), bx, ru);
     const int q = int(bx) / bps;
-    const int cell = (int(bx) % bps) * 256 + threadIdx.x;
+    const int cell = (int(bx) % bps) * BS + threadIdx.x;
     const int sl = NCOL == 16 ? colour : colour + 4 * q;
     const int r = int(ru);
     const unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
@@ -569,17 +571,21 @@ template <typename Real>
 hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
                               int kind, unsigned long long pass_index, hipStream_t s) {
     const int ncol = J.shells == 3 ? 16 : 4;
-    const int bps = (st.n_cells + 255) / 256;
+    // One wave per workgroup: measured on the L=12 x 64 tempering lattice,
+    // 64-thread workgroups take 5.17 us per colour launch against 5.39 us
+    // for 128, 256 or 512 threads (same sites, same results).
+    constexpr int BS = kPyroColourThreads;
+    const int bps = (st.n_cells + BS - 1) / BS;
     const dim3 grid(unsigned(bps * (16 / ncol)) * unsigned(st.R));  // replica-major, XCD-aware (xcd_split)
     auto go = [&](auto per_rep) -> hipError_t {
         constexpr bool PR = decltype(per_rep)::value;
         return dispatch_pyro(J, [&](auto shells) -> hipError_t {
             constexpr int SH = decltype(shells)::value;
             switch (kind) {
-                case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
-                case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
-                case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
-                case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3, PR>), grid, dim3(256), 0, s, st, J, colour, pass_index); break;
+                case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
+                case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
+                case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
+                case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
                 default: return hipErrorInvalidValue;
             }
             return hipGetLastError();
