@@ -504,7 +504,7 @@ struct pyro_ctx {
                                                              epart.as<double>(), stream);
             });
             timed(kFamEnergy, [&] {
-                return pyro::launch_energy_finish(epart.as<double>(), pyro::energy_blocks(int(N())), int(nc()),
+                return pyro::launch_energy_finish(epart.as<double>(), pyro::pyro_energy_blocks(int(N())), int(nc()),
                                                   e.as<double>(), slot_of.as<int32_t>(), es, e2, ne, R, stream);
             });
             return;
@@ -846,7 +846,8 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
             ctx->acc_wave.alloc(sizeof(unsigned int) * size_t(R) * size_t(ctx->acc_waves));
         }
         ctx->counters.alloc(sizeof(unsigned long long) * 3);  // pass base, round, anneal step
-        ctx->epart.alloc(sizeof(double) * R * pyro::energy_blocks(int(N)));
+        ctx->epart.alloc(sizeof(double) * R *
+                         std::max(pyro::energy_blocks(int(N)), pyro::pyro_energy_blocks(int(N))));
         ctx->e.alloc(sizeof(double) * R);
         ctx->Esum.alloc(sizeof(double) * R);
         ctx->E2sum.alloc(sizeof(double) * R);

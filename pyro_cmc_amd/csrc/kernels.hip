@@ -552,10 +552,12 @@ __global__ __launch_bounds__(256) void dft_mfma_kernel(const float* __restrict__
 
 // Contracted correlators (ssf_manager.hpp:260-293 with the separable phase
 // fold): b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K); S_c = conj(b^alpha) b^beta.
+constexpr int kContractBlock = 64;  // one wave per workgroup
+
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void contract_kernel(SsfState<Real> st, CorrArgs c,
-                                                          int r_begin) {
-    const int K = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kContractBlock) void contract_kernel(SsfState<Real> st, CorrArgs c,
+                                                                  int r_begin) {
+    const int K = blockIdx.x * kContractBlock + threadIdx.x;
     const int rl = blockIdx.y;
     const int r = r_begin + rl;
     const int slot = c.slot_of[r];
@@ -901,15 +903,20 @@ hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
 // per pass: it keeps the line's L values in registers and forms all L
 // outputs with compile-time twiddle indices (2 L^2 independent FMA chains),
 // then writes them back over its own line, so lines never overlap.
+// a thread per line: L^2 lines rounded up to whole waves
+template <int L>
+constexpr int dft3_threads() { return (L * L + 63) / 64 * 64; }
+
 template <typename Real, int L>
-__global__ __launch_bounds__(kBlock) void dft3_kernel(const Real* __restrict__ spins, Real* __restrict__ F,
-                                                      const Real* __restrict__ tw, int n_sl) {
-    constexpr int NC = L * L * L, LINES = L * L;
+__global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __restrict__ spins,
+                                                                 Real* __restrict__ F,
+                                                                 const Real* __restrict__ tw, int n_sl) {
+    constexpr int NC = L * L * L, LINES = L * L, BS = dft3_threads<L>();
     __shared__ Real x[2 * NC];
     __shared__ Real w[2 * L];
     const size_t f_off = (size_t(blockIdx.y) * 3 * n_sl + size_t(blockIdx.x)) * size_t(NC);
     const Real* src = spins + f_off;
-    for (int i = threadIdx.x; i < NC; i += kBlock) {
+    for (int i = threadIdx.x; i < NC; i += BS) {
         x[2 * i] = src[i];
         x[2 * i + 1] = Real(0);
     }
@@ -918,7 +925,7 @@ __global__ __launch_bounds__(kBlock) void dft3_kernel(const Real* __restrict__ s
 #pragma unroll
     for (int pass = 0; pass < 3; ++pass) {
         const int sa = pass == 0 ? 1 : (pass == 1 ? L : L * L);
-        for (int line = threadIdx.x; line < LINES; line += kBlock) {
+        for (int line = threadIdx.x; line < LINES; line += BS) {
             const int base = (line / sa) * (sa * L) + line % sa;
             Real xr[L], xi[L], yr[L], yi[L];
 #pragma unroll
@@ -950,7 +957,7 @@ __global__ __launch_bounds__(kBlock) void dft3_kernel(const Real* __restrict__ s
         __syncthreads();
     }
     Real* dst = F + 2 * f_off;
-    for (int i = threadIdx.x; i < 2 * NC; i += kBlock) dst[i] = x[i];
+    for (int i = threadIdx.x; i < 2 * NC; i += BS) dst[i] = x[i];
 }
 
 // One axis of the per-sublattice DFT: the LDS line kernel for La <= 64,
@@ -1009,7 +1016,8 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
     if (st.L0 == st.L1 && st.L1 == st.L2) {
         const dim3 grid(unsigned(3 * st.n_sl), unsigned(r_count));
         auto go = [&](auto l) -> hipError_t {
-            hipLaunchKernelGGL((dft3_kernel<Real, decltype(l)::value>), grid, dim3(kBlock), 0, s, spins, st.F,
+            constexpr int LL = decltype(l)::value;
+            hipLaunchKernelGGL((dft3_kernel<Real, LL>), grid, dim3(dft3_threads<LL>()), 0, s, spins, st.F,
                                st.tw0, st.n_sl);
             return hipGetLastError();
         };
@@ -1031,8 +1039,8 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
 template <typename Real>
 hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_begin, int r_count,
                            hipStream_t s) {
-    const dim3 grid(unsigned((st.n_cells + kBlock - 1) / kBlock), unsigned(r_count));
-    hipLaunchKernelGGL(contract_kernel<Real>, grid, dim3(kBlock), 0, s, st, c, r_begin);
+    const dim3 grid(unsigned((st.n_cells + kContractBlock - 1) / kContractBlock), unsigned(r_count));
+    hipLaunchKernelGGL(contract_kernel<Real>, grid, dim3(kContractBlock), 0, s, st, c, r_begin);
     return hipGetLastError();
 }
 

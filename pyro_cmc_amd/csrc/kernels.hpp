@@ -131,6 +131,8 @@ template <typename Real>
 hipError_t launch_energy(const DeviceState<Real>& st, double* partial, double* e,
                          double* Esum, double* E2sum, unsigned long long* nE, hipStream_t s);
 int energy_blocks(int N);
+// Partials per replica of launch_pyro_energy (64-site blocks).
+inline int pyro_energy_blocks(int N) { return (N + 63) / 64; }
 // Stencil form of the partial sums for contexts with pyrochlore J1-J2-J3
 // couplings: the same partial[R][energy_blocks(N)] layout, no table read.
 template <typename Real>

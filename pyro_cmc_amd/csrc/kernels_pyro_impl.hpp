@@ -224,11 +224,11 @@ __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
 // Energy terms per site (MC.cpp:305-312) with the stencil: S_i . h_i over
 // the bonds to higher sublattices (each bond once, the same sum as
 // 1/2 sum_i S_i . h_i with half the gathers) minus B . S_i; one partial sum
-// per 256-site block and replica.
+// per 64-site (one-wave) block and replica.
 template <typename Real, int SHELLS>
-__global__ __launch_bounds__(256) void pyro_energy_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
-                                                          double* __restrict__ partial) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(64) void pyro_energy_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
+                                                         double* __restrict__ partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
     const int r = blockIdx.y;
     double e = 0.0;
     if (i < st.N) {
@@ -263,10 +263,7 @@ __global__ __launch_bounds__(256) void pyro_energy_kernel(DeviceState<Real> st, 
         e = double(bond) - double(bs);
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
-    __shared__ double red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[size_t(r) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) partial[size_t(r) * gridDim.x + blockIdx.x] = e;
 }
 
 // Local field of every site of one replica (MC.cpp:87-96), out[3][N].
@@ -597,10 +594,10 @@ hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<R
 template <typename Real>
 hipError_t launch_pyro_energy(const DeviceState<Real>& st, const PyroCouplings<Real>& J, double* partial,
                               hipStream_t s) {
-    const dim3 grid(unsigned(energy_blocks(st.N)), unsigned(st.R));
+    const dim3 grid(unsigned(pyro_energy_blocks(st.N)), unsigned(st.R));
     return dispatch_pyro(J, [&](auto shells) -> hipError_t {
         constexpr int SH = decltype(shells)::value;
-        hipLaunchKernelGGL((pyro_energy_kernel<Real, SH>), grid, dim3(256), 0, s, st, J, partial);
+        hipLaunchKernelGGL((pyro_energy_kernel<Real, SH>), grid, dim3(64), 0, s, st, J, partial);
         return hipGetLastError();
     });
 }
