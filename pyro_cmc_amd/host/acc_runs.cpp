@@ -11,6 +11,10 @@
 //     every file holds static_corr_2), with the per-seed sample count taken
 //     as n_samples / K; /geometry and the /ssf metadata copied from the
 //     first file, checked for agreement across files;
+//   * the readers' derived quantities, written beside the sums: /energy/C =
+//     var / T^2 per temperature (plot_ssf.py:181-186) and /ssf/S_mean =
+//     Re static_corr / n_samples, the sample-normalised contracted S(q)
+//     [n_corr][n_T][n_k] (plot_ssf.py:42-57);
 //   * the output is <name with seed=<hex>_ replaced by mergeK_>.avg.h5 in the
 //     directory of the first file.
 // Pre-existing .avg.h5 files among the inputs are listed and deleted only
@@ -292,11 +296,12 @@ std::string merge_group(const std::vector<std::string>& files_in) {
         }
     }
 
-    Array e_mean = E, e2_mean = E2, var = E;
+    Array e_mean = E, e2_mean = E2, var = E, heat = E;
     for (size_t i = 0; i < E.size(); ++i) {
         e_mean.v[i] = E.v[i] / n.v[i];
         e2_mean.v[i] = E2.v[i] / n.v[i];
         var.v[i] = e2_mean.v[i] - e_mean.v[i] * e_mean.v[i];
+        heat.v[i] = var.v[i] / (T.v[i] * T.v[i]);
     }
     // per-seed sample count n_T / K broadcast along axis 1 of [n_corr, n_T, ...]
     if (C.dims.size() < 2 || C.dims[1] != ns.size()) throw std::runtime_error("static_corr has no T axis");
@@ -304,6 +309,14 @@ std::string merge_group(const std::vector<std::string>& files_in) {
     for (size_t d = 2; d < C.dims.size(); ++d) inner *= size_t(C.dims[d]);
     const size_t nT = ns.size();
     Array var_inter = C, var_intra = C;
+    // S_mean: the real parts of static_corr over the per-T sample count
+    Array s_mean;
+    s_mean.dims.assign(C.dims.begin(), C.dims.end() - 1);
+    s_mean.v.resize(C.size() / 2);
+    for (size_t i = 0; i < s_mean.v.size(); ++i) {
+        const size_t t = (i / (inner / 2)) % nT;
+        s_mean.v[i] = C.v[2 * i] / ns.v[t];
+    }
     for (size_t i = 0; i < C.size(); ++i) {
         const size_t t = (i / inner) % nT;
         const double nps = ns.v[t] / double(K);
@@ -322,6 +335,7 @@ std::string merge_group(const std::vector<std::string>& files_in) {
     write_d(g, "e_mean", e_mean);
     write_d(g, "e2_mean", e2_mean);
     write_d(g, "var", var);
+    write_d(g, "C", heat);
     const unsigned long long kk = K;
     write(g, "n_seeds", H5T_NATIVE_ULLONG, {}, &kk);
     H5Gclose(g);
@@ -337,6 +351,7 @@ std::string merge_group(const std::vector<std::string>& files_in) {
     write_d(g, "static_corr", C);
     write_d(g, "static_corr_2", C2);
     write_d(g, "var_inter", var_inter);
+    write_d(g, "S_mean", s_mean);
     if (have_intra) write_d(g, "var_intra", var_intra);
     std::vector<long long> sp(sl_pos.v.begin(), sl_pos.v.end());
     write(g, "sl_positions", H5T_NATIVE_LLONG, sl_pos.dims, sp.data());

@@ -70,11 +70,13 @@ def test_merge_against_per_sample_statistics(pyro, tmp_path):
         assert int(f.read("/energy/n_seeds")) == K
         np.testing.assert_allclose(f.read("/energy/e_mean"), energies.mean(0), rtol=1e-13)
         np.testing.assert_allclose(f.read("/energy/var"), energies.var(0), rtol=1e-10)
+        np.testing.assert_allclose(f.read("/energy/C"), energies.var(0) / T_LIST ** 2, rtol=1e-10)
         per_seed_mean = samples.mean(1)                          # [K][corr][T][k][2]
         np.testing.assert_allclose(f.read("/ssf/static_corr"), samples.sum((0, 1)), rtol=1e-13)
         np.testing.assert_allclose(f.read("/ssf/static_corr_2"), (samples.sum(1) ** 2).sum(0), rtol=1e-13)
         np.testing.assert_array_equal(f.read("/ssf/n_samples"), [K * n_s] * 2)
         np.testing.assert_allclose(f.read("/ssf/var_inter"), per_seed_mean.var(0), rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(f.read("/ssf/S_mean"), samples.sum((0, 1))[..., 0] / (K * n_s), rtol=1e-12)
         np.testing.assert_allclose(f.read("/ssf/var_intra"), samples.var(1).sum(0), rtol=1e-9, atol=1e-12)
         assert f.read("/ssf/corr_lookup") == LABELS
         np.testing.assert_array_equal(f.attr("/ssf", "k_dims"), [L, L, L])
