@@ -122,14 +122,16 @@ def large_lattice_sweep(pyro, sweeps=6):
     return updates / (ms / 1e3), sweep_ms / launches, upd_per_launch, strategy
 
 
-def munu_correlators(pyro, samples=128, buffer=64):
+def munu_correlators(pyro, samples=384, buffer=192):
     """Sublattice-resolved correlators of the large tempering lattice:
     L = 48 x 16 replicas, FP32 spins, all 9 (alpha, beta) pairs over the 16
     sublattices (the 48 x 48 Hermitian matrix per k), one accumulator slot
     per replica.  Samples are staged as FP32 rows X = [Re | Im] of the 48
     transformed fields, `buffer` deep, and flushed as X^T X on FP32 MFMA
-    into FP64 sums (tile pairs I <= J of the Hermitian matrix).  Returns the
-    flush kernel's timing and work."""
+    into FP64 sums (tile pairs I <= J of the Hermitian matrix).  192 rows
+    per flush: 130 GB of staging beside the 65 GB of FP64 accumulators, so
+    the accumulator round trip is spread over three times as many samples as
+    at 64.  Returns the flush kernel's timing and work."""
     R, L = 16, 48
     ctx = pyro.Context.pyrochlore(L, n_replicas=R, precision=pyro.FP32, seed=48)
     ctx.define_J1J2J3(1.0, 0.1, 0.1)
