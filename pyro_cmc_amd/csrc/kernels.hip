@@ -458,41 +458,71 @@ __global__ __launch_bounds__(256) void dft_mfma_kernel(const float* __restrict__
     float* wtr = xi + Kp * XS;                       // [Kp][P]  W^T, real part
     float* wti = wtr + Kp * P;                       // [Kp][P]  W^T, imaginary part
     const unsigned c0 = blockIdx.x * 64u;  // launcher keeps 2 * La * n_cols < 2^31
-    for (int e = threadIdx.x; e < Kp * P; e += 256) {
-        const int n = e / P, K = e - n * P;
-        float c = 0.f, d = 0.f;
-        if (n < La && K < La) {
-            const int j = (K * n) % La;
-            c = tw[2 * j];
-            d = tw[2 * j + 1];
-        }
-        wtr[e] = c;
-        wti[e] = d;
-    }
-    for (int e = threadIdx.x; e < Kp * 64; e += 256) {
-        int j, n;
-        unsigned a;
-        if (s == 1) {
-            j = e / Kp;
-            n = e - j * Kp;
-            a = (c0 + j) * La + n;
-        } else {
-            n = e >> 6;
-            j = e & 63;
-            const unsigned c = c0 + j, o = c / unsigned(s);
-            a = (o * La + n) * unsigned(s) + (c - o * unsigned(s));
-        }
-        float re = 0.f, im = 0.f;
-        if (n < La && c0 + j < n_cols) {
-            if constexpr (REAL_IN) {
-                re = in[a];
-            } else {
-                re = in[2 * a];
-                im = in[2 * a + 1];
+    // Staging in two phases over a fixed 16 rounds (Kp, P <= 64): every
+    // round's global load is issued before the first LDS store waits for
+    // one (a loop that stored each value as it arrived made the kernel
+    // latency-bound).
+    float vr[16], vi[16];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        vr[it] = 0.f;
+        vi[it] = 0.f;
+        if (e < Kp * P) {
+            const int n = e / P, K = e - n * P;
+            if (n < La && K < La) {
+                const int j = (K * n) % La;
+                vr[it] = tw[2 * j];
+                vi[it] = tw[2 * j + 1];
             }
         }
-        xr[n * XS + j] = re;
-        xi[n * XS + j] = im;
+    }
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        if (e < Kp * P) {
+            wtr[e] = vr[it];
+            wti[e] = vi[it];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        vr[it] = 0.f;
+        vi[it] = 0.f;
+        if (e < Kp * 64) {
+            int j, n;
+            unsigned a;
+            if (s == 1) {
+                j = e / Kp;
+                n = e - j * Kp;
+                a = (c0 + j) * La + n;
+            } else {
+                n = e >> 6;
+                j = e & 63;
+                const unsigned c = c0 + j, o = c / unsigned(s);
+                a = (o * La + n) * unsigned(s) + (c - o * unsigned(s));
+            }
+            if (n < La && c0 + j < n_cols) {
+                if constexpr (REAL_IN) {
+                    vr[it] = in[a];
+                } else {
+                    const float2 v = reinterpret_cast<const float2*>(in)[a];  // one 8-byte access
+                    vr[it] = v.x;
+                    vi[it] = v.y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        if (e < Kp * 64) {
+            const int j = s == 1 ? e / Kp : (e & 63);
+            const int n = s == 1 ? e - j * Kp : (e >> 6);
+            xr[n * XS + j] = vr[it];
+            xi[n * XS + j] = vi[it];
+        }
     }
     __syncthreads();
 
@@ -543,10 +573,7 @@ __global__ __launch_bounds__(256) void dft_mfma_kernel(const float* __restrict__
             const unsigned c = c0 + jj, o = c / unsigned(s);
             a = (o * La + K) * unsigned(s) + (c - o * unsigned(s));
         }
-        if (c0 + jj < n_cols) {
-            out[2 * a] = xr[K * XS + jj];
-            out[2 * a + 1] = xi[K * XS + jj];
-        }
+        if (c0 + jj < n_cols) reinterpret_cast<float2*>(out)[a] = float2{xr[K * XS + jj], xi[K * XS + jj]};
     }
 }
 
