@@ -13,9 +13,11 @@ namespace pyro {
 // FP64 uses the correctly rounded library functions (the reference's
 // arithmetic).  FP32 uses the hardware approximations: v_log_f32 / v_exp_f32
 // (base 2, ~1 ulp), v_sin_f32 / v_cos_f32, which take their argument in
-// revolutions, so sin(2 pi u) is one instruction, and v_rsq_f32.
+// revolutions, so sin(2 pi u) is one instruction, v_sqrt_f32, v_rsq_f32 and
+// v_rcp_f32 (about 1 ulp each, one instruction each instead of the
+// correctly rounded expansions).
 template <typename Real> __device__ __forceinline__ Real dsqrt(Real x);
-template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
 template <typename Real> __device__ __forceinline__ Real dexp(Real x);
 template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
@@ -36,9 +38,9 @@ template <> __device__ __forceinline__ void dsincos_rev<float>(float u, float* s
 template <> __device__ __forceinline__ void dsincos_rev<double>(double u, double* s, double* c) {
     sincos(6.28318530717958647692 * u, s, c);
 }
-// a / b: fast reciprocal-multiply in FP32, a true division in FP64
+// a / b: reciprocal and multiply in FP32, a true division in FP64
 template <typename Real> __device__ __forceinline__ Real ddiv(Real a, Real b);
-template <> __device__ __forceinline__ float ddiv<float>(float a, float b) { return __fdividef(a, b); }
+template <> __device__ __forceinline__ float ddiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <> __device__ __forceinline__ double ddiv<double>(double a, double b) { return a / b; }
 // v <- v / |v|: v * rsq(|v|^2) in FP32, division by the square root in FP64
 template <typename Real>
@@ -74,14 +76,8 @@ __device__ __forceinline__ Real ld32(const Real* __restrict__ base, unsigned idx
 template <typename Real> struct Renorm { static constexpr bool value = false; };
 template <> struct Renorm<float> { static constexpr bool value = true; };
 
-// One single-site update of kind KIND of the spin (px, py, pz) at `site` of
-// replica r, pass index `pass`, given the local field (hx, hy, hz) without
-// the global field.  Returns whether the proposal was accepted (OR: always
-// false); `changed` tells whether the spin was modified.  dp points at the
-// site's lifted direction (read and flipped by the lifted kinds only).
 // The five 24-bit uniforms of (site, replica, pass): one Philox4x32-10
-// block.  They do not depend on the field, so kernels draw them next to the
-// neighbour gathers, where their arithmetic hides the load latency.
+// block.  They do not depend on the field.
 template <typename Real>
 __device__ __forceinline__ Uniform5 site_uniforms(const DeviceState<Real>& st, int site, int r,
                                                   unsigned long long pass) {
@@ -89,23 +85,38 @@ __device__ __forceinline__ Uniform5 site_uniforms(const DeviceState<Real>& st, i
         Words4{uint32_t(site), uint32_t(r), uint32_t(pass), uint32_t(pass >> 32)}, st.k0, st.k1));
 }
 
-// update_spin with the site's uniforms already drawn (site_uniforms).
+// What an update needs of its replica: the temperature of the slot the
+// replica occupies, the proposal width sqrt(T / T_ref) (computed in FP64 as
+// the reference does) and the global field.  Constant within a launch, so
+// kernels that update many sites of one replica compute it once.
+template <typename Real>
+struct ReplicaParams {
+    Real T, sig, Bx, By, Bz;
+};
+template <typename Real>
+__device__ __forceinline__ ReplicaParams<Real> replica_params(const DeviceState<Real>& st, int r) {
+    const double Td = st.T_slot[st.slot_of[r]];
+    return ReplicaParams<Real>{Real(Td), Real(sqrt(Td / st.T_ref)), st.B[3 * r + 0], st.B[3 * r + 1],
+                               st.B[3 * r + 2]};
+}
+
+// update_spin with the replica's parameters and the site's uniforms already
+// drawn (replica_params, site_uniforms).
 template <typename Real, int KIND>
-__device__ __forceinline__ bool update_spin_u(const DeviceState<Real>& st, Real& px, Real& py, Real& pz,
-                                              bool& changed, int r, const Uniform5& u, Real hx, Real hy,
-                                              Real hz, int8_t* dp) {
+__device__ __forceinline__ bool update_spin_u(const ReplicaParams<Real>& p, Real& px, Real& py, Real& pz,
+                                              bool& changed, const Uniform5& u, Real hx, Real hy, Real hz,
+                                              int8_t* dp) {
     changed = false;
-    hx -= st.B[3 * r + 0];  // h_loc = local_field - global_field
-    hy -= st.B[3 * r + 1];
-    hz -= st.B[3 * r + 2];
+    hx -= p.Bx;  // h_loc = local_field - global_field
+    hy -= p.By;
+    hz -= p.Bz;
 
     const Real sx = px, sy = py, sz = pz;
-    const double Td = st.T_slot[st.slot_of[r]];
-    const Real T = Real(Td);
+    const Real T = p.T;
 
     if constexpr (KIND == 0) {
         // Gaussian-kick Metropolis (MC.cpp:111-130).
-        const Real sig = Real(sqrt(Td / st.T_ref));
+        const Real sig = p.sig;
         Real s1, c1, s2, c2;
         const Real r1 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[0])));
         dsincos_rev<Real>(u24<Real>(u.m[1]), &s1, &c1);
@@ -157,7 +168,7 @@ __device__ __forceinline__ bool update_spin_u(const DeviceState<Real>& st, Real&
         }
         const Real ex = ddiv<Real>(hx, hn), ey = ddiv<Real>(hy, hn), ez = ddiv<Real>(hz, hn);
         const Real uu = ex * sx + ey * sy + ez * sz;
-        const Real du = (Real(d) * Real(sqrt(Td / st.T_ref))) * (-dlog<Real>(u24<Real>(u.m[0])));
+        const Real du = (Real(d) * p.sig) * (-dlog<Real>(u24<Real>(u.m[0])));
         const Real nu = uu + du;
         if (nu > Real(1) || nu < Real(-1) || fabs(Real(1) - uu * uu) < Real(1e-16)) {
             *dp = int8_t(-d);
@@ -189,7 +200,7 @@ __device__ __forceinline__ bool update_spin_u(const DeviceState<Real>& st, Real&
             *dp = int8_t(-d);
             return false;
         }
-        const Real th = Real(d) * Real(sqrt(Td / st.T_ref));
+        const Real th = Real(d) * p.sig;
         Real s, c;
         dsincos<Real>(th, &s, &c);
         const Real dd = ax * sx + ay * sy + az * sz;
@@ -220,23 +231,28 @@ __device__ __forceinline__ bool update_spin_u(const DeviceState<Real>& st, Real&
     }
 }
 
-
+// One single-site update of kind KIND of the spin (px, py, pz) at `site` of
+// replica r (parameters p = replica_params(st, r)), pass index `pass`, given
+// the local field (hx, hy, hz) without the global field.  Returns whether
+// the proposal was accepted (OR: always false); `changed` tells whether the
+// spin was modified.  dp points at the site's lifted direction (read and
+// flipped by the lifted kinds only).
 template <typename Real, int KIND>
-__device__ __forceinline__ bool update_spin(const DeviceState<Real>& st, Real& px, Real& py, Real& pz,
-                                            bool& changed, int site, int r, unsigned long long pass,
-                                            Real hx, Real hy, Real hz, int8_t* dp) {
-    return update_spin_u<Real, KIND>(st, px, py, pz, changed, r, site_uniforms(st, site, r, pass), hx, hy, hz,
-                                     dp);
+__device__ __forceinline__ bool update_spin_p(const ReplicaParams<Real>& p, const DeviceState<Real>& st, Real& px,
+                                              Real& py, Real& pz, bool& changed, int site, int r,
+                                              unsigned long long pass, Real hx, Real hy, Real hz, int8_t* dp) {
+    return update_spin_u<Real, KIND>(p, px, py, pz, changed, site_uniforms(st, site, r, pass), hx, hy, hz, dp);
 }
 
-// update_spin_u on the spin stored in global memory (S[r][c][site]).
+// update_spin_p on the spin stored in global memory (S[r][c][site]).
 template <typename Real, int KIND>
-__device__ __forceinline__ bool apply_update_u(const DeviceState<Real>& st, Real* __restrict__ Sx,
-                                               Real* __restrict__ Sy, Real* __restrict__ Sz, int site, int r,
-                                               const Uniform5& u, Real hx, Real hy, Real hz) {
+__device__ __forceinline__ bool apply_update_p(const ReplicaParams<Real>& p, const DeviceState<Real>& st,
+                                               Real* __restrict__ Sx, Real* __restrict__ Sy, Real* __restrict__ Sz,
+                                               int site, int r, unsigned long long pass, Real hx, Real hy,
+                                               Real hz) {
     Real x = Sx[site], y = Sy[site], z = Sz[site];
     bool changed;
-    const bool acc = update_spin_u<Real, KIND>(st, x, y, z, changed, r, u, hx, hy, hz,
+    const bool acc = update_spin_p<Real, KIND>(p, st, x, y, z, changed, site, r, pass, hx, hy, hz,
                                                st.dir + size_t(r) * size_t(st.N) + site);
     if (changed) {
         Sx[site] = x;
@@ -249,7 +265,7 @@ template <typename Real, int KIND>
 __device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sx,
                                              Real* __restrict__ Sy, Real* __restrict__ Sz, int site,
                                              int r, unsigned long long pass, Real hx, Real hy, Real hz) {
-    return apply_update_u<Real, KIND>(st, Sx, Sy, Sz, site, r, site_uniforms(st, site, r, pass), hx, hy, hz);
+    return apply_update_p<Real, KIND>(replica_params(st, r), st, Sx, Sy, Sz, site, r, pass, hx, hy, hz);
 }
 
 // Wave-level count of accepted updates added to acc[r].

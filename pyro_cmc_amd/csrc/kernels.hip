@@ -71,15 +71,15 @@ __device__ inline void local_field(const DeviceState<Real>& st, const Real* __re
 }
 
 template <typename Real, int NS, bool ISO, int KIND>
-__device__ inline bool update_site(const DeviceState<Real>& st, int site, int sl, int r,
-                                   unsigned long long pass) {
+__device__ inline bool update_site(const DeviceState<Real>& st, const ReplicaParams<Real>& p, int site, int sl,
+                                   int r, unsigned long long pass) {
     const size_t N = size_t(st.N);
     Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
     Real* __restrict__ Sy = Sx + N;
     Real* __restrict__ Sz = Sy + N;
     Real hx, hy, hz;
     local_field<Real, NS, ISO>(st, Sx, Sy, Sz, site, sl, hx, hy, hz);
-    return apply_update<Real, KIND>(st, Sx, Sy, Sz, site, r, pass, hx, hy, hz);
+    return apply_update_p<Real, KIND>(p, st, Sx, Sy, Sz, site, r, pass, hx, hy, hz);
 }
 
 // Strategy 1: one launch per colour; grid.x = classes x blocks-per-class,
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void sweep_colour_kernel(DeviceState<Real> 
     bool accepted = false;
     if (m < spc) {
         const int site = decode_site(desc, m, st.n_cells, st.L1, st.L2, st.p0, st.p1, st.p2);
-        accepted = update_site<Real, NS, ISO, KIND>(st, site, sl, r, pass);
+        accepted = update_site<Real, NS, ISO, KIND>(st, replica_params(st, r), site, sl, r, pass);
     }
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
 }
@@ -117,6 +117,7 @@ __global__ __launch_bounds__(BS) void sweep_resident_kernel(
     const int r = blockIdx.x;
     const int spc = st.n_cells / (st.p0 * st.p1 * st.p2);
     unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    const ReplicaParams<Real> rp = replica_params(st, r);
     unsigned int my_acc = 0;
     for (int sw = 0; sw < n_sweeps; ++sw) {
         for (int ps = 0; ps <= n_or; ++ps, ++pass) {
@@ -129,9 +130,9 @@ __global__ __launch_bounds__(BS) void sweep_resident_kernel(
                         const int site =
                             decode_site(desc, m, st.n_cells, st.L1, st.L2, st.p0, st.p1, st.p2);
                         if (ps == 0) {
-                            my_acc += update_site<Real, NS, ISO, FIRST>(st, site, sl, r, pass) ? 1u : 0u;
+                            my_acc += update_site<Real, NS, ISO, FIRST>(st, rp, site, sl, r, pass) ? 1u : 0u;
                         } else {
-                            update_site<Real, NS, ISO, 3>(st, site, sl, r, pass);
+                            update_site<Real, NS, ISO, 3>(st, rp, site, sl, r, pass);
                         }
                     }
                 }

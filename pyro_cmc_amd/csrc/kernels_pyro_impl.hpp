@@ -300,6 +300,7 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     Real* __restrict__ Sy = Sx + N;
     Real* __restrict__ Sz = Sy + N;
     unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    const ReplicaParams<Real> rp = replica_params(st, r);
     unsigned int my_acc = 0;
     static_assert(CPT == 2, "the resident kernel owns two cells per thread");
     const int cell_b = threadIdx.x + BS < nc ? threadIdx.x + BS : 0;
@@ -326,10 +327,10 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                         Real hx, hy, hz;
                         pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, Jr, hx, hy, hz);
                         if (ps == 0)
-                            my_acc += apply_update<Real, FIRST>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy,
-                                                                hz) ? 1u : 0u;
+                            my_acc += apply_update_p<Real, FIRST>(rp, st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx,
+                                                                  hy, hz) ? 1u : 0u;
                         else
-                            apply_update<Real, 3>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
+                            apply_update_p<Real, 3>(rp, st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
                     }
                 }
                 __syncthreads();
@@ -468,6 +469,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     __syncthreads();
 
     const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    const ReplicaParams<Real> rp = replica_params(st, r);
     static_assert(NI <= BS, "one tile cell per thread");
     const int my_cell = kTileLanes<T0, T1, T2, BS>.cell[threadIdx.x];
     unsigned int my_acc = 0;
@@ -507,13 +509,13 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                             wp = ldw + sl * W + (m >> 5);
                             d0 = d = (*wp & bit) ? int8_t(-1) : int8_t(1);
                         }
-                        my_acc += update_spin<Real, FIRST>(st, x, y, z, ch, site, r, pass0, hx, hy, hz,
-                                                           LIFTED ? &d : nullptr) ? 1u : 0u;
+                        my_acc += update_spin_p<Real, FIRST>(rp, st, x, y, z, ch, site, r, pass0, hx, hy, hz,
+                                                             LIFTED ? &d : nullptr) ? 1u : 0u;
                         if constexpr (LIFTED) {
                             if (d != d0) atomicXor(wp, bit);
                         }
                     } else {
-                        update_spin<Real, 3>(st, x, y, z, ch, site, r, pass0 + k, hx, hy, hz, nullptr);
+                        update_spin_p<Real, 3>(rp, st, x, y, z, ch, site, r, pass0 + k, hx, hy, hz, nullptr);
                     }
                     if (ch) {
                         lx[li] = x;
