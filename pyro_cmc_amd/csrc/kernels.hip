@@ -180,12 +180,16 @@ __device__ inline double block_sum(double v) {
     return t;
 }
 
-// Energy terms 1/2 S.h - B.S per site (MC.cpp:305-312), block partial sums.
+// Energy terms 1/2 S.h - B.S per site (MC.cpp:305-312), block partial sums
+// partial[r][block]; 1-D grid of blocks x R placed XCD-aware (xcd_split).
 template <typename Real, int NS, bool ISO>
 __global__ __launch_bounds__(kBlock) void energy_partial_kernel(DeviceState<Real> st,
                                                                 double* __restrict__ partial) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const int r = blockIdx.y;
+    const unsigned gx = gridDim.x / unsigned(st.R);
+    unsigned bx, ru;
+    xcd_split(blockIdx.x, gx, unsigned(st.R), bx, ru);
+    const int i = int(bx) * kBlock + threadIdx.x;
+    const int r = int(ru);
     double e = 0.0;
     if (i < st.N) {
         const size_t N = size_t(st.N);
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void energy_partial_kernel(DeviceState<Real
         e = double(half) - double(bs);
     }
     const double t = block_sum(e);
-    if (threadIdx.x == 0) partial[size_t(r) * gridDim.x + blockIdx.x] = t;
+    if (threadIdx.x == 0) partial[size_t(r) * gx + bx] = t;
 }
 
 __global__ __launch_bounds__(kBlock) void energy_finish_kernel(
@@ -878,7 +882,7 @@ template <typename Real>
 hipError_t launch_energy(const DeviceState<Real>& st, double* partial, double* e, double* Esum,
                          double* E2sum, unsigned long long* nE, hipStream_t s) {
     const int nblk = energy_blocks(st.N);
-    const dim3 grid(unsigned(nblk), unsigned(st.R));
+    const dim3 grid(unsigned(nblk) * unsigned(st.R));  // XCD-aware (xcd_split)
     hipError_t err = dispatch_ns(st.NS, [&](auto ns) -> hipError_t {
         constexpr int NS = decltype(ns)::value;
         if (st.iso) hipLaunchKernelGGL((energy_partial_kernel<Real, NS, true>), grid, dim3(kBlock), 0, s, st, partial);

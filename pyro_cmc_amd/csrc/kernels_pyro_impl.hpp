@@ -224,12 +224,17 @@ __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
 // Energy terms per site (MC.cpp:305-312) with the stencil: S_i . h_i over
 // the bonds to higher sublattices (each bond once, the same sum as
 // 1/2 sum_i S_i . h_i with half the gathers) minus B . S_i; one partial sum
-// per 64-site (one-wave) block and replica.
+// per 64-site (one-wave) block and replica, partial[r][block].  1-D grid of
+// blocks x R with XCD-aware replica placement (xcd_split), so a replica's
+// neighbour gathers are served by one L2.
 template <typename Real, int SHELLS>
 __global__ __launch_bounds__(64) void pyro_energy_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                          double* __restrict__ partial) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int r = blockIdx.y;
+    const unsigned gx = gridDim.x / unsigned(st.R);
+    unsigned bx, ru;
+    xcd_split(blockIdx.x, gx, unsigned(st.R), bx, ru);
+    const int i = int(bx) * 64 + threadIdx.x;
+    const int r = int(ru);
     double e = 0.0;
     if (i < st.N) {
         const int nc = st.n_cells;
@@ -263,7 +268,7 @@ __global__ __launch_bounds__(64) void pyro_energy_kernel(DeviceState<Real> st, P
         e = double(bond) - double(bs);
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
-    if (threadIdx.x == 0) partial[size_t(r) * gridDim.x + blockIdx.x] = e;
+    if (threadIdx.x == 0) partial[size_t(r) * gx + bx] = e;
 }
 
 // Local field of every site of one replica (MC.cpp:87-96), out[3][N].
@@ -596,7 +601,7 @@ hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<R
 template <typename Real>
 hipError_t launch_pyro_energy(const DeviceState<Real>& st, const PyroCouplings<Real>& J, double* partial,
                               hipStream_t s) {
-    const dim3 grid(unsigned(pyro_energy_blocks(st.N)), unsigned(st.R));
+    const dim3 grid(unsigned(pyro_energy_blocks(st.N)) * unsigned(st.R));  // XCD-aware (xcd_split)
     return dispatch_pyro(J, [&](auto shells) -> hipError_t {
         constexpr int SH = decltype(shells)::value;
         hipLaunchKernelGGL((pyro_energy_kernel<Real, SH>), grid, dim3(64), 0, s, st, J, partial);
