@@ -582,6 +582,164 @@ __global__ __launch_bounds__(256) void dft_mfma_kernel(const float* __restrict__
     }
 }
 
+// FP32 line DFT of La = 16 M points (M = 2, 3, 4) by one Cooley-Tukey step,
+// for the same 64-column view as dft_mfma_kernel: with n = M n1 + n2 and
+// K = k1 + 16 k2,
+//   Y_n2[k1] = sum_n1 w16^(k1 n1) x[M n1 + n2]       (M complex GEMMs 16 x 16
+//                                                       on v_mfma_f32_16x16x4f32),
+//   X[k1 + 16 k2] = sum_n2 wM^(n2 k2) wLa^(n2 k1) Y_n2[k1]   (in registers),
+// with wP = e^(-2 pi i / P).  A lane's MFMA accumulators hold Y_n2 of the
+// same (k1, column) for every n2, so the twiddles and the M-point butterflies
+// need no data exchange; the GEMM work is 1/M of the direct product's and no
+// La x La table is built.  tw[j] = e^(-2 pi i j / La) as (re, im).
+template <bool REAL_IN, int M>
+__global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                     const float* __restrict__ tw, int s, unsigned n_cols) {
+    static_assert(M >= 2 && M <= 4, "lines of 32, 48 or 64 points");
+    constexpr int La = 16 * M;
+    // row stride (floats): the B-operand rows M apart (kq = 0..3) start 16
+    // banks apart, so a wave's reads hit 64 distinct banks
+    constexpr int XS = M == 2 ? 72 : (M == 3 ? 80 : 68);
+    constexpr int ROUNDS = La * 64 / 256;  // staging: 4 M elements per thread
+    __shared__ float xr[La * XS];
+    __shared__ float xi[La * XS];
+    const unsigned c0 = blockIdx.x * 64u;  // launcher keeps 2 * La * n_cols < 2^31
+    auto where = [&](int e, int& j, int& n, unsigned& a) __attribute__((always_inline)) {
+        if (s == 1) {
+            j = e / La;
+            n = e - j * La;
+            a = (c0 + j) * La + n;
+        } else {
+            n = e >> 6;
+            j = e & 63;
+            const unsigned c = c0 + j, o = c / unsigned(s);
+            a = (o * La + n) * unsigned(s) + (c - o * unsigned(s));
+        }
+    };
+    // every global load issued before the first LDS store
+    float vr[ROUNDS], vi[ROUNDS];
+#pragma unroll
+    for (int it = 0; it < ROUNDS; ++it) {
+        int j, n;
+        unsigned a;
+        where(threadIdx.x + 256 * it, j, n, a);
+        vr[it] = 0.f;
+        vi[it] = 0.f;
+        if (c0 + j < n_cols) {
+            if constexpr (REAL_IN) {
+                vr[it] = in[a];
+            } else {
+                const float2 v = reinterpret_cast<const float2*>(in)[a];
+                vr[it] = v.x;
+                vi[it] = v.y;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int kq = lane >> 4, ci = lane & 15;
+    // A operands: w16^(k1 n1) for row k1 = ci and n1 = 4 t + kq
+    float Ar[4], Ai[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int jt = (M * ci * (4 * t + kq)) % La;
+        Ar[t] = tw[2 * jt];
+        Ai[t] = tw[2 * jt + 1];
+    }
+#pragma unroll
+    for (int it = 0; it < ROUNDS; ++it) {
+        int j, n;
+        unsigned a;
+        where(threadIdx.x + 256 * it, j, n, a);
+        xr[n * XS + j] = vr[it];
+        xi[n * XS + j] = vi[it];
+    }
+    __syncthreads();
+
+    v4f dr[M], di[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) dr[q] = di[q] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int n = M * (4 * t + kq) + q;
+            const float br = xr[n * XS + 16 * w + ci];
+            dr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[t], br, dr[q], 0, 0, 0);
+            di[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ai[t], br, di[q], 0, 0, 0);
+            if constexpr (!REAL_IN) {
+                const float bi = xi[n * XS + 16 * w + ci];
+                dr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-Ai[t], bi, dr[q], 0, 0, 0);
+                di[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[t], bi, di[q], 0, 0, 0);
+            }
+        }
+    }
+    // twiddles wLa^(n2 k1) and the M-point DFT over n2, per accumulator row
+    float yr[M][4], yi[M][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int k1 = 4 * kq + v;
+        float ur[M], ui[M];
+        ur[0] = dr[0][v];
+        ui[0] = di[0][v];
+#pragma unroll
+        for (int q = 1; q < M; ++q) {
+            const float c = tw[2 * (q * k1)], d = tw[2 * (q * k1) + 1];
+            ur[q] = dr[q][v] * c - di[q][v] * d;
+            ui[q] = dr[q][v] * d + di[q][v] * c;
+        }
+        if constexpr (M == 2) {
+            yr[0][v] = ur[0] + ur[1];
+            yi[0][v] = ui[0] + ui[1];
+            yr[1][v] = ur[0] - ur[1];
+            yi[1][v] = ui[0] - ui[1];
+        } else if constexpr (M == 3) {
+            // w3 = -1/2 - i sqrt(3)/2: X1 = u0 + w3 u1 + w3^2 u2, X2 = u0 + w3^2 u1 + w3 u2
+            constexpr float h = 0.8660254037844386f;  // sqrt(3) / 2
+            const float sr = ur[1] + ur[2], si = ui[1] + ui[2];
+            const float dr_ = ur[1] - ur[2], di_ = ui[1] - ui[2];
+            yr[0][v] = ur[0] + sr;
+            yi[0][v] = ui[0] + si;
+            const float mr = ur[0] - 0.5f * sr, mi = ui[0] - 0.5f * si;
+            yr[1][v] = mr + h * di_;
+            yi[1][v] = mi - h * dr_;
+            yr[2][v] = mr - h * di_;
+            yi[2][v] = mi + h * dr_;
+        } else {
+            // w4 = -i
+            const float ar = ur[0] + ur[2], ai = ui[0] + ui[2];
+            const float br = ur[0] - ur[2], bi = ui[0] - ui[2];
+            const float cr = ur[1] + ur[3], cim = ui[1] + ui[3];
+            const float er = ur[1] - ur[3], ei = ui[1] - ui[3];
+            yr[0][v] = ar + cr;
+            yi[0][v] = ai + cim;
+            yr[1][v] = br + ei;
+            yi[1][v] = bi - er;
+            yr[2][v] = ar - cr;
+            yi[2][v] = ai - cim;
+            yr[3][v] = br - ei;
+            yi[3][v] = bi + er;
+        }
+    }
+    __syncthreads();  // every B read is done: reuse xr/xi for the outputs
+#pragma unroll
+    for (int q = 0; q < M; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int K = 4 * kq + v + 16 * q;
+            xr[K * XS + 16 * w + ci] = yr[q][v];
+            xi[K * XS + 16 * w + ci] = yi[q][v];
+        }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ROUNDS; ++it) {
+        int jj, K;
+        unsigned a;
+        where(threadIdx.x + 256 * it, jj, K, a);
+        if (c0 + jj < n_cols) reinterpret_cast<float2*>(out)[a] = float2{xr[K * XS + jj], xi[K * XS + jj]};
+    }
+}
+
 // Contracted correlators (ssf_manager.hpp:260-293 with the separable phase
 // fold): b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K); S_c = conj(b^alpha) b^beta.
 constexpr int kContractBlock = 64;  // one wave per workgroup
@@ -1006,7 +1164,18 @@ hipError_t dft_axis(const Real* in, Real* out, const Real* tw, int La, int strid
     const unsigned n_cols = unsigned(total / La);
     const dim3 grid((n_cols + 63u) / 64u);
     if constexpr (std::is_same_v<Real, float>) {
-        // FP32: the DFT as a complex GEMM on MFMA
+        // FP32, 32, 48 or 64 points: one Cooley-Tukey step over 16-point GEMMs
+        if (La == 32 || La == 48 || La == 64) {
+            auto ct = [&](auto m) -> hipError_t {
+                constexpr int M = decltype(m)::value;
+                hipLaunchKernelGGL((dft_ct_kernel<REAL_IN, M>), grid, dim3(256), 0, s, in, out, tw, stride, n_cols);
+                return hipGetLastError();
+            };
+            if (La == 32) return ct(std::integral_constant<int, 2>{});
+            if (La == 48) return ct(std::integral_constant<int, 3>{});
+            return ct(std::integral_constant<int, 4>{});
+        }
+        // other lengths: the DFT as one complex GEMM on MFMA
         const int Kp = (La + 3) & ~3, MT = (La + 15) / 16;
         const size_t lds = (size_t(2) * Kp * 80 + size_t(2) * Kp * 16 * MT) * sizeof(float);
         auto go = [&](auto mt) -> hipError_t {

@@ -352,6 +352,21 @@ def test_spin_ft_against_direct_dft(gpu, oracle, prec, L):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("L", [(32, 3, 2), (2, 48, 3), (3, 2, 64), (48, 2, 4), (2, 3, 32), (4, 64, 2), (2, 2, 48)])
+def test_spin_ft_radix_lines(gpu, oracle, prec, L):
+    """Line lengths 32, 48 and 64 on each axis (strided and contiguous lines):
+    the FP32 Cooley-Tukey kernel (16-point GEMMs, then twiddles and 2-, 3- or
+    4-point butterflies) against the direct DFT."""
+    ctx, os_ = make(gpu, oracle, L, prec=prec)
+    S = random_spins(ctx.N, np.random.default_rng(sum(L)))
+    ctx.set_spins(S, 0)
+    os_[0].set_spins(S)
+    ft = ctx.spin_ft(0)
+    want = os_[0].spin_ft()
+    assert np.max(np.abs(ft - want)) <= TOL[prec] * np.prod(L)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
 def test_spin_ft_parseval_full_size(gpu, prec):
     """L = 48 (the large tempering lattice): sum_K |A_mu(K)|^2 = L^3 sum_n |S_mu(n)|^2
     per component and sublattice, and A(0) = sum_n S(n)."""
