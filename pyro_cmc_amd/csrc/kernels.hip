@@ -582,62 +582,22 @@ __global__ __launch_bounds__(256) void dft_mfma_kernel(const float* __restrict__
     }
 }
 
-// FP32 line DFT of La = 16 M points (M = 2, 3, 4) by one Cooley-Tukey step,
-// for the same 64-column view as dft_mfma_kernel: with n = M n1 + n2 and
-// K = k1 + 16 k2,
+// One Cooley-Tukey step for La = 16 M points (M = 2, 3, 4) on the 16
+// columns of a wave: with n = M n1 + n2 and K = k1 + 16 k2,
 //   Y_n2[k1] = sum_n1 w16^(k1 n1) x[M n1 + n2]       (M complex GEMMs 16 x 16
 //                                                       on v_mfma_f32_16x16x4f32),
 //   X[k1 + 16 k2] = sum_n2 wM^(n2 k2) wLa^(n2 k1) Y_n2[k1]   (in registers),
 // with wP = e^(-2 pi i / P).  A lane's MFMA accumulators hold Y_n2 of the
 // same (k1, column) for every n2, so the twiddles and the M-point butterflies
-// need no data exchange; the GEMM work is 1/M of the direct product's and no
-// La x La table is built.  tw[j] = e^(-2 pi i j / La) as (re, im).
-template <bool REAL_IN, int M>
-__global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                                     const float* __restrict__ tw, int s, unsigned n_cols) {
+// need no data exchange; the GEMM work is 1/M of the direct product's.
+// load(n, re, im) gives point n of the lane's column ci; on return lane
+// (kq, ci) holds X[4 kq + v + 16 q] of its column in (yr, yi)[q][v].
+// tw[j] = e^(-2 pi i j / La) as (re, im).
+template <bool REAL_IN, int M, typename Load>
+__device__ __forceinline__ void ct16_lines(Load&& load, const float* __restrict__ tw, int kq, int ci,
+                                           float (&yr)[M][4], float (&yi)[M][4]) {
     static_assert(M >= 2 && M <= 4, "lines of 32, 48 or 64 points");
     constexpr int La = 16 * M;
-    // row stride (floats): the B-operand rows M apart (kq = 0..3) start 16
-    // banks apart, so a wave's reads hit 64 distinct banks
-    constexpr int XS = M == 2 ? 72 : (M == 3 ? 80 : 68);
-    constexpr int ROUNDS = La * 64 / 256;  // staging: 4 M elements per thread
-    __shared__ float xr[La * XS];
-    __shared__ float xi[La * XS];
-    const unsigned c0 = blockIdx.x * 64u;  // launcher keeps 2 * La * n_cols < 2^31
-    auto where = [&](int e, int& j, int& n, unsigned& a) __attribute__((always_inline)) {
-        if (s == 1) {
-            j = e / La;
-            n = e - j * La;
-            a = (c0 + j) * La + n;
-        } else {
-            n = e >> 6;
-            j = e & 63;
-            const unsigned c = c0 + j, o = c / unsigned(s);
-            a = (o * La + n) * unsigned(s) + (c - o * unsigned(s));
-        }
-    };
-    // every global load issued before the first LDS store
-    float vr[ROUNDS], vi[ROUNDS];
-#pragma unroll
-    for (int it = 0; it < ROUNDS; ++it) {
-        int j, n;
-        unsigned a;
-        where(threadIdx.x + 256 * it, j, n, a);
-        vr[it] = 0.f;
-        vi[it] = 0.f;
-        if (c0 + j < n_cols) {
-            if constexpr (REAL_IN) {
-                vr[it] = in[a];
-            } else {
-                const float2 v = reinterpret_cast<const float2*>(in)[a];
-                vr[it] = v.x;
-                vi[it] = v.y;
-            }
-        }
-    }
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int kq = lane >> 4, ci = lane & 15;
     // A operands: w16^(k1 n1) for row k1 = ci and n1 = 4 t + kq
     float Ar[4], Ai[4];
 #pragma unroll
@@ -646,16 +606,6 @@ __global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ i
         Ar[t] = tw[2 * jt];
         Ai[t] = tw[2 * jt + 1];
     }
-#pragma unroll
-    for (int it = 0; it < ROUNDS; ++it) {
-        int j, n;
-        unsigned a;
-        where(threadIdx.x + 256 * it, j, n, a);
-        xr[n * XS + j] = vr[it];
-        xi[n * XS + j] = vi[it];
-    }
-    __syncthreads();
-
     v4f dr[M], di[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) dr[q] = di[q] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -663,19 +613,16 @@ __global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ i
     for (int t = 0; t < 4; ++t) {
 #pragma unroll
         for (int q = 0; q < M; ++q) {
-            const int n = M * (4 * t + kq) + q;
-            const float br = xr[n * XS + 16 * w + ci];
+            float br, bi;
+            load(M * (4 * t + kq) + q, br, bi);
             dr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[t], br, dr[q], 0, 0, 0);
             di[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ai[t], br, di[q], 0, 0, 0);
             if constexpr (!REAL_IN) {
-                const float bi = xi[n * XS + 16 * w + ci];
                 dr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-Ai[t], bi, dr[q], 0, 0, 0);
                 di[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[t], bi, di[q], 0, 0, 0);
             }
         }
     }
-    // twiddles wLa^(n2 k1) and the M-point DFT over n2, per accumulator row
-    float yr[M][4], yi[M][4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int k1 = 4 * kq + v;
@@ -721,6 +668,72 @@ __global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ i
             yi[3][v] = bi + er;
         }
     }
+}
+
+// FP32 line DFT of La = 16 M points (M = 2, 3, 4) by ct16_lines, for the
+// same 64-column view as dft_mfma_kernel (wave w owns columns 16 w .. 16 w +
+// 15), with no La x La table.
+template <bool REAL_IN, int M>
+__global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                     const float* __restrict__ tw, int s, unsigned n_cols) {
+    constexpr int La = 16 * M;
+    // row stride (floats): the B-operand rows M apart (kq = 0..3) start 16
+    // banks apart, so a wave's reads hit 64 distinct banks
+    constexpr int XS = M == 2 ? 72 : (M == 3 ? 80 : 68);
+    constexpr int ROUNDS = La * 64 / 256;  // staging: 4 M elements per thread
+    __shared__ float xr[La * XS];
+    __shared__ float xi[La * XS];
+    const unsigned c0 = blockIdx.x * 64u;  // launcher keeps 2 * La * n_cols < 2^31
+    auto where = [&](int e, int& j, int& n, unsigned& a) __attribute__((always_inline)) {
+        if (s == 1) {
+            j = e / La;
+            n = e - j * La;
+            a = (c0 + j) * La + n;
+        } else {
+            n = e >> 6;
+            j = e & 63;
+            const unsigned c = c0 + j, o = c / unsigned(s);
+            a = (o * La + n) * unsigned(s) + (c - o * unsigned(s));
+        }
+    };
+    // every global load issued before the first LDS store
+    float vr[ROUNDS], vi[ROUNDS];
+#pragma unroll
+    for (int it = 0; it < ROUNDS; ++it) {
+        int j, n;
+        unsigned a;
+        where(threadIdx.x + 256 * it, j, n, a);
+        vr[it] = 0.f;
+        vi[it] = 0.f;
+        if (c0 + j < n_cols) {
+            if constexpr (REAL_IN) {
+                vr[it] = in[a];
+            } else {
+                const float2 v = reinterpret_cast<const float2*>(in)[a];
+                vr[it] = v.x;
+                vi[it] = v.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ROUNDS; ++it) {
+        int j, n;
+        unsigned a;
+        where(threadIdx.x + 256 * it, j, n, a);
+        xr[n * XS + j] = vr[it];
+        xi[n * XS + j] = vi[it];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int kq = lane >> 4, ci = lane & 15;
+    float yr[M][4], yi[M][4];
+    ct16_lines<REAL_IN, M>(
+        [&](int n, float& br, float& bi) __attribute__((always_inline)) {
+            br = xr[n * XS + 16 * w + ci];
+            bi = REAL_IN ? 0.f : xi[n * XS + 16 * w + ci];
+        },
+        tw, kq, ci, yr, yi);
     __syncthreads();  // every B read is done: reuse xr/xi for the outputs
 #pragma unroll
     for (int q = 0; q < M; ++q)
@@ -737,6 +750,78 @@ __global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ i
         unsigned a;
         where(threadIdx.x + 256 * it, jj, K, a);
         if (c0 + jj < n_cols) reinterpret_cast<float2*>(out)[a] = float2{xr[K * XS + jj], xi[K * XS + jj]};
+    }
+}
+
+// FP32 DFT along axes 2 and 1 of one (field, n0) plane of real values, L1 =
+// 16 M1 by L2 = 16 M2 points, in LDS: read once, written once as complex
+// (the axis-0 pass follows), so the two passes cost 12 B per point of HBM
+// traffic instead of 4 + 8 + 8 + 8.  Pass A transforms the rows (column i1 of
+// the Cooley-Tukey step = wave i1 / 16) and stores its result transposed,
+// P'[k2][i1]; pass B then reads columns of P' with the same transposed
+// access.  The row stride 68 puts a wave's transposed reads on 64 distinct
+// banks for 48-point lines (4 ci + 3 kq mod 64 distinct).
+template <int M1, int M2>
+__global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                        const float* __restrict__ tw1,
+                                                        const float* __restrict__ tw2) {
+    constexpr int L1 = 16 * M1, L2 = 16 * M2, NP = L1 * L2;
+    constexpr int XS = (M1 == 3 && M2 == 3) ? 68 : 71;
+    constexpr int ROWS = L1 > L2 ? L1 : L2;
+    constexpr int RND = (NP + 255) / 256;
+    __shared__ float pr[ROWS * XS];
+    __shared__ float pi[ROWS * XS];
+    const size_t base = size_t(blockIdx.x) * NP;  // plane p = field * L0 + n0
+    float v[RND];
+#pragma unroll
+    for (int it = 0; it < RND; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        v[it] = e < NP ? in[base + e] : 0.f;
+    }
+#pragma unroll
+    for (int it = 0; it < RND; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        if (e < NP) pr[(e / L2) * XS + e % L2] = v[it];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int kq = lane >> 4, ci = lane & 15;
+    // pass A: rows i1 = 16 w + ci (M1 waves), points along n2
+    float yr[M2][4], yi[M2][4];
+    if (w < M1)
+        ct16_lines<true, M2>(
+            [&](int n, float& br, float& bi) __attribute__((always_inline)) {
+                br = pr[(16 * w + ci) * XS + n];
+                bi = 0.f;
+            },
+            tw2, kq, ci, yr, yi);
+    __syncthreads();  // every read of the real plane is done
+    if (w < M1) {
+#pragma unroll
+        for (int q = 0; q < M2; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k2 = 4 * kq + u + 16 * q;
+                pr[k2 * XS + 16 * w + ci] = yr[q][u];
+                pi[k2 * XS + 16 * w + ci] = yi[q][u];
+            }
+    }
+    __syncthreads();
+    // pass B: columns k2 = 16 w + ci (M2 waves), points along n1
+    if (w < M2) {
+        float zr[M1][4], zi[M1][4];
+        ct16_lines<false, M1>(
+            [&](int n, float& br, float& bi) __attribute__((always_inline)) {
+                br = pr[(16 * w + ci) * XS + n];
+                bi = pi[(16 * w + ci) * XS + n];
+            },
+            tw1, kq, ci, zr, zi);
+        float2* o = reinterpret_cast<float2*>(out) + base;
+#pragma unroll
+        for (int q = 0; q < M1; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) o[(4 * kq + u + 16 * q) * L2 + 16 * w + ci] = float2{zr[q][u], zi[q][u]};
     }
 }
 
@@ -1228,6 +1313,33 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
             case 12: return go(std::integral_constant<int, 12>{});
             case 16: return go(std::integral_constant<int, 16>{});
             default: break;
+        }
+    }
+    if constexpr (std::is_same_v<Real, float>) {
+        // axes 2 and 1 of every plane in one pass (spins -> G), then axis 0: G -> F
+        auto plane = [&](auto m1, auto m2) -> hipError_t {
+            constexpr int M1 = decltype(m1)::value, M2 = decltype(m2)::value;
+            const long long planes = total / ((long long)st.L1 * st.L2);
+            hipLaunchKernelGGL((dft_plane_kernel<M1, M2>), dim3(unsigned(planes)), dim3(256), 0, s, spins, st.G,
+                               st.tw1, st.tw2);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = dft_axis<Real, false>(st.G, st.F, st.tw0, st.L0, st.L1 * st.L2, total, s);
+            return e;
+        };
+        auto m2 = [&](auto m1) -> hipError_t {
+            switch (st.L2) {
+                case 32: return plane(m1, std::integral_constant<int, 2>{});
+                case 48: return plane(m1, std::integral_constant<int, 3>{});
+                default: return plane(m1, std::integral_constant<int, 4>{});
+            }
+        };
+        const bool ct1 = st.L1 == 32 || st.L1 == 48 || st.L1 == 64, ct2 = st.L2 == 32 || st.L2 == 48 || st.L2 == 64;
+        if (ct1 && ct2 && total < (1ll << 30)) {
+            switch (st.L1) {
+                case 32: return m2(std::integral_constant<int, 2>{});
+                case 48: return m2(std::integral_constant<int, 3>{});
+                default: return m2(std::integral_constant<int, 4>{});
+            }
         }
     }
     // axis 2 (stride 1): spins -> F; axis 1 (stride L2): F -> G; axis 0: G -> F

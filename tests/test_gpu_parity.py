@@ -367,6 +367,35 @@ def test_spin_ft_radix_lines(gpu, oracle, prec, L):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("L", [(2, 32, 48), (3, 48, 32), (2, 64, 64), (4, 48, 48), (48, 48, 48)])
+def test_spin_ft_planes(gpu, oracle, prec, L):
+    """L1 and L2 of 32, 48 or 64 points: axes 2 and 1 of each plane in one FP32
+    kernel, then axis 0.  The direct DFT of the oracle is too slow at these
+    sizes, so the check is numpy's FFT of each (component, sublattice) field;
+    the same FFT is first pinned to the oracle's direct DFT (sign, order,
+    no normalisation) on a small lattice."""
+    small = (2, 3, 4)
+    ctx, os_ = make(gpu, oracle, small, prec=64)
+    S = random_spins(ctx.N, np.random.default_rng(1))
+    os_[0].set_spins(S)
+    fft = np.fft.fftn(S.T.reshape(3, 16, *small), axes=(2, 3, 4)).reshape(3, 16, -1)
+    assert np.max(np.abs(fft.real - os_[0].spin_ft()[..., 0])) < 1e-12
+    assert np.max(np.abs(fft.imag - os_[0].spin_ft()[..., 1])) < 1e-12
+    ctx.close()
+
+    ctx = gpu.Context.pyrochlore(L, precision=prec, seed=3)
+    ctx.define_J1J2J3(1.0)
+    ctx.setup()
+    S = random_spins(ctx.N, np.random.default_rng(sum(L)))
+    ctx.set_spins(S, 0)
+    ft = ctx.spin_ft(0)
+    want = np.fft.fftn(S.T.reshape(3, 16, *L), axes=(2, 3, 4)).reshape(3, 16, -1)
+    nc = int(np.prod(L))
+    assert np.max(np.abs(ft[..., 0] - want.real)) <= TOL[prec] * nc
+    assert np.max(np.abs(ft[..., 1] - want.imag)) <= TOL[prec] * nc
+
+
+@pytest.mark.parametrize("prec", [64, 32])
 def test_spin_ft_parseval_full_size(gpu, prec):
     """L = 48 (the large tempering lattice): sum_K |A_mu(K)|^2 = L^3 sum_n |S_mu(n)|^2
     per component and sublattice, and A(0) = sum_n S(n)."""
