@@ -590,7 +590,8 @@ __global__ __launch_bounds__(256) void dft_mfma_kernel(const float* __restrict__
 // with wP = e^(-2 pi i / P).  A lane's MFMA accumulators hold Y_n2 of the
 // same (k1, column) for every n2, so the twiddles and the M-point butterflies
 // need no data exchange; the GEMM work is 1/M of the direct product's.
-// load(n, re, im) gives point n of the lane's column ci; on return lane
+// load(t, q, n, re, im) gives point n = M (4 t + kq) + q of the lane's
+// column ci (t and q are compile-time after unrolling); on return lane
 // (kq, ci) holds X[4 kq + v + 16 q] of its column in (yr, yi)[q][v].
 // tw[j] = e^(-2 pi i j / La) as (re, im).
 template <bool REAL_IN, int M, typename Load>
@@ -614,7 +615,7 @@ __device__ __forceinline__ void ct16_lines(Load&& load, const float* __restrict_
 #pragma unroll
         for (int q = 0; q < M; ++q) {
             float br, bi;
-            load(M * (4 * t + kq) + q, br, bi);
+            load(t, q, M * (4 * t + kq) + q, br, bi);
             dr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[t], br, dr[q], 0, 0, 0);
             di[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ai[t], br, di[q], 0, 0, 0);
             if constexpr (!REAL_IN) {
@@ -729,7 +730,7 @@ __global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ i
     const int kq = lane >> 4, ci = lane & 15;
     float yr[M][4], yi[M][4];
     ct16_lines<REAL_IN, M>(
-        [&](int n, float& br, float& bi) __attribute__((always_inline)) {
+        [&](int, int, int n, float& br, float& bi) __attribute__((always_inline)) {
             br = xr[n * XS + 16 * w + ci];
             bi = REAL_IN ? 0.f : xi[n * XS + 16 * w + ci];
         },
@@ -751,6 +752,46 @@ __global__ __launch_bounds__(256) void dft_ct_kernel(const float* __restrict__ i
         where(threadIdx.x + 256 * it, jj, K, a);
         if (c0 + jj < n_cols) reinterpret_cast<float2*>(out)[a] = float2{xr[K * XS + jj], xi[K * XS + jj]};
     }
+}
+
+// Complex lines of La = 16 M points with stride s > 1 (axis 0, or axis 1
+// of a lattice the plane kernel does not take): each lane loads its MFMA
+// B operands straight from global memory and stores its outputs straight
+// back, no LDS and no barrier.  For each (t, q) a wave reads rows n = M (4 t
+// + kq) + q of its 16 consecutive columns: four 128-byte segments when s is
+// a multiple of 16.  Same grid as dft_ct_kernel (wave w of workgroup b owns
+// columns 64 b + 16 w .. + 15).
+template <int M>
+__global__ __launch_bounds__(256) void dft_ct_strided_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                             const float* __restrict__ tw, int s, unsigned n_cols) {
+    constexpr int La = 16 * M;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int kq = lane >> 4, ci = lane & 15;
+    const unsigned c = blockIdx.x * 64u + 16u * unsigned(w) + unsigned(ci);
+    const bool ok = c < n_cols;
+    const unsigned o = c / unsigned(s);
+    const unsigned base = o * unsigned(La) * unsigned(s) + (c - o * unsigned(s));  // point n at base + n s
+    const float2* __restrict__ in2 = reinterpret_cast<const float2*>(in);
+    float2 b[4][M];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < M; ++q)
+            b[t][q] = ok ? in2[base + unsigned(M * (4 * t + kq) + q) * unsigned(s)] : float2{0.f, 0.f};
+    float yr[M][4], yi[M][4];
+    ct16_lines<false, M>(
+        [&](int t, int q, int, float& br, float& bi) __attribute__((always_inline)) {
+            br = b[t][q].x;
+            bi = b[t][q].y;
+        },
+        tw, kq, ci, yr, yi);
+    if (!ok) return;
+    float2* __restrict__ out2 = reinterpret_cast<float2*>(out);
+#pragma unroll
+    for (int q = 0; q < M; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out2[base + unsigned(4 * kq + v + 16 * q) * unsigned(s)] = float2{yr[q][v], yi[q][v]};
 }
 
 // FP32 DFT along axes 2 and 1 of one (field, n0) plane of real values, L1 =
@@ -791,7 +832,7 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict_
     float yr[M2][4], yi[M2][4];
     if (w < M1)
         ct16_lines<true, M2>(
-            [&](int n, float& br, float& bi) __attribute__((always_inline)) {
+            [&](int, int, int n, float& br, float& bi) __attribute__((always_inline)) {
                 br = pr[(16 * w + ci) * XS + n];
                 bi = 0.f;
             },
@@ -812,7 +853,7 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict_
     if (w < M2) {
         float zr[M1][4], zi[M1][4];
         ct16_lines<false, M1>(
-            [&](int n, float& br, float& bi) __attribute__((always_inline)) {
+            [&](int, int, int n, float& br, float& bi) __attribute__((always_inline)) {
                 br = pr[(16 * w + ci) * XS + n];
                 bi = pi[(16 * w + ci) * XS + n];
             },
@@ -1253,7 +1294,10 @@ hipError_t dft_axis(const Real* in, Real* out, const Real* tw, int La, int strid
         if (La == 32 || La == 48 || La == 64) {
             auto ct = [&](auto m) -> hipError_t {
                 constexpr int M = decltype(m)::value;
-                hipLaunchKernelGGL((dft_ct_kernel<REAL_IN, M>), grid, dim3(256), 0, s, in, out, tw, stride, n_cols);
+                if (!REAL_IN && stride > 1)
+                    hipLaunchKernelGGL((dft_ct_strided_kernel<M>), grid, dim3(256), 0, s, in, out, tw, stride, n_cols);
+                else
+                    hipLaunchKernelGGL((dft_ct_kernel<REAL_IN, M>), grid, dim3(256), 0, s, in, out, tw, stride, n_cols);
                 return hipGetLastError();
             };
             if (La == 32) return ct(std::integral_constant<int, 2>{});
