@@ -20,7 +20,7 @@ HBM traffic per launch ("traffic") comes from the rocprofv3 FETCH_SIZE and
 WRITE_SIZE summaries committed under profiles/ for the same kernel and
 workload (lines "traffic tag=<tag> bytes_per_launch=<B>"), or is null.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs c3,fp64,large,munu,cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs c3,fp64,large,munu,c5,batch,cpu]
 """
 from __future__ import annotations
 
@@ -47,7 +47,7 @@ N_SWEEP = 4
 CORR = ("xx", "yy", "zz", "xy", "xz", "yz", "yx", "zx", "zy")
 PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E spec
 PEAK_FP32_MATRIX = 157.3e12  # flop/s, MI355X dense FP32 MFMA spec
-ALL_LEGS = ("c3", "fp64", "large", "munu", "batch", "cpu")
+ALL_LEGS = ("c3", "fp64", "large", "munu", "c5", "batch", "cpu")
 
 
 def profiled_traffic(tag: str):
@@ -169,6 +169,51 @@ def munu_correlators(pyro, samples=384, buffer=192):
             "bytes_per_launch": acc_bytes + row_bytes,
             "traffic": profiled_traffic("munu_flush_L48x16"),
         },
+    }
+
+
+def large_tempering_run(pyro, rounds=384, buffer=192):
+    """The large tempering workload end to end: L = 48 (1,769,472 spins) x 16
+    replicas, FP32, J1 = 1, J2 = J3 = 0.1, B = (0, 0, 0.2), T log-spaced in
+    [0.05, 1]; each round is one sweep (1 lifted Metropolis + 1 OR pass), an
+    SSF sample of all 9 correlators over the 16 sublattices (contracted sums
+    and the sublattice-resolved matrices, buffered `buffer` deep and flushed
+    on MFMA into FP64), the energies and one replica exchange, captured as
+    one graph per round.  The timed region holds `rounds` rounds and the
+    flushes they trigger.  Exchange runs every sweep here, since SSF samples
+    are taken once per round."""
+    R, L = 16, 48
+    ctx = pyro.Context.pyrochlore(L, n_replicas=R, precision=pyro.FP32, seed=48)
+    ctx.define_J1J2J3(1.0, 0.1, 0.1)
+    ctx.setup()
+    ctx.set_field(np.array([0.0, 0.0, 0.2]))
+    ctx.randomize(49)
+    ctx.pt_init(np.geomspace(1.0, 0.05, R), 50)
+    ctx.ssf_configure(CORR, n_slots=R, store_error_term=True, munu=True, munu_buffer=buffer)
+    ctx.pt_rounds(pyro.LIFTED, 1, 1, True, 4)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.pt_rounds(pyro.LIFTED, 1, 1, True, rounds)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    n = ctx.N
+    st = ctx.pt_stats()
+    # per-kernel HIP-event times of 16 rounds run eagerly (no flush among them)
+    ctx.profile(True)
+    ctx.pt_rounds(pyro.LIFTED, 1, 1, True, 16)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.close()
+    return {
+        "workload": f"parallel tempering, L={L} (1769472 spins) x {R} replicas, FP32, J1=1 J2=J3=0.1, "
+                    "B=(0,0,0.2), T log-spaced 1.0..0.05, sweep = 1 lifted Metropolis + 1 OR, SSF of 9 "
+                    f"correlators over 16 sublattices every sweep ({buffer}-sample MFMA flushes into FP64), "
+                    "exchange every sweep",
+        "site_updates_per_s": n * R * 2 * rounds / wall,
+        "ms_per_sweep": 1e3 * wall / rounds,
+        "rounds": rounds,
+        "swap_acceptance_min": float(np.min(st["accepts"] / np.maximum(st["attempts"], 1))),
+        "kernel_ms_per_round": {k: v[0] / 16 for k, v in prof.items() if v[1]},
     }
 
 
@@ -315,6 +360,8 @@ def main():
             out["large_lattice_sweep"] = large_lattice_sweep_line(pyro)
         if "munu" in legs:
             out["munu_correlators"] = munu_correlators(pyro)
+        if "c5" in legs:
+            out["large_tempering_run"] = large_tempering_run(pyro)
         if "batch" in legs:
             out["uneven_batch"] = uneven_batch(pyro)
         print(json.dumps(out))
@@ -367,6 +414,7 @@ def main():
     cpu = cpu_baseline(args.cpu_seconds) if "cpu" in legs else None
     big = large_lattice_sweep_line(pyro) if "large" in legs else None
     munu = munu_correlators(pyro) if "munu" in legs else None
+    c5 = large_tempering_run(pyro) if "c5" in legs else None
     unev = uneven_batch(pyro) if "batch" in legs else None
 
     line = {
@@ -410,6 +458,7 @@ def main():
         "swap_acceptance_min": float(min(acc_rate)) if acc_rate else None,
         "large_lattice_sweep": big,
         "munu_correlators": munu,
+        "large_tempering_run": c5,
         "uneven_batch": unev,
         "cpu_baseline": cpu,
     }

@@ -496,7 +496,17 @@ struct pyro_ctx {
         double* e2 = accumulate ? E2sum.as<double>() : nullptr;
         unsigned long long* ne = accumulate ? nE.as<unsigned long long>() : nullptr;
         if (stencil_observables()) {
+            // lattices the tiled sweep takes: tiles read once into LDS
+            // (partials per tile); others: one 64-site block per partial
+            int nblk = pyro::pyro_energy_blocks(int(N()));
             timed(kFamEnergy, [&] {
+                if (tile_kind != 0)
+                    return prec == 64 ? pyro::launch_pyro_energy_tiled<double>(state<double>(nullptr),
+                                                                               pyro_couplings<double>(), tile_kind,
+                                                                               epart.as<double>(), &nblk, stream)
+                                      : pyro::launch_pyro_energy_tiled<float>(state<float>(nullptr),
+                                                                              pyro_couplings<float>(), tile_kind,
+                                                                              epart.as<double>(), &nblk, stream);
                 return prec == 64
                            ? pyro::launch_pyro_energy<double>(state<double>(nullptr), pyro_couplings<double>(),
                                                               epart.as<double>(), stream)
@@ -504,8 +514,8 @@ struct pyro_ctx {
                                                              epart.as<double>(), stream);
             });
             timed(kFamEnergy, [&] {
-                return pyro::launch_energy_finish(epart.as<double>(), pyro::pyro_energy_blocks(int(N())), int(nc()),
-                                                  e.as<double>(), slot_of.as<int32_t>(), es, e2, ne, R, stream);
+                return pyro::launch_energy_finish(epart.as<double>(), nblk, int(nc()), e.as<double>(),
+                                                  slot_of.as<int32_t>(), es, e2, ne, R, stream);
             });
             return;
         }

@@ -138,6 +138,13 @@ inline int pyro_energy_blocks(int N) { return (N + 63) / 64; }
 template <typename Real>
 hipError_t launch_pyro_energy(const DeviceState<Real>& st, const PyroCouplings<Real>& J, double* partial,
                               hipStream_t s);
+// Tiled form of launch_pyro_energy for a tile kind of pyro_tile_dims whose
+// edges divide the lattice: one workgroup per (tile, replica) reads the tile
+// and its one-sided halo into LDS once; partial[R][*nblk], *nblk = tiles per
+// replica.  Each site's term is computed as launch_pyro_energy computes it.
+template <typename Real>
+hipError_t launch_pyro_energy_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
+                                    double* partial, int* nblk, hipStream_t s);
 // Local field (without the global field) of one replica, out[3][N], with
 // the stencil field.
 template <typename Real>

@@ -155,10 +155,10 @@ __device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Re
 // B0 x B1 x B2 = (T0 + 1) x (T1 + 1) x (T2 + 1) cells shifted down one cell
 // on the axes where its halo is on the lower side (pyrostencil::halo_lo);
 // lc = (i0 * B1 + i1) * B2 + i2 for the site's tile cell (i0, i1, i2).
-template <typename Real, int SHELLS, int SL, int B1, int B2, int BC>
+template <typename Real, int SHELLS, int SL, int B1, int B2, int BC, bool FWD = false>
 __device__ __forceinline__ void pyro_field_lds(const Real* lx, const Real* ly, const Real* lz, int lc,
                                                const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
-    pyro_field_with<Real, SHELLS, SL>(
+    pyro_field_with<Real, SHELLS, SL, FWD>(
         [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
             using pyrostencil::halo_lo;
             const int j = lc + (n.sl * BC + (n.d[0] + halo_lo(n.sl, 0)) * B1 * B2 +
@@ -392,6 +392,34 @@ struct TileLanes {
 template <int T0, int T1, int T2, int BS>
 inline constexpr TileLanes<T0, T1, T2, BS> kTileLanes{};
 
+// Copies the tile at cell (o0, o1, o2) with its one-sided halo into LDS,
+// [sl][box cell]: sublattice sl's box of (T0 + 1)(T1 + 1)(T2 + 1) cells is
+// shifted down one cell on the axes where its halo is on the lower side
+// (halo_lo(sl, a) = 1 iff R_fcc[sl / 4] has 4 on axis a), coordinates
+// wrapped periodically.
+template <typename Real, int T0, int T1, int T2, int BS>
+__device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, const Real* __restrict__ Sx,
+                                              const Real* __restrict__ Sy, const Real* __restrict__ Sz, int o0,
+                                              int o1, int o2, int L0, int L1, int L2, int nc) {
+    constexpr int B1 = T1 + 1, B2 = T2 + 1, BC = (T0 + 1) * B1 * B2;
+    for (int e = threadIdx.x; e < 16 * BC; e += BS) {
+        const int sl = e / BC, bc = e % BC;
+        const int a2 = bc % B2, a1 = (bc / B2) % B1, a0 = bc / (B1 * B2);
+        const int f = sl >> 2;
+        int g0 = o0 + a0 - (f == 2 || f == 3), g1 = o1 + a1 - (f == 1 || f == 3), g2 = o2 + a2 - (f == 1 || f == 2);
+        g0 += g0 < 0 ? L0 : 0;
+        g0 -= g0 >= L0 ? L0 : 0;
+        g1 += g1 < 0 ? L1 : 0;
+        g1 -= g1 >= L1 ? L1 : 0;
+        g2 += g2 < 0 ? L2 : 0;
+        g2 -= g2 >= L2 ? L2 : 0;
+        const unsigned gi = unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2);
+        lx[e] = ld32(Sx, gi);
+        ly[e] = ld32(Sy, gi);
+        lz[e] = ld32(Sz, gi);
+    }
+}
+
 // Tiled sweep: one workgroup per (tile, replica) of one checkerboard phase.
 // The T0 x T1 x T2 tile and its one-sided one-cell halo (per sublattice the
 // side its bonds reach, so (T0 + 1)(T1 + 1)(T2 + 1) cells of each of the 16
@@ -444,22 +472,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     Real* __restrict__ Sy = Sx + N;
     Real* __restrict__ Sz = Sy + N;
 
-    for (int e = threadIdx.x; e < 16 * BC; e += BS) {
-        const int sl = e / BC, bc = e % BC;
-        const int a2 = bc % B2, a1 = (bc / B2) % B1, a0 = bc / (B1 * B2);
-        const int f = sl >> 2;  // halo_lo(sl, a) = 1 iff R_fcc[f] has 4 on axis a
-        int g0 = o0 + a0 - (f == 2 || f == 3), g1 = o1 + a1 - (f == 1 || f == 3), g2 = o2 + a2 - (f == 1 || f == 2);
-        g0 += g0 < 0 ? L0 : 0;
-        g0 -= g0 >= L0 ? L0 : 0;
-        g1 += g1 < 0 ? L1 : 0;
-        g1 -= g1 >= L1 ? L1 : 0;
-        g2 += g2 < 0 ? L2 : 0;
-        g2 -= g2 >= L2 ? L2 : 0;
-        const unsigned gi = unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2);
-        lx[e] = ld32(Sx, gi);
-        ly[e] = ld32(Sy, gi);
-        lz[e] = ld32(Sz, gi);
-    }
+    load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sx, Sy, Sz, o0, o1, o2, L0, L1, L2, nc);
     int8_t* const dir_r = st.dir + size_t(r) * N;
     if constexpr (LIFTED) {
         for (int w = threadIdx.x; w < 16 * W; w += BS) ldw[w] = 0u;
@@ -556,6 +569,77 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
         unsigned long long t = 0;
         for (int i = 0; i < BS / 64; ++i) t += red[i];
         atomicAdd(st.acc + r, t);
+    }
+}
+
+// Energy partial sums of pyro_energy_kernel from tiles in LDS, for lattices
+// the tiled sweep takes: one workgroup per (tile, replica) over all tiles
+// (no phases: the kernel only reads), the tile and its one-sided halo read
+// once.  The tile's sites are then taken as (sublattice, 64-cell chunk)
+// items, item i by wave i mod (BS / 64), so a wave's lanes share one
+// sublattice and the 16 sublattices are spread over the waves; each site
+// adds S_i . h_i over the bonds to higher sublattices minus B . S_i, with
+// the same per-site arithmetic as pyro_energy_kernel.  partial[r][tile].
+template <typename Real, int SHELLS, int T0, int T1, int T2, int BS>
+__global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
+                                                               double* __restrict__ partial) {
+    constexpr int B1 = T1 + 1, B2 = T2 + 1, BC = (T0 + 1) * B1 * B2;
+    extern __shared__ __align__(16) unsigned char pyro_smem[];
+    Real* lx = reinterpret_cast<Real*>(pyro_smem);
+    Real* ly = lx + 16 * BC;
+    Real* lz = ly + 16 * BC;
+    const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica
+    unsigned g = blockIdx.x;
+    if ((gridDim.x & 7u) == 0u) g = (g & 7u) * (gridDim.x >> 3) + (g >> 3);
+    const int r = int(g / tiles);
+    const int tile = int(g - unsigned(r) * tiles);
+    const int q1n = st.L1 / T1, q2n = st.L2 / T2;
+    const int m2 = tile % q2n;
+    const int t12 = tile / q2n;
+    const int m1 = t12 % q1n, m0 = t12 / q1n;
+    const size_t N = size_t(st.N);
+    const Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
+    load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sx, Sx + N, Sx + 2 * N, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1,
+                                        st.L2, st.n_cells);
+    __syncthreads();
+    const PyroCouplings<Real> Jr = J.at(r);
+    const Real bx = st.B[3 * r], by = st.B[3 * r + 1], bz = st.B[3 * r + 2];
+    double e = 0.0;
+    constexpr int NI = T0 * T1 * T2, CH = (NI + 63) / 64, WAVES = BS / 64;
+    for (int item = threadIdx.x >> 6; item < 16 * CH; item += WAVES) {
+        const int sl = item / CH;  // uniform across the wave
+        const int m = (item - sl * CH) * 64 + (threadIdx.x & 63);
+        if (m < NI) {
+            const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
+            const int lc = (i0 * B1 + i1) * B2 + i2;
+            const int f = sl >> 2;
+            const int own = ((f == 2 || f == 3) * B1 + (f == 1 || f == 3)) * B2 + (f == 1 || f == 2);
+            Real hx = Real(0), hy = Real(0), hz = Real(0);
+            switch (sl) {
+#define PYRO_SL_CASE(X) \
+    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, BC, true>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
+                PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+                PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+                PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+                PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+                default: break;
+            }
+            const int li = sl * BC + lc + own;
+            const Real sx = lx[li], sy = ly[li], sz = lz[li];
+            const Real bond = sx * hx + sy * hy + sz * hz;
+            const Real bs = sx * bx + sy * by + sz * bz;
+            e += double(bond) - double(bs);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    __shared__ double red[BS / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < BS / 64; ++i) t += red[i];
+        partial[size_t(r) * tiles + tile] = t;
     }
 }
 
@@ -663,6 +747,40 @@ hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Re
 #undef PYRO_TILED_LAUNCH
                 default: return hipErrorInvalidValue;
             }
+            return hipGetLastError();
+        });
+    };
+    using I4 = std::integral_constant<int, 4>;
+    using I6 = std::integral_constant<int, 6>;
+    using I8 = std::integral_constant<int, 8>;
+    switch (tile_kind) {
+        case 1: return go(I4{}, I4{}, I4{});
+        case 2: return go(I6{}, I6{}, I4{});
+        case 3: return go(I8{}, I4{}, I4{});
+        case 4: return go(I4{}, I4{}, I8{});
+        case 5: return go(I8{}, I8{}, I4{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename Real>
+hipError_t launch_pyro_energy_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
+                                    double* partial, int* nblk, hipStream_t s) {
+    auto go = [&](auto t0, auto t1, auto t2) -> hipError_t {
+        constexpr int T0 = decltype(t0)::value, T1 = decltype(t1)::value, T2 = decltype(t2)::value;
+        constexpr int BS = 512;  // 8 waves share the tile's (sublattice, chunk) items
+        if (st.L0 % T0 || st.L1 % T1 || st.L2 % T2) return hipErrorInvalidValue;
+        const size_t lds = pyro_tile_lds(T0, T1, T2, int(sizeof(Real)), false);
+        const unsigned tiles = unsigned((st.L0 / T0) * (st.L1 / T1) * (st.L2 / T2));
+        *nblk = int(tiles);
+        const dim3 grid(tiles * unsigned(st.R));  // (replica, tile), XCD-chunked in the kernel
+        return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+            constexpr int SH = decltype(shells)::value;
+            auto* fn = pyro_energy_tiled_kernel<Real, SH, T0, T1, T2, BS>;
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(fn, grid, dim3(BS), lds, s, st, J, partial);
             return hipGetLastError();
         });
     };

@@ -73,6 +73,39 @@ def test_fields_and_energy(gpu, oracle, prec, J, Delta, B, state):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("L,tile", [(8, None), (16, None), (24, None), (16, (8, 4, 4)), (32, (8, 8, 4)),
+                                    (16, (4, 4, 8)), (24, (6, 6, 4))])
+def test_energy_tiled(gpu, oracle, prec, L, tile):
+    """Lattices the tiled sweep takes get their energies from tiles in LDS
+    (one workgroup per tile and replica): against the oracle for XXZ
+    couplings, all three shells, a field, and per-replica models."""
+    J, Delta, B = (1.0, -0.05, 0.05), 0.7, (0.05, -0.1, 0.2)
+    R = 3
+    ctx, os_ = make(gpu, oracle, L, J, Delta, prec, R=R, B=B)
+    if tile is not None:
+        ctx.set_tile_shape(tile)
+    assert ctx.tile_shape() != (0, 0, 0)
+    rng = np.random.default_rng(L)
+    for r in range(R):
+        S = random_spins(ctx.N, rng)
+        ctx.set_spins(S, r)
+        os_[r].set_spins(S)
+    e_scale = 16 * 0.5 * field_scale(J, B)
+    e = ctx.energy_per_cell()
+    for r in range(R):
+        assert abs(e[r] - os_[r].energy_per_cell()) <= TOL[prec] * e_scale
+    # per-replica models: each replica against an oracle of its own couplings
+    Jrep = np.array([[1.0, 0.1, -0.1, 1.0], [-0.5, 0.2, 0.0, 1.3], [1.0, 0.0, 0.15, 0.8]])
+    ctx.set_replica_couplings(Jrep)
+    e = ctx.energy_per_cell()
+    for r in range(R):
+        o = oracle.Oracle.pyrochlore(L, *Jrep[r])
+        o.set_field(np.asarray(B, float))
+        o.set_spins(ctx.get_spins(r))
+        assert abs(e[r] - o.energy_per_cell()) <= TOL[prec] * 16 * 0.5 * field_scale(tuple(Jrep[r, :3]), B)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
 def test_randomize(gpu, oracle, prec):
     ctx, os_ = make(gpu, oracle, 2, prec=prec, R=3)
     ctx.randomize(987654321)
