@@ -549,8 +549,10 @@ struct pyro_ctx {
             } else {
                 const auto st = ssf_state<float>();
                 timed(kFamDft, [&] { return pyro::launch_dft<float>(st, r0, rc, s); });
-                timed(kFamContract, [&] { return pyro::launch_contract<float>(st, c, r0, rc, s); });
-                if (munu) timed(kFamMunu, [&] { return pyro::launch_munu_stage<float>(st, slots, r0, rc, munu_args(), s); });
+                if (munu)  // contraction and staging from one read of the fields
+                    timed(kFamContract, [&] { return pyro::launch_contract_stage(st, c, r0, rc, munu_args(), s); });
+                else
+                    timed(kFamContract, [&] { return pyro::launch_contract<float>(st, c, r0, rc, s); });
             }
         }
     }

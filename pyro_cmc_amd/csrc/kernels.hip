@@ -961,6 +961,83 @@ __global__ __launch_bounds__(kBlock) void munu_stage_kernel(SsfState<Real> st,
     }
 }
 
+// contract_kernel and munu_stage_kernel from one read of the transformed
+// fields, for FP32 contexts with the sublattice-resolved correlators: the
+// 64 k of a workgroup are transposed through LDS as for the staging and the
+// row block is written to the buffer; wave a < 3 then folds component a of
+// the same 64 k from LDS (b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K), in the
+// order of contract_kernel) and every wave updates the correlators cc = w,
+// w + 4, ... of its slot.
+template <int T>
+__global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> st, CorrArgs c, int r_begin,
+                                                                float* __restrict__ buf,
+                                                                const unsigned int* __restrict__ cnt, int Bmax) {
+    constexpr int FW = 2 * 16 * T;
+    __shared__ float xs[FW][65];  // [feature][k], padded against bank conflicts
+    const int rl = blockIdx.y;
+    const int slot = c.slot_of[r_begin + rl];
+    if (slot < 0) return;
+    const int nc = st.n_cells;
+    const int K0 = int(blockIdx.x) * 64;
+    const int ns = st.n_sl, nf = 3 * ns;
+    const int kl = threadIdx.x & 63;
+    for (int f = int(threadIdx.x >> 6); f < 16 * T; f += kBlock / 64) {
+        float re = 0.f, im = 0.f;
+        const int K = K0 + kl;
+        if (f < nf && K < nc) {
+            const int a = f / ns, mu = f % ns;
+            const float* F = st.F + 2 * (((size_t(rl) * 3 + a) * ns + mu) * size_t(nc) + K);
+            re = F[0];
+            im = F[1];
+        }
+        xs[f][kl] = re;
+        xs[16 * T + f][kl] = im;
+    }
+    __syncthreads();
+    const unsigned int b = cnt[slot];
+    if (b < unsigned(Bmax)) {
+        const int nk = min(64, nc - K0);
+        float* dst = buf + ((size_t(slot) * Bmax + b) * size_t(nc) + K0) * FW;
+        for (int e = threadIdx.x; e < nk * FW; e += kBlock) {
+            const int k = e / FW, f = e - k * FW;
+            dst[e] = xs[f][k];
+        }
+    }
+    const int K = K0 + kl;
+    const int w = int(threadIdx.x >> 6);
+    __shared__ float bsh[2][3][64];
+    if (w < 3 && K < nc) {
+        float sr = 0.f, si = 0.f;
+        for (int mu = 0; mu < ns; ++mu) {
+            const float fr = xs[w * ns + mu][kl], fi = xs[16 * T + w * ns + mu][kl];
+            const float pr = st.phase[2 * (mu * size_t(nc) + K)], pi = st.phase[2 * (mu * size_t(nc) + K) + 1];
+            sr += fr * pr - fi * pi;
+            si += fr * pi + fi * pr;
+        }
+        bsh[0][w][kl] = sr;
+        bsh[1][w][kl] = si;
+    }
+    __syncthreads();
+    if (K < nc) {
+        const float br[3] = {bsh[0][0][kl], bsh[0][1][kl], bsh[0][2][kl]};
+        const float bi[3] = {bsh[1][0][kl], bsh[1][1][kl], bsh[1][2][kl]};
+        for (int cc = w; cc < c.n_corr; cc += kBlock / 64) {
+            const int al = c.alpha[cc], be = c.beta[cc];
+            const double re = double(br[al] * br[be] + bi[al] * bi[be]);
+            const double im = double(br[al] * bi[be] - bi[al] * br[be]);
+            double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+            A[0] += re;
+            A[1] += im;
+            if (c.acc2) {
+                double* A2 = c.acc2 + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+                A2[0] += re * re;
+                A2[1] += im * im;
+            }
+        }
+    }
+    if (K0 == 0 && threadIdx.x == 0) c.n_samples[slot] += 1ull;
+}
+
 __global__ void munu_count_kernel(const int32_t* __restrict__ slot_of, int r_begin, int r_count,
                                   unsigned int* cnt, int Bmax) {
     for (int i = threadIdx.x; i < r_count; i += blockDim.x) {
@@ -1413,6 +1490,19 @@ hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, i
     }
     hipLaunchKernelGGL(munu_count_kernel, dim3(1), dim3(kBlock), 0, s, slot_of, r_begin, r_count,
                        m.cnt, m.Bmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_stage(const SsfState<float>& st, const CorrArgs& c, int r_begin, int r_count,
+                                 const MunuArgs& m, hipStream_t s) {
+    const dim3 grid(unsigned((st.n_cells + 63) / 64), unsigned(r_count));
+    switch (m.T) {
+        case 1: hipLaunchKernelGGL((contract_stage_kernel<1>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 2: hipLaunchKernelGGL((contract_stage_kernel<2>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 3: hipLaunchKernelGGL((contract_stage_kernel<3>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(munu_count_kernel, dim3(1), dim3(kBlock), 0, s, c.slot_of, r_begin, r_count, m.cnt, m.Bmax);
     return hipGetLastError();
 }
 

@@ -245,6 +245,10 @@ struct MunuArgs {
 template <typename Real>
 hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
                              int r_count, const MunuArgs& m, hipStream_t s);
+// launch_contract and launch_munu_stage of FP32 contexts in one kernel (one
+// read of the transformed fields), with the same results.
+hipError_t launch_contract_stage(const SsfState<float>& st, const CorrArgs& c, int r_begin, int r_count,
+                                 const MunuArgs& m, hipStream_t s);
 hipError_t launch_munu_flush(const MunuArgs& m, hipStream_t s);
 
 // Luttinger-Tisza spectrum over the L[0] x L[1] x L[2] grid of k with
