@@ -961,14 +961,14 @@ __global__ __launch_bounds__(kBlock) void munu_stage_kernel(SsfState<Real> st,
     }
 }
 
-// contract_kernel and munu_stage_kernel from one read of the transformed
-// fields, for FP32 contexts with the sublattice-resolved correlators: the
-// 64 k of a workgroup are transposed through LDS as for the staging and the
-// row block is written to the buffer; wave a < 3 then folds component a of
-// the same 64 k from LDS (b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K), in the
-// order of contract_kernel) and every wave updates the correlators cc = w,
-// w + 4, ... of its slot.
-template <int T>
+// contract_kernel for FP32 contexts, and with STAGE also munu_stage_kernel,
+// from one read of the transformed fields: the 64 k of a workgroup are
+// transposed through LDS as for the staging (and with STAGE the row block is
+// written to the buffer); wave a < 3 then folds component a of the same 64 k
+// from LDS (b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K), in the order of
+// contract_kernel) and every wave updates the correlators cc = w, w + 4, ...
+// of its slot.
+template <int T, bool STAGE>
 __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> st, CorrArgs c, int r_begin,
                                                                 float* __restrict__ buf,
                                                                 const unsigned int* __restrict__ cnt, int Bmax) {
@@ -994,8 +994,8 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
         xs[16 * T + f][kl] = im;
     }
     __syncthreads();
-    const unsigned int b = cnt[slot];
-    if (b < unsigned(Bmax)) {
+    const unsigned int b = STAGE ? cnt[slot] : 0u;
+    if (STAGE && b < unsigned(Bmax)) {
         const int nk = min(64, nc - K0);
         float* dst = buf + ((size_t(slot) * Bmax + b) * size_t(nc) + K0) * FW;
         for (int e = threadIdx.x; e < nk * FW; e += kBlock) {
@@ -1473,6 +1473,17 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
 template <typename Real>
 hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_begin, int r_count,
                            hipStream_t s) {
+    if constexpr (std::is_same_v<Real, float>) {
+        // FP32: fields through LDS, the fold split across waves
+        const int T = (3 * st.n_sl + 15) / 16;
+        const dim3 g64(unsigned((st.n_cells + 63) / 64), unsigned(r_count));
+        switch (T) {
+            case 1: hipLaunchKernelGGL((contract_stage_kernel<1, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
+            case 2: hipLaunchKernelGGL((contract_stage_kernel<2, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
+            case 3: hipLaunchKernelGGL((contract_stage_kernel<3, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
+            default: break;
+        }
+    }
     const dim3 grid(unsigned((st.n_cells + kContractBlock - 1) / kContractBlock), unsigned(r_count));
     hipLaunchKernelGGL(contract_kernel<Real>, grid, dim3(kContractBlock), 0, s, st, c, r_begin);
     return hipGetLastError();
@@ -1497,9 +1508,9 @@ hipError_t launch_contract_stage(const SsfState<float>& st, const CorrArgs& c, i
                                  const MunuArgs& m, hipStream_t s) {
     const dim3 grid(unsigned((st.n_cells + 63) / 64), unsigned(r_count));
     switch (m.T) {
-        case 1: hipLaunchKernelGGL((contract_stage_kernel<1>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
-        case 2: hipLaunchKernelGGL((contract_stage_kernel<2>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
-        case 3: hipLaunchKernelGGL((contract_stage_kernel<3>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 1: hipLaunchKernelGGL((contract_stage_kernel<1, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 2: hipLaunchKernelGGL((contract_stage_kernel<2, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 3: hipLaunchKernelGGL((contract_stage_kernel<3, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
         default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(munu_count_kernel, dim3(1), dim3(kBlock), 0, s, c.slot_of, r_begin, r_count, m.cnt, m.Bmax);
