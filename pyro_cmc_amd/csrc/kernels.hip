@@ -1293,12 +1293,128 @@ hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
 // cubic lattices of edge L = 4, 8, 12 or 16.  The field (L^3 reals) is read
 // once, transformed along axis 2, 1, 0 in place and written once as
 // F[r][a][mu][K], the layout of the per-axis path.  A thread owns one line
-// per pass: it keeps the line's L values in registers and forms all L
-// outputs with compile-time twiddle indices (2 L^2 independent FMA chains),
-// then writes them back over its own line, so lines never overlap.
+// per pass: it keeps the line's L values in registers, transforms them with
+// dft_line (one mixed-radix step over 4-point DFTs, compile-time twiddles;
+// the first pass on a real line), then writes them back over its own line,
+// so lines never overlap.
 // a thread per line: L^2 lines rounded up to whole waves
 template <int L>
 constexpr int dft3_threads() { return (L * L + 63) / 64 * 64; }
+
+// Compile-time twiddles e^{-2 pi i m / L} for the whole-field DFT: Taylor
+// series in double precision (exact 0 and +-1 at the quarter turns, so the
+// compiler folds those products away).
+namespace dftc {
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double taylor_sin(double x) {
+    double t = x, s = x;
+    for (int n = 1; n < 24; ++n) {
+        t *= -x * x / double((2 * n) * (2 * n + 1));
+        s += t;
+    }
+    return s;
+}
+constexpr double taylor_cos(double x) {
+    double t = 1.0, s = 1.0;
+    for (int n = 1; n < 24; ++n) {
+        t *= -x * x / double((2 * n - 1) * (2 * n));
+        s += t;
+    }
+    return s;
+}
+// real and imaginary parts of e^{-2 pi i m / L}
+constexpr double wre(int m, int L) {
+    m = ((m % L) + L) % L;
+    if ((4 * m) % L == 0) return (4 * m) / L == 0 ? 1.0 : ((4 * m) / L == 2 ? -1.0 : 0.0);
+    const int mc = 2 * m > L ? m - L : m;  // angle in (-pi, pi]
+    return taylor_cos(kTwoPi * double(mc) / double(L));
+}
+constexpr double wim(int m, int L) {
+    m = ((m % L) + L) % L;
+    if ((4 * m) % L == 0) return (4 * m) / L == 1 ? -1.0 : ((4 * m) / L == 3 ? 1.0 : 0.0);
+    const int mc = 2 * m > L ? m - L : m;
+    return -taylor_sin(kTwoPi * double(mc) / double(L));
+}
+}  // namespace dftc
+
+// In-register DFT of P = 2, 3 or 4 points (e^{-2 pi i n k / P}).
+template <typename Real, int P>
+__device__ __forceinline__ void dft_small(Real (&r)[P], Real (&i)[P]) {
+    if constexpr (P == 2) {
+        const Real ar = r[0] + r[1], ai = i[0] + i[1];
+        r[1] = r[0] - r[1];
+        i[1] = i[0] - i[1];
+        r[0] = ar;
+        i[0] = ai;
+    } else if constexpr (P == 3) {
+        constexpr Real h = Real(0.86602540378443864676);  // sqrt(3) / 2
+        const Real sr = r[1] + r[2], si = i[1] + i[2], dr = r[1] - r[2], di = i[1] - i[2];
+        const Real mr = r[0] - Real(0.5) * sr, mi = i[0] - Real(0.5) * si;
+        r[0] += sr;
+        i[0] += si;
+        r[1] = mr + h * di;
+        i[1] = mi - h * dr;
+        r[2] = mr - h * di;
+        i[2] = mi + h * dr;
+    } else {
+        static_assert(P == 4, "2, 3 or 4 points");
+        const Real ar = r[0] + r[2], ai = i[0] + i[2], br = r[0] - r[2], bi = i[0] - i[2];
+        const Real cr = r[1] + r[3], ci = i[1] + i[3], er = r[1] - r[3], ei = i[1] - i[3];
+        r[0] = ar + cr;
+        i[0] = ai + ci;
+        r[2] = ar - cr;
+        i[2] = ai - ci;
+        r[1] = br + ei;
+        i[1] = bi - er;
+        r[3] = br - ei;
+        i[3] = bi + er;
+    }
+}
+
+// In-register DFT of L = 4, 8, 12 or 16 points by one mixed-radix step,
+// L = P Q with P = 4 (L = 4: P = 4, Q = 1): n = Q n1 + n2, k = k1 + P k2,
+//   Z[n2][k1] = DFT_P over n1 of x[Q n1 + n2],  Z *= w_L^(n2 k1),
+//   X[k1 + P k2] = DFT_Q over n2 of Z[n2][k1],
+// with compile-time twiddles.
+template <typename Real, int L>
+__device__ __forceinline__ void dft_line(Real (&r)[L], Real (&i)[L]) {
+    constexpr int P = 4, Q = L / 4;
+    static_assert(L % 4 == 0 && Q >= 1 && Q <= 4, "4, 8, 12 or 16 points");
+    Real zr[Q][P], zi[Q][P];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) {
+            zr[n2][n1] = r[Q * n1 + n2];
+            zi[n2][n1] = i[Q * n1 + n2];
+        }
+        dft_small<Real, P>(zr[n2], zi[n2]);
+    }
+    if constexpr (Q == 1) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            r[k] = zr[0][k];
+            i[k] = zi[0][k];
+        }
+    } else {
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) {
+            Real ur[Q], ui[Q];
+#pragma unroll
+            for (int n2 = 0; n2 < Q; ++n2) {
+                const Real c = Real(dftc::wre(n2 * k1, L)), d = Real(dftc::wim(n2 * k1, L));
+                ur[n2] = zr[n2][k1] * c - zi[n2][k1] * d;
+                ui[n2] = zr[n2][k1] * d + zi[n2][k1] * c;
+            }
+            if constexpr (Q == 2 || Q == 3 || Q == 4) dft_small<Real, Q>(ur, ui);
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2) {
+                r[k1 + P * k2] = ur[k2];
+                i[k1 + P * k2] = ui[k2];
+            }
+        }
+    }
+}
 
 template <typename Real, int L>
 __global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __restrict__ spins,
@@ -1306,45 +1422,32 @@ __global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __r
                                                                  const Real* __restrict__ tw, int n_sl) {
     constexpr int NC = L * L * L, LINES = L * L, BS = dft3_threads<L>();
     __shared__ Real x[2 * NC];
-    __shared__ Real w[2 * L];
+    (void)tw;  // twiddles are compile-time constants (dft_line)
     const size_t f_off = (size_t(blockIdx.y) * 3 * n_sl + size_t(blockIdx.x)) * size_t(NC);
     const Real* src = spins + f_off;
     for (int i = threadIdx.x; i < NC; i += BS) {
         x[2 * i] = src[i];
         x[2 * i + 1] = Real(0);
     }
-    if (threadIdx.x < 2 * L) w[threadIdx.x] = tw[threadIdx.x];
     __syncthreads();
 #pragma unroll
     for (int pass = 0; pass < 3; ++pass) {
         const int sa = pass == 0 ? 1 : (pass == 1 ? L : L * L);
         for (int line = threadIdx.x; line < LINES; line += BS) {
             const int base = (line / sa) * (sa * L) + line % sa;
-            Real xr[L], xi[L], yr[L], yi[L];
+            Real xr[L], xi[L];
 #pragma unroll
             for (int j = 0; j < L; ++j) {
                 xr[j] = x[2 * (base + j * sa)];
-                xi[j] = x[2 * (base + j * sa) + 1];
+                // the first pass transforms real spins: a literal zero lets
+                // the compiler drop the imaginary half of its arithmetic
+                xi[j] = pass == 0 ? Real(0) : x[2 * (base + j * sa) + 1];
             }
+            dft_line<Real, L>(xr, xi);
 #pragma unroll
             for (int k = 0; k < L; ++k) {
-                yr[k] = xr[0];
-                yi[k] = xi[0];
-            }
-#pragma unroll
-            for (int j = 1; j < L; ++j) {
-#pragma unroll
-                for (int k = 0; k < L; ++k) {
-                    const int t = (j * k) % L;
-                    const Real cr = w[2 * t], ci = w[2 * t + 1];
-                    yr[k] += xr[j] * cr - xi[j] * ci;
-                    yi[k] += xr[j] * ci + xi[j] * cr;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < L; ++k) {
-                x[2 * (base + k * sa)] = yr[k];
-                x[2 * (base + k * sa) + 1] = yi[k];
+                x[2 * (base + k * sa)] = xr[k];
+                x[2 * (base + k * sa) + 1] = xi[k];
             }
         }
         __syncthreads();
