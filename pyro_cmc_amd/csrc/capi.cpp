@@ -309,6 +309,22 @@ struct pyro_ctx {
             if (tile_applies(kind)) return kind;
         return 0;
     }
+    // Tile shape of the tiled energy kernel: the sweep's when it has one,
+    // else the first shape (same order) whose edges divide L (the energy
+    // only reads, so no checkerboard of tiles is needed) and whose tile +
+    // halo fits the LDS limit; 0 = the global-memory stencil kernel.
+    int energy_tile_kind() const {
+        if (tile_kind) return tile_kind;
+        if (!spec) return 0;
+        for (int kind : kTileOrder) {
+            int d[3];
+            pyro::pyro_tile_dims(kind, d);
+            if (geom.L[0] % d[0] == 0 && geom.L[1] % d[1] == 0 && geom.L[2] % d[2] == 0 &&
+                pyro::pyro_tile_lds(d[0], d[1], d[2], int(real_size()), false) <= max_lds)
+                return kind;
+        }
+        return 0;
+    }
 
     template <typename Real>
     pyro::SsfState<Real> ssf_state() const {
@@ -496,16 +512,17 @@ struct pyro_ctx {
         double* e2 = accumulate ? E2sum.as<double>() : nullptr;
         unsigned long long* ne = accumulate ? nE.as<unsigned long long>() : nullptr;
         if (stencil_observables()) {
-            // lattices the tiled sweep takes: tiles read once into LDS
+            // lattices a tile shape divides: tiles read once into LDS
             // (partials per tile); others: one 64-site block per partial
             int nblk = pyro::pyro_energy_blocks(int(N()));
+            const int ek = energy_tile_kind();
             timed(kFamEnergy, [&] {
-                if (tile_kind != 0)
+                if (ek != 0)
                     return prec == 64 ? pyro::launch_pyro_energy_tiled<double>(state<double>(nullptr),
-                                                                               pyro_couplings<double>(), tile_kind,
+                                                                               pyro_couplings<double>(), ek,
                                                                                epart.as<double>(), &nblk, stream)
                                       : pyro::launch_pyro_energy_tiled<float>(state<float>(nullptr),
-                                                                              pyro_couplings<float>(), tile_kind,
+                                                                              pyro_couplings<float>(), ek,
                                                                               epart.as<double>(), &nblk, stream);
                 return prec == 64
                            ? pyro::launch_pyro_energy<double>(state<double>(nullptr), pyro_couplings<double>(),
