@@ -1011,17 +1011,13 @@ int pyro_set_spins(pyro_ctx* ctx, int replica, const double* S) {
         const int64_t N = ctx->N();
         for (int64_t i = 0; i < 3 * N; ++i)
             require(std::isfinite(S[i]), PYRO_E_INVALID, "spin components must be finite");
-        const size_t off = size_t(replica) * 3 * N;
+        const size_t off = size_t(replica) * 3 * N;  // device spins S[r][i][a], the host layout
         if (ctx->prec == 64) {
-            std::vector<double> soa(3 * N);
-            for (int64_t i = 0; i < N; ++i)
-                for (int a = 0; a < 3; ++a) soa[a * N + i] = S[3 * i + a];
-            ctx->h2d(ctx->S.as<double>() + off, soa.data(), soa.size() * 8);
+            ctx->h2d(ctx->S.as<double>() + off, S, size_t(3 * N) * 8);
         } else {
-            std::vector<float> soa(3 * N);
-            for (int64_t i = 0; i < N; ++i)
-                for (int a = 0; a < 3; ++a) soa[a * N + i] = float(S[3 * i + a]);
-            ctx->h2d(ctx->S.as<float>() + off, soa.data(), soa.size() * 4);
+            std::vector<float> v(3 * N);
+            for (int64_t i = 0; i < 3 * N; ++i) v[i] = float(S[i]);
+            ctx->h2d(ctx->S.as<float>() + off, v.data(), v.size() * 4);
         }
     });
 }
@@ -1035,15 +1031,11 @@ int pyro_get_spins(pyro_ctx* ctx, int replica, double* S) {
         const size_t off = size_t(replica) * 3 * N;
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (ctx->prec == 64) {
-            std::vector<double> soa(3 * N);
-            ctx->d2h(soa.data(), ctx->S.as<double>() + off, soa.size() * 8);
-            for (int64_t i = 0; i < N; ++i)
-                for (int a = 0; a < 3; ++a) S[3 * i + a] = soa[a * N + i];
+            ctx->d2h(S, ctx->S.as<double>() + off, size_t(3 * N) * 8);
         } else {
-            std::vector<float> soa(3 * N);
-            ctx->d2h(soa.data(), ctx->S.as<float>() + off, soa.size() * 4);
-            for (int64_t i = 0; i < N; ++i)
-                for (int a = 0; a < 3; ++a) S[3 * i + a] = soa[a * N + i];
+            std::vector<float> v(3 * N);
+            ctx->d2h(v.data(), ctx->S.as<float>() + off, v.size() * 4);
+            for (int64_t i = 0; i < 3 * N; ++i) S[i] = v[i];
         }
     });
 }

@@ -71,6 +71,25 @@ __device__ __forceinline__ Real ld32(const Real* __restrict__ base, unsigned idx
                                           static_cast<size_t>(idx * unsigned(sizeof(Real))));
 }
 
+// One site's spin (x, y, z), the unit of the spin layout S[r][i][3].
+template <typename Real>
+struct Vec3 {
+    Real x, y, z;
+};
+// Spin i of a replica's block Sr (= S + r * 3 N) in one 12-byte (FP32) or
+// 24-byte (FP64) access, through a 32-bit byte offset from a wave-uniform
+// base as ld32 does.
+template <typename Real>
+__device__ __forceinline__ Vec3<Real> ld3(const Real* __restrict__ Sr, unsigned i) {
+    return *reinterpret_cast<const Vec3<Real>*>(reinterpret_cast<const char*>(Sr) +
+                                                static_cast<size_t>(i * unsigned(3 * sizeof(Real))));
+}
+template <typename Real>
+__device__ __forceinline__ void st3(Real* __restrict__ Sr, unsigned i, Real x, Real y, Real z) {
+    *reinterpret_cast<Vec3<Real>*>(reinterpret_cast<char*>(Sr) + static_cast<size_t>(i * unsigned(3 * sizeof(Real)))) =
+        Vec3<Real>{x, y, z};
+}
+
 // FP32 spins are renormalised after every accepted change (drift control);
 // FP64 follows the reference exactly (no renormalisation after OR).
 template <typename Real> struct Renorm { static constexpr bool value = false; };
@@ -244,28 +263,24 @@ __device__ __forceinline__ bool update_spin_p(const ReplicaParams<Real>& p, cons
     return update_spin_u<Real, KIND>(p, px, py, pz, changed, site_uniforms(st, site, r, pass), hx, hy, hz, dp);
 }
 
-// update_spin_p on the spin stored in global memory (S[r][c][site]).
+// update_spin_p on the spin stored in global memory (Sr = S + r * 3 N,
+// site i at Sr[3 i .. 3 i + 2]).
 template <typename Real, int KIND>
 __device__ __forceinline__ bool apply_update_p(const ReplicaParams<Real>& p, const DeviceState<Real>& st,
-                                               Real* __restrict__ Sx, Real* __restrict__ Sy, Real* __restrict__ Sz,
-                                               int site, int r, unsigned long long pass, Real hx, Real hy,
-                                               Real hz) {
-    Real x = Sx[site], y = Sy[site], z = Sz[site];
+                                               Real* __restrict__ Sr, int site, int r, unsigned long long pass,
+                                               Real hx, Real hy, Real hz) {
+    const Vec3<Real> v = ld3(Sr, unsigned(site));
+    Real x = v.x, y = v.y, z = v.z;
     bool changed;
     const bool acc = update_spin_p<Real, KIND>(p, st, x, y, z, changed, site, r, pass, hx, hy, hz,
                                                st.dir + size_t(r) * size_t(st.N) + site);
-    if (changed) {
-        Sx[site] = x;
-        Sy[site] = y;
-        Sz[site] = z;
-    }
+    if (changed) st3(Sr, unsigned(site), x, y, z);
     return acc;
 }
 template <typename Real, int KIND>
-__device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sx,
-                                             Real* __restrict__ Sy, Real* __restrict__ Sz, int site,
-                                             int r, unsigned long long pass, Real hx, Real hy, Real hz) {
-    return apply_update_p<Real, KIND>(replica_params(st, r), st, Sx, Sy, Sz, site, r, pass, hx, hy, hz);
+__device__ __forceinline__ bool apply_update(const DeviceState<Real>& st, Real* __restrict__ Sr, int site, int r,
+                                             unsigned long long pass, Real hx, Real hy, Real hz) {
+    return apply_update_p<Real, KIND>(replica_params(st, r), st, Sr, site, r, pass, hx, hy, hz);
 }
 
 // Wave-level count of accepted updates added to acc[r].

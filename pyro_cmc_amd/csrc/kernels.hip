@@ -3,7 +3,7 @@
 // structure-factor correlators and the sublattice-resolved MFMA products.
 //
 // Data layout in HBM (per context):
-//   spins      S[r][c][i]      Real, c = x,y,z; i = sl*n_cells + cell
+//   spins      S[r][i][c]      Real, c = x,y,z; i = sl*n_cells + cell
 //   lifted dir dir[r][i]       int8
 //   neighbours nbr[k][i]       int32, slot-major so a wave's loads coalesce
 //   couplings  cpl[sl][k]      Real scalar (isotropic) or [sl][k][9]
@@ -39,9 +39,8 @@ __device__ inline int decode_site(int desc, int m, int n_cells, int L1, int L2, 
 
 // Local field h_i = sum_k J_k S_{nbr(k)} of site i in replica r (MC.cpp:87-96).
 template <typename Real, int NS, bool ISO>
-__device__ inline void local_field(const DeviceState<Real>& st, const Real* __restrict__ Sx,
-                                   const Real* __restrict__ Sy, const Real* __restrict__ Sz,
-                                   int site, int sl, Real& hx, Real& hy, Real& hz) {
+__device__ inline void local_field(const DeviceState<Real>& st, const Real* __restrict__ Sr, int site, int sl,
+                                   Real& hx, Real& hy, Real& hz) {
     unsigned nb[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) nb[k] = unsigned(ld32(st.nbr, unsigned(k * st.N + site)));
@@ -53,15 +52,17 @@ __device__ inline void local_field(const DeviceState<Real>& st, const Real* __re
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const Real ck = c[k];
-            hx += ck * ld32(Sx, nb[k]);
-            hy += ck * ld32(Sy, nb[k]);
-            hz += ck * ld32(Sz, nb[k]);
+            const Vec3<Real> v = ld3(Sr, nb[k]);
+            hx += ck * v.x;
+            hy += ck * v.y;
+            hz += ck * v.z;
         }
     } else {
         const Real* __restrict__ J = st.cpl + size_t(sl) * NS * 9;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const Real x = ld32(Sx, nb[k]), y = ld32(Sy, nb[k]), z = ld32(Sz, nb[k]);
+            const Vec3<Real> v = ld3(Sr, nb[k]);
+            const Real x = v.x, y = v.y, z = v.z;
             const Real* m = J + 9 * k;
             hx += m[0] * x + m[1] * y + m[2] * z;
             hy += m[3] * x + m[4] * y + m[5] * z;
@@ -73,13 +74,10 @@ __device__ inline void local_field(const DeviceState<Real>& st, const Real* __re
 template <typename Real, int NS, bool ISO, int KIND>
 __device__ inline bool update_site(const DeviceState<Real>& st, const ReplicaParams<Real>& p, int site, int sl,
                                    int r, unsigned long long pass) {
-    const size_t N = size_t(st.N);
-    Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
-    Real* __restrict__ Sy = Sx + N;
-    Real* __restrict__ Sz = Sy + N;
+    Real* __restrict__ Sr = st.S + size_t(r) * 3 * size_t(st.N);
     Real hx, hy, hz;
-    local_field<Real, NS, ISO>(st, Sx, Sy, Sz, site, sl, hx, hy, hz);
-    return apply_update_p<Real, KIND>(p, st, Sx, Sy, Sz, site, r, pass, hx, hy, hz);
+    local_field<Real, NS, ISO>(st, Sr, site, sl, hx, hy, hz);
+    return apply_update_p<Real, KIND>(p, st, Sr, site, r, pass, hx, hy, hz);
 }
 
 // Strategy 1: one launch per colour; grid.x = classes x blocks-per-class,
@@ -160,10 +158,10 @@ __global__ __launch_bounds__(kBlock) void fields_kernel(DeviceState<Real> st, in
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= st.N) return;
     const size_t N = size_t(st.N);
-    const Real* Sx = st.S + size_t(r) * 3 * N;
+    const Real* Sr = st.S + size_t(r) * 3 * N;
     const int sl = i / st.n_cells;
     Real hx, hy, hz;
-    local_field<Real, NS, ISO>(st, Sx, Sx + N, Sx + 2 * N, i, sl, hx, hy, hz);
+    local_field<Real, NS, ISO>(st, Sr, i, sl, hx, hy, hz);
     out[i] = hx;
     out[N + i] = hy;
     out[2 * N + i] = hz;
@@ -192,12 +190,12 @@ __global__ __launch_bounds__(kBlock) void energy_partial_kernel(DeviceState<Real
     const int r = int(ru);
     double e = 0.0;
     if (i < st.N) {
-        const size_t N = size_t(st.N);
-        const Real* Sx = st.S + size_t(r) * 3 * N;
+        const Real* Sr = st.S + size_t(r) * 3 * size_t(st.N);
         const int sl = i / st.n_cells;
         Real hx, hy, hz;
-        local_field<Real, NS, ISO>(st, Sx, Sx + N, Sx + 2 * N, i, sl, hx, hy, hz);
-        const Real sx = Sx[i], sy = Sx[N + i], sz = Sx[2 * N + i];
+        local_field<Real, NS, ISO>(st, Sr, i, sl, hx, hy, hz);
+        const Vec3<Real> si = ld3(Sr, unsigned(i));
+        const Real sx = si.x, sy = si.y, sz = si.z;
         const Real half = Real(0.5) * (sx * hx + sy * hy + sz * hz);
         const Real bs = sx * st.B[3 * r] + sy * st.B[3 * r + 1] + sz * st.B[3 * r + 2];
         e = double(half) - double(bs);
@@ -238,10 +236,7 @@ __global__ __launch_bounds__(kBlock) void randomize_kernel(DeviceState<Real> st,
     const double phi = 6.283185307179586 * u24<double>(u.m[1]);
     const double s = sqrt(1.0 - z * z);
     const size_t N = size_t(st.N);
-    Real* Sx = st.S + size_t(r) * 3 * N;
-    Sx[i] = Real(s * cos(phi));
-    Sx[N + i] = Real(s * sin(phi));
-    Sx[2 * N + i] = Real(z);
+    st3(st.S + size_t(r) * 3 * N, unsigned(i), Real(s * cos(phi)), Real(s * sin(phi)), Real(z));
     st.dir[size_t(r) * N + i] = 1;
 }
 
@@ -805,19 +800,24 @@ __global__ __launch_bounds__(256) void dft_ct_strided_kernel(const float* __rest
 template <int M1, int M2>
 __global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict__ in, float* __restrict__ out,
                                                         const float* __restrict__ tw1,
-                                                        const float* __restrict__ tw2) {
+                                                        const float* __restrict__ tw2, int n_sl, int L0) {
     constexpr int L1 = 16 * M1, L2 = 16 * M2, NP = L1 * L2;
     constexpr int XS = (M1 == 3 && M2 == 3) ? 68 : 71;
     constexpr int ROWS = L1 > L2 ? L1 : L2;
     constexpr int RND = (NP + 255) / 256;
     __shared__ float pr[ROWS * XS];
     __shared__ float pi[ROWS * XS];
-    const size_t base = size_t(blockIdx.x) * NP;  // plane p = field * L0 + n0
+    const size_t base = size_t(blockIdx.x) * NP;  // plane p = field * L0 + n0 of the output
+    // input: field = (r * 3 + a) * n_sl + mu of spins S[r][i][a], every third value
+    const int field = int(blockIdx.x) / L0, n0 = int(blockIdx.x) - field * L0;
+    const int mu = field % n_sl, ra = field / n_sl, a = ra % 3, r = ra / 3;
+    const size_t nc = size_t(L0) * NP;
+    const float* src = in + (size_t(r) * n_sl + mu) * 3 * nc + 3 * size_t(n0) * NP + a;
     float v[RND];
 #pragma unroll
     for (int it = 0; it < RND; ++it) {
         const int e = threadIdx.x + 256 * it;
-        v[it] = e < NP ? in[base + e] : 0.f;
+        v[it] = e < NP ? src[3 * e] : 0.f;
     }
 #pragma unroll
     for (int it = 0; it < RND; ++it) {
@@ -1301,6 +1301,17 @@ hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
 template <int L>
 constexpr int dft3_threads() { return (L * L + 63) / 64 * 64; }
 
+// Real fields out[r][a][i] = S[r][i][a] (N sites per replica) for the
+// per-axis DFT path; one thread per output value.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void deinterleave_kernel(const Real* __restrict__ S, Real* __restrict__ out,
+                                                              long long N, long long total) {
+    const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= total) return;
+    const long long ra = e / N, i = e - ra * N, r = ra / 3, a = ra - 3 * r;
+    out[e] = S[(r * N + i) * 3 + a];
+}
+
 // Compile-time twiddles e^{-2 pi i m / L} for the whole-field DFT: Taylor
 // series in double precision (exact 0 and +-1 at the quarter turns, so the
 // compiler folds those products away).
@@ -1424,9 +1435,13 @@ __global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __r
     __shared__ Real x[2 * NC];
     (void)tw;  // twiddles are compile-time constants (dft_line)
     const size_t f_off = (size_t(blockIdx.y) * 3 * n_sl + size_t(blockIdx.x)) * size_t(NC);
-    const Real* src = spins + f_off;
+    // field (component a, sublattice mu) of spins S[r][i][a]: every third
+    // value from 3 (mu NC) + a; the three components' workgroups are n_sl
+    // apart in the grid, on one XCD when n_sl is a multiple of 8
+    const int a = int(blockIdx.x) / n_sl, mu = int(blockIdx.x) - a * n_sl;
+    const Real* src = spins + (size_t(blockIdx.y) * n_sl + mu) * 3 * size_t(NC) + a;
     for (int i = threadIdx.x; i < NC; i += BS) {
-        x[2 * i] = src[i];
+        x[2 * i] = src[3 * i];
         x[2 * i + 1] = Real(0);
     }
     __syncthreads();
@@ -1545,7 +1560,7 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
             constexpr int M1 = decltype(m1)::value, M2 = decltype(m2)::value;
             const long long planes = total / ((long long)st.L1 * st.L2);
             hipLaunchKernelGGL((dft_plane_kernel<M1, M2>), dim3(unsigned(planes)), dim3(256), 0, s, spins, st.G,
-                               st.tw1, st.tw2);
+                               st.tw1, st.tw2, st.n_sl, st.L0);
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) e = dft_axis<Real, false>(st.G, st.F, st.tw0, st.L0, st.L1 * st.L2, total, s);
             return e;
@@ -1566,8 +1581,12 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
             }
         }
     }
-    // axis 2 (stride 1): spins -> F; axis 1 (stride L2): F -> G; axis 0: G -> F
-    hipError_t e = dft_axis<Real, true>(spins, st.F, st.tw2, st.L2, 1, total, s);
+    // spins S[r][i][a] -> real fields G[r][a][mu][cell]; axis 2 (stride 1):
+    // G -> F; axis 1 (stride L2): F -> G; axis 0: G -> F
+    hipLaunchKernelGGL((deinterleave_kernel<Real>), dim3(unsigned((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       spins, st.G, (long long)st.n_sl * (long long)nc, total);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = dft_axis<Real, true>(st.G, st.F, st.tw2, st.L2, 1, total, s);
     if (e == hipSuccess) e = dft_axis<Real, false>(st.F, st.G, st.tw1, st.L1, st.L2, total, s);
     if (e == hipSuccess) e = dft_axis<Real, false>(st.G, st.F, st.tw0, st.L0, st.L1 * st.L2, total, s);
     return e;

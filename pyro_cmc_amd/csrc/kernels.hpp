@@ -13,10 +13,11 @@
 namespace pyro {
 
 // Device view of a context's sweep state, shared by the sweep, field and
-// energy kernels.  Spins are SoA per replica: S[r][c][i], c = x, y, z.
+// energy kernels.  Spins are S[r][i][c], c = x, y, z: one site's three
+// components are adjacent, so a neighbour gather is one 12-byte (FP32) load.
 template <typename Real>
 struct DeviceState {
-    Real* S = nullptr;                   // [R][3][N]
+    Real* S = nullptr;                   // [R][N][3]
     int8_t* dir = nullptr;               // [R][N]
     const int32_t* nbr = nullptr;        // [NS][N] slot-major neighbour table
     const Real* cpl = nullptr;           // isotropic [n_sl][NS] else [n_sl][NS][9]
@@ -201,7 +202,7 @@ hipError_t launch_reduce_accepts(const unsigned int* acc_wave, int n, unsigned l
 // ---------------------------------------------------------------------------
 template <typename Real>
 struct SsfState {
-    const Real* S = nullptr;         // [R][3][N]
+    const Real* S = nullptr;         // [R][N][3]
     Real* F = nullptr;               // [R][3][n_sl][n_cells] complex (2 Real)
     Real* G = nullptr;               // same, scratch
     const Real* tw0 = nullptr;       // twiddles per axis, [L_a] complex

@@ -136,17 +136,18 @@ __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplin
     }
 }
 
-// Field from spins in global memory, neighbour cells from the wrapped axes.
+// Field from spins in global memory (Sr = S + r * 3 N), neighbour cells
+// from the wrapped axes, one 3-component load per neighbour.
 template <typename Real, int SHELLS, int SL>
-__device__ __forceinline__ void pyro_field(const Real* __restrict__ Sx, const Real* __restrict__ Sy,
-                                           const Real* __restrict__ Sz, int nc, const CellAxes& c,
+__device__ __forceinline__ void pyro_field(const Real* __restrict__ Sr, int nc, const CellAxes& c,
                                            const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
     pyro_field_with<Real, SHELLS, SL>(
         [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
             const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
-            x = ld32(Sx, j);
-            y = ld32(Sy, j);
-            z = ld32(Sz, j);
+            const Vec3<Real> v = ld3(Sr, j);
+            x = v.x;
+            y = v.y;
+            z = v.z;
         },
         J, hx, hy, hz);
 }
@@ -172,12 +173,11 @@ __device__ __forceinline__ void pyro_field_lds(const Real* lx, const Real* ly, c
 
 // Runtime sublattice -> compile-time stencil.
 template <typename Real, int SHELLS>
-__device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sx, const Real* Sy, const Real* Sz, int nc,
-                                     const CellAxes& c, const PyroCouplings<Real>& J, Real& hx,
-                                     Real& hy, Real& hz) {
+__device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sr, int nc, const CellAxes& c,
+                                              const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
     switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field<Real, SHELLS, X>(Sx, Sy, Sz, nc, c, J, hx, hy, hz); break;
+    case X: pyro_field<Real, SHELLS, X>(Sr, nc, c, J, hx, hy, hz); break;
         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
         PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
         PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -210,13 +210,11 @@ __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
     bool accepted = false;
     if (cell < nc) {
         const size_t N = size_t(st.N);
-        Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
-        Real* __restrict__ Sy = Sx + N;
-        Real* __restrict__ Sz = Sy + N;
+        Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
         const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
         Real hx, hy, hz;
-        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, PER_REP ? J.at(r) : J, hx, hy, hz);
-        accepted = apply_update<Real, KIND>(st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
+        pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, PER_REP ? J.at(r) : J, hx, hy, hz);
+        accepted = apply_update<Real, KIND>(st, Sr, sl * nc + cell, r, pass, hx, hy, hz);
     }
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
 }
@@ -239,17 +237,16 @@ __global__ __launch_bounds__(64) void pyro_energy_kernel(DeviceState<Real> st, P
     if (i < st.N) {
         const int nc = st.n_cells;
         const size_t N = size_t(st.N);
-        const Real* Sx = st.S + size_t(r) * 3 * N;
-        const Real* Sy = Sx + N;
-        const Real* Sz = Sy + N;
+        const Real* Sr = st.S + size_t(r) * 3 * N;
         const int sl = i / nc;
         const CellAxes c = cell_axes_of(i - sl * nc, st.L0, st.L1, st.L2);
         const PyroCouplings<Real> Jr = J.at(r);
         auto fetch = [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
             const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
-            x = ld32(Sx, j);
-            y = ld32(Sy, j);
-            z = ld32(Sz, j);
+            const Vec3<Real> v = ld3(Sr, j);
+            x = v.x;
+            y = v.y;
+            z = v.z;
         };
         Real hx = Real(0), hy = Real(0), hz = Real(0);
         switch (sl) {
@@ -262,7 +259,8 @@ __global__ __launch_bounds__(64) void pyro_energy_kernel(DeviceState<Real> st, P
 #undef PYRO_SL_CASE
             default: break;
         }
-        const Real sx = Sx[i], sy = Sy[i], sz = Sz[i];
+        const Vec3<Real> si = ld3(Sr, unsigned(i));
+        const Real sx = si.x, sy = si.y, sz = si.z;
         const Real bond = sx * hx + sy * hy + sz * hz;
         const Real bs = sx * st.B[3 * r] + sy * st.B[3 * r + 1] + sz * st.B[3 * r + 2];
         e = double(bond) - double(bs);
@@ -279,11 +277,11 @@ __global__ __launch_bounds__(256) void pyro_fields_kernel(DeviceState<Real> st, 
     if (i >= st.N) return;
     const int nc = st.n_cells;
     const size_t N = size_t(st.N);
-    const Real* Sx = st.S + size_t(r) * 3 * N;
+    const Real* Sr = st.S + size_t(r) * 3 * N;
     const int sl = i / nc;
     const CellAxes c = cell_axes_of(i - sl * nc, st.L0, st.L1, st.L2);
     Real hx, hy, hz;
-    pyro_field_rt<Real, SHELLS>(sl, Sx, Sx + N, Sx + 2 * N, nc, c, J.at(r), hx, hy, hz);
+    pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, J.at(r), hx, hy, hz);
     out[i] = hx;
     out[N + i] = hy;
     out[2 * N + i] = hz;
@@ -301,9 +299,7 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     const PyroCouplings<Real> Jr = J.at(r);
     const int nc = st.n_cells;
     const size_t N = size_t(st.N);
-    Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
-    Real* __restrict__ Sy = Sx + N;
-    Real* __restrict__ Sz = Sy + N;
+    Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
     unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
     const ReplicaParams<Real> rp = replica_params(st, r);
     unsigned int my_acc = 0;
@@ -330,12 +326,13 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                             c.a2[a] = j ? ax_b.a2[a] : ax_a.a2[a];
                         }
                         Real hx, hy, hz;
-                        pyro_field_rt<Real, SHELLS>(sl, Sx, Sy, Sz, nc, c, Jr, hx, hy, hz);
+                        pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, Jr, hx, hy, hz);
                         if (ps == 0)
-                            my_acc += apply_update_p<Real, FIRST>(rp, st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx,
-                                                                  hy, hz) ? 1u : 0u;
+                            my_acc += apply_update_p<Real, FIRST>(rp, st, Sr, sl * nc + cell, r, pass, hx, hy, hz)
+                                          ? 1u
+                                          : 0u;
                         else
-                            apply_update_p<Real, 3>(rp, st, Sx, Sy, Sz, sl * nc + cell, r, pass, hx, hy, hz);
+                            apply_update_p<Real, 3>(rp, st, Sr, sl * nc + cell, r, pass, hx, hy, hz);
                     }
                 }
                 __syncthreads();
@@ -398,8 +395,7 @@ inline constexpr TileLanes<T0, T1, T2, BS> kTileLanes{};
 // (halo_lo(sl, a) = 1 iff R_fcc[sl / 4] has 4 on axis a), coordinates
 // wrapped periodically.
 template <typename Real, int T0, int T1, int T2, int BS>
-__device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, const Real* __restrict__ Sx,
-                                              const Real* __restrict__ Sy, const Real* __restrict__ Sz, int o0,
+__device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, const Real* __restrict__ Sr, int o0,
                                               int o1, int o2, int L0, int L1, int L2, int nc) {
     constexpr int B1 = T1 + 1, B2 = T2 + 1, BC = (T0 + 1) * B1 * B2;
     for (int e = threadIdx.x; e < 16 * BC; e += BS) {
@@ -413,10 +409,10 @@ __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, cons
         g1 -= g1 >= L1 ? L1 : 0;
         g2 += g2 < 0 ? L2 : 0;
         g2 -= g2 >= L2 ? L2 : 0;
-        const unsigned gi = unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2);
-        lx[e] = ld32(Sx, gi);
-        ly[e] = ld32(Sy, gi);
-        lz[e] = ld32(Sz, gi);
+        const Vec3<Real> v = ld3(Sr, unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2));
+        lx[e] = v.x;
+        ly[e] = v.y;
+        lz[e] = v.z;
     }
 }
 
@@ -468,11 +464,9 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const int o1 = (((phase >> 1) & 1) + 2 * m1) * T1;
     const int o2 = (((phase >> 2) & 1) + 2 * m2) * T2;
     const size_t N = size_t(st.N);
-    Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
-    Real* __restrict__ Sy = Sx + N;
-    Real* __restrict__ Sz = Sy + N;
+    Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
 
-    load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sx, Sy, Sz, o0, o1, o2, L0, L1, L2, nc);
+    load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
     int8_t* const dir_r = st.dir + size_t(r) * N;
     if constexpr (LIFTED) {
         for (int w = threadIdx.x; w < 16 * W; w += BS) ldw[w] = 0u;
@@ -553,9 +547,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
         const int li = sl * BC + ((i0 + (f == 2 || f == 3)) * B1 + (i1 + (f == 1 || f == 3))) * B2 +
                        (i2 + (f == 1 || f == 2));
         const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
-        Sx[site] = lx[li];
-        Sy[site] = ly[li];
-        Sz[site] = lz[li];
+        st3(Sr, unsigned(site), lx[li], ly[li], lz[li]);
         if constexpr (LIFTED) dir_r[site] = ((ldw[sl * W + (m >> 5)] >> (m & 31)) & 1u) ? int8_t(-1) : int8_t(1);
     }
 
@@ -598,9 +590,8 @@ __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real>
     const int t12 = tile / q2n;
     const int m1 = t12 % q1n, m0 = t12 / q1n;
     const size_t N = size_t(st.N);
-    const Real* __restrict__ Sx = st.S + size_t(r) * 3 * N;
-    load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sx, Sx + N, Sx + 2 * N, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1,
-                                        st.L2, st.n_cells);
+    const Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
+    load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sr, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1, st.L2, st.n_cells);
     __syncthreads();
     const PyroCouplings<Real> Jr = J.at(r);
     const Real bx = st.B[3 * r], by = st.B[3 * r + 1], bz = st.B[3 * r + 2];
