@@ -295,6 +295,7 @@ def large_lattice_sweep_line(pyro):
     # table is read), kept at its formula value for the comparison with the
     # table kernels
     b_big = bytes_per_update(4, 30, 16, lifted_frac=0.5)
+    traffic = profiled_traffic("sweep_L48x16_lifted")
     return {
         "workload": "L=48 (1769472 spins) x 16 replicas, FP32, J1=1 J2=J3=0.1, B=(0,0,0.2), "
                     "sweep = 1 lifted Metropolis + 1 OR",
@@ -304,7 +305,9 @@ def large_lattice_sweep_line(pyro):
             "bound": "hbm", "achieved": b_big * upl / (launch_ms / 1e3) / 1e9,
             "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": b_big * upl / (launch_ms / 1e3) / PEAK_HBM,
-            "traffic": profiled_traffic("sweep_L48x16_lifted"),
+            "traffic": traffic,
+            # the measured bytes per launch moved in the measured launch time
+            "traffic_rate_frac": (traffic / (launch_ms / 1e3) / PEAK_HBM) if traffic else None,
             "bytes_per_update": b_big, "updates_per_launch": upl,
             "avg_launch_us": launch_ms * 1e3,
         },
@@ -400,6 +403,7 @@ def main():
     avg_launch_s = sweep_ms / 1e3 / sweep_launches
     b_alg = bytes_per_update(4, info["n_slot"], R_REP)
     achieved = b_alg * upd_per_launch / avg_launch_s
+    traffic_c3 = profiled_traffic("sweep_c3_L12x64")
     st = ctx.pt_stats()
     acc_rate = (st["accepts"] / np.maximum(st["attempts"], 1)).tolist()
 
@@ -447,7 +451,9 @@ def main():
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM,
-            "traffic": profiled_traffic("sweep_c3_L12x64"),
+            "traffic": traffic_c3,
+            # the measured bytes per launch moved in the measured launch time
+            "traffic_rate_frac": (traffic_c3 / avg_launch_s / PEAK_HBM) if traffic_c3 else None,
             "kernel": {1: "sweep_colour_kernel", 2: "sweep_resident_kernel", 3: "pyro_colour_kernel",
                        4: "pyro_resident_kernel", 5: "pyro_tiled_kernel"}[strategy],
             "bytes_per_update": b_alg,
