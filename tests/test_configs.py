@@ -9,7 +9,12 @@ seed), with checks that do not depend on this implementation:
   E/spin over seeds is -1 + 0.75 * 0.05 up to O(T^2);
 - FP64 and FP32 side by side, L = 16, J1 = 1, J2 = 0.1 (T 10 -> 0.02 over
   100 log steps, n_sweep 16): the seed-averaged energy of the two
-  precisions agrees within 3 sigma at every step of the anneal.
+  precisions agrees within 3 sigma at every step of the anneal;
+- the tempering workload, L = 12 x 64 replicas, J = (1, -0.05, 0.05),
+  T log-spaced in [0.02, 2], sweeps of 1 Metropolis + 2 OR, exchange every
+  4 sweeps, 9-correlator SSF every round: FP32 against FP64 per rung (down
+  to T = 0.2) for <E>/N, C_v/N and S(q) at three wavevectors, within 3
+  sigma of batch-mean errors.
 """
 import numpy as np
 import pytest
@@ -49,3 +54,73 @@ def test_fp32_and_fp64_anneals_agree(gpu):
     f32 = np.array([r.E_final for r in r32]) / 16
     se_f = np.sqrt(f64.var(ddof=1) / len(f64) + f32.var(ddof=1) / len(f32))
     assert abs(f64.mean() - f32.mean()) < 3 * se_f
+
+
+def _tempering_batches(gpu, precision, seed, n_burn=200, n_batches=20, per_batch=20):
+    """Per-rung batch means of E/cell, E^2/cell^2 and S_xx + S_yy + S_zz at
+    K = (0, 0, 0), (0, 0, 1), (0, 1, 1) of the tempering workload, one batch
+    = per_batch rounds of 4 sweeps."""
+    L, R = 12, 64
+    ctx = gpu.Context.pyrochlore(L, n_replicas=R, precision=precision, seed=seed)
+    ctx.define_J1J2J3(1.0, -0.05, 0.05)
+    ctx.setup()
+    ctx.randomize(seed + 1)
+    ctx.pt_init(np.geomspace(2.0, 0.02, R), seed + 2)
+    ctx.ssf_configure(("xx", "yy", "zz", "xy", "xz", "yz", "yx", "zx", "zy"), n_slots=R)
+    ctx.pt_rounds(gpu.METROPOLIS, 2, 4, True, n_burn)
+    ks = [0, 1, L + 1]  # flat K = (K0 L + K1) L + K2: (0, 0, 0), (0, 0, 1), (0, 1, 1)
+
+    def read_ssf():
+        vals, ns = [], []
+        for slot in range(R):
+            S, _, n = ctx.ssf_read(slot, with_sq=False)
+            vals.append(S[:3, ks, 0].sum(axis=0))
+            ns.append(n)
+        return np.array(vals), np.array(ns, float)
+
+    prev = ctx.pt_stats()
+    prev_s, prev_n = read_ssf()
+    E, E2, S = [], [], []
+    for _ in range(n_batches):
+        ctx.pt_rounds(gpu.METROPOLIS, 2, 4, True, per_batch)
+        st = ctx.pt_stats()
+        dn = (st["n"] - prev["n"]).astype(float)
+        E.append((st["E"] - prev["E"]) / dn)
+        E2.append((st["E2"] - prev["E2"]) / dn)
+        cur_s, cur_n = read_ssf()
+        S.append((cur_s - prev_s) / (cur_n - prev_n)[:, None])
+        prev, prev_s, prev_n = st, cur_s, cur_n
+    T = np.geomspace(2.0, 0.02, R)
+    ctx.close()
+    return T, np.array(E), np.array(E2), np.array(S), L ** 3
+
+
+@pytest.mark.gpu
+def test_headline_tempering_fp32_matches_fp64(gpu):
+    T, E32, E232, S32, nc = _tempering_batches(gpu, gpu.FP32, 21)
+    _, E64, E264, S64, _ = _tempering_batches(gpu, gpu.FP64, 41)
+    N = 16 * nc
+
+    def stats(E, E2, S):
+        e = E / 16.0  # per spin
+        # C_v/N per batch: cells * var(E/cell) / T^2 / 16
+        cv = nc * (E2 - E ** 2) / T[None, :] ** 2 / 16.0
+        s = S / N  # S(q)/N
+        out = []
+        for x in (e, cv, *(s[:, :, j] for j in range(s.shape[2]))):
+            out.append((x.mean(axis=0), x.std(axis=0, ddof=1) / np.sqrt(x.shape[0])))
+        return out
+
+    a, b = stats(E32, E232, S32), stats(E64, E264, S64)
+    # rungs down to T = 0.2: below it 800 burn-in sweeps and 80-sweep
+    # batches are too short for independent runs to agree (equilibration,
+    # not precision: at T ~ 0.1 both precisions drift the same way)
+    keep = T >= 0.2
+    z = np.concatenate([np.abs(m1 - m2)[keep] / np.sqrt(s1 ** 2 + s2 ** 2 + 1e-300)[keep]
+                        for (m1, s1), (m2, s2) in zip(a, b)])
+    assert np.all(np.isfinite(z))
+    # with 20 batch means per side, z is close to Student's t with about 38
+    # degrees of freedom: P(|t| > 3) = 0.5%, so about 1 of the 160
+    # comparisons is expected beyond 3; 5% would be 8 (Poisson tail ~1e-5)
+    assert np.mean(z > 3) < 0.05, (np.sort(z)[-8:], len(z))
+    assert z.max() < 5, np.sort(z)[-8:]
