@@ -51,6 +51,15 @@ def test_missing_required_arguments(pyro, name):
     assert "usage:" in r.stderr
 
 
+@pytest.mark.parametrize("name", ["bench_lifted", "bench_speed"])
+def test_harness_rejects_unknown_option(pyro, name):
+    """The reference's test harnesses exit with status 2 on a bad command
+    line (bench_lifted.cpp:172-176, bench_speed.cpp:128-133)."""
+    r = _run([_tool(pyro, name), "--no_such_option", "1"])
+    assert r.returncode == 2
+    assert "unknown option --no_such_option" in r.stderr
+
+
 def test_unknown_option_and_bad_outdir(pyro, tmp_path):
     exe = _tool(pyro, "anneal")
     r = _run([exe, "-o", str(tmp_path), "-s", "0", "--T_sample", "1", "--J1", "1", "4", "--bogus", "1"])
@@ -359,3 +368,35 @@ def test_fieldcool_file_matches_library_replay(gpu, tmp_path):
         ctx.sweep(pyro.METROPOLIS, 1, 2)
         ctx.ssf_sample([0])
     np.testing.assert_array_equal(S[:, 0], ctx.ssf_read(0)[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [[], ["--fp32"]])
+def test_bench_lifted_passes(pyro, extra):
+    """Lifted and standard Metropolis on the J1-J2 square lattice agree on
+    <E> and <E^2> within 0.1 * min(|J1|, |J2|) (bench_lifted.cpp:254-272),
+    at a shorter run than the harness defaults."""
+    r = _run([_tool(pyro, "bench_lifted"), "-L", "8", "--n_burn", "500", "--n_sample", "4000", *extra], timeout=300)
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "RESULT: PASS" in r.stdout
+    # the table row "<E>  standard  lifted  |difference|" (not the verdict line)
+    row = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("<E> ") and ":" not in ln][0]
+    e_std, e_lft = float(row[1]), float(row[2])
+    # ferromagnetic J1 = -1 with J2 = 0.3 (below |J1| / 2: the ferromagnet is
+    # the ground state, E/spin = -2 + 2 * 0.3 = -1.4) at T = 0.5: above the
+    # ground state, and still well below zero
+    assert -1.4 < e_std < -0.4 and -1.4 < e_lft < -0.4
+
+
+@pytest.mark.gpu
+def test_bench_speed_reports_both_variants(pyro):
+    r = _run([_tool(pyro, "bench_speed"), "-L", "16", "--n_warm", "20", "--n_sweep", "200", "--reps", "2"], timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert sum(ln.startswith("rep ") for ln in lines) == 2
+    ns = [ln for ln in lines if ln.startswith("ns / sweep")][0].split()
+    assert float(ns[3]) > 0 and float(ns[4]) > 0
+    acc = [ln for ln in lines if ln.startswith("accept")][0].replace("%", " ").split()
+    assert 0 < float(acc[1]) <= 100 and 0 < float(acc[2]) <= 100
+    assert any(ln.startswith("Faster: ") for ln in lines)
