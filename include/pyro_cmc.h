@@ -181,6 +181,15 @@ int pyro_pt_stats(pyro_ctx* ctx, uint64_t* attempts, uint64_t* accepts,
 int pyro_pt_permutation(pyro_ctx* ctx, int32_t* replica_of_slot);
 /* Zero the swap statistics and energy sums (burn-in -> sampling). */
 int pyro_pt_reset_stats(pyro_ctx* ctx);
+/* Exchange after every `every`-th round only (default 1, the reference's
+ * temper.cpp:418-474 cadence): the rounds in between still sweep, sample
+ * the SSF and the energies, so a schedule with samples every sweep and
+ * exchanges every k sweeps is pyro_pt_rounds with n_sweep = 1 and this
+ * interval k.  Exchange x (x = 0, 1, ... at the end of rounds k - 1,
+ * 2k - 1, ...) has parity x mod 2 and its own random numbers; with k = 1
+ * the exchanges are those without the setting.  No counterpart in the
+ * reference, whose rounds always end with an exchange. */
+int pyro_pt_set_exchange_interval(pyro_ctx* ctx, int every);
 /* Accumulate the current energy per cell of every replica into the
  * energy sums of its slot (energy_manager::sample, energy_manager.hpp:38-44). */
 int pyro_energy_sample(pyro_ctx* ctx);

@@ -58,6 +58,7 @@ _SIGS = {
     "pyro_pt_stats": [_vp, _vp, _vp, _vp, _vp, _vp],
     "pyro_pt_permutation": [_vp, _i32p],
     "pyro_pt_reset_stats": [_vp],
+    "pyro_pt_set_exchange_interval": [_vp, C.c_int],
     "pyro_set_replica_couplings": [_vp, _vp],
     "pyro_anneal_schedule": [_vp, _f64p, C.c_int],
     "pyro_anneal_steps": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
@@ -336,6 +337,10 @@ class Context:
 
     def pt_reset_stats(self):
         _check(lib().pyro_pt_reset_stats(self._h))
+
+    def pt_exchange_interval(self, every):
+        """Exchange after every `every`-th tempering round only (default 1)."""
+        _check(lib().pyro_pt_set_exchange_interval(self._h, int(every)))
 
     # batches of independent anneals
     def set_replica_couplings(self, J1J2J3Delta):

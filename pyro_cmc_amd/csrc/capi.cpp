@@ -144,6 +144,7 @@ struct pyro_ctx {
     // 3 pyrochlore per-colour, 4 pyrochlore resident, 5 pyrochlore tiled
     int strategy = 0;
     int tile_kind = 0;          // tiled strategy tile shape (0: not applicable)
+    int exchange_every = 1;     // tempering rounds per replica exchange
     int tile_forced = 0;        // tile shape requested by pyro_set_tile_shape (0: automatic)
     size_t max_lds = 65536;     // per-workgroup LDS limit of the device
     int tuned_key = -1;        // (kind, n_or) the auto choice was made for
@@ -1251,6 +1252,7 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
             a.round = ctr + 1;
             a.pass_base = ctr;
             a.passes_per_round = ppr;
+            a.every = (unsigned long long)ctx->exchange_every;
             a.n_cells = double(ctx->nc());
             a.k0 = ctx->swap_k0;
             a.k1 = ctx->swap_k1;
@@ -1476,6 +1478,15 @@ int pyro_pt_reset_stats(pyro_ctx* ctx) {
         for (DevMem* m : {&ctx->Esum, &ctx->E2sum, &ctx->nE, &ctx->att, &ctx->accp})
             HIPCHK(pyro::launch_zero(m->p, m->bytes / 4, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_pt_set_exchange_interval(pyro_ctx* ctx, int every) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        require(every >= 1, PYRO_E_INVALID, "the exchange interval must be at least 1");
+        ctx->exchange_every = every;
+        ctx->invalidate_graph();
     });
 }
 

@@ -245,13 +245,17 @@ __global__ __launch_bounds__(kBlock) void randomize_kernel(DeviceState<Real> st,
 // arg >= 0 or U < exp(arg).  Pairs are disjoint, so one thread per pair.
 __global__ __launch_bounds__(kBlock) void exchange_kernel(ExchangeArgs a) {
     const unsigned long long round = *a.round;
-    const int parity = int(round & 1ull);
-    for (int s = parity + 2 * int(threadIdx.x); s + 1 < a.R; s += 2 * kBlock) {
+    // exchange x = round / every happens at the end of rounds every - 1,
+    // 2 every - 1, ...; its parity and Philox counter come from x (x = round
+    // when every = 1)
+    const unsigned long long x = round / a.every;
+    const bool swap_now = (round + 1ull) % a.every == 0ull;
+    const int parity = int(x & 1ull);
+    for (int s = parity + 2 * int(threadIdx.x); swap_now && s + 1 < a.R; s += 2 * kBlock) {
         const int ra = a.rep_of[s], rb = a.rep_of[s + 1];
         const double Ea = a.e[ra] * a.n_cells, Eb = a.e[rb] * a.n_cells;
         const double arg = (1.0 / a.T_slot[s] - 1.0 / a.T_slot[s + 1]) * (Ea - Eb);
-        const Words4 w = philox4x32_10(
-            Words4{uint32_t(s), 0xFFFFFFFFu, uint32_t(round), uint32_t(round >> 32)}, a.k0, a.k1);
+        const Words4 w = philox4x32_10(Words4{uint32_t(s), 0xFFFFFFFFu, uint32_t(x), uint32_t(x >> 32)}, a.k0, a.k1);
         const bool acc = arg >= 0.0 || u53(w.x, w.y) < exp(arg);
         a.att[s] += 1ull;
         if (acc) {

@@ -180,6 +180,7 @@ struct ExchangeArgs {
     unsigned long long* round = nullptr;
     unsigned long long* pass_base = nullptr;
     unsigned long long passes_per_round = 0;
+    unsigned long long every = 1;  // exchange after every `every`-th round
     double n_cells = 1.0;
     uint32_t k0 = 0, k1 = 0;
     int R = 0;

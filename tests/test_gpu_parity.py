@@ -281,6 +281,40 @@ def test_tempering_rounds(gpu, oracle, prec):
     assert np.allclose(st["E2"], E2s, rtol=0, atol=tol * n_rounds * 100)
 
 
+def test_tempering_exchange_interval(gpu, oracle):
+    """With an exchange every 3rd round, exchange x (at the end of rounds 2,
+    5, 8, ...) has parity x mod 2 and counter x; the other rounds only
+    sample energies.  Spins stay fixed (no sweeps), so the oracle replays
+    the permutation and statistics draw for draw."""
+    L, R, swap_seed, every = 2, 6, 4242, 3
+    T = np.geomspace(3.0, 0.2, R)
+    ctx, os_ = make(gpu, oracle, L, (1.0, 0.1, 0.0), 1.0, 64, R=R)
+    rng = np.random.default_rng(5)
+    for r in range(R):
+        S = random_spins(ctx.N, rng)
+        ctx.set_spins(S, r)
+        os_[r].set_spins(S)
+    ctx.pt_init(T, swap_seed)
+    ctx.pt_exchange_interval(every)
+    e = np.array([o.energy_per_cell() for o in os_])
+    rep = np.arange(R, dtype=np.int32)
+    att = np.zeros(R, np.uint64)
+    accp = np.zeros(R, np.uint64)
+    n_rounds = 10
+    for rnd in range(n_rounds):
+        if (rnd + 1) % every == 0:
+            rep = oracle.exchange(T, e, L ** 3, rep, rnd // every, swap_seed, att, accp)
+    ctx.pt_rounds(n_sweep=0, n_rounds=n_rounds)
+    st = ctx.pt_stats()
+    assert np.array_equal(ctx.pt_permutation(), rep)
+    assert np.array_equal(st["attempts"], att[: R - 1])
+    assert np.array_equal(st["accepts"], accp[: R - 1])
+    assert att.sum() > 0 and accp.sum() > 0  # the test must see swaps
+    assert np.all(st["n"] == n_rounds)  # energies are sampled every round
+    with pytest.raises(gpu.PyroError):
+        ctx.pt_exchange_interval(0)
+
+
 def test_tempering_ssf_slot_map(gpu, oracle):
     """SSF samples taken inside tempering rounds land in the accumulator the
     map names for the rung a replica occupies at that round (temper.cpp:426-429:
