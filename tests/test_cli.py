@@ -380,7 +380,7 @@ def test_bench_lifted_passes(pyro, extra):
     print(r.stdout[-1500:])
     assert r.returncode == 0, r.stdout + r.stderr
     assert "RESULT: PASS" in r.stdout
-    # the table row "<E>  standard  lifted  |difference|" (not the verdict line)
+    # the table row "<E>  standard  lifted  |difference|" (not the PASS/FAIL line)
     row = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("<E> ") and ":" not in ln][0]
     e_std, e_lft = float(row[1]), float(row[2])
     # ferromagnetic J1 = -1 with J2 = 0.3 (below |J1| / 2: the ferromagnet is
