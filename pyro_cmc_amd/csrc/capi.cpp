@@ -374,14 +374,17 @@ struct pyro_ctx {
 
     // One pass of `kind` over all colours, one launch per colour
     // (strategy 1 generic, 3 pyrochlore stencil).
+    // epart (stencil kernels only): the pass also leaves each replica's
+    // energy partials there (pyro::launch_pyro_colour).
     template <typename Real>
     void pass_colours(int kind, const unsigned long long* pass_base, unsigned long long pass_index,
-                      bool use_spec) {
+                      bool use_spec, double* epart = nullptr) {
         const auto st = state<Real>(pass_base);
         if (use_spec) {
             const auto J = pyro_couplings<Real>();
             for (int c = 0; c < topo.n_colours; ++c)
-                timed(kFamSweep, [&] { return pyro::launch_pyro_colour<Real>(st, J, c, kind, pass_index, stream); });
+                timed(kFamSweep,
+                      [&] { return pyro::launch_pyro_colour<Real>(st, J, c, kind, pass_index, stream, epart); });
             return;
         }
         for (int c = 0; c < topo.n_colours; ++c) {
@@ -392,9 +395,11 @@ struct pyro_ctx {
         }
     }
     // n_sweeps sweeps of (kind + n_or OR passes) with the given strategy.
+    // epart_last (strategy 3 only): the last pass also leaves the energy
+    // partials of the final state there.
     template <typename Real>
     void sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
-                unsigned long long pass_index) {
+                unsigned long long pass_index, double* epart_last = nullptr) {
         if (strat == 2) {
             const auto st = state<Real>(pass_base);
             timed(kFamSweep, [&] {
@@ -425,15 +430,21 @@ struct pyro_ctx {
         }
         unsigned long long p = pass_index;
         for (int s = 0; s < n_sweeps; ++s) {
-            pass_colours<Real>(kind, pass_base, p++, strat == 3);
-            for (int o = 0; o < n_or; ++o) pass_colours<Real>(3, pass_base, p++, strat == 3);
+            const bool last = s == n_sweeps - 1 && strat == 3;
+            pass_colours<Real>(kind, pass_base, p++, strat == 3, last && n_or == 0 ? epart_last : nullptr);
+            for (int o = 0; o < n_or; ++o)
+                pass_colours<Real>(3, pass_base, p++, strat == 3, last && o == n_or - 1 ? epart_last : nullptr);
         }
     }
     void run_sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
-                    unsigned long long pass_index) {
-        if (prec == 64) sweeps<double>(strat, kind, n_or, n_sweeps, pass_base, pass_index);
-        else sweeps<float>(strat, kind, n_or, n_sweeps, pass_base, pass_index);
+                    unsigned long long pass_index, double* epart_last = nullptr) {
+        if (prec == 64) sweeps<double>(strat, kind, n_or, n_sweeps, pass_base, pass_index, epart_last);
+        else sweeps<float>(strat, kind, n_or, n_sweeps, pass_base, pass_index, epart_last);
     }
+    // Whether run_sweeps(strat, ...) with n_sweeps > 0 can leave the energy
+    // partials (energy_finish then replaces energy): per-colour stencil
+    // launches, with the observables on the stencil too.
+    bool sweeps_leave_energy(int strat) const { return strat == 3 && stencil_observables(); }
 
     // Pick the faster launch strategy for (kind, n_or) by timing both on a
     // saved copy of the state, then restore it (results do not depend on the
@@ -508,6 +519,18 @@ struct pyro_ctx {
         return best_s;
     }
 
+    // Per-replica energies (and with accumulate the slot sums) from nblk
+    // partials per replica in epart.
+    void energy_finish(bool accumulate, int nblk) {
+        double* es = accumulate ? Esum.as<double>() : nullptr;
+        double* e2 = accumulate ? E2sum.as<double>() : nullptr;
+        unsigned long long* ne = accumulate ? nE.as<unsigned long long>() : nullptr;
+        timed(kFamEnergy, [&] {
+            return pyro::launch_energy_finish(epart.as<double>(), nblk, int(nc()), e.as<double>(),
+                                              slot_of.as<int32_t>(), es, e2, ne, R, stream);
+        });
+    }
+
     void energy(bool accumulate) {
         double* es = accumulate ? Esum.as<double>() : nullptr;
         double* e2 = accumulate ? E2sum.as<double>() : nullptr;
@@ -531,10 +554,7 @@ struct pyro_ctx {
                            : pyro::launch_pyro_energy<float>(state<float>(nullptr), pyro_couplings<float>(),
                                                              epart.as<double>(), stream);
             });
-            timed(kFamEnergy, [&] {
-                return pyro::launch_energy_finish(epart.as<double>(), nblk, int(nc()), e.as<double>(),
-                                                  slot_of.as<int32_t>(), es, e2, ne, R, stream);
-            });
+            energy_finish(accumulate, nblk);
             return;
         }
         timed(kFamEnergy, [&] {
@@ -877,7 +897,8 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
         }
         ctx->counters.alloc(sizeof(unsigned long long) * 3);  // pass base, round, anneal step
         ctx->epart.alloc(sizeof(double) * R *
-                         std::max(pyro::energy_blocks(int(N)), pyro::pyro_energy_blocks(int(N))));
+                         std::max({pyro::energy_blocks(int(N)), pyro::pyro_energy_blocks(int(N)),
+                                   pyro::pyro_colour_energy_blocks(int(ctx->nc()))}));
         ctx->e.alloc(sizeof(double) * R);
         ctx->Esum.alloc(sizeof(double) * R);
         ctx->E2sum.alloc(sizeof(double) * R);
@@ -1227,8 +1248,12 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
         HIPCHK(hipMemcpyAsync(ctx->counters.p, hc, sizeof(hc), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
 
+        // the round's last colour pass leaves the energy partials when it can
+        const bool fused_energy = n_sweep > 0 && ctx->sweeps_leave_energy(strat);
         auto round_body = [&]() {
-            if (n_sweep > 0) ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull);
+            if (n_sweep > 0)
+                ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
+                                fused_energy ? ctx->epart.as<double>() : nullptr);
             // One stream: running the SSF branch on a second stream beside the
             // energy (fork and join by events inside the captured round) made
             // each round 0.19 ms slower, more than the branch itself takes.
@@ -1241,7 +1266,8 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
                     ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
                 }
             }
-            ctx->energy(true);
+            if (fused_energy) ctx->energy_finish(true, pyro::pyro_colour_energy_blocks(int(ctx->nc())));
+            else ctx->energy(true);
             pyro::ExchangeArgs a;
             a.e = ctx->e.as<double>();
             a.rep_of = ctx->rep_of.as<int32_t>();
@@ -1384,12 +1410,17 @@ int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int
         auto* ctr = ctx->counters.as<unsigned long long>();
         const unsigned long long hc[3] = {ctx->pass_count, ctx->round_count, (unsigned long long)ctx->anneal_step};
         ctx->h2d(ctx->counters.p, hc, sizeof(hc));
+        // the step's last colour pass leaves the energy partials when it can
+        const bool fused_energy = n_sweep > 0 && ctx->sweeps_leave_energy(strat);
         auto step_body = [&]() {
             HIPCHK(pyro::launch_anneal_set_T(ctx->T_slot.as<double>(), ctx->R, ctx->sched.as<double>(), ctx->n_sched,
                                              ctr + 2, ctx->stream));
-            if (n_sweep > 0) ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull);
+            if (n_sweep > 0)
+                ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
+                                fused_energy ? ctx->epart.as<double>() : nullptr);
             if (sample_ssf) ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
-            ctx->energy(false);
+            if (fused_energy) ctx->energy_finish(false, pyro::pyro_colour_energy_blocks(int(ctx->nc())));
+            else ctx->energy(false);
             HIPCHK(pyro::launch_anneal_record(ctx->e.as<double>(), ctx->R, ctx->ehist.as<double>(), ctx->n_sched,
                                               ctr + 2, ctr, ppr, ctx->stream));
         };

@@ -45,13 +45,16 @@ template <> __device__ __forceinline__ double ddiv<double>(double a, double b) {
 // v <- v / |v|: v * rsq(|v|^2) in FP32, division by the square root in FP64
 template <typename Real>
 __device__ __forceinline__ void normalize3(Real& x, Real& y, Real& z) {
+    // explicit fma: the contraction is the same in every kernel that
+    // inlines this, whatever code surrounds it
+    const Real n2 = fma(z, z, fma(y, y, x * x));
     if constexpr (sizeof(Real) == 4) {
-        const Real inv = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+        const Real inv = __builtin_amdgcn_rsqf(n2);
         x *= inv;
         y *= inv;
         z *= inv;
     } else {
-        const Real nrm = sqrt(x * x + y * y + z * z);
+        const Real nrm = sqrt(n2);
         x /= nrm;
         y /= nrm;
         z /= nrm;
@@ -158,17 +161,19 @@ __device__ __forceinline__ bool update_spin_u(const ReplicaParams<Real>& p, Real
         return false;
     } else if constexpr (KIND == 3) {
         // Overrelaxation: rotation by 2*pi*U about h_loc (MC.cpp:102-109, 145-153).
-        const Real n2 = hx * hx + hy * hy + hz * hz;
+        // Products and sums as explicit fma, so that every kernel inlining
+        // this (with or without other work around it) rounds alike.
+        const Real n2 = fma(hz, hz, fma(hy, hy, hx * hx));
         if (!(n2 > Real(0))) return false;  // h_loc = 0: no axis, spin kept
         Real s, c;
         dsincos_rev<Real>(u24<Real>(u.m[0]), &s, &c);
-        const Real d = hx * sx + hy * sy + hz * sz;
+        const Real d = fma(hz, sz, fma(hy, sy, hx * sx));
         const Real f = ddiv<Real>(Real(1) - c, n2) * d;
         const Real g = ddiv<Real>(s, dsqrt<Real>(n2));
-        const Real cx = hy * sz - hz * sy, cy = hz * sx - hx * sz, cz = hx * sy - hy * sx;
-        Real nx = c * sx + f * hx + g * cx;
-        Real ny = c * sy + f * hy + g * cy;
-        Real nz = c * sz + f * hz + g * cz;
+        const Real cx = fma(hy, sz, -(hz * sy)), cy = fma(hz, sx, -(hx * sz)), cz = fma(hx, sy, -(hy * sx));
+        Real nx = fma(g, cx, fma(c, sx, f * hx));
+        Real ny = fma(g, cy, fma(c, sy, f * hy));
+        Real nz = fma(g, cz, fma(c, sz, f * hz));
         if constexpr (Renorm<Real>::value) {
             normalize3<Real>(nx, ny, nz);
         }

@@ -86,10 +86,14 @@ struct PyroCouplings {
     }
 };
 // One pass of `kind` over one colour (4 colours = pyro_sl when shells < 3,
-// else 16 colours = sublattice) for every replica.
+// else 16 colours = sublattice) for every replica.  With epart (the last
+// pass of a sweep, every colour in order): also each replica's energy as
+// pyro_colour_energy_blocks(n_cells) partials epart[r][...], which
+// launch_energy_finish sums once the pass's last colour has run.
 template <typename Real>
 hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
-                              int kind, unsigned long long pass_index, hipStream_t s);
+                              int kind, unsigned long long pass_index, hipStream_t s, double* epart = nullptr);
+inline int pyro_colour_energy_blocks(int n_cells) { return 16 * ((n_cells + 63) / 64); }
 // Whole sweeps with one resident workgroup (1024 threads, two cells per
 // thread) per replica; lattices of at most 2048 cells.
 constexpr int kPyroResidentMaxCells = 2048;

@@ -86,16 +86,29 @@ __device__ __forceinline__ CellAxes cell_axes_of(int cell, int L0, int L1, int L
     return cell_axes(n0, n1, n2, L0, L1, L2);
 }
 
+// Neighbour sums over the bonds to sites of earlier colours: per shell,
+// and the J1 neighbours again per target pyro_sl (for the XXZ term).
+template <typename Real>
+struct LowSums {
+    Real s[3][3];
+    Real v[4][3];
+};
+
 // Field of a site of sublattice SL; fetch(n, x, y, z) loads the spin of
 // stencil neighbour n (compile-time target sublattice and cell offset).
 // FWD: only the neighbours on higher sublattices (no bond joins equal
 // sublattices), so summing S_i . h_i over all sites counts every bond once.
-template <typename Real, int SHELLS, int SL, bool FWD = false, typename Fetch>
+// SPLIT = NCOL > 0: also the sums over the neighbours of earlier colours
+// (colour = sublattice mod NCOL) into *low, from the same loads.  They are
+// sums of the loaded values only (no products), so the full field's
+// arithmetic is that of SPLIT = 0.
+template <typename Real, int SHELLS, int SL, bool FWD = false, int SPLIT = 0, typename Fetch>
 __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplings<Real>& J, Real& hx,
-                                                Real& hy, Real& hz) {
+                                                Real& hy, Real& hz, LowSums<Real>* low = nullptr) {
     constexpr int NS = pyrostencil::slots_for(SHELLS);
     Real s[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // per shell x, y, z
     Real zs = Real(0);
+    if constexpr (SPLIT > 0) *low = LowSums<Real>{};
     static_for<NS>([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
         constexpr pyrostencil::Nbr n = kStencil.n[SL][k];
@@ -107,10 +120,19 @@ __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplin
             s[shell][0] += x;
             s[shell][1] += y;
             s[shell][2] += z;
-            if constexpr (shell == 0) {
-                constexpr int tp = n.sl % 4;
+            constexpr int tp = n.sl % 4;
+            if constexpr (shell == 0)
                 zs += Real(pyrostencil::kLink[tp][0]) * x + Real(pyrostencil::kLink[tp][1]) * y +
                       Real(pyrostencil::kLink[tp][2]) * z;
+            if constexpr (SPLIT > 0 && n.sl % SPLIT < SL % SPLIT) {
+                low->s[shell][0] += x;
+                low->s[shell][1] += y;
+                low->s[shell][2] += z;
+                if constexpr (shell == 0) {
+                    low->v[tp][0] += x;
+                    low->v[tp][1] += y;
+                    low->v[tp][2] += z;
+                }
             }
         }
     });
@@ -136,12 +158,37 @@ __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplin
     }
 }
 
+// S . h_low for the spin S = (x, y, z) of a site of pyro_sl p and the sums
+// of LowSums: J_shell S . s[shell], plus for XXZ Jx (z_p . S) sum_tp
+// z_tp . v[tp], written as sum_tp ((z_p . S) z_tp) . v[tp] so that none of
+// its products is one the field computes.
+template <typename Real, int SHELLS>
+__device__ __forceinline__ Real low_bond_energy(const LowSums<Real>& lo, const PyroCouplings<Real>& J, int p,
+                                                Real x, Real y, Real z) {
+    Real e = J.J1 * (x * lo.s[0][0] + y * lo.s[0][1] + z * lo.s[0][2]);
+    if constexpr (SHELLS >= 2) e += J.J2 * (x * lo.s[1][0] + y * lo.s[1][1] + z * lo.s[1][2]);
+    if constexpr (SHELLS >= 3) e += J.J3 * (x * lo.s[2][0] + y * lo.s[2][1] + z * lo.s[2][2]);
+    if (J.xxz) {
+        const Real a = x * Real(pyrostencil::kLink[p][0]) + y * Real(pyrostencil::kLink[p][1]) +
+                       z * Real(pyrostencil::kLink[p][2]);
+        Real w = Real(0);
+#pragma unroll
+        for (int tp = 0; tp < 4; ++tp)
+            w += (a * Real(pyrostencil::kLink[tp][0])) * lo.v[tp][0] +
+                 (a * Real(pyrostencil::kLink[tp][1])) * lo.v[tp][1] +
+                 (a * Real(pyrostencil::kLink[tp][2])) * lo.v[tp][2];
+        e += J.Jx * w;
+    }
+    return e;
+}
+
 // Field from spins in global memory (Sr = S + r * 3 N), neighbour cells
 // from the wrapped axes, one 3-component load per neighbour.
-template <typename Real, int SHELLS, int SL>
+template <typename Real, int SHELLS, int SL, int SPLIT = 0>
 __device__ __forceinline__ void pyro_field(const Real* __restrict__ Sr, int nc, const CellAxes& c,
-                                           const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
-    pyro_field_with<Real, SHELLS, SL>(
+                                           const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
+                                           LowSums<Real>* low = nullptr) {
+    pyro_field_with<Real, SHELLS, SL, false, SPLIT>(
         [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
             const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
             const Vec3<Real> v = ld3(Sr, j);
@@ -149,7 +196,7 @@ __device__ __forceinline__ void pyro_field(const Real* __restrict__ Sr, int nc, 
             y = v.y;
             z = v.z;
         },
-        J, hx, hy, hz);
+        J, hx, hy, hz, low);
 }
 
 // Field from a tile in LDS laid out [sl][box cell], every sublattice's box
@@ -172,12 +219,13 @@ __device__ __forceinline__ void pyro_field_lds(const Real* lx, const Real* ly, c
 }
 
 // Runtime sublattice -> compile-time stencil.
-template <typename Real, int SHELLS>
+template <typename Real, int SHELLS, int SPLIT = 0>
 __device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sr, int nc, const CellAxes& c,
-                                              const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
+                                              const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
+                                              LowSums<Real>* low = nullptr) {
     switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field<Real, SHELLS, X>(Sr, nc, c, J, hx, hy, hz); break;
+    case X: pyro_field<Real, SHELLS, X, SPLIT>(Sr, nc, c, J, hx, hy, hz, low); break;
         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
         PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
         PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -187,17 +235,52 @@ __device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sr, int nc, co
     }
 }
 
+// Sum of v over the 64 lanes of a wave (every lane active), the same in
+// every lane, in a fixed order: an inclusive scan within each row of 16
+// lanes by DPP row shifts (lanes shifted in from outside the row read 0),
+// then the four row totals (lanes 15, 31, 47, 63) added in order.  Short
+// latency next to a shuffle tree, which makes each step an LDS-crossbar
+// round trip.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    auto shr = [](double x, auto k) __attribute__((always_inline)) {
+        constexpr int ctrl = 0x110 + decltype(k)::value;  // DPP row_shr:k
+        const long long b = __double_as_longlong(x);
+        const int lo = __builtin_amdgcn_update_dpp(0, int(b), ctrl, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), ctrl, 0xF, 0xF, true);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    };
+    v += shr(v, std::integral_constant<int, 1>{});
+    v += shr(v, std::integral_constant<int, 2>{});
+    v += shr(v, std::integral_constant<int, 4>{});
+    v += shr(v, std::integral_constant<int, 8>{});
+    const long long b = __double_as_longlong(v);
+    const int lo = int(b), hi = int(b >> 32);
+    auto row = [&](int lane) __attribute__((always_inline)) {
+        const int l = __builtin_amdgcn_readlane(lo, lane), h = __builtin_amdgcn_readlane(hi, lane);
+        return __longlong_as_double((static_cast<long long>(h) << 32) | static_cast<unsigned int>(l));
+    };
+    return ((row(15) + row(31)) + row(47)) + row(63);
+}
+
 // One launch per colour; grid.x = (sublattices of the colour) x (cell
 // blocks), grid.y = replicas.  PER_REP: per-replica couplings (a batch of
 // models); the single-model instance carries no branch on them, which
 // would otherwise sit ahead of the neighbour gathers.
+// EN (the last pass of a sweep): also the energy terms (MC.cpp:305-312) of
+// the sites it updates, S_i . h_i over the bonds to sites of earlier colours
+// (already final in this pass) minus B . S_i, with S_i the updated spin.
+// Over the pass's colours this counts every bond once, when its later
+// endpoint is visited, so the partials sum to the energy of the state the
+// pass leaves.  One partial per (one-wave) block: epart[r * e_stride + bx].
 constexpr int kPyroColourThreads = 64;
 
-template <typename Real, int SHELLS, int KIND, bool PER_REP, int BS>
+template <typename Real, int SHELLS, int KIND, bool PER_REP, int BS, bool EN>
 __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
                                                           PyroCouplings<Real> J, int colour,
-                                                          unsigned long long pass_index) {
+                                                          unsigned long long pass_index,
+                                                          double* __restrict__ epart, int e_stride) {
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
+    static_assert(!EN || BS == 64, "one energy partial per wave");
     const int nc = st.n_cells;
     const int bps = (nc + BS - 1) / BS;
     unsigned bx, ru;
@@ -208,15 +291,37 @@ __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
     const int r = int(ru);
     const unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
     bool accepted = false;
+    double e = 0.0;
     if (cell < nc) {
         const size_t N = size_t(st.N);
         Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
         const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
         Real hx, hy, hz;
-        pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, PER_REP ? J.at(r) : J, hx, hy, hz);
-        accepted = apply_update<Real, KIND>(st, Sr, sl * nc + cell, r, pass, hx, hy, hz);
+        if constexpr (EN) {
+            LowSums<Real> lo;
+            const PyroCouplings<Real> Jr = PER_REP ? J.at(r) : J;
+            pyro_field_rt<Real, SHELLS, NCOL>(sl, Sr, nc, c, Jr, hx, hy, hz, &lo);
+            const ReplicaParams<Real> p = replica_params(st, r);
+            const int site = sl * nc + cell;
+            const Vec3<Real> v = ld3(Sr, unsigned(site));
+            Real x = v.x, y = v.y, z = v.z;
+            bool changed;
+            accepted = update_spin_p<Real, KIND>(p, st, x, y, z, changed, site, r, pass, hx, hy, hz,
+                                                 st.dir + size_t(r) * N + site);
+            if (changed) st3(Sr, unsigned(site), x, y, z);
+            const Real bond = low_bond_energy<Real, SHELLS>(lo, Jr, sl % 4, x, y, z);
+            const Real bs = x * p.Bx + y * p.By + z * p.Bz;
+            e = double(bond) - double(bs);
+        } else {
+            pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, PER_REP ? J.at(r) : J, hx, hy, hz);
+            accepted = apply_update<Real, KIND>(st, Sr, sl * nc + cell, r, pass, hx, hy, hz);
+        }
     }
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
+    if constexpr (EN) {
+        const double t = wave_sum_f64(e);
+        if (threadIdx.x == 0) epart[size_t(r) * unsigned(e_stride) + bx] = t;
+    }
 }
 
 // Energy terms per site (MC.cpp:305-312) with the stencil: S_i . h_i over
@@ -648,29 +753,35 @@ hipError_t dispatch_pyro(const PyroCouplings<Real>& J, F&& f) {
 
 template <typename Real>
 hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
-                              int kind, unsigned long long pass_index, hipStream_t s) {
+                              int kind, unsigned long long pass_index, hipStream_t s, double* epart) {
     const int ncol = J.shells == 3 ? 16 : 4;
     // One wave per workgroup: measured on the L=12 x 64 tempering lattice,
     // 64-thread workgroups take 5.17 us per colour launch against 5.39 us
     // for 128, 256 or 512 threads (same sites, same results).
     constexpr int BS = kPyroColourThreads;
     const int bps = (st.n_cells + BS - 1) / BS;
-    const dim3 grid(unsigned(bps * (16 / ncol)) * unsigned(st.R));  // replica-major, XCD-aware (xcd_split)
-    auto go = [&](auto per_rep) -> hipError_t {
+    const int per_launch = bps * (16 / ncol);
+    const dim3 grid(unsigned(per_launch) * unsigned(st.R));  // replica-major, XCD-aware (xcd_split)
+    // energy partials of colour c at [r][c * per_launch + block]: 16 bps per replica
+    double* ep = epart ? epart + size_t(colour) * size_t(per_launch) : nullptr;
+    const int es = pyro_colour_energy_blocks(st.n_cells);
+    auto go = [&](auto per_rep, auto en) -> hipError_t {
         constexpr bool PR = decltype(per_rep)::value;
+        constexpr bool EN = decltype(en)::value;
         return dispatch_pyro(J, [&](auto shells) -> hipError_t {
             constexpr int SH = decltype(shells)::value;
             switch (kind) {
-                case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
-                case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
-                case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
-                case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3, PR, BS>), grid, dim3(BS), 0, s, st, J, colour, pass_index); break;
+                case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
+                case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
+                case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
+                case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
                 default: return hipErrorInvalidValue;
             }
             return hipGetLastError();
         });
     };
-    return J.per_replica ? go(std::true_type{}) : go(std::false_type{});
+    if (epart) return J.per_replica ? go(std::true_type{}, std::true_type{}) : go(std::false_type{}, std::true_type{});
+    return J.per_replica ? go(std::true_type{}, std::false_type{}) : go(std::false_type{}, std::false_type{});
 }
 
 template <typename Real>

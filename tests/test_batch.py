@@ -170,7 +170,12 @@ def test_anneal_steps_equal_eager_steps(gpu, oracle, prec, strategy):
                     ctx.pass_count()))
     (Eg, Sg, Qg, pg), (Ee, Se, Qe, pe) = out
     assert pg == pe
-    assert np.array_equal(Eg, Ee)
+    if strategy == 3:
+        # per-colour launches: the step's last colour pass sums the energy
+        # (bonds to earlier colours), the eager steps use the energy kernel
+        assert np.allclose(Eg, Ee, rtol=0, atol=1e-5 * 16 * 8)
+    else:
+        assert np.array_equal(Eg, Ee)
     for a, b in zip(Sg, Se):
         assert np.array_equal(a, b)
     for a, b in zip(Qg, Qe):

@@ -315,6 +315,82 @@ def test_tempering_exchange_interval(gpu, oracle):
         ctx.pt_exchange_interval(0)
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize(
+    "L,R,J,kind,n_or",
+    [
+        (3, 5, (1.0, -0.05, 0.05), 0, 2),  # 16 colours; 27 cells: one partly filled block
+        (3, 4, (1.0, 0.1, 0.0), 1, 1),  # 4 colours, lifted
+        (4, 3, (1.0, 0.1, 0.1), 0, 0),  # no OR pass: the Metropolis pass is the last
+        (12, 64, (1.0, -0.05, 0.05), 0, 2),  # the L=12 x 64 tempering lattice, 27 blocks
+    ],
+)
+def test_tempering_fused_energy(gpu, oracle, prec, L, R, J, kind, n_or):
+    """With per-colour stencil launches the round's last colour pass also
+    leaves the energy partials (bonds to earlier colours, counted when the
+    later endpoint is updated), replacing the energy launch.  The energies
+    each round samples equal those of the state the round leaves: from the
+    energy kernel and from the oracle on the downloaded spins."""
+    T = np.geomspace(2.0, 0.05, R)
+    ctx, os_ = make(gpu, oracle, L, J, 0.8, prec, R=R, B=(0.1, -0.2, 0.3))
+    ctx.set_strategy(3)
+    ctx.randomize(11)
+    ctx.pt_init(T, 9)
+    ctx.pt_rounds(kind, n_or, 2, n_rounds=1)
+    st = ctx.pt_stats()
+    assert np.all(st["n"] == 1)
+    e_dev = ctx.energy_per_cell()
+    e_orc = []
+    for r in range(R):
+        o = os_[0]
+        o.set_spins(ctx.get_spins(r))
+        e_orc.append(o.energy_per_cell())
+    # the exchange permutes slots, not spins: compare the sampled multiset
+    tol = TOL[prec] * 16 * 8
+    assert np.allclose(np.sort(st["E"]), np.sort(e_dev), rtol=0, atol=tol)
+    assert np.allclose(np.sort(st["E"]), np.sort(e_orc), rtol=0, atol=tol)
+    assert np.ptp(st["E"]) > 100 * tol  # distinct energies, so the sort pairs them
+    # further rounds (the second replays the captured round): each adds the
+    # energies of the state it leaves, whatever the slot order
+    prev = st["E"].sum()
+    for n in (2, 3):
+        ctx.pt_rounds(kind, n_or, 1, n_rounds=1)
+        s = ctx.pt_stats()
+        assert np.all(s["n"] == n)
+        assert abs(s["E"].sum() - prev - ctx.energy_per_cell().sum()) < tol * R
+        prev = s["E"].sum()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 0), (2, 1), (1, 1)])
+def test_energy_pass_is_bit_identical(gpu, prec, kind, n_or):
+    """The colour kernel that also sums the energy (the last pass of an
+    anneal step or tempering round) updates every site exactly as the plain
+    kernel does: anneal steps and plain sweeps end in the same spins, bit
+    for bit, with XXZ per-replica couplings and fields."""
+    out = []
+    for fused in (True, False):
+        ctx = gpu.Context.pyrochlore(3, n_replicas=3, precision=prec, seed=21)
+        ctx.define_J1J2J3(1.0, 0.1, 0.1)
+        ctx.setup()
+        ctx.set_replica_couplings(np.array([[1.0, -0.2, 0.15, 0.7], [-1.0, 0.1, -0.05, 1.3], [0.5, 0.2, 0.1, 1.0]]))
+        for r, B in enumerate([(0.0, 0.0, 0.3), (0.1, -0.3, 0.2), (0.0, 0.0, 0.0)]):
+            ctx.set_field(np.array(B), r)
+        ctx.set_strategy(3)
+        ctx.randomize(8)
+        if fused:
+            ctx.anneal_schedule(np.array([1.5, 0.6]))
+            ctx.anneal_steps(kind, n_or, 2, False, 2)
+        else:
+            for T in (1.5, 0.6):
+                ctx.set_temperatures(np.full(ctx.R, T))
+                ctx.sweep(kind, n_or, 2, count=False)
+        out.append([(ctx.get_spins(r), ctx.get_lifted_dir(r)) for r in range(ctx.R)])
+    for (Sa, da), (Sb, db) in zip(*out):
+        assert np.array_equal(Sa, Sb)
+        assert np.array_equal(da, db)
+
+
 def test_tempering_ssf_slot_map(gpu, oracle):
     """SSF samples taken inside tempering rounds land in the accumulator the
     map names for the rung a replica occupies at that round (temper.cpp:426-429:
