@@ -266,7 +266,7 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // blocks), grid.y = replicas.  PER_REP: per-replica couplings (a batch of
 // models); the single-model instance carries no branch on them, which
 // would otherwise sit ahead of the neighbour gathers.
-// EN (the last pass of a sweep): also the energy terms (MC.cpp:305-312) of
+// EN (the last pass of a tempering round or anneal step): also the energy terms (MC.cpp:305-312) of
 // the sites it updates, S_i . h_i over the bonds to sites of earlier colours
 // (already final in this pass) minus B . S_i, with S_i the updated spin.
 // Over the pass's colours this counts every bond once, when its later
