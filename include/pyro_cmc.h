@@ -213,7 +213,8 @@ int pyro_set_replica_couplings(pyro_ctx* ctx, const double* J1J2J3Delta);
 int pyro_anneal_schedule(pyro_ctx* ctx, const double* T, int n_steps);
 /* n_steps anneal steps, each: every replica at the schedule's next
  * temperature, n_sweep sweeps (kind + n_overrelax OR passes), if
- * sample_ssf one SSF sample of every replica into slot r, then its energy
+ * sample_ssf one SSF sample of every replica into slot r (or the slot
+ * pyro_pt_ssf_map gives replica r), then its energy
  * per cell recorded in the history (energy_manager::sample per step,
  * anneal.cpp:203-206).  Captured once as a HIP graph and only enqueued:
  * contexts (each with its own stream) run concurrently until synchronised
@@ -239,7 +240,11 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha,
  * pyro_pt_rounds: map[s] in [-1, n_slots), -1 = that rung is not sampled
  * (the reference samples only its T_sample rungs, temper.cpp:426-429).
  * No two rungs may share a slot.  NULL clears the map (then tempering slot
- * s accumulates into slot s). */
+ * s accumulates into slot s).  Anneal steps (pyro_anneal_steps, where slot
+ * r is replica r) follow the same map, so a schedule with several sampling
+ * temperatures (ssf_manager::new_T per T_sample, anneal.cpp:208-221) sends
+ * each temperature's samples to its own slots by changing the map between
+ * calls; a new map takes effect without re-capturing the step graph. */
 int pyro_pt_ssf_map(pyro_ctx* ctx, const int32_t* map);
 /* One SSF sample of every replica r with slot_of_replica[r] >= 0 into that
  * slot.  slot_of_replica == NULL: use each replica's tempering slot. */

@@ -37,6 +37,7 @@
 // Both modes guard the measure-zero h_loc = 0 case (the reference divides
 // by |h| there): OR keeps the spin, lifted flips its direction.
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -161,6 +162,17 @@ struct Draws {
     std::uniform_real_distribution<double>* rand01 = nullptr;
     // counter mode
     const double* u = nullptr;
+    // Conditioning record of the last update, for FP32 error bounds:
+    //   [0] the smallest perturbation |dh| of the local field that could
+    //       flip one of its decision tests, to first order (the test's
+    //       distance from its threshold over its sensitivity to h; +inf
+    //       when it takes no decision);
+    //   [1] |h_loc|;
+    //   [2] sensitivity of the new spin to the local field, |dS'| / |dh| to
+    //       first order (0 when the update does not follow h);
+    //   [3] amplification of the relative rounding of its own arithmetic
+    //       (1 for well-conditioned formulas).
+    double info[4] = {HUGE_VAL, 0.0, 0.0, 1.0};
 };
 
 struct Oracle {
@@ -226,11 +238,18 @@ struct Oracle {
         const double nrm = std::sqrt(dot(ns, ns));
         ns = {ns[0] / nrm, ns[1] / nrm, ns[2] / nrm};
         const double dE = dot(ns, h) - curE;
+        const double dS = std::sqrt(dot(sub(ns, s->S), sub(ns, s->S)));  // d dE / dh
+        d.info[0] = std::abs(dE) / dS;
+        d.info[1] = std::sqrt(dot(h, h));
         if (dE < 0) {
             s->S = ns;
             return 1;
         }
         if (U < 0) U = (*d.rand01)(*d.rng);
+        {
+            const double p = std::exp(-dE / T);
+            d.info[0] = std::min(d.info[0], std::abs(U - p) / (p * dS / T));
+        }
         if (U < std::exp(-dE / T)) {
             s->S = ns;
             return 1;
@@ -243,6 +262,8 @@ struct Oracle {
         const V3 h = sub(local_field(s), B);
         const double U = d.u ? d.u[0] : (*d.rand01)(*d.rng);
         if (!(dot(h, h) > 0)) return;
+        d.info[1] = std::sqrt(dot(h, h));
+        d.info[2] = 1.0 / d.info[1];  // the axis direction moves by |dh| / |h|
         rotate_about_vector(s->S, h, 2 * M_PI * U);
     }
 
@@ -260,6 +281,17 @@ struct Oracle {
         const double u = dot(hh, s->S);
         const double du = s->lifted_dir * std::sqrt(T / T_ref) * E1;
         const double nu = u + du;
+        // u moves by at most |dh| / |h|
+        d.info[0] = std::min(std::abs(1 - nu), std::abs(nu + 1)) * hn;
+        d.info[1] = hn;
+        // S' = u' h^ + sqrt(1 - u'^2) v^ with v^ = (S - u h^) / |S - u h^|:
+        // u' moves by |dh| / |h|, which moves S' by that over sqrt(1 - u'^2),
+        // and v^ by |dh| / (|h| sin theta); the ratio (1 - u'^2) / (1 - u^2)
+        // carries the rounding of 1 - u^2 amplified by 1 / sin^2 theta
+        // (recorded only if the move is accepted: a rejected spin stays put)
+        const double s2 = std::max(std::abs(1 - u * u), 1e-300);
+        const double c2 = std::max(std::abs(1 - nu * nu), 1e-300);
+        const double sens_h = (1.0 / std::sqrt(c2) + std::sqrt(c2 / s2)) / hn;
         if (nu > 1 || nu < -1 || std::abs(1 - u * u) < 1e-16) {
             s->lifted_dir = int8_t(-s->lifted_dir);
             return 0;
@@ -269,7 +301,15 @@ struct Oracle {
         ns = {ns[0] / nrm, ns[1] / nrm, ns[2] / nrm};
         const double dE = dot(ns, h) - curE;
         const double U = d.u ? d.u[1] : (*d.rand01)(*d.rng);
+        {
+            // dE = (S' - S) . h moves by |S' - S| |dh| + |h| |dS'|
+            const double p = std::exp(-dE / T);
+            const double sens = std::sqrt(dot(sub(ns, s->S), sub(ns, s->S))) + hn * sens_h;
+            d.info[0] = std::min(d.info[0], std::abs(U - p) / (p * sens / T));
+        }
         if (U < std::exp(-dE / T)) {
+            d.info[2] = sens_h;
+            d.info[3] = 1.0 / s2 + 1.0 / std::sqrt(c2);
             s->S = ns;
             return 1;
         }
@@ -284,16 +324,20 @@ struct Oracle {
         const double delta = std::sqrt(T / T_ref);
         const V3 axis = cross(s->S, h);
         const double a2 = dot(axis, axis);
+        d.info[1] = std::sqrt(dot(h, h));
         if (a2 <= 1e-20) {
             s->lifted_dir = int8_t(-s->lifted_dir);
             return 0;
         }
+        const double sens_h = 1.0 / std::sqrt(a2);  // the rotation axis S x h moves by |dh| / |S x h|
         V3 ns = s->S;
         rotate_about_vector(ns, axis, s->lifted_dir * delta);
         const double nrm = std::sqrt(dot(ns, ns));
         ns = {ns[0] / nrm, ns[1] / nrm, ns[2] / nrm};
         const V3 axis_new = cross(ns, h);
         const double b2 = dot(axis_new, axis_new);
+        // a . b with a = S x h, b = S' x h moves by (|a| + |b|) |dh|
+        d.info[0] = std::abs(dot(axis, axis_new)) / (std::sqrt(a2) + std::sqrt(b2));
         if (dot(axis, axis_new) < 0) {
             s->lifted_dir = int8_t(-s->lifted_dir);
             return 0;
@@ -301,7 +345,14 @@ struct Oracle {
         const double dE = dot(ns, h) - curE;
         const double ratio = std::sqrt(b2 / a2) * std::exp(-dE / T);
         const double U = d.u ? d.u[0] : (*d.rand01)(*d.rng);
+        {
+            // log ratio moves by (1 / |a| + 1 / |b| + |S' - S| / T) |dh|
+            const double sens = 1.0 / std::sqrt(a2) + 1.0 / std::sqrt(std::max(b2, 1e-300)) +
+                                std::sqrt(dot(sub(ns, s->S), sub(ns, s->S))) / T;
+            d.info[0] = std::min(d.info[0], std::abs(U - ratio) / (ratio * sens));
+        }
         if (U < ratio) {
+            d.info[2] = sens_h;  // an accepted move follows the axis
             s->S = ns;
             return 1;
         }
@@ -310,6 +361,10 @@ struct Oracle {
     }
 
     int update(int kind, double T, Spin* s, Draws& d) {
+        d.info[0] = HUGE_VAL;
+        d.info[1] = 0.0;
+        d.info[2] = 0.0;
+        d.info[3] = 1.0;
         switch (kind) {
             case 0: return metropolis(T, s, d);
             case 1: return lifted(T, s, d);
@@ -398,6 +453,22 @@ int orc_define_coupling(void* h, const char* name, int n_kinds, const int32_t* c
         }
     o->couplings.push_back(c);
     return 0;
+}
+
+// Replace the matrix of the coupling called `name` (the bond shells point at
+// it, so no new setup is needed); -1 if there is no such coupling.
+int orc_set_coupling(void* h, const char* name, const double* J) {
+    auto* o = static_cast<Oracle*>(h);
+    for (auto& c : o->couplings)
+        if (c.name == name) {
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    c.J[3 * i + j] = J[3 * i + j];
+                    c.Jt[3 * i + j] = J[3 * j + i];
+                }
+            return 0;
+        }
+    return -1;
 }
 
 // setup_lattice (MC.cpp:15-52).  Returns -1 if a displacement misses a site
@@ -497,8 +568,10 @@ uint64_t orc_sweep_ref(void* h, int kind, double T, int n_or) {
 // Counter-mode pass: one pass of `kind` (0..3) visiting colours 0..n_col-1
 // in order (sites of a colour in storage order) with Philox uniforms keyed
 // by (seed ^ tag, seed >> 32) and counter (site, replica, pass).
-uint64_t orc_pass_ctr(void* h, int kind, double T, const int32_t* colour_of_site, int n_col,
-                      uint64_t seed, uint32_t replica, uint64_t pass) {
+// info (may be NULL) receives per site the conditioning record of its
+// update, info[4 i .. 4 i + 3] (Draws::info).
+uint64_t orc_pass_ctr_m(void* h, int kind, double T, const int32_t* colour_of_site, int n_col,
+                        uint64_t seed, uint32_t replica, uint64_t pass, double* info) {
     auto* o = static_cast<Oracle*>(h);
     uint64_t acc = 0;
     const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
@@ -511,16 +584,24 @@ uint64_t orc_pass_ctr(void* h, int kind, double T, const int32_t* colour_of_site
             Draws d;
             d.u = u;
             acc += o->update(kind, T, &o->spins[i], d);
+            if (info) std::copy(d.info, d.info + 4, info + 4 * i);
         }
     return acc;
+}
+
+uint64_t orc_pass_ctr(void* h, int kind, double T, const int32_t* colour_of_site, int n_col,
+                      uint64_t seed, uint32_t replica, uint64_t pass) {
+    return orc_pass_ctr_m(h, kind, T, colour_of_site, n_col, seed, replica, pass, nullptr);
 }
 
 // Counter-mode updates in an explicit order: update k visits site[k] with
 // kind first_kind if pass_offset[k] == 0 and overrelaxation otherwise, pass
 // index pass0 + pass_offset[k].  Returns the accepted count of the first kind.
-uint64_t orc_sweep_order(void* h, int first_kind, double T, const int32_t* site,
-                         const int32_t* pass_offset, int64_t n, uint64_t seed, uint32_t replica,
-                         uint64_t pass0) {
+// info (may be NULL) receives the conditioning record of update k at
+// info[4 k .. 4 k + 3].
+uint64_t orc_sweep_order_m(void* h, int first_kind, double T, const int32_t* site,
+                           const int32_t* pass_offset, int64_t n, uint64_t seed, uint32_t replica,
+                           uint64_t pass0, double* info) {
     auto* o = static_cast<Oracle*>(h);
     uint64_t acc = 0;
     const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
@@ -534,9 +615,16 @@ uint64_t orc_sweep_order(void* h, int first_kind, double T, const int32_t* site,
         d.u = u;
         const int kind = pass_offset[k] == 0 ? first_kind : 3;
         const int a = o->update(kind, T, &o->spins[i], d);
+        if (info) std::copy(d.info, d.info + 4, info + 4 * k);
         if (pass_offset[k] == 0) acc += a;
     }
     return acc;
+}
+
+uint64_t orc_sweep_order(void* h, int first_kind, double T, const int32_t* site,
+                         const int32_t* pass_offset, int64_t n, uint64_t seed, uint32_t replica,
+                         uint64_t pass0) {
+    return orc_sweep_order_m(h, first_kind, T, site, pass_offset, n, seed, replica, pass0, nullptr);
 }
 
 // Uniform-on-sphere initial spins of replica r (z = 2u0 - 1, phi = 2 pi u1).

@@ -28,6 +28,7 @@ _SIGS = {
     "orc_num_sites": (C.c_int64, [_vp]),
     "orc_define_coupling": (C.c_int, [_vp, C.c_char_p, C.c_int, _i32p, _i32p, _f64p]),
     "orc_setup": (C.c_int, [_vp]),
+    "orc_set_coupling": (C.c_int, [_vp, C.c_char_p, _f64p]),
     "orc_set_field": (None, [_vp, _f64p]),
     "orc_set_T_ref": (None, [_vp, C.c_double]),
     "orc_set_spins": (None, [_vp, _f64p]),
@@ -42,6 +43,10 @@ _SIGS = {
     "orc_pass_ctr": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64]),
     "orc_randomize_ctr": (None, [_vp, C.c_uint64, C.c_uint32]),
     "orc_sweep_order": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, _i32p, C.c_int64, C.c_uint64, C.c_uint32, C.c_uint64]),
+    "orc_pass_ctr_m": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64,
+                                    _vp]),
+    "orc_sweep_order_m": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, _i32p, C.c_int64, C.c_uint64, C.c_uint32,
+                                       C.c_uint64, _vp]),
     "orc_philox": (None, [_u32p, _u32p, _u32p]),
     "orc_xoroshiro": (None, [C.c_uint64, C.c_int, _u64p]),
     "orc_exchange": (None, [C.c_int, _f64p, _f64p, C.c_double, _i32p, C.c_uint64, C.c_uint64, _u64p, _u64p]),
@@ -241,6 +246,17 @@ class Oracle:
         if rc != 0:
             raise ValueError("Coupling names must be unique")
 
+    def set_coupling(self, name, J):
+        """New 3x3 matrix for an existing coupling (no new setup)."""
+        if lib().orc_set_coupling(self.h, name.encode(), np.ascontiguousarray(J, np.float64).ravel()) != 0:
+            raise KeyError(name)
+
+    def set_J1J2J3(self, J1, J2=0.0, J3=0.0, Delta=1.0):
+        """The nine build_J1J2J3_h matrices of another model on an oracle
+        made by pyrochlore() (the zero couplings are defined there too)."""
+        for name, _, J in j1j2j3_couplings(J1, J2, J3, Delta):
+            self.set_coupling(name, J)
+
     def setup(self):
         if lib().orc_setup(self.h) != 0:
             raise ValueError("displacement does not land on a site")
@@ -290,15 +306,25 @@ class Oracle:
     def sweep_ref(self, kind, T, n_or=1):
         return int(lib().orc_sweep_ref(self.h, kind, float(T), n_or))
 
-    def pass_ctr(self, kind, T, colour_of_site, n_colours, seed, replica, pass_index):
-        return int(lib().orc_pass_ctr(self.h, kind, float(T), np.ascontiguousarray(colour_of_site, np.int32),
-                                      int(n_colours), int(seed), int(replica), int(pass_index)))
+    def pass_ctr(self, kind, T, colour_of_site, n_colours, seed, replica, pass_index, info=None):
+        """Counter-mode pass.  info (float64[N][4], optional) receives each
+        site's conditioning record: the distance of its closest decision
+        test from the threshold (+inf where it decides nothing), |h_loc|,
+        the sensitivity |dS'|/|dh| of its new spin to the field (0 where the
+        update does not follow h) and the amplification of its own rounding."""
+        p = None if info is None else info.ctypes.data_as(C.c_void_p)
+        return int(lib().orc_pass_ctr_m(self.h, kind, float(T), np.ascontiguousarray(colour_of_site, np.int32),
+                                        int(n_colours), int(seed), int(replica), int(pass_index), p))
 
-    def sweep_order(self, first_kind, T, site, pass_offset, seed, replica, pass0):
+    def sweep_order(self, first_kind, T, site, pass_offset, seed, replica, pass0, info=None):
+        """Counter-mode updates in an explicit order; info (float64[n][4],
+        optional) receives the conditioning record of every update (see
+        pass_ctr)."""
         site = np.ascontiguousarray(site, np.int32)
         koff = np.ascontiguousarray(pass_offset, np.int32)
-        return int(lib().orc_sweep_order(self.h, int(first_kind), float(T), site, koff, len(site),
-                                         int(seed), int(replica), int(pass0)))
+        p = None if info is None else info.ctypes.data_as(C.c_void_p)
+        return int(lib().orc_sweep_order_m(self.h, int(first_kind), float(T), site, koff, len(site),
+                                           int(seed), int(replica), int(pass0), p))
 
     def randomize_ctr(self, seed, replica=0):
         lib().orc_randomize_ctr(self.h, int(seed), int(replica))

@@ -8,10 +8,12 @@ context follows the same device-side temperature schedule as a replayed HIP
 graph on its own stream, and the contexts' steps are enqueued in turn so the
 small lattices fill the GPU beside the large ones.
 
-Schedule per item, as in anneal.cpp:193-221 with T_sample = {T_cold}:
-  n_burn_in sweeps at T_hot; then for every T of generate_T_profile(T_hot,
-  T_cold, [T_cold], n_steps) n_sweep sweeps and one energy sample; then at
-  T_cold n_sample x (n_sweep sweeps + one SSF sample).
+Schedule per item, as in anneal.cpp:193-221: n_burn_in sweeps at T_hot;
+then for every T of generate_T_profile(T_hot, T_cold, T_sample, n_steps)
+n_sweep sweeps and one energy sample, and where T is one of the T_sample
+entries n_sample x (n_sweep sweeps + one SSF sample) at T into that
+temperature's accumulator (ssf_manager::new_T).  T_sample defaults to
+{T_cold}.
 """
 from __future__ import annotations
 
@@ -35,9 +37,10 @@ class AnnealResult:
     item: AnnealItem
     T_list: np.ndarray              # the anneal grid, descending
     E: np.ndarray                   # energy per cell after each grid step
-    S: np.ndarray                   # SSF sums at T_cold, [n_corr][n_k][2]
-    S2: np.ndarray                  # (Re S)^2, (Im S)^2 sums
-    n_samples: int
+    T_sample: np.ndarray            # sampling temperatures, descending (ssf_manager T order)
+    S: np.ndarray                   # SSF sums per T_sample, [n_T][n_corr][n_k][2]
+    S2: np.ndarray                  # (Re S)^2, (Im S)^2 sums, same shape
+    n_samples: np.ndarray           # samples per T_sample, [n_T]
     E_final: float                  # energy per cell of the final configuration
     spins: np.ndarray | None = field(default=None, repr=False)  # final [N][3] when kept
 
@@ -81,10 +84,29 @@ def uneven_items(n=256, sizes=(4, 8, 12, 16, 24), seed=0):
 
 def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_burn_in=16, n_sample=64,
                      n_overrelax=1, kind=METROPOLIS, precision=FP32, correlators=("xx", "yy", "zz"),
-                     seed=0, keep_spins=False, strategy=0):
+                     seed=0, keep_spins=False, strategy=0, T_sample=None):
     """Run every item's anneal; returns (results in item order, stats)."""
-    grid, _ = generate_T_profile(T_hot, T_cold, [T_cold], n_steps)
-    sched = np.concatenate([[T_hot], grid, np.full(n_sample, T_cold)])
+    T_sample = [T_cold] if T_sample is None else list(T_sample)
+    grid, s_idx = generate_T_profile(T_hot, T_cold, T_sample, n_steps)
+    n_ts = len(s_idx)
+    # device schedule: the burn-in step at T_hot, then every grid step, each
+    # sampling step followed by its n_sample SSF steps at the same T
+    sched = [T_hot]
+    e_at = []  # schedule index of each grid step's energy sample
+    phases = [(n_burn_in, False, 1, -1)]  # (sweeps, ssf, steps, T_sample index)
+    run = 0
+    for i, T in enumerate(grid):
+        e_at.append(len(sched))
+        sched.append(T)
+        run += 1
+        if i in s_idx:
+            phases.append((n_sweep, False, run, -1))
+            run = 0
+            phases.append((n_sweep, True, n_sample, s_idx.index(i)))
+            sched.extend([T] * n_sample)
+    if run:
+        phases.append((n_sweep, False, run, -1))
+    sched = np.array(sched)
     groups = {}
     for i, it in enumerate(items):
         groups.setdefault(it.L, []).append(i)
@@ -99,7 +121,8 @@ def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_
         for r, i in enumerate(idx):
             c.set_field(np.array(items[i].B, np.float64), r)
         c.randomize(seed * 7919 + L)
-        c.ssf_configure(correlators, n_slots=len(idx), store_error_term=True)
+        # slot j R + r holds replica r's samples at T_sample j
+        c.ssf_configure(correlators, n_slots=n_ts * len(idx), store_error_term=True)
         c.anneal_schedule(sched)
         ctxs.append((c, idx))
     for c, _ in ctxs:
@@ -107,22 +130,26 @@ def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_
     # timed: everything from the first step on, including each context's
     # strategy choice and graph capture on its first call
     t0 = time.perf_counter()
-    phases = [(n_burn_in, False, 1), (n_sweep, False, len(grid)), (n_sweep, True, n_sample)]
-    for n_sw, ssf, steps in phases:
+    for n_sw, ssf, steps, j in phases:
         for c, _ in ctxs:  # enqueue only: the contexts' streams run side by side
+            if ssf and n_ts > 1:  # (setting a map waits for the context's queued steps)
+                c.pt_ssf_map(np.arange(c.R, dtype=np.int32) + j * c.R)
             c.anneal_steps(kind, n_overrelax, n_sw, ssf, steps)
     for c, _ in ctxs:
         c.synchronize()
     wall = time.perf_counter() - t0
     results = [None] * len(items)
     updates = 0
+    e_at = np.array(e_at)
     for c, idx in ctxs:
         hist = c.anneal_history()
         for r, i in enumerate(idx):
-            S, S2, n = c.ssf_read(r)
-            results[i] = AnnealResult(items[i], grid.copy(), hist[r, 1:1 + len(grid)].copy(), S, S2, n,
-                                      float(hist[r, -1]), c.get_spins(r) if keep_spins else None)
-        passes = (1 + n_overrelax) * (n_burn_in + n_sweep * (len(grid) + n_sample))
+            reads = [c.ssf_read(j * c.R + r) for j in range(n_ts)]
+            results[i] = AnnealResult(items[i], grid.copy(), hist[r, e_at].copy(), grid[s_idx].copy(),
+                                      np.array([q[0] for q in reads]), np.array([q[1] for q in reads]),
+                                      np.array([q[2] for q in reads]), float(hist[r, -1]),
+                                      c.get_spins(r) if keep_spins else None)
+        passes = (1 + n_overrelax) * (n_burn_in + n_sweep * (len(sched) - 1))
         updates += c.N * c.R * passes
         c.close()
     stats = {"wall_s": wall, "site_updates": updates, "site_updates_per_s": updates / wall,

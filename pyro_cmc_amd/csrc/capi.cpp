@@ -32,6 +32,52 @@ thread_local std::string g_last_error;
 constexpr int kTileOrder[] = {2, 5, 1, 4, 3};
 constexpr int kMinTilesPerPhase = 8;
 
+// Whether tile kind `kind` applies to the tiled sweep of an L lattice: each
+// L_a divisible by twice the tile edge (checkerboard of tiles) and the tile
+// + halo within the LDS limit.
+bool tile_applies_to(const int L[3], int kind, size_t real_size, size_t max_lds) {
+    if (kind < 1 || kind > pyro::kPyroTileKinds) return false;
+    int d[3];
+    pyro::pyro_tile_dims(kind, d);
+    for (int a = 0; a < 3; ++a)
+        if (L[a] % (2 * d[a]) != 0) return false;
+    return pyro::pyro_tile_lds(d[0], d[1], d[2], int(real_size), true) <= max_lds;
+}
+// The tiled sweep's automatic tile kind (0: none applies).
+int auto_tile_kind(const int L[3], size_t real_size, size_t max_lds) {
+    for (int kind : kTileOrder) {
+        int d[3];
+        pyro::pyro_tile_dims(kind, d);
+        const int64_t tiles = int64_t(L[0] / (2 * d[0])) * (L[1] / (2 * d[1])) * (L[2] / (2 * d[2]));
+        if (tile_applies_to(L, kind, real_size, max_lds) && tiles >= kMinTilesPerPhase) return kind;
+    }
+    for (int kind : kTileOrder)
+        if (tile_applies_to(L, kind, real_size, max_lds)) return kind;
+    return 0;
+}
+// Tile shape of the tiled energy kernel: the sweep's when it has one, else
+// the first shape (same order) whose edges divide L (the energy only reads,
+// so no checkerboard of tiles is needed) and whose tile + halo fits the LDS
+// limit; 0 = the global-memory stencil kernel.
+int energy_tile_kind_for(const int L[3], size_t real_size, size_t max_lds, int sweep_tile_kind) {
+    if (sweep_tile_kind) return sweep_tile_kind;
+    for (int kind : kTileOrder) {
+        int d[3];
+        pyro::pyro_tile_dims(kind, d);
+        if (L[0] % d[0] == 0 && L[1] % d[1] == 0 && L[2] % d[2] == 0 &&
+            pyro::pyro_tile_lds(d[0], d[1], d[2], int(real_size), false) <= max_lds)
+            return kind;
+    }
+    return 0;
+}
+// Per-workgroup LDS limit of the current device.
+size_t device_max_lds(int device) {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        return size_t(lds);
+    return 65536;
+}
+
 struct Status : std::runtime_error {
     int code;
     Status(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -288,43 +334,17 @@ struct pyro_ctx {
     }
     bool resident_fits() const { return nc() <= pyro::kPyroResidentMaxCells; }
 
-    // Tile shape for the tiled strategy: L_a divisible by twice the tile
-    // edge (checkerboard of tiles) and the tile + halo within the LDS limit.
-    bool tile_applies(int kind) const {
-        if (!spec || kind < 1 || kind > pyro::kPyroTileKinds) return false;
-        int d[3];
-        pyro::pyro_tile_dims(kind, d);
-        for (int a = 0; a < 3; ++a)
-            if (geom.L[a] % (2 * d[a]) != 0) return false;
-        return pyro::pyro_tile_lds(d[0], d[1], d[2], int(real_size()), true) <= max_lds;
-    }
+    // Tile shape for the tiled strategy (tile_applies_to, auto_tile_kind).
+    bool tile_applies(int kind) const { return spec && tile_applies_to(geom.L.data(), kind, real_size(), max_lds); }
     int pick_tile_kind() const {
         if (tile_forced) return tile_applies(tile_forced) ? tile_forced : 0;
-        for (int kind : kTileOrder) {
-            int d[3];
-            pyro::pyro_tile_dims(kind, d);
-            const int64_t tiles = int64_t(geom.L[0] / (2 * d[0])) * (geom.L[1] / (2 * d[1])) * (geom.L[2] / (2 * d[2]));
-            if (tile_applies(kind) && tiles >= kMinTilesPerPhase) return kind;
-        }
-        for (int kind : kTileOrder)
-            if (tile_applies(kind)) return kind;
-        return 0;
+        return spec ? auto_tile_kind(geom.L.data(), real_size(), max_lds) : 0;
     }
-    // Tile shape of the tiled energy kernel: the sweep's when it has one,
-    // else the first shape (same order) whose edges divide L (the energy
-    // only reads, so no checkerboard of tiles is needed) and whose tile +
-    // halo fits the LDS limit; 0 = the global-memory stencil kernel.
+    // Tile shape of the tiled energy kernel (energy_tile_kind_for).
     int energy_tile_kind() const {
         if (tile_kind) return tile_kind;
         if (!spec) return 0;
-        for (int kind : kTileOrder) {
-            int d[3];
-            pyro::pyro_tile_dims(kind, d);
-            if (geom.L[0] % d[0] == 0 && geom.L[1] % d[1] == 0 && geom.L[2] % d[2] == 0 &&
-                pyro::pyro_tile_lds(d[0], d[1], d[2], int(real_size()), false) <= max_lds)
-                return kind;
-        }
-        return 0;
+        return energy_tile_kind_for(geom.L.data(), real_size(), max_lds, 0);
     }
 
     template <typename Real>
@@ -395,8 +415,8 @@ struct pyro_ctx {
         }
     }
     // n_sweeps sweeps of (kind + n_or OR passes) with the given strategy.
-    // epart_last (strategy 3 only): the last pass also leaves the energy
-    // partials of the final state there.
+    // epart_last (strategies 3 and 4): the last pass also leaves the energy
+    // partials of the final state there, the same bits from either.
     template <typename Real>
     void sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
                 unsigned long long pass_index, double* epart_last = nullptr) {
@@ -424,7 +444,8 @@ struct pyro_ctx {
             const auto st = state<Real>(pass_base);
             const auto J = pyro_couplings<Real>();
             timed(kFamSweep, [&] {
-                return pyro::launch_pyro_resident<Real>(st, J, kind, n_or, n_sweeps, pass_index, stream);
+                return pyro::launch_pyro_resident<Real>(st, J, kind, n_or, n_sweeps, pass_index, stream,
+                                                        epart_last);
             });
             return;
         }
@@ -442,9 +463,11 @@ struct pyro_ctx {
         else sweeps<float>(strat, kind, n_or, n_sweeps, pass_base, pass_index, epart_last);
     }
     // Whether run_sweeps(strat, ...) with n_sweeps > 0 can leave the energy
-    // partials (energy_finish then replaces energy): per-colour stencil
-    // launches, with the observables on the stencil too.
-    bool sweeps_leave_energy(int strat) const { return strat == 3 && stencil_observables(); }
+    // partials (energy_finish then replaces energy): the stencil strategies
+    // 3 and 4, which leave identical partials (so the autotuner's choice
+    // between them never changes a sampled energy), with the observables on
+    // the stencil too.
+    bool sweeps_leave_energy(int strat) const { return (strat == 3 || strat == 4) && stencil_observables(); }
 
     // Pick the faster launch strategy for (kind, n_or) by timing both on a
     // saved copy of the state, then restore it (results do not depend on the
@@ -805,12 +828,7 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
         require(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, PYRO_E_HIP, "no HIP device available");
         HIPCHK(hipGetDevice(&ctx->device));
         if (!ctx->stream) HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        {
-            int lds = 0;
-            if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess &&
-                lds > 0)
-                ctx->max_lds = size_t(lds);
-        }
+        ctx->max_lds = device_max_lds(ctx->device);
         const int64_t N = ctx->N();
         const int R = ctx->R;
         const int NS = ctx->NS;
@@ -1402,8 +1420,8 @@ int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int
         require(n_overrelax >= 0 && n_sweep >= 0 && n_steps >= 0, PYRO_E_INVALID, "negative count");
         require(ctx->anneal_step + n_steps <= ctx->n_sched, PYRO_E_INVALID, "steps beyond the end of the schedule");
         require(!sample_ssf || ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
-        require(!sample_ssf || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
-                "SSF sampling in anneal steps needs one accumulator slot per replica");
+        require(!sample_ssf || ctx->has_ssf_map || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
+                "SSF sampling in anneal steps needs a slot map or one accumulator slot per replica");
         if (n_steps == 0) return;
         const int strat = n_sweep > 0 ? ctx->choose_strategy(kind, n_overrelax) : 1;
         const unsigned long long ppr = (unsigned long long)n_sweep * (1 + n_overrelax);
@@ -1418,7 +1436,15 @@ int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int
             if (n_sweep > 0)
                 ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
                                 fused_energy ? ctx->epart.as<double>() : nullptr);
-            if (sample_ssf) ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
+            if (sample_ssf) {
+                if (ctx->has_ssf_map) {  // replica r (slot r) -> accumulator map[r]
+                    HIPCHK(pyro::launch_map_slots(ctx->slot_of.as<int32_t>(), ctx->ssf_map.as<int32_t>(),
+                                                  ctx->ssf_slot.as<int32_t>(), ctx->R, ctx->stream));
+                    ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>(), ctx->stream);
+                } else {
+                    ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
+                }
+            }
             if (fused_energy) ctx->energy_finish(false, pyro::pyro_colour_energy_blocks(int(ctx->nc())));
             else ctx->energy(false);
             HIPCHK(pyro::launch_anneal_record(ctx->e.as<double>(), ctx->R, ctx->ehist.as<double>(), ctx->n_sched,
@@ -1582,6 +1608,7 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha, const in
         }
         HIPCHK(hipDeviceSynchronize());
         ctx->munu_pending = 0;
+        ctx->has_ssf_map = false;  // a map names slots of the previous configuration
         ctx->ssf_ready = true;
     });
 }
@@ -1590,8 +1617,8 @@ int pyro_pt_ssf_map(pyro_ctx* ctx, const int32_t* map) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
         require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
-        ctx->invalidate_graph();
         if (!map) {
+            if (ctx->has_ssf_map) ctx->invalidate_graph();
             ctx->has_ssf_map = false;
             return;
         }
@@ -1602,8 +1629,11 @@ int pyro_pt_ssf_map(pyro_ctx* ctx, const int32_t* map) {
             // round update each slot from a single replica
             if (map[s] >= 0) require(used[map[s]]++ == 0, PYRO_E_INVALID, "two rungs map to one SSF slot");
         }
-        ctx->ssf_map.alloc(sizeof(int32_t) * ctx->R);
+        // the map lives at a fixed device address that captured rounds and
+        // anneal steps read when replayed, so a new map needs no new graph
+        if (!ctx->ssf_map.p) ctx->ssf_map.alloc(sizeof(int32_t) * ctx->R);
         ctx->h2d(ctx->ssf_map.p, map, sizeof(int32_t) * ctx->R);
+        if (!ctx->has_ssf_map) ctx->invalidate_graph();
         ctx->has_ssf_map = true;
     });
 }

@@ -60,6 +60,21 @@ __device__ __forceinline__ void normalize3(Real& x, Real& y, Real& z) {
         z /= nrm;
     }
 }
+// a . b and a x b with explicit fma: every kernel that inlines an update
+// rounds its dot and cross products alike, whatever code surrounds it (under
+// -ffp-contract=fast the compiler's choice of which product to fuse follows
+// the code around it)
+template <typename Real>
+__device__ __forceinline__ Real dot3(Real ax, Real ay, Real az, Real bx, Real by, Real bz) {
+    return fma(az, bz, fma(ay, by, ax * bx));
+}
+template <typename Real>
+__device__ __forceinline__ void cross3(Real ax, Real ay, Real az, Real bx, Real by, Real bz, Real& cx, Real& cy,
+                                       Real& cz) {
+    cx = fma(ay, bz, -(az * by));
+    cy = fma(az, bx, -(ax * bz));
+    cz = fma(ax, by, -(ay * bx));
+}
 // 1 / sqrt(x)
 template <typename Real> __device__ __forceinline__ Real dinv_sqrt(Real x);
 template <> __device__ __forceinline__ float dinv_sqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -144,12 +159,12 @@ __device__ __forceinline__ bool update_spin_u(const ReplicaParams<Real>& p, Real
         dsincos_rev<Real>(u24<Real>(u.m[1]), &s1, &c1);
         const Real r2 = dsqrt<Real>(Real(-2) * dlog<Real>(u24<Real>(u.m[2])));
         dsincos_rev<Real>(u24<Real>(u.m[3]), &s2, &c2);
-        Real nx = sig * (r1 * c1) + sx;
-        Real ny = sig * (r1 * s1) + sy;
-        Real nz = sig * (r2 * c2) + sz;
+        Real nx = fma(sig, r1 * c1, sx);
+        Real ny = fma(sig, r1 * s1, sy);
+        Real nz = fma(sig, r2 * c2, sz);
         normalize3<Real>(nx, ny, nz);
-        const Real curE = sx * hx + sy * hy + sz * hz;
-        const Real newE = nx * hx + ny * hy + nz * hz;
+        const Real curE = dot3(sx, sy, sz, hx, hy, hz);
+        const Real newE = dot3(nx, ny, nz, hx, hy, hz);
         const Real dE = newE - curE;
         if (dE < Real(0) || u24<Real>(u.m[4]) < dexp<Real>(ddiv<Real>(-dE, T))) {
             px = nx;
@@ -170,7 +185,8 @@ __device__ __forceinline__ bool update_spin_u(const ReplicaParams<Real>& p, Real
         const Real d = fma(hz, sz, fma(hy, sy, hx * sx));
         const Real f = ddiv<Real>(Real(1) - c, n2) * d;
         const Real g = ddiv<Real>(s, dsqrt<Real>(n2));
-        const Real cx = fma(hy, sz, -(hz * sy)), cy = fma(hz, sx, -(hx * sz)), cz = fma(hx, sy, -(hy * sx));
+        Real cx, cy, cz;
+        cross3(hx, hy, hz, sx, sy, sz, cx, cy, cz);
         Real nx = fma(g, cx, fma(c, sx, f * hx));
         Real ny = fma(g, cy, fma(c, sy, f * hy));
         Real nz = fma(g, cz, fma(c, sz, f * hz));
@@ -185,26 +201,27 @@ __device__ __forceinline__ bool update_spin_u(const ReplicaParams<Real>& p, Real
     } else if constexpr (KIND == 1) {
         // Lifted Metropolis, displacement form (MC.cpp:249-281).
         const int8_t d = *dp;
-        const Real hn = dsqrt<Real>(hx * hx + hy * hy + hz * hz);
+        const Real hn = dsqrt<Real>(dot3(hx, hy, hz, hx, hy, hz));
         if (!(hn > Real(0))) {  // no field direction: bounce
             *dp = int8_t(-d);
             return false;
         }
         const Real ex = ddiv<Real>(hx, hn), ey = ddiv<Real>(hy, hn), ez = ddiv<Real>(hz, hn);
-        const Real uu = ex * sx + ey * sy + ez * sz;
+        const Real uu = dot3(ex, ey, ez, sx, sy, sz);
         const Real du = (Real(d) * p.sig) * (-dlog<Real>(u24<Real>(u.m[0])));
         const Real nu = uu + du;
-        if (nu > Real(1) || nu < Real(-1) || fabs(Real(1) - uu * uu) < Real(1e-16)) {
+        const Real one_uu = fma(-uu, uu, Real(1));
+        if (nu > Real(1) || nu < Real(-1) || fabs(one_uu) < Real(1e-16)) {
             *dp = int8_t(-d);
             return false;
         }
-        const Real f = dsqrt<Real>(ddiv<Real>(Real(1) - nu * nu, Real(1) - uu * uu));
-        Real nx = nu * ex + f * (sx - uu * ex);
-        Real ny = nu * ey + f * (sy - uu * ey);
-        Real nz = nu * ez + f * (sz - uu * ez);
+        const Real f = dsqrt<Real>(ddiv<Real>(fma(-nu, nu, Real(1)), one_uu));
+        Real nx = fma(nu, ex, f * fma(-uu, ex, sx));
+        Real ny = fma(nu, ey, f * fma(-uu, ey, sy));
+        Real nz = fma(nu, ez, f * fma(-uu, ez, sz));
         normalize3<Real>(nx, ny, nz);
-        const Real curE = sx * hx + sy * hy + sz * hz;
-        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
+        const Real curE = dot3(sx, sy, sz, hx, hy, hz);
+        const Real dE = dot3(nx, ny, nz, hx, hy, hz) - curE;
         if (u24<Real>(u.m[1]) < dexp<Real>(ddiv<Real>(-dE, T))) {
             px = nx;
             py = ny;
@@ -217,9 +234,10 @@ __device__ __forceinline__ bool update_spin_u(const ReplicaParams<Real>& p, Real
     } else {
         // Lifted rotation Metropolis with reflecting poles (MC.cpp:194-244).
         const int8_t d = *dp;
-        const Real curE = sx * hx + sy * hy + sz * hz;
-        const Real ax = sy * hz - sz * hy, ay = sz * hx - sx * hz, az = sx * hy - sy * hx;
-        const Real a2 = ax * ax + ay * ay + az * az;
+        const Real curE = dot3(sx, sy, sz, hx, hy, hz);
+        Real ax, ay, az;
+        cross3(sx, sy, sz, hx, hy, hz, ax, ay, az);
+        const Real a2 = dot3(ax, ay, az, ax, ay, az);
         if (a2 <= Real(1e-20)) {
             *dp = int8_t(-d);
             return false;
@@ -227,21 +245,23 @@ __device__ __forceinline__ bool update_spin_u(const ReplicaParams<Real>& p, Real
         const Real th = Real(d) * p.sig;
         Real s, c;
         dsincos<Real>(th, &s, &c);
-        const Real dd = ax * sx + ay * sy + az * sz;
+        const Real dd = dot3(ax, ay, az, sx, sy, sz);
         const Real f = ddiv<Real>(Real(1) - c, a2) * dd;
         const Real g = ddiv<Real>(s, dsqrt<Real>(a2));
-        const Real cx = ay * sz - az * sy, cy = az * sx - ax * sz, cz = ax * sy - ay * sx;
-        Real nx = c * sx + f * ax + g * cx;
-        Real ny = c * sy + f * ay + g * cy;
-        Real nz = c * sz + f * az + g * cz;
+        Real cx, cy, cz;
+        cross3(ax, ay, az, sx, sy, sz, cx, cy, cz);
+        Real nx = fma(g, cx, fma(f, ax, c * sx));
+        Real ny = fma(g, cy, fma(f, ay, c * sy));
+        Real nz = fma(g, cz, fma(f, az, c * sz));
         normalize3<Real>(nx, ny, nz);
-        const Real bx = ny * hz - nz * hy, by = nz * hx - nx * hz, bz = nx * hy - ny * hx;
-        const Real b2 = bx * bx + by * by + bz * bz;
-        if (ax * bx + ay * by + az * bz < Real(0)) {
+        Real bx, by, bz;
+        cross3(nx, ny, nz, hx, hy, hz, bx, by, bz);
+        const Real b2 = dot3(bx, by, bz, bx, by, bz);
+        if (dot3(ax, ay, az, bx, by, bz) < Real(0)) {
             *dp = int8_t(-d);
             return false;
         }
-        const Real dE = (nx * hx + ny * hy + nz * hz) - curE;
+        const Real dE = dot3(nx, ny, nz, hx, hy, hz) - curE;
         const Real ratio = dsqrt<Real>(ddiv<Real>(b2, a2)) * dexp<Real>(ddiv<Real>(-dE, T));
         if (u24<Real>(u.m[0]) < ratio) {
             px = nx;

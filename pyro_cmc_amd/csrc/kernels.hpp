@@ -93,14 +93,15 @@ struct PyroCouplings {
 template <typename Real>
 hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
                               int kind, unsigned long long pass_index, hipStream_t s, double* epart = nullptr);
-inline int pyro_colour_energy_blocks(int n_cells) { return 16 * ((n_cells + 63) / 64); }
+__host__ __device__ constexpr int pyro_colour_energy_blocks(int n_cells) { return 16 * ((n_cells + 63) / 64); }
 // Whole sweeps with one resident workgroup (1024 threads, two cells per
-// thread) per replica; lattices of at most 2048 cells.
+// thread) per replica; lattices of at most 2048 cells.  With epart the last
+// pass also leaves the energy partials launch_pyro_colour leaves, bit for bit.
 constexpr int kPyroResidentMaxCells = 2048;
 template <typename Real>
 hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,
                                 int n_or, int n_sweeps, unsigned long long pass_index,
-                                hipStream_t s);
+                                hipStream_t s, double* epart = nullptr);
 
 // Tiled sweeps: one call launches one of the 8 checkerboard phases of a
 // sweep; its workgroups hold a tile + one-sided halo in LDS and run every

@@ -161,25 +161,35 @@ __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplin
 // S . h_low for the spin S = (x, y, z) of a site of pyro_sl p and the sums
 // of LowSums: J_shell S . s[shell], plus for XXZ Jx (z_p . S) sum_tp
 // z_tp . v[tp], written as sum_tp ((z_p . S) z_tp) . v[tp] so that none of
-// its products is one the field computes.
+// its products is one the field computes.  Explicit fma throughout, so every
+// kernel that inlines it (per-colour or resident) rounds alike.
 template <typename Real, int SHELLS>
 __device__ __forceinline__ Real low_bond_energy(const LowSums<Real>& lo, const PyroCouplings<Real>& J, int p,
                                                 Real x, Real y, Real z) {
-    Real e = J.J1 * (x * lo.s[0][0] + y * lo.s[0][1] + z * lo.s[0][2]);
-    if constexpr (SHELLS >= 2) e += J.J2 * (x * lo.s[1][0] + y * lo.s[1][1] + z * lo.s[1][2]);
-    if constexpr (SHELLS >= 3) e += J.J3 * (x * lo.s[2][0] + y * lo.s[2][1] + z * lo.s[2][2]);
+    Real e = J.J1 * dot3(x, y, z, lo.s[0][0], lo.s[0][1], lo.s[0][2]);
+    if constexpr (SHELLS >= 2) e = fma(J.J2, dot3(x, y, z, lo.s[1][0], lo.s[1][1], lo.s[1][2]), e);
+    if constexpr (SHELLS >= 3) e = fma(J.J3, dot3(x, y, z, lo.s[2][0], lo.s[2][1], lo.s[2][2]), e);
     if (J.xxz) {
-        const Real a = x * Real(pyrostencil::kLink[p][0]) + y * Real(pyrostencil::kLink[p][1]) +
-                       z * Real(pyrostencil::kLink[p][2]);
+        const Real a = dot3(x, y, z, Real(pyrostencil::kLink[p][0]), Real(pyrostencil::kLink[p][1]),
+                            Real(pyrostencil::kLink[p][2]));
         Real w = Real(0);
 #pragma unroll
         for (int tp = 0; tp < 4; ++tp)
-            w += (a * Real(pyrostencil::kLink[tp][0])) * lo.v[tp][0] +
-                 (a * Real(pyrostencil::kLink[tp][1])) * lo.v[tp][1] +
-                 (a * Real(pyrostencil::kLink[tp][2])) * lo.v[tp][2];
-        e += J.Jx * w;
+            w += dot3(a * Real(pyrostencil::kLink[tp][0]), a * Real(pyrostencil::kLink[tp][1]),
+                      a * Real(pyrostencil::kLink[tp][2]), lo.v[tp][0], lo.v[tp][1], lo.v[tp][2]);
+        e = fma(J.Jx, w, e);
     }
     return e;
+}
+
+// Energy term of a site updated in a pass that also sums the energy:
+// S . h_low - B . S in FP64 (see pyro_colour_kernel).
+template <typename Real, int SHELLS>
+__device__ __forceinline__ double low_site_energy(const LowSums<Real>& lo, const PyroCouplings<Real>& J, int p,
+                                                  const ReplicaParams<Real>& rp, Real x, Real y, Real z) {
+    const Real bond = low_bond_energy<Real, SHELLS>(lo, J, p, x, y, z);
+    const Real bs = dot3(x, y, z, rp.Bx, rp.By, rp.Bz);
+    return double(bond) - double(bs);
 }
 
 // Field from spins in global memory (Sr = S + r * 3 N), neighbour cells
@@ -309,9 +319,7 @@ __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
             accepted = update_spin_p<Real, KIND>(p, st, x, y, z, changed, site, r, pass, hx, hy, hz,
                                                  st.dir + size_t(r) * N + site);
             if (changed) st3(Sr, unsigned(site), x, y, z);
-            const Real bond = low_bond_energy<Real, SHELLS>(lo, Jr, sl % 4, x, y, z);
-            const Real bs = x * p.Bx + y * p.By + z * p.Bz;
-            e = double(bond) - double(bs);
+            e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, p, x, y, z);
         } else {
             pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, PER_REP ? J.at(r) : J, hx, hy, hz);
             accepted = apply_update<Real, KIND>(st, Sr, sl * nc + cell, r, pass, hx, hy, hz);
@@ -393,11 +401,17 @@ __global__ __launch_bounds__(256) void pyro_fields_kernel(DeviceState<Real> st, 
 }
 
 // One workgroup per replica; thread t owns cells t, t + BS, ... (CPT cells).
-template <typename Real, int SHELLS, int FIRST, int BS, int CPT>
+// EN: the last pass of the last sweep also leaves the energy partials of
+// pyro_colour_kernel<..., EN>, at the same indices and with the same
+// per-site arithmetic and wave sums: wave w of the workgroup covers cells
+// 64 w .. 64 w + 63 + j BS, which is block j (BS / 64) + w of the per-colour
+// launch, so the two strategies give bit-identical energies.
+template <typename Real, int SHELLS, int FIRST, int BS, int CPT, bool EN>
 __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                                                            PyroCouplings<Real> J, int n_or,
                                                            int n_sweeps,
-                                                           unsigned long long pass_index) {
+                                                           unsigned long long pass_index,
+                                                           double* __restrict__ epart) {
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     constexpr int SPC = 16 / NCOL;
     const int r = blockIdx.x;
@@ -412,8 +426,11 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     const int cell_b = threadIdx.x + BS < nc ? threadIdx.x + BS : 0;
     const CellAxes ax_a = cell_axes_of(threadIdx.x < nc ? threadIdx.x : 0, st.L0, st.L1, st.L2);
     const CellAxes ax_b = cell_axes_of(cell_b, st.L0, st.L1, st.L2);
+    const int bps = (nc + 63) / 64;
+    const size_t es = size_t(pyro_colour_energy_blocks(nc));
     for (int sw = 0; sw < n_sweeps; ++sw) {
         for (int ps = 0; ps <= n_or; ++ps, ++pass) {
+            const bool en_pass = EN && sw == n_sweeps - 1 && ps == n_or;  // uniform
             for (int col = 0; col < NCOL; ++col) {
                 for (int q = 0; q < SPC; ++q) {
                     const int sl = NCOL == 16 ? col : col + 4 * q;
@@ -422,7 +439,6 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
 #pragma unroll 1
                     for (int j = 0; j < CPT; ++j) {
                         const int cell = threadIdx.x + j * BS;
-                        if (cell >= nc) break;
                         CellAxes c;
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
@@ -430,6 +446,35 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                             c.a1[a] = j ? ax_b.a1[a] : ax_a.a1[a];
                             c.a2[a] = j ? ax_b.a2[a] : ax_a.a2[a];
                         }
+                        if constexpr (EN) {
+                            if (en_pass) {
+                                // every lane reaches the wave sum below
+                                double e = 0.0;
+                                if (cell < nc) {
+                                    LowSums<Real> lo;
+                                    Real hx, hy, hz;
+                                    pyro_field_rt<Real, SHELLS, NCOL>(sl, Sr, nc, c, Jr, hx, hy, hz, &lo);
+                                    const int site = sl * nc + cell;
+                                    const Vec3<Real> v = ld3(Sr, unsigned(site));
+                                    Real x = v.x, y = v.y, z = v.z;
+                                    bool changed;
+                                    const bool a = ps == 0
+                                        ? update_spin_p<Real, FIRST>(rp, st, x, y, z, changed, site, r, pass, hx, hy,
+                                                                     hz, st.dir + size_t(r) * N + site)
+                                        : update_spin_p<Real, 3>(rp, st, x, y, z, changed, site, r, pass, hx, hy,
+                                                                 hz, st.dir + size_t(r) * N + site);
+                                    if (ps == 0 && a) ++my_acc;
+                                    if (changed) st3(Sr, unsigned(site), x, y, z);
+                                    e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, x, y, z);
+                                }
+                                const double t = wave_sum_f64(e);
+                                const int cb = j * (BS / 64) + int(threadIdx.x >> 6);
+                                if ((threadIdx.x & 63) == 0 && cb < bps)
+                                    epart[size_t(r) * es + size_t(col * SPC + q) * unsigned(bps) + unsigned(cb)] = t;
+                                continue;
+                            }
+                        }
+                        if (cell >= nc) break;
                         Real hx, hy, hz;
                         pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, Jr, hx, hy, hz);
                         if (ps == 0)
@@ -809,20 +854,24 @@ hipError_t launch_pyro_fields(const DeviceState<Real>& st, const PyroCouplings<R
 template <typename Real>
 hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,
                                 int n_or, int n_sweeps, unsigned long long pass_index,
-                                hipStream_t s) {
+                                hipStream_t s, double* epart) {
     constexpr int BS = 1024, CPT = 2;
     if (st.n_cells > BS * CPT) return hipErrorInvalidValue;
     const dim3 grid(unsigned(st.R));
-    return dispatch_pyro(J, [&](auto shells) -> hipError_t {
-        constexpr int SH = decltype(shells)::value;
-        switch (kind) {
-            case 0: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 0, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
-            case 1: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 1, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
-            case 2: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 2, BS, CPT>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index); break;
-            default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    });
+    auto go = [&](auto en) -> hipError_t {
+        constexpr bool EN = decltype(en)::value;
+        return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+            constexpr int SH = decltype(shells)::value;
+            switch (kind) {
+                case 0: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 0, BS, CPT, EN>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
+                case 1: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 1, BS, CPT, EN>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
+                case 2: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 2, BS, CPT, EN>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        });
+    };
+    return epart ? go(std::true_type{}) : go(std::false_type{});
 }
 
 template <typename Real>

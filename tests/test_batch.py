@@ -170,9 +170,9 @@ def test_anneal_steps_equal_eager_steps(gpu, oracle, prec, strategy):
                     ctx.pass_count()))
     (Eg, Sg, Qg, pg), (Ee, Se, Qe, pe) = out
     assert pg == pe
-    if strategy == 3:
-        # per-colour launches: the step's last colour pass sums the energy
-        # (bonds to earlier colours), the eager steps use the energy kernel
+    if strategy in (3, 4):
+        # stencil strategies: the step's last pass sums the energy (bonds to
+        # earlier colours), the eager steps use the energy kernel
         assert np.allclose(Eg, Ee, rtol=0, atol=1e-5 * 16 * 8)
     else:
         assert np.array_equal(Eg, Ee)
@@ -225,10 +225,70 @@ def test_uneven_batch_driver(gpu, oracle):
     assert stats["contexts"] == {L: sum(it.L == L for it in items) for L in (2, 4, 8) if any(it.L == L for it in items)}
     for it, r in zip(items, res):
         assert r.item is it
-        assert np.array_equal(r.T_list, grid) and r.E.shape == (7,) and r.n_samples == 3
+        assert np.array_equal(r.T_list, grid) and r.E.shape == (7,) and np.array_equal(r.n_samples, [3])
         o = oracle.Oracle.pyrochlore(it.L, *it.J)
         o.set_field(np.array(it.B))
         o.set_spins(r.spins)
         e = o.energy_per_cell()
         assert abs(r.E_final - e) <= 1e-5 * 16 * _scale(it.J, it.B)
-        assert np.all(np.isfinite(r.S)) and r.S.shape == (3, it.L ** 3, 2)
+        assert np.all(np.isfinite(r.S)) and r.S.shape == (1, 3, it.L ** 3, 2)
+
+
+def test_batch_samples_every_T_sample(gpu, oracle):
+    """With several sampling temperatures (C2's {1.0, 0.2, 0.05}) the batch
+    runs anneal.cpp:193-221: at each T_sample grid step, after its energy
+    sample, n_sample x (n_sweep sweeps + SSF sample) into that temperature's
+    accumulator.  The replayed device schedule equals the same loop issued
+    as eager calls (spins and SSF sums bit for bit, energies to rounding:
+    the graph takes them from the last colour pass), every T_sample holds
+    n_sample samples, and the final spins give each item's own energy."""
+    from pyro_cmc_amd import batch
+
+    items = batch.uneven_items(n=5, sizes=(2, 4), seed=8)
+    kw = dict(T_hot=5.0, T_cold=0.05, n_steps=8, n_sweep=2, n_burn_in=3, n_sample=2, seed=3)
+    Ts = [1.0, 0.2, 0.05]
+    res, _ = batch.run_anneal_batch(items, keep_spins=True, T_sample=Ts, **kw)
+    grid, s_idx = batch.generate_T_profile(5.0, 0.05, Ts, 8)
+    assert len(grid) == 8 + 3 and len(s_idx) == 3  # T_sample merged into the 8-step grid
+    for it, r in zip(items, res):
+        assert np.array_equal(r.T_sample, Ts) and np.array_equal(r.n_samples, [2, 2, 2])
+        assert r.S.shape == (3, 3, it.L ** 3, 2) and r.E.shape == (len(grid),)
+        o = oracle.Oracle.pyrochlore(it.L, *it.J)
+        o.set_field(np.array(it.B))
+        o.set_spins(r.spins)
+        assert abs(r.E_final - o.energy_per_cell()) <= 1e-5 * 16 * _scale(it.J, it.B)
+    # eager replay of the loop for the L = 4 group
+    idx = [i for i, it in enumerate(items) if it.L == 4]
+    c = gpu.Context.pyrochlore(4, n_replicas=len(idx), precision=gpu.FP32, seed=3 * 1000003 + 4)
+    c.define_J1J2J3(1.0, 1.0, 1.0)
+    c.setup()
+    c.set_replica_couplings(np.array([items[i].J for i in idx], np.float64))
+    for r, i in enumerate(idx):
+        c.set_field(np.array(items[i].B, np.float64), r)
+    c.randomize(3 * 7919 + 4)
+    R = c.R
+    c.ssf_configure(("xx", "yy", "zz"), n_slots=3 * R, store_error_term=True)
+
+    def step(T, n_sw):
+        c.set_temperatures(np.full(R, T))
+        c.sweep(gpu.METROPOLIS, 1, n_sw, count=False)
+
+    step(5.0, 3)
+    E = []
+    for i, T in enumerate(grid):
+        step(T, 2)
+        E.append(c.energy_per_cell())
+        if i in s_idx:
+            j = s_idx.index(i)
+            for _ in range(2):
+                step(T, 2)
+                c.ssf_sample(np.arange(R, dtype=np.int32) + j * R)
+    E = np.array(E).T
+    for r, i in enumerate(idx):
+        assert np.array_equal(res[i].spins, c.get_spins(r))
+        assert np.allclose(res[i].E, E[r], rtol=0, atol=1e-5 * 16 * 8)
+        for j in range(3):
+            S, S2, n = c.ssf_read(j * R + r)
+            assert n == 2
+            assert np.array_equal(res[i].S[j], S) and np.array_equal(res[i].S2[j], S2)
+    c.close()

@@ -229,11 +229,29 @@ def _temper_args(tmp_path, R, L, seed, munu):
     return args
 
 
+def _oracle_colour_sweeps(o, r, T, colours, n_col, key, p, n_sweeps, n_or=1):
+    """n_sweeps counter-mode Metropolis + n_or OR sweeps of oracle o as
+    replica r from pass index p; returns the next pass index."""
+    for _ in range(n_sweeps):
+        o.pass_ctr(0, T, colours, n_col, key, r, p)
+        p += 1
+        for _ in range(n_or):
+            o.pass_ctr(3, T, colours, n_col, key, r, p)
+            p += 1
+    return p
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("munu", [False, True])
-def test_temper_file_matches_library_replay(gpu, tmp_path, munu):
+def test_temper_file_matches_oracle_replay(gpu, oracle, tmp_path, munu):
+    """The whole temper run (temper.cpp:418-508) replayed by the oracle:
+    counter-mode colour sweeps of every replica at its rung's temperature,
+    the SSF of the T_sample rungs, one energy sample per rung and round,
+    and the checkerboard exchange decided as temper.cpp:437-473 decides it.
+    Energies, swap table, SSF sums (contracted and sublattice-resolved) and
+    the coldest replica's spins of the files equal the replay."""
     pyro = gpu
-    R, L, seed = 6, 2, 5
+    R, L, seed, n_rounds = 6, 2, 5, 7
     r = _run([_tool(pyro, "temper"), *_temper_args(tmp_path, R, L, seed, munu)])
     assert r.returncode == 0, r.stderr
     stem = f"run_L={L}_J1=1_J2=-0.05_J3=0.05_Delta=1_B=0,0,0_seed={seed}_Tc=0.25_sw=2_"
@@ -249,51 +267,72 @@ def test_temper_file_matches_library_replay(gpu, tmp_path, munu):
         Mf = f.read("/ssf/static_corr_munu") if munu else None
         assert f.exists("/ssf/static_corr_munu") == munu
     assert len(T_asc) == R and np.all(np.diff(T_asc) > 0) and T_asc[-1] == 2.0
-    T_grid = T_asc[::-1].copy()
+    T_grid = T_asc[::-1].copy()  # slot 0 hottest
     assert Sf.shape == (3, 2, L ** 3, 2)
-    np.testing.assert_array_equal(nE, np.full(R, 7, np.uint64))
-    np.testing.assert_array_equal(nS, [7, 7])
+    np.testing.assert_array_equal(nE, np.full(R, n_rounds, np.uint64))
+    np.testing.assert_array_equal(nS, [n_rounds, n_rounds])
 
     key = pyro.seed_key(seed)
-    ctx = pyro.Context.pyrochlore(L, R, pyro.FP64, seed=key)
+    ctx = pyro.Context.pyrochlore(L, 1, pyro.FP64, seed=key)  # for the colouring only
     ctx.define_J1J2J3(1.0, -0.05, 0.05, 1.0)
     ctx.setup()
-    ctx.set_T_ref(1.0)
-    ctx.set_field([0, 0, 0])
-    ctx.randomize(key)
-    ctx.pt_init(T_grid, pyro.seed_key(seed + R))
+    colours, n_col = ctx.colouring(), ctx.info()["n_colours"]
+    ctx.close()
+    os_ = []
+    for rep in range(R):
+        o = oracle.Oracle.pyrochlore(L, 1.0, -0.05, 0.05)
+        o.randomize_ctr(key, rep)
+        os_.append(o)
+    p = 0
+    for rep in range(R):  # burn-in: 4 sweeps at the starting rung (slot = replica)
+        p_end = _oracle_colour_sweeps(os_[rep], rep, T_grid[rep], colours, n_col, key, p, 4)
+    p = p_end
     sampled = sorted([int(np.flatnonzero(T_grid == T)[0]) for T in (0.4, 0.25)], key=lambda s: T_grid[s])
-    ctx.ssf_configure(("xx", "yy", "zz"), len(sampled), True, munu=munu, munu_buffer=16)
-    m = np.full(R, -1, np.int32)
-    for j, s in enumerate(sampled):
-        m[s] = j
-    ctx.pt_ssf_map(m)
-    ctx.sweep(pyro.METROPOLIS, 1, 4)
-    ctx.pt_rounds(pyro.METROPOLIS, 1, 2, True, 7)
-    st = ctx.pt_stats()
-    np.testing.assert_array_equal(E, st["E"][::-1])
-    np.testing.assert_array_equal(E2, st["E2"][::-1])
-    for j in range(len(sampled)):
-        S, S2, n = ctx.ssf_read(j)
-        assert n == 7
-        np.testing.assert_array_equal(Sf[:, j], S)
-        np.testing.assert_array_equal(S2f[:, j], S2)
-        if munu:
-            for c in range(3):
-                np.testing.assert_array_equal(Mf[c, j], ctx.ssf_read_munu(j, c))
-    # per-pair swap summary printed by the driver
+    rep_of = np.arange(R, dtype=np.int32)
+    att = np.zeros(R, np.uint64)
+    accp = np.zeros(R, np.uint64)
+    Es, E2s = np.zeros(R), np.zeros(R)
+    S = np.zeros((3, 2, L ** 3, 2))
+    S2 = np.zeros_like(S)
+    M = np.zeros((3, 2, L ** 3, 16, 16, 2))
+    swap_seed = pyro.seed_key(seed + R)
+    for rnd in range(n_rounds):
+        slot_of = np.argsort(rep_of)
+        for rep in range(R):
+            p_end = _oracle_colour_sweeps(os_[rep], rep, T_grid[slot_of[rep]], colours, n_col, key, p, 2)
+        p = p_end
+        for j, sl in enumerate(sampled):
+            out, m = os_[rep_of[sl]].ssf_sample(("xx", "yy", "zz"), munu=True)
+            S[:, j] += out
+            S2[:, j] += out ** 2
+            M[:, j] += m
+        e = np.array([o.energy_per_cell() for o in os_])
+        Es += e[rep_of]
+        E2s += e[rep_of] ** 2
+        rep_of = oracle.exchange(T_grid, e, L ** 3, rep_of, rnd, swap_seed, att, accp)
+    scale = 6 + 12 * 0.05 + 12 * 0.05
+    np.testing.assert_allclose(E, Es[::-1], rtol=0, atol=1e-12 * 16 * scale * n_rounds)
+    np.testing.assert_allclose(E2, E2s[::-1], rtol=0, atol=1e-12 * (16 * scale) ** 2 * n_rounds)
+    N = 16 * L ** 3
+    np.testing.assert_allclose(Sf, S, rtol=0, atol=1e-12 * N ** 2 * n_rounds)
+    np.testing.assert_allclose(S2f, S2, rtol=0, atol=1e-12 * N ** 4 * n_rounds)
+    if munu:
+        # FP32 MFMA products: ~1e-7 relative to |A|^2 ~ L^3 per term
+        assert np.abs(Mf - M).max() <= 2e-5 * L ** 3 * n_rounds
+    # per-pair swap summary printed by the driver, decided as the oracle decides
     pairs = re.findall(r"pair\s+(\d+)<->\s*(\d+).*\((\d+)/(\d+)\)", r.stdout)
     assert len(pairs) == R - 1
-    for a, b, acc, att in pairs:
+    for a, b, acc, at in pairs:
         a = int(a)
         assert int(b) == a + 1
-        assert int(acc) == st["accepts"][a] and int(att) == st["attempts"][a]
+        assert int(acc) == accp[a] and int(at) == att[a]
         # checkerboard: pair a is attempted in the rounds of parity a % 2
-        assert int(att) == len([k for k in range(7) if k % 2 == a % 2])
-    # coldest replica's spins
-    rep = ctx.pt_permutation()[R - 1]
+        assert int(at) == len([k for k in range(n_rounds) if k % 2 == a % 2])
+    assert accp.sum() > 0  # the run must see swaps
     with _h5(tmp_path / (stem + ".spins.h5")) as f:
-        np.testing.assert_array_equal(f.read("spin_orientation"), ctx.get_spins(int(rep)))
+        # rounding of the shell-grouped field sums against the oracle's
+        # per-bond sums, accumulated over 36 passes
+        np.testing.assert_allclose(f.read("spin_orientation"), os_[rep_of[R - 1]].get_spins(), rtol=0, atol=1e-9)
 
 
 @pytest.mark.gpu
@@ -325,12 +364,13 @@ def test_anneal_seeds_merge(gpu, tmp_path):
 
 
 @pytest.mark.gpu
-def test_fieldcool_file_matches_library_replay(gpu, tmp_path):
-    """fieldcool.cpp:31-214: anneal in a field, release it, re-burn, sample."""
+def test_fieldcool_file_matches_oracle_replay(gpu, oracle, tmp_path):
+    """fieldcool.cpp:148-205 replayed by the oracle: anneal in a field,
+    release it, re-burn at the final temperature, sample the SSF."""
     pyro = gpu
     args = ["-o", str(tmp_path), "-s", "6", "--J1", "1", "--J2", "0.1", "-B", "0", "0", "0.3",
             "--T_hot", "2", "--T_cold", "0.2", "--n_steps", "4", "--n_sweep", "2", "--n_burn_in", "3",
-            "--n_sample", "2", "2"]
+            "--n_sample", "2", "--save_state", "2"]
     r = _run([_tool(pyro, "fieldcool"), *args])
     assert r.returncode == 0, r.stderr
     stem = "run_L=2_J1=1_J2=0.1_J3=0_Delta=1_B=0,0,0.3_seed=6_Tc=0.2_"
@@ -339,6 +379,7 @@ def test_fieldcool_file_matches_library_replay(gpu, tmp_path):
         E = f.read("/energy/E")
         T_final = f.read("/ssf/T_list")
         S = f.read("/ssf/static_corr")
+        S2 = f.read("/ssf/static_corr_2")
         assert f.read("/ssf/corr_lookup") == ["xx", "yy", "zz"]
         np.testing.assert_array_equal(f.read("/ssf/n_samples"), [2])
     f = (0.2 / 2.0) ** (1 / 4)
@@ -346,28 +387,34 @@ def test_fieldcool_file_matches_library_replay(gpu, tmp_path):
     np.testing.assert_allclose(T_final, [2.0 * f ** 4], rtol=1e-14)
 
     key = pyro.seed_key(6)
-    ctx = pyro.Context.pyrochlore(2, 1, pyro.FP64, seed=key)
+    ctx = pyro.Context.pyrochlore(2, 1, pyro.FP64, seed=key)  # for the colouring only
     ctx.define_J1J2J3(1.0, 0.1, 0.0, 1.0)
     ctx.setup()
-    ctx.set_T_ref(1.0)
-    ctx.set_field([0, 0, 0.3])
-    ctx.randomize(key)
-    ctx.set_temperatures([2.0])
-    ctx.sweep(pyro.METROPOLIS, 1, 3)
+    colours, n_col = ctx.colouring(), ctx.info()["n_colours"]
+    ctx.close()
+    o = oracle.Oracle.pyrochlore(2, 1.0, 0.1, 0.0)
+    o.set_field(np.array([0, 0, 0.3]))
+    o.randomize_ctr(key, 0)
+    p = _oracle_colour_sweeps(o, 0, 2.0, colours, n_col, key, 0, 3)
     E_rep = []
     for T in T_steps:
-        ctx.set_temperatures([T])
-        ctx.sweep(pyro.METROPOLIS, 1, 2)
-        E_rep.append(ctx.energy_per_cell()[0])
-    np.testing.assert_array_equal(E, E_rep)
-    ctx.ssf_configure(("xx", "yy", "zz"), 1, True)
-    ctx.set_field([0, 0, 0])
-    ctx.set_temperatures([T_final[0]])
-    ctx.sweep(pyro.METROPOLIS, 1, 3)
+        p = _oracle_colour_sweeps(o, 0, T, colours, n_col, key, p, 2)
+        E_rep.append(o.energy_per_cell())
+    scale = 6 + 12 * 0.1 + 0.3
+    np.testing.assert_allclose(E, E_rep, rtol=0, atol=1e-12 * 16 * scale)
+    o.set_field(np.array([0.0, 0.0, 0.0]))
+    p = _oracle_colour_sweeps(o, 0, T_final[0], colours, n_col, key, p, 3)
+    want, want2 = 0.0, 0.0
     for _ in range(2):
-        ctx.sweep(pyro.METROPOLIS, 1, 2)
-        ctx.ssf_sample([0])
-    np.testing.assert_array_equal(S[:, 0], ctx.ssf_read(0)[0])
+        p = _oracle_colour_sweeps(o, 0, T_final[0], colours, n_col, key, p, 2)
+        out = o.ssf_sample(("xx", "yy", "zz"))
+        want = want + out
+        want2 = want2 + out ** 2
+    N = 16 * 8
+    np.testing.assert_allclose(S[:, 0], want, rtol=0, atol=1e-12 * N ** 2 * 2)
+    np.testing.assert_allclose(S2[:, 0], want2, rtol=0, atol=1e-12 * N ** 4 * 2)
+    with _h5(tmp_path / (stem + ".spins.h5")) as f:
+        np.testing.assert_allclose(f.read("spin_orientation"), o.get_spins(), rtol=0, atol=1e-9)
 
 
 @pytest.mark.gpu

@@ -10,6 +10,7 @@ accept/reject decision and lifted direction matches.
 import numpy as np
 import pytest
 
+import fp32_bound
 from helpers import byhand_spiral, random_spins
 
 pytestmark = pytest.mark.gpu
@@ -159,24 +160,37 @@ def test_sweep_trajectory_fp64(gpu, oracle, strategy, kind, n_or, J, Delta, B):
     assert ctx.pass_count() == pass0 + 3 * (1 + n_or)
 
 
-@pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 1), (2, 1)])
+@pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 1), (2, 1), (1, 0)])
 def test_sweep_fp32_tracks_fp64(gpu, oracle, kind, n_or):
+    """FP32 passes against the FP64 oracle pass by pass, each from the GPU's
+    own state: every site's deviation is bounded by FP32_SITE_TOL unless a
+    near-threshold decision upstream of it in the pass explains it."""
     L, seed, T = 3, 77, 0.6
     J = (1.0, -0.05, 0.05)
     ctx, (o,) = make(gpu, oracle, L, J, 1.0, 32, seed=seed)
     ctx.set_temperatures([T])
     S = random_spins(ctx.N, np.random.default_rng(9))
     ctx.set_spins(S)
-    o.set_spins(S)
     colours = ctx.colouring()
     n_col = ctx.info()["n_colours"]
-    ctx.sweep(kind, n_or, 1)
-    _oracle_sweeps(o, 0, kind, n_or, 1, T, colours, n_col, seed, 0)
-    diff = np.abs(ctx.get_spins() - o.get_spins()).max(1)
-    # identical uniforms: only decisions within FP32 rounding of the
-    # threshold may differ, so almost every site agrees to FP32 precision
-    assert np.mean(diff > 1e-4) < 0.01
-    assert np.median(diff) < 1e-5
+    adj = fp32_bound.padded_adjacency(o, (*J, 1.0))
+    sites = np.arange(ctx.N)
+    for sweep in range(3):
+        for p in range(1 + n_or):
+            o.set_spins(ctx.get_spins())
+            o.set_dir(ctx.get_lifted_dir())
+            pidx = ctx.pass_count()
+            if p == 0:
+                ctx.sweep(kind, 0, 1)
+            else:
+                ctx.overrelax(1)
+            info = np.empty((ctx.N, 4))
+            o.pass_ctr(kind if p == 0 else 3, T, colours, n_col, seed, 0, pidx, info=info)
+            E, taint, seeds = fp32_bound.propagate(sites, colours, info, adj, T, field_scale(J), ctx.N)
+            fp32_bound.check(ctx.get_spins(), o.get_spins(), E, taint, seeds, ctx.N, f"kind {kind} sweep {sweep} pass {p}")
+            if kind in (1, 2) and p == 0:  # a lifted direction may differ only where a decision may flip
+                dd = ctx.get_lifted_dir() != o.get_dir()
+                assert np.all(taint[dd])
     assert np.allclose(np.linalg.norm(ctx.get_spins(), axis=1), 1.0, atol=1e-6)
 
 
@@ -222,6 +236,38 @@ def test_stencil_and_table_kernels_agree(gpu, prec):
         out.append(ctx.get_spins(1))
     tol = 1e-12 if prec == 64 else 1e-4
     assert np.mean(np.abs(out[0] - out[1]).max(1) > tol) < 0.01
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_stencil_strategies_sample_identical_energies(gpu, prec):
+    """Tempering rounds and anneal steps take their energies from the last
+    sweep pass under both stencil strategies (per-colour launches, 3, and
+    resident workgroups, 4) and under the automatic choice between them:
+    spins, sampled E / E^2 sums, swap statistics and anneal energy
+    histories agree bit for bit, so the autotuner's timing never reaches a
+    result."""
+    out = []
+    for strategy in (3, 4, 0):
+        ctx = gpu.Context.pyrochlore(3, n_replicas=6, precision=prec, seed=31)
+        ctx.define_J1J2J3(1.0, -0.05, 0.05, 0.9)
+        ctx.setup()
+        ctx.set_field(np.array([0.05, 0.0, -0.1]))
+        ctx.set_strategy(strategy)
+        ctx.randomize(2)
+        ctx.pt_init(np.geomspace(2.0, 0.1, 6), 5)
+        ctx.pt_rounds(gpu.METROPOLIS, 2, 2, n_rounds=5)
+        st = ctx.pt_stats()
+        ctx.anneal_schedule(np.geomspace(3.0, 0.2, 4))
+        ctx.anneal_steps(gpu.LIFTED, 1, 2, False, 4)
+        hist = ctx.anneal_history()
+        out.append((st, hist, [ctx.get_spins(r) for r in range(ctx.R)]))
+        ctx.close()
+    for st, hist, spins in out[1:]:
+        for key in ("E", "E2", "n", "attempts", "accepts"):
+            assert np.array_equal(st[key], out[0][0][key]), key
+        assert np.array_equal(hist, out[0][1])
+        for a, b in zip(spins, out[0][2]):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("stencil", [True, False])
@@ -362,21 +408,32 @@ def test_tempering_fused_energy(gpu, oracle, prec, L, R, J, kind, n_or):
 
 
 @pytest.mark.parametrize("prec", [32, 64])
-@pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 0), (2, 1), (1, 1)])
-def test_energy_pass_is_bit_identical(gpu, prec, kind, n_or):
-    """The colour kernel that also sums the energy (the last pass of an
-    anneal step or tempering round) updates every site exactly as the plain
-    kernel does: anneal steps and plain sweeps end in the same spins, bit
-    for bit, with XXZ per-replica couplings and fields."""
+@pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 0), (2, 1), (1, 1), (0, 0), (2, 0)])
+@pytest.mark.parametrize("per_rep", [True, False])
+@pytest.mark.parametrize("strategy", [3, 4])
+def test_energy_pass_is_bit_identical(gpu, prec, kind, n_or, per_rep, strategy):
+    """The kernels that also sum the energy (the last pass of an anneal step
+    or tempering round; per-colour launches and resident workgroups) update
+    every site exactly as the plain kernels do: anneal steps and plain
+    sweeps end in the same spins, bit for bit, for every update kind as the
+    last pass, with XXZ per-replica couplings and fields or the single-model
+    kernels."""
     out = []
     for fused in (True, False):
         ctx = gpu.Context.pyrochlore(3, n_replicas=3, precision=prec, seed=21)
-        ctx.define_J1J2J3(1.0, 0.1, 0.1)
+        if per_rep:
+            ctx.define_J1J2J3(1.0, 0.1, 0.1)
+        else:
+            ctx.define_J1J2J3(1.0, -0.05, 0.05, 0.8)
         ctx.setup()
-        ctx.set_replica_couplings(np.array([[1.0, -0.2, 0.15, 0.7], [-1.0, 0.1, -0.05, 1.3], [0.5, 0.2, 0.1, 1.0]]))
-        for r, B in enumerate([(0.0, 0.0, 0.3), (0.1, -0.3, 0.2), (0.0, 0.0, 0.0)]):
-            ctx.set_field(np.array(B), r)
-        ctx.set_strategy(3)
+        if per_rep:
+            ctx.set_replica_couplings(np.array([[1.0, -0.2, 0.15, 0.7], [-1.0, 0.1, -0.05, 1.3],
+                                                [0.5, 0.2, 0.1, 1.0]]))
+            for r, B in enumerate([(0.0, 0.0, 0.3), (0.1, -0.3, 0.2), (0.0, 0.0, 0.0)]):
+                ctx.set_field(np.array(B), r)
+        else:
+            ctx.set_field(np.array([0.1, -0.2, 0.3]))
+        ctx.set_strategy(strategy)
         ctx.randomize(8)
         if fused:
             ctx.anneal_schedule(np.array([1.5, 0.6]))
@@ -607,7 +664,9 @@ def test_ssf_munu_mfma(gpu, oracle, prec):
                                                 (16, (1.0, 0.1, 0.1), 1, 1, (8, 4, 4)),
                                                 (16, (1.0, -0.05, 0.05), 0, 2, (4, 4, 8)),
                                                 (16, (1.0, 0.1, 0.0), 2, 1, (8, 8, 4)),
-                                                (24, (1.0, 0.1, 0.1), 1, 1, (4, 4, 4))])
+                                                (24, (1.0, 0.1, 0.1), 1, 1, (4, 4, 4)),
+                                                # the large tempering run's sweep: lifted + OR on 6x6x4 tiles
+                                                (24, (1.0, 0.1, 0.1), 1, 1, (6, 6, 4))])
 def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, tile):
     """The tiled schedule (tiles in LDS, all passes per tile) equals the
     sequential order pyro_sweep_order reports, for every tile shape; the
@@ -631,16 +690,16 @@ def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, t
         os_[r].set_spins(S)
     site, koff = ctx.sweep_order(n_or)
     assert np.array_equal(np.sort(site[koff == 0]), np.arange(ctx.N))
-    try:
-        acc = ctx.sweep(kind, n_or, 2)
-    except gpu.PyroError as e:
-        assert e.code == 5  # tile + halo exceeds this device's LDS limit
-        pytest.skip(str(e))
-    for r, o in enumerate(os_):
-        a = o.sweep_order(kind, T[r], site, koff, seed, r, 0)
-        a += o.sweep_order(kind, T[r], site, koff, seed, r, 1 + n_or)
-        diff = np.abs(ctx.get_spins(r) - o.get_spins()).max(1)
-        if prec == 64:
+    if prec == 64:
+        try:
+            acc = ctx.sweep(kind, n_or, 2)
+        except gpu.PyroError as e:
+            assert e.code == 5  # tile + halo exceeds this device's LDS limit
+            pytest.skip(str(e))
+        for r, o in enumerate(os_):
+            a = o.sweep_order(kind, T[r], site, koff, seed, r, 0)
+            a += o.sweep_order(kind, T[r], site, koff, seed, r, 1 + n_or)
+            diff = np.abs(ctx.get_spins(r) - o.get_spins()).max(1)
             # every decision equal; rounding of the shell-grouped field sum
             # against the oracle's per-bond sum accumulates over the passes
             # (most at lifted moves close to the field axis, where the
@@ -648,8 +707,28 @@ def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, t
             # L = 24 reaches 1.1e-10)
             assert int(acc[r]) == a
             assert diff.max() < 1e-9
-        else:
-            assert np.mean(diff > 1e-4) < 0.01
+        return
+    # FP32: sweep by sweep from the GPU's own state, every site within its
+    # first-order error bound
+    adj = fp32_bound.padded_adjacency(os_[0], (*J, 1.0))
+    stage = fp32_bound.tiled_stages(ctx, site, koff)
+    for sw in range(2):
+        before = [(ctx.get_spins(r), ctx.get_lifted_dir(r)) for r in range(R)]
+        p0 = ctx.pass_count()
+        try:
+            ctx.sweep(kind, n_or, 1)
+        except gpu.PyroError as e:
+            assert e.code == 5  # tile + halo exceeds this device's LDS limit
+            pytest.skip(str(e))
+        for r, o in enumerate(os_):
+            o.set_spins(before[r][0])
+            o.set_dir(before[r][1])
+            info = np.empty((len(site), 4))
+            o.sweep_order(kind, T[r], site, koff, seed, r, p0, info=info)
+            E, taint, seeds = fp32_bound.propagate(site, stage, info, adj, T[r], field_scale(J, (0.05, -0.1, 0.2)),
+                                                   ctx.N)
+            fp32_bound.check(ctx.get_spins(r), o.get_spins(), E, taint, seeds, len(site),
+                             f"L {L} tile {ctx.tile_shape()} replica {r}")
 
 
 def test_tile_shape_rules(gpu):
