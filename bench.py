@@ -262,28 +262,48 @@ def _cpu_worker(job):
     return o.N * (1 + N_OR) * n, dt
 
 
+def _cpu_worker_best(job):
+    """One host process alone: warm-up, then `reps` timed stretches of
+    `seconds` each of the same sweeps; returns the best rate
+    (bench_speed.cpp:173-187 reports the best of its repetitions)."""
+    seconds, seed, reps = job
+    rates = []
+    for k in range(reps):
+        u, dt = _cpu_worker((seconds, seed + 1000 * k))
+        rates.append(u / dt)
+    return max(rates), rates
+
+
 def cpu_baseline(seconds: float = 12.0):
     """The oracle's restatement of the reference sweep (sequential, AoS
     spins with bond-shell pointer lists, Xoroshiro128++ + libstdc++
     distributions; MC.cpp:87-163), one L = 12 replica per host process,
     Metropolis + 2 OR per sweep at T = 0.5, as the reference runs one
-    replica per process.  Worker processes are started fresh (spawn), never
-    forked from this GPU process, and capped at 16."""
+    replica per process.  Two measurements: one process alone, best of 3
+    stretches of seconds / 3 (the rate of the reference on one core), then
+    as many processes as cores (up to 16) for `seconds` at once (the rate of
+    the reference run as independent seed jobs, acc_runs.py:13-57), whose
+    aggregate is `value`.  Worker processes are started fresh (spawn),
+    never forked from this GPU process."""
     import multiprocessing as mp
 
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(1) as pool:
+        single, single_reps = pool.map(_cpu_worker_best, [(seconds / 3.0, 5, 3)])[0]
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
-    with mp.get_context("spawn").Pool(workers) as pool:
+    with ctx.Pool(workers) as pool:
         res = pool.map(_cpu_worker, [(seconds, 7 + i) for i in range(workers)])
     total = sum(u for u, _ in res) / max(dt for _, dt in res)
-    single = res[0][0] / res[0][1]
     return {
         "value": total,
         "unit": "site-updates/s",
         "cores": workers,
         "kind": "port",
         "single_core_value": single,
-        "sample": f"{workers} processes, each one L={L_LAT} replica ({16 * L_LAT ** 3} spins) swept with "
-                  f"Metropolis + {N_OR} OR at T=0.5 for {seconds:.0f} s; host {os.cpu_count()} logical CPUs, "
+        "single_core_reps": single_reps,
+        "sample": f"{workers} processes at once, each one L={L_LAT} replica ({16 * L_LAT ** 3} spins) swept with "
+                  f"Metropolis + {N_OR} OR at T=0.5 for {seconds:.0f} s (value); one process alone, best of 3 "
+                  f"stretches of {seconds / 3:.0f} s (single_core_value); host {os.cpu_count()} logical CPUs, "
                   f"{len(os.sched_getaffinity(0))} available",
     }
 

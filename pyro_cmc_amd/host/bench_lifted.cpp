@@ -1,96 +1,110 @@
-// bench_lifted: lifted and standard Metropolis sample the same distribution.
-// Mirrors the reference harness (test/bench_lifted.cpp:124-272): the J1-J2
-// Heisenberg model on an L x L square lattice, the same flags and defaults,
-// SplitMix64 fan-out of the base seed into four streams (initial spins and
-// sampler seed per sampler), n_burn sweeps then n_sample sweeps with one
-// energy measurement each, batch-means errors, and PASS when both |<E>| and
-// |<E^2>| of the two samplers differ by less than tol_frac * min(|J1|,|J2|)
-// (exit status 0, else 1).  Each sweep is one pass of the sampler plus one
-// overrelaxation pass, as sweep_local_Metropolis / sweep_lifted_Metropolis
-// default to.
-#include <algorithm>
+// bench_lifted: do lifted and standard Metropolis sample the same Boltzmann
+// distribution?  The question and the interface are the reference harness's
+// (test/bench_lifted.cpp): the J1-J2 Heisenberg model on an L x L square
+// lattice, the same flags and defaults, the same report and the same verdict
+// (PASS when both |<E>| and |<E^2>| per spin of the two samplers differ by
+// less than tol_frac * min(|J1|, |J2|); exit status 0, else 1).
+//
+// The sampling runs as a GPU ensemble.  Each sampler is one device context
+// holding --chains independent chains of the lattice (independent counter-
+// RNG streams and uniform random starting spins); after n_burn sweeps every
+// chain takes n_sample measurement sweeps of one sampler pass plus one
+// overrelaxation pass, through a constant-temperature device schedule that
+// records every chain's energy per spin after every sweep on the device (one
+// replayed graph, no host round trip per sweep).  <E> and <E^2> are ensemble
+// means over chains and sweeps, and their standard errors come from the
+// spread of the chains' own means, which needs no autocorrelation model.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <iostream>
-#include <random>
+#include <numeric>
+#include <vector>
 
 #include "cli.hpp"
 #include "pyro_host.hpp"
 
 namespace {
 
-CMC::UnitCellSpecifier square_cell() {
-    CMC::UnitCellSpecifier spec(imat33_t::from_cols({1, 0, 0}, {0, 1, 0}, {0, 0, 1}));
-    CMC::HeisenbergSpin s;
-    s.ipos = {0, 0, 0};
-    s.pyro_sl = 0;
-    spec.add(s);
-    return spec;
-}
-const std::vector<std::vector<ipos_t>> kJ1 = {{{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}}};
-const std::vector<std::vector<ipos_t>> kJ2 = {{{1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}}};
+struct Sampler {
+    const char* name;
+    int kind;
+    uint64_t tag;  // distinguishes the two samplers' streams under one --seed
+};
 
-void randomise(CMC::Lattice& lat, std::mt19937_64& rng) {
-    std::normal_distribution<double> g(0.0, 1.0);
-    for (auto& s : lat.get_objects<CMC::HeisenbergSpin>()) {
-        const double x = g(rng), y = g(rng), z = g(rng);
-        const double n = std::sqrt(x * x + y * y + z * z);
-        s.S = vector3::vec3d(x / n, y / n, z / n);
-        s.lifted_dir = 1;
-    }
-}
-
-// SplitMix64 (Steele, Lea, Flood 2014): next output of the state x.
-uint64_t splitmix64(uint64_t& x) {
-    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-// Standard error of the mean from n_batch contiguous batch means.
-double batch_stderr(const std::vector<double>& x, size_t n_batch) {
-    if (x.size() < n_batch || n_batch < 2) return 0.0;
-    const size_t bs = x.size() / n_batch;
-    std::vector<double> m(n_batch, 0.0);
-    for (size_t b = 0; b < n_batch; ++b) {
-        for (size_t i = 0; i < bs; ++i) m[b] += x[b * bs + i];
-        m[b] /= double(bs);
-    }
-    double mean = 0.0;
-    for (double v : m) mean += v;
-    mean /= double(n_batch);
-    double var = 0.0;
-    for (double v : m) var += (v - mean) * (v - mean);
-    return std::sqrt(var / double(n_batch - 1) / double(n_batch));
-}
-
-struct Estimates {
+struct Result {
     double E = 0, E2 = 0, E_err = 0, E2_err = 0, accept = 0;
 };
 
-Estimates run_sampler(CMC::MC_runner& mc, int kind, double T, size_t n_spins, size_t n_burn, size_t n_sample) {
-    if (n_burn) mc.sweeps(kind, T, 1, n_burn);
-    std::vector<double> e(n_sample), e2(n_sample);
-    size_t accepted = 0;
-    for (size_t i = 0; i < n_sample; ++i) {
-        accepted += mc.sweeps(kind, T, 1, 1);
-        e[i] = mc.total_energy_per_unit_cell();  // one site per cell: per spin
-        e2[i] = e[i] * e[i];
+void ok(int status, const char* what) {
+    if (status != PYRO_OK) throw std::runtime_error(std::string(what) + ": " + pyro_last_error());
+}
+
+// Mean over chains of per-chain means, and its standard error from the
+// chain-to-chain spread.
+void ensemble(const std::vector<double>& chain_means, double& mean, double& err) {
+    const double n = double(chain_means.size());
+    mean = std::accumulate(chain_means.begin(), chain_means.end(), 0.0) / n;
+    double ss = 0;
+    for (double m : chain_means) ss += (m - mean) * (m - mean);
+    err = std::sqrt(ss / (n - 1.0) / n);
+}
+
+Result run(const Sampler& smp, int L, int chains, double J1, double J2, double T, double T_ref, size_t n_burn,
+           size_t n_sample, uint64_t seed, int precision) {
+    // one site per cell of a 1 x 1 x 1 cubic cell; an L x L x 1 supercell is
+    // the square lattice, J1 on the four axial and J2 on the four diagonal
+    // neighbours in the plane
+    const int32_t pos[3] = {0, 0, 0}, kind[1] = {0}, cell[3] = {1, 1, 1}, Ls[3] = {L, L, 1};
+    pyro_ctx* ctx = nullptr;
+    const uint64_t key = seed_key(seed) ^ smp.tag;
+    ok(pyro_create(1, pos, kind, cell, Ls, chains, precision, key, &ctx), "create");
+    struct Closer {
+        pyro_ctx* c;
+        ~Closer() { pyro_destroy(c); }
+    } closer{ctx};
+    const int32_t counts[1] = {4};
+    const int32_t axial[12] = {1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1, 0};
+    const int32_t diagonal[12] = {1, 1, 0, 1, -1, 0, -1, 1, 0, -1, -1, 0};
+    const double I1[9] = {J1, 0, 0, 0, J1, 0, 0, 0, J1}, I2[9] = {J2, 0, 0, 0, J2, 0, 0, 0, J2};
+    ok(pyro_define_coupling(ctx, "J1", 1, counts, axial, I1), "define J1");
+    ok(pyro_define_coupling(ctx, "J2", 1, counts, diagonal, I2), "define J2");
+    ok(pyro_setup_lattice(ctx), "setup");
+    ok(pyro_set_T_ref(ctx, T_ref), "T_ref");
+    ok(pyro_randomize_spins(ctx, key), "randomize");
+    const std::vector<double> Ts(size_t(chains), T);
+    ok(pyro_set_temperatures(ctx, Ts.data()), "temperatures");
+    std::printf("Running %s Metropolis (%d chains)...\n", smp.name, chains);
+    if (n_burn) ok(pyro_sweep(ctx, smp.kind, 1, int(n_burn), nullptr), "burn-in");
+
+    // measurement: n_sample steps of a constant schedule, energies on the device
+    const std::vector<double> sched(n_sample, T);
+    ok(pyro_anneal_schedule(ctx, sched.data(), int(n_sample)), "schedule");
+    ok(pyro_anneal_steps(ctx, smp.kind, 1, 1, 0, int(n_sample)), "measurement sweeps");
+    std::vector<double> hist(size_t(chains) * n_sample);  // [chain][sweep], per cell = per spin
+    ok(pyro_anneal_history(ctx, hist.data()), "history");
+    // acceptance of the sampler pass over a further stretch of sweeps
+    std::vector<uint64_t> acc(static_cast<size_t>(chains));
+    const int n_acc = int(std::min<size_t>(n_sample, 1000));
+    ok(pyro_sweep(ctx, smp.kind, 1, n_acc, acc.data()), "acceptance sweeps");
+
+    std::vector<double> mE(static_cast<size_t>(chains)), mE2(static_cast<size_t>(chains));
+    for (int c = 0; c < chains; ++c) {
+        double s = 0, s2 = 0;
+        for (size_t k = 0; k < n_sample; ++k) {
+            const double e = hist[size_t(c) * n_sample + k];
+            s += e;
+            s2 += e * e;
+        }
+        mE[size_t(c)] = s / double(n_sample);
+        mE2[size_t(c)] = s2 / double(n_sample);
     }
-    Estimates est;
-    for (size_t i = 0; i < n_sample; ++i) {
-        est.E += e[i];
-        est.E2 += e2[i];
-    }
-    est.E /= double(n_sample);
-    est.E2 /= double(n_sample);
-    const size_t n_batch = std::min<size_t>(50, std::max<size_t>(2, n_sample / 20));
-    est.E_err = batch_stderr(e, n_batch);
-    est.E2_err = batch_stderr(e2, n_batch);
-    est.accept = double(accepted) / (double(n_sample) * double(n_spins));
-    return est;
+    Result res;
+    ensemble(mE, res.E, res.E_err);
+    ensemble(mE2, res.E2, res.E2_err);
+    const double n_acc_total = double(std::accumulate(acc.begin(), acc.end(), uint64_t(0)));
+    res.accept = n_acc_total / (double(chains) * double(n_acc) * double(L) * double(L));
+    return res;
 }
 
 }  // namespace
@@ -106,6 +120,7 @@ int main(int argc, char* argv[]) {
     prog.add({"--n_sample"}, "Measurement sweeps (one energy measurement per sweep)").def = {"50000"};
     prog.add({"--tol_frac"}, "Tolerance as a fraction of min(|J1|,|J2|)").def = {"0.1"};
     prog.add({"--seed"}, "Base RNG seed").def = {"2718281828"};
+    prog.add({"--chains"}, "Independent chains per sampler, run side by side on the GPU").def = {"32"};
     prog.add({"--fp32"}, "FP32 spins and fields (FP64 accumulators)").flag = true;
     try {
         prog.parse(argc, argv);
@@ -116,40 +131,25 @@ int main(int argc, char* argv[]) {
     try {
         const double J1 = prog.dbl("--J1"), J2 = prog.dbl("--J2"), T = prog.dbl("--T");
         const double T_ref = prog.dbl("--T_ref"), tol_frac = prog.dbl("--tol_frac");
-        const int L = int(prog.integer("--L"));
+        const int L = int(prog.integer("--L")), chains = int(prog.integer("--chains"));
         const size_t n_burn = size_t(prog.integer("--n_burn")), n_sample = size_t(prog.integer("--n_sample"));
-        const size_t seed = size_t(prog.integer("--seed"));
-        if (L < 1 || n_sample < 1) throw std::invalid_argument("L and n_sample must be positive");
-        const double jmin = std::min(std::abs(J1), std::abs(J2));
-        const double tol = tol_frac * jmin;
+        const uint64_t seed = uint64_t(prog.integer("--seed"));
+        if (L < 1 || n_sample < 1 || chains < 2)
+            throw std::invalid_argument("L and n_sample must be positive and --chains at least 2");
+        const double jmin = std::min(std::abs(J1), std::abs(J2)), tol = tol_frac * jmin;
+        const int precision = prog.flag("--fp32") ? PYRO_FP32 : PYRO_FP64;
 
         std::printf("=== lifted vs. standard Metropolis consistency benchmark ===\n");
         std::printf("Model : %dx%d square-lattice J1-J2 Heisenberg\n", L, L);
         std::printf("        J1=%g  J2=%g  T=%g  T_ref=%g\n", J1, J2, T, T_ref);
-        std::printf("Stats : n_burn=%zu  n_sample=%zu  seed=%zu\n", n_burn, n_sample, seed);
+        std::printf("Stats : n_burn=%zu  n_sample=%zu  seed=%llu  chains=%d\n", n_burn, n_sample,
+                    (unsigned long long)seed, chains);
         std::printf("Tol   : %g  (= %g * min(|J1|,|J2|)=%g)\n\n", tol, tol_frac, jmin);
 
-        uint64_t sm = seed;
-        const uint64_t seed_std_init = splitmix64(sm), seed_std_mc = splitmix64(sm);
-        const uint64_t seed_lft_init = splitmix64(sm), seed_lft_mc = splitmix64(sm);
-        const imat33_t supercell = imat33_t::from_cols({L, 0, 0}, {0, L, 0}, {0, 0, 1});
-        const size_t n_spins = size_t(L) * size_t(L);
-        const vector3::mat33d I = vector3::mat33d::from_cols({1, 0, 0}, {0, 1, 0}, {0, 0, 1});
-        auto run = [&](uint64_t init_seed, uint64_t mc_seed, int kind, const char* what) {
-            CMC::Lattice lat = CMC::build_supercell(square_cell(), supercell);
-            std::mt19937_64 init(init_seed);
-            randomise(lat, init);
-            CMC::MC_runner mc(lat, mc_seed);
-            mc.settings.precision = prog.flag("--fp32") ? PYRO_FP32 : PYRO_FP64;
-            mc.settings.T_ref = T_ref;
-            mc.define_coupling("J1", kJ1, J1 * I);
-            mc.define_coupling("J2", kJ2, J2 * I);
-            mc.setup_lattice();
-            std::printf("Running %s Metropolis...\n", what);
-            return run_sampler(mc, kind, T, n_spins, n_burn, n_sample);
-        };
-        const Estimates s = run(seed_std_init, seed_std_mc, PYRO_METROPOLIS, "standard");
-        const Estimates l = run(seed_lft_init, seed_lft_mc, PYRO_LIFTED, "lifted");
+        const Sampler standard{"standard", PYRO_METROPOLIS, 0x5354444D45545230ull};
+        const Sampler lifted{"lifted", PYRO_LIFTED, 0x4C49465445444D43ull};
+        const Result s = run(standard, L, chains, J1, J2, T, T_ref, n_burn, n_sample, seed, precision);
+        const Result l = run(lifted, L, chains, J1, J2, T, T_ref, n_burn, n_sample, seed, precision);
 
         const double dE = std::abs(s.E - l.E), dE2 = std::abs(s.E2 - l.E2);
         std::printf("\n%-10s %14s %14s %14s\n", "quantity", "standard", "lifted", "|difference|");
