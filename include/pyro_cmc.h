@@ -158,6 +158,29 @@ int pyro_local_fields(pyro_ctx* ctx, int replica, double* h);
  * e[R], summed in FP64. */
 int pyro_energy_per_cell(pyro_ctx* ctx, double* e);
 
+/* Allocation-free variant of pyro_energy_per_cell.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Needs no context and calls no hipMalloc / hipFree.
+ *   spins: device memory, [n_replicas][16 L0 L1 L2][3] FP32 or FP64 as
+ *          `precision` says, in the context's site order (the pyrochlore
+ *          cubic cell); read only.
+ *   J1 (non-zero), J2, J3, Delta: the couplings of pyro_define_J1J2J3,
+ *   B[3] (host): the global field of every replica.
+ *   e: device memory, double[n_replicas], receives the energy per cell.
+ *   stream: a hipStream_t (NULL: the null stream).  The call only enqueues:
+ *          e is ready once the stream reaches that point.
+ * Results equal pyro_energy_per_cell's on a context holding the same
+ * spins, couplings and field, bit for bit (same kernels).  The partial sums
+ * live in one 4 MiB device array per GPU, held until the process ends:
+ * n_replicas x (energy blocks per replica, L0 L1 L2 / 4 or fewer) + 3
+ * n_replicas doubles must fit in it (else PYRO_E_INVALID).  Calls that
+ * overlap on one GPU (two streams, or two threads) share that array and
+ * race; serialise them. */
+int pyro_energy_per_cell_noalloc(const void* spins, int precision, const int32_t L[3], int n_replicas,
+                                 double J1, double J2, double J3, double Delta, const double B[3],
+                                 double* e, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Replica exchange (temper.cpp:418-474 without MPI).                   */
 /* T_ladder[R]: slot temperatures, slot 0 hottest (descending, like the */

@@ -52,6 +52,8 @@ _SIGS = {
     "pyro_overrelax": [_vp, C.c_int],
     "pyro_local_fields": [_vp, C.c_int, _f64p],
     "pyro_energy_per_cell": [_vp, _f64p],
+    "pyro_energy_per_cell_noalloc": [_vp, C.c_int, _i32p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     _f64p, _vp, _vp],
     "pyro_energy_sample": [_vp],
     "pyro_pt_init": [_vp, _f64p, C.c_uint64],
     "pyro_pt_rounds": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
@@ -166,6 +168,20 @@ def lt_spectrum(L, J1, J2=0.0, J3=0.0, Delta=1.0):
     ev = np.zeros((48, 2), np.float64)
     _check(lib().pyro_lt_spectrum(Ls, float(J1), float(J2), float(J3), float(Delta), lam, k_star, ev))
     return lam, k_star, ev[:, 0] + 1j * ev[:, 1]
+
+
+def energy_per_cell_noalloc(spins_ptr: int, precision: int, L, n_replicas: int, J1: float, J2: float = 0.0,
+                            J3: float = 0.0, Delta: float = 1.0, B=(0.0, 0.0, 0.0), e_ptr: int = 0,
+                            stream: int = 0) -> None:
+    """pyro_energy_per_cell_noalloc: energy per cell of n_replicas replicas
+    whose spins ([R][16 L0 L1 L2][3], FP32 or FP64) are already in device
+    memory at spins_ptr, into the double[R] device buffer at e_ptr, enqueued
+    on `stream` (a hipStream_t handle, 0 = the null stream); allocates
+    nothing."""
+    Ls = np.array([L] * 3 if np.isscalar(L) else list(L), np.int32)
+    _check(lib().pyro_energy_per_cell_noalloc(_vp(spins_ptr), int(precision), Ls, int(n_replicas), float(J1),
+                                              float(J2), float(J3), float(Delta), np.asarray(B, np.float64),
+                                              _vp(e_ptr), _vp(stream)))
 
 
 def pyrochlore_cell():

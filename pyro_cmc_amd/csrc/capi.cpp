@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -134,6 +135,33 @@ struct DevMem {
     }
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
+
+// Scratch of the allocation-free entry point (pyro_energy_per_cell_noalloc):
+// the energy partials and the field, in a file-scope device array that every
+// GPU holds until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: kernels get the address hipGetSymbolAddress
+// returns for the current device (noalloc_scratch), never the name.
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 19;  // 4 MiB
+__device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+size_t g_scratch_lds[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+double* noalloc_scratch(int device) {
+    require(device >= 0 && device < kMaxDevices, PYRO_E_UNSUPPORTED, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, PYRO_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device] = static_cast<double*>(p);
+        g_scratch_lds[device] = device_max_lds(device);
+    }
+    return g_scratch_ptr[device];
+}
 
 constexpr int kFamilies = 7;
 enum Family { kFamSweep = 0, kFamEnergy, kFamExchange, kFamDft, kFamContract, kFamMunu, kFamMunuFlush };
@@ -1210,6 +1238,72 @@ int pyro_energy_per_cell(pyro_ctx* ctx, double* e) {
         ctx->energy(false);
         HIPCHK(hipMemcpyAsync(e, ctx->e.p, sizeof(double) * ctx->R, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pyro_energy_per_cell_noalloc(const void* spins, int precision, const int32_t L[3], int n_replicas, double J1,
+                                 double J2, double J3, double Delta, const double B[3], double* e, void* stream) {
+    return guarded([&] {
+        require(spins && L && B && e, PYRO_E_INVALID, "NULL argument");
+        require(precision == PYRO_FP32 || precision == PYRO_FP64, PYRO_E_INVALID, "precision must be 32 or 64");
+        require(n_replicas >= 1, PYRO_E_INVALID, "need at least one replica");
+        for (double v : {J1, J2, J3, Delta, B[0], B[1], B[2]})
+            require(std::isfinite(v), PYRO_E_INVALID, "couplings and field must be finite");
+        require(J1 != 0.0, PYRO_E_UNSUPPORTED, "the stencil energy kernels need J1 != 0");
+        int Ls[3];
+        int64_t nc = 1;
+        for (int a = 0; a < 3; ++a) {
+            require(L[a] >= 1 && L[a] <= 1024, PYRO_E_INVALID, "lattice extent out of range");
+            Ls[a] = L[a];
+            nc *= L[a];
+        }
+        const int64_t N = 16 * nc;
+        require(N <= (int64_t(1) << 30), PYRO_E_INVALID, "lattice too large");
+        int device = 0;
+        HIPCHK(hipGetDevice(&device));
+        double* scr = noalloc_scratch(device);
+        const size_t rs = precision == PYRO_FP64 ? 8 : 4;
+        // the kernel pyro_energy_per_cell would use for this lattice
+        const int ek = energy_tile_kind_for(Ls, rs, g_scratch_lds[device], auto_tile_kind(Ls, rs, g_scratch_lds[device]));
+        int64_t nblk = (N + 63) / 64;
+        if (ek) {
+            int d[3];
+            pyro::pyro_tile_dims(ek, d);
+            nblk = int64_t(Ls[0] / d[0]) * (Ls[1] / d[1]) * (Ls[2] / d[2]);
+        }
+        // partials [R][nblk], then the field [R][3] on a 256-byte boundary
+        const int64_t R = n_replicas;
+        const int64_t b_off = (R * nblk + 31) / 32 * 32;
+        require(b_off + 3 * R <= int64_t(kNoAllocScratchDoubles), PYRO_E_INVALID,
+                "replicas x energy blocks exceed the allocation-free scratch (4 MiB)");
+        auto run = [&](auto real) {
+            using Real = decltype(real);
+            Real* Bd = reinterpret_cast<Real*>(scr + b_off);
+            pyro::DeviceState<Real> st;
+            st.S = static_cast<Real*>(const_cast<void*>(spins));  // read only
+            st.B = Bd;
+            st.N = int(N);
+            st.n_cells = int(nc);
+            st.R = n_replicas;
+            st.L0 = Ls[0];
+            st.L1 = Ls[1];
+            st.L2 = Ls[2];
+            pyro::PyroCouplings<Real> J;
+            J.J1 = Real(J1);
+            J.J2 = Real(J2);
+            J.J3 = Real(J3);
+            J.Jx = Real(J1 * (Delta - 1.0) / 3.0);
+            J.shells = J3 != 0.0 ? 3 : (J2 != 0.0 ? 2 : 1);
+            J.xxz = Delta != 1.0;
+            const hipStream_t s = static_cast<hipStream_t>(stream);
+            HIPCHK(pyro::launch_fill_field<Real>(Bd, n_replicas, B[0], B[1], B[2], s));
+            int nb = int(nblk);
+            if (ek) HIPCHK(pyro::launch_pyro_energy_tiled<Real>(st, J, ek, scr, &nb, s));
+            else HIPCHK(pyro::launch_pyro_energy<Real>(st, J, scr, s));
+            HIPCHK(pyro::launch_energy_finish(scr, nb, int(nc), e, nullptr, nullptr, nullptr, nullptr, n_replicas, s));
+        };
+        if (precision == PYRO_FP64) run(double{});
+        else run(float{});
     });
 }
 

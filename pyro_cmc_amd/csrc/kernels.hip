@@ -1307,6 +1307,25 @@ hipError_t launch_map_slots(const int32_t* slot_of, const int32_t* map, int32_t*
     return hipGetLastError();
 }
 
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void fill_field_kernel(Real* __restrict__ B, int R, Real bx, Real by, Real bz) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r < R) {
+        B[3 * r] = bx;
+        B[3 * r + 1] = by;
+        B[3 * r + 2] = bz;
+    }
+}
+
+template <typename Real>
+hipError_t launch_fill_field(Real* B, int R, double bx, double by, double bz, hipStream_t s) {
+    hipLaunchKernelGGL(fill_field_kernel<Real>, dim3(unsigned((R + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, B, R,
+                       Real(bx), Real(by), Real(bz));
+    return hipGetLastError();
+}
+template hipError_t launch_fill_field<float>(float*, int, double, double, double, hipStream_t);
+template hipError_t launch_fill_field<double>(double*, int, double, double, double, hipStream_t);
+
 hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
     if (n_words32 == 0) return hipSuccess;
     size_t blocks = (n_words32 + kBlock - 1) / kBlock;

@@ -199,6 +199,9 @@ hipError_t launch_map_slots(const int32_t* slot_of, const int32_t* map, int32_t*
                             hipStream_t s);
 // Sets n words to zero (graph-safe replacement for a memset).
 hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s);
+// B[r][0..2] = (bx, by, bz) for r < R (a field from kernel arguments, graph-safe).
+template <typename Real>
+hipError_t launch_fill_field(Real* B, int R, double bx, double by, double bz, hipStream_t s);
 // acc[r] += sum of the n per-wave counters acc_wave[r][0..n), r < R.
 hipError_t launch_reduce_accepts(const unsigned int* acc_wave, int n, unsigned long long* acc, int R,
                                  hipStream_t s);
