@@ -100,6 +100,7 @@ def large_lattice_sweep(pyro, sweeps=6):
     ctx = pyro.Context.pyrochlore(48, n_replicas=R, precision=pyro.FP32, seed=48)
     ctx.define_J1J2J3(1.0, 0.1, 0.1)
     ctx.setup()
+    ctx.set_fused_passes(LARGE_FUSED)
     ctx.set_field(np.array([0.0, 0.0, 0.2]))
     ctx.randomize(49)
     ctx.set_temperatures(np.geomspace(1.0, 0.05, R))
@@ -173,48 +174,59 @@ def munu_correlators(pyro, samples=384, buffer=192):
     }
 
 
-def large_tempering_run(pyro, rounds=384, buffer=192):
+# The large runs' sweeps take a site's lifted and OR updates back to back
+# within its colour stage (pyro_set_fused_passes): one field evaluation and
+# one Philox block per site and sweep instead of two.
+LARGE_FUSED = True
+LARGE_ORDER = ("fused: per tile and colour, each site's lifted then OR update from one field evaluation"
+               if LARGE_FUSED else "pass by pass within each tile")
+
+
+def large_tempering_run(pyro, rounds=192, buffer=192):
     """The large tempering workload end to end: L = 48 (1,769,472 spins) x 16
     replicas, FP32, J1 = 1, J2 = J3 = 0.1, B = (0, 0, 0.2), T log-spaced in
-    [0.05, 1]; each round is one sweep (1 lifted Metropolis + 1 OR pass), an
-    SSF sample of all 9 correlators over the 16 sublattices (contracted sums
-    and the sublattice-resolved matrices, buffered `buffer` deep and flushed
-    on MFMA into FP64) and the energies, captured as one graph per round,
-    with a replica exchange after every second round.  The timed region
-    holds `rounds` rounds and the flushes they trigger."""
+    [0.05, 1]; each round is two sweeps (1 lifted Metropolis + 1 OR pass
+    each) with an SSF sample of all 9 correlators over the 16 sublattices
+    after each sweep (contracted sums and the sublattice-resolved matrices,
+    buffered `buffer` deep and flushed on MFMA into FP64), then the
+    energies (one sample per round, as temper.cpp:432-435 takes one per
+    exchange) and a replica exchange; captured as one graph per round.  The
+    timed region holds `rounds` rounds and the flushes they trigger."""
     R, L = 16, 48
     ctx = pyro.Context.pyrochlore(L, n_replicas=R, precision=pyro.FP32, seed=48)
     ctx.define_J1J2J3(1.0, 0.1, 0.1)
     ctx.setup()
+    ctx.set_fused_passes(LARGE_FUSED)
     ctx.set_field(np.array([0.0, 0.0, 0.2]))
     ctx.randomize(49)
     ctx.pt_init(np.geomspace(1.0, 0.05, R), 50)
-    ctx.pt_exchange_interval(2)  # exchange every 2 sweeps, SSF every sweep
     ctx.ssf_configure(CORR, n_slots=R, store_error_term=True, munu=True, munu_buffer=buffer)
-    ctx.pt_rounds(pyro.LIFTED, 1, 1, True, 4)
+    ctx.pt_rounds(pyro.LIFTED, 1, 2, 2, 2)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.pt_rounds(pyro.LIFTED, 1, 1, True, rounds)
+    ctx.pt_rounds(pyro.LIFTED, 1, 2, 2, rounds)  # SSF after each of the 2 sweeps
     ctx.synchronize()
     wall = time.perf_counter() - t0
     n = ctx.N
     st = ctx.pt_stats()
-    # per-kernel HIP-event times of 16 rounds run eagerly (no flush among them)
+    # per-kernel HIP-event times of 8 rounds run eagerly (no flush among them)
     ctx.profile(True)
-    ctx.pt_rounds(pyro.LIFTED, 1, 1, True, 16)
+    ctx.pt_rounds(pyro.LIFTED, 1, 2, 2, 8)
     prof = ctx.profile_read()
     ctx.profile(False)
     ctx.close()
+    sweeps = 2 * rounds
     return {
         "workload": f"parallel tempering, L={L} (1769472 spins) x {R} replicas, FP32, J1=1 J2=J3=0.1, "
                     "B=(0,0,0.2), T log-spaced 1.0..0.05, sweep = 1 lifted Metropolis + 1 OR, SSF of 9 "
                     f"correlators over 16 sublattices every sweep ({buffer}-sample MFMA flushes into FP64), "
-                    "exchange every 2 sweeps",
-        "site_updates_per_s": n * R * 2 * rounds / wall,
-        "ms_per_sweep": 1e3 * wall / rounds,
-        "rounds": rounds,
+                    "exchange and energy sample every 2 sweeps",
+        "pass_order": LARGE_ORDER,
+        "site_updates_per_s": n * R * 2 * sweeps / wall,
+        "ms_per_sweep": 1e3 * wall / sweeps,
+        "sweeps": sweeps,
         "swap_acceptance_min": float(np.min(st["accepts"] / np.maximum(st["attempts"], 1))),
-        "kernel_ms_per_round": {k: v[0] / 16 for k, v in prof.items() if v[1]},
+        "kernel_ms_per_sweep": {k: v[0] / 16 for k, v in prof.items() if v[1]},
     }
 
 
@@ -319,6 +331,7 @@ def large_lattice_sweep_line(pyro):
     return {
         "workload": "L=48 (1769472 spins) x 16 replicas, FP32, J1=1 J2=J3=0.1, B=(0,0,0.2), "
                     "sweep = 1 lifted Metropolis + 1 OR",
+        "pass_order": LARGE_ORDER,
         "site_updates_per_s": rate,
         "strategy": strat_big,
         "roofline": {

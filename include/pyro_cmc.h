@@ -189,10 +189,12 @@ int pyro_energy_per_cell_noalloc(const void* spins, int precision, const int32_t
 int pyro_pt_init(pyro_ctx* ctx, const double* T_ladder, uint64_t swap_seed);
 /* One tempering round: n_sweep sweeps (kind + n_overrelax OR passes) of
  * every replica at its slot temperature; if sample_ssf, one SSF sample of
- * every replica into its slot's accumulator; the energy per cell of every
- * replica is added to its slot's E, E^2, n; then the checkerboard
- * exchange of slot pairs (s, s+1) with s = round parity (mod 2).
- * n_rounds rounds are run back to back (captured once as a HIP graph). */
+ * every replica into its slot's accumulator (sample_ssf = 1: after the
+ * sweeps, as temper.cpp:424-429; sample_ssf = 2: after each of the
+ * sweeps); the energy per cell of every replica is added to its slot's E,
+ * E^2, n; then the checkerboard exchange of slot pairs (s, s+1) with
+ * s = round parity (mod 2).  n_rounds rounds are run back to back
+ * (captured once as a HIP graph). */
 int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep,
                    int sample_ssf, int n_rounds);
 /* Per-pair swap statistics attempts[R-1], accepts[R-1] (pair s <-> s+1),
@@ -310,6 +312,21 @@ int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
  * is a fixed rule, never a timing choice.  pyro_get_tile_shape returns the
  * shape in use (0, 0, 0 when none applies). */
 int pyro_set_tile_shape(pyro_ctx* ctx, const int32_t T[3]);
+/* Pass order of the tiled sweep (strategy 5).  0 (default): pass by pass
+ * within each tile, the order the reference's sweeps follow
+ * (MC.cpp:283-303: every site's first-kind update, then every site's OR
+ * update).  1: within each colour stage every site takes all passes of the
+ * sweep back to back, the first kind then the OR passes, from one
+ * evaluation of its local field (no bonded site changes in between), and
+ * its uniforms for the sweep are drawn in order from its Philox blocks:
+ * uniform j is component j % 5 of the block with pass index p0 + j / 5 (p0
+ * the sweep's first pass), the first kind taking 5 (Metropolis), 2
+ * (lifted) or 1 (lifted rotation) and each OR pass the next one, so a
+ * lifted + OR sweep draws one block per site.  Both orders are sequential
+ * single-site schedules of the same updates; the order changes the Markov
+ * chain, not its stationary distribution, and pyro_sweep_order reports it.
+ * No counterpart in the reference. */
+int pyro_set_fused_passes(pyro_ctx* ctx, int on);
 int pyro_get_tile_shape(pyro_ctx* ctx, int32_t T[3]);
 /* The site visiting order of one sweep under the strategy the next sweep
  * will use: site[k] and pass_offset[k] (0 = the sweep's first pass kind,

@@ -598,19 +598,31 @@ uint64_t orc_pass_ctr(void* h, int kind, double T, const int32_t* colour_of_site
 // kind first_kind if pass_offset[k] == 0 and overrelaxation otherwise, pass
 // index pass0 + pass_offset[k].  Returns the accepted count of the first kind.
 // info (may be NULL) receives the conditioning record of update k at
-// info[4 k .. 4 k + 3].
+// info[4 k .. 4 k + 3].  shared != 0: the sweep's uniforms of a site are
+// drawn in order from its blocks (the product's fused passes): uniform j is
+// component j % 5 of block pass0 + j / 5, the first kind taking 5 / 2 / 1
+// (Metropolis / lifted / lifted rotation) and OR pass o >= 1 uniform
+// (first kind's count) + o - 1; else every pass draws its own block.
 uint64_t orc_sweep_order_m(void* h, int first_kind, double T, const int32_t* site,
                            const int32_t* pass_offset, int64_t n, uint64_t seed, uint32_t replica,
-                           uint64_t pass0, double* info) {
+                           uint64_t pass0, double* info, int shared) {
     auto* o = static_cast<Oracle*>(h);
     uint64_t acc = 0;
     const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+    const int n_first = first_kind == 0 ? 5 : (first_kind == 1 ? 2 : 1);
     for (int64_t k = 0; k < n; ++k) {
         const int64_t i = site[k];
-        const uint64_t pass = pass0 + uint64_t(pass_offset[k]);
+        uint64_t pass = pass0 + uint64_t(pass_offset[k]);
+        int comp = 0;
+        if (shared && pass_offset[k] > 0) {
+            const int j = n_first + pass_offset[k] - 1;
+            pass = pass0 + uint64_t(j / 5);
+            comp = j % 5;
+        }
         const W4 w = philox10(W4{{uint32_t(i), replica, uint32_t(pass), uint32_t(pass >> 32)}}, k0, k1);
         double u[5];
         uniforms5(w, u);
+        u[0] = u[comp];
         Draws d;
         d.u = u;
         const int kind = pass_offset[k] == 0 ? first_kind : 3;
@@ -624,7 +636,7 @@ uint64_t orc_sweep_order_m(void* h, int first_kind, double T, const int32_t* sit
 uint64_t orc_sweep_order(void* h, int first_kind, double T, const int32_t* site,
                          const int32_t* pass_offset, int64_t n, uint64_t seed, uint32_t replica,
                          uint64_t pass0) {
-    return orc_sweep_order_m(h, first_kind, T, site, pass_offset, n, seed, replica, pass0, nullptr);
+    return orc_sweep_order_m(h, first_kind, T, site, pass_offset, n, seed, replica, pass0, nullptr, 0);
 }
 
 // Uniform-on-sphere initial spins of replica r (z = 2u0 - 1, phi = 2 pi u1).

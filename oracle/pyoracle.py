@@ -46,7 +46,7 @@ _SIGS = {
     "orc_pass_ctr_m": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64,
                                     _vp]),
     "orc_sweep_order_m": (C.c_uint64, [_vp, C.c_int, C.c_double, _i32p, _i32p, C.c_int64, C.c_uint64, C.c_uint32,
-                                       C.c_uint64, _vp]),
+                                       C.c_uint64, _vp, C.c_int]),
     "orc_philox": (None, [_u32p, _u32p, _u32p]),
     "orc_xoroshiro": (None, [C.c_uint64, C.c_int, _u64p]),
     "orc_exchange": (None, [C.c_int, _f64p, _f64p, C.c_double, _i32p, C.c_uint64, C.c_uint64, _u64p, _u64p]),
@@ -316,15 +316,16 @@ class Oracle:
         return int(lib().orc_pass_ctr_m(self.h, kind, float(T), np.ascontiguousarray(colour_of_site, np.int32),
                                         int(n_colours), int(seed), int(replica), int(pass_index), p))
 
-    def sweep_order(self, first_kind, T, site, pass_offset, seed, replica, pass0, info=None):
+    def sweep_order(self, first_kind, T, site, pass_offset, seed, replica, pass0, info=None, shared=False):
         """Counter-mode updates in an explicit order; info (float64[n][4],
         optional) receives the conditioning record of every update (see
-        pass_ctr)."""
+        pass_ctr).  shared: a site's OR passes take the next uniforms of the
+        sweep's blocks (the product's fused passes) instead of a block each."""
         site = np.ascontiguousarray(site, np.int32)
         koff = np.ascontiguousarray(pass_offset, np.int32)
         p = None if info is None else info.ctypes.data_as(C.c_void_p)
         return int(lib().orc_sweep_order_m(self.h, int(first_kind), float(T), site, koff, len(site),
-                                           int(seed), int(replica), int(pass0), p))
+                                           int(seed), int(replica), int(pass0), p, int(bool(shared))))
 
     def randomize_ctr(self, seed, replica=0):
         lib().orc_randomize_ctr(self.h, int(seed), int(replica))

@@ -77,6 +77,7 @@ _SIGS = {
     "pyro_set_strategy": [_vp, C.c_int],
     "pyro_get_strategy": [_vp, C.POINTER(C.c_int)],
     "pyro_set_tile_shape": [_vp, _vp],
+    "pyro_set_fused_passes": [_vp, C.c_int],
     "pyro_get_tile_shape": [_vp, _i32p],
     "pyro_profile_enable": [_vp, C.c_int],
     "pyro_profile_read": [_vp, _f64p, _u64p],
@@ -334,7 +335,9 @@ class Context:
         _check(lib().pyro_pt_init(self._h, np.ascontiguousarray(T_ladder, np.float64), int(swap_seed)))
 
     def pt_rounds(self, kind=METROPOLIS, n_overrelax=1, n_sweep=1, sample_ssf=False, n_rounds=1):
-        _check(lib().pyro_pt_rounds(self._h, kind, n_overrelax, n_sweep, int(bool(sample_ssf)), n_rounds))
+        """sample_ssf: False / True (one SSF sample per round, after its
+        sweeps) or 2 (one after each sweep of the round)."""
+        _check(lib().pyro_pt_rounds(self._h, kind, n_overrelax, n_sweep, int(sample_ssf), n_rounds))
 
     def pt_stats(self):
         R = self.R
@@ -435,6 +438,11 @@ class Context:
         """Tile shape (cells per axis) of the tiled sweep; None = automatic."""
         t = None if T is None else np.ascontiguousarray(T, np.int32)
         _check(lib().pyro_set_tile_shape(self._h, _ptr(t)))
+
+    def set_fused_passes(self, on=True):
+        """Tiled sweeps: every pass of a site back to back within its colour
+        stage, from one field evaluation (pyro_set_fused_passes)."""
+        _check(lib().pyro_set_fused_passes(self._h, int(bool(on))))
 
     def tile_shape(self):
         t = np.zeros(3, np.int32)

@@ -219,6 +219,7 @@ struct pyro_ctx {
     int strategy = 0;
     int tile_kind = 0;          // tiled strategy tile shape (0: not applicable)
     int exchange_every = 1;     // tempering rounds per replica exchange
+    bool fused_passes = false;  // tiled sweeps: a site's passes back to back (pyro_set_fused_passes)
     int tile_forced = 0;        // tile shape requested by pyro_set_tile_shape (0: automatic)
     size_t max_lds = 65536;     // per-workgroup LDS limit of the device
     int tuned_key = -1;        // (kind, n_or) the auto choice was made for
@@ -464,7 +465,8 @@ struct pyro_ctx {
             for (int sw = 0; sw < n_sweeps; ++sw, p += 1 + n_or)
                 for (int phase = 0; phase < 8; ++phase)
                     timed(kFamSweep, [&] {
-                        return pyro::launch_pyro_tiled<Real>(st, J, tile_kind, phase, kind, n_or, p, stream);
+                        return pyro::launch_pyro_tiled<Real>(st, J, tile_kind, phase, kind, n_or, p, stream,
+                                                             fused_passes);
                     });
             return;
         }
@@ -1348,9 +1350,14 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
         require(kind == PYRO_METROPOLIS || kind == PYRO_LIFTED || kind == PYRO_LIFTED_ROT, PYRO_E_INVALID,
                 "sweep kind must be METROPOLIS, LIFTED or LIFTED_ROT");
         require(n_overrelax >= 0 && n_sweep >= 0 && n_rounds >= 0, PYRO_E_INVALID, "negative count");
+        require(sample_ssf >= 0 && sample_ssf <= 2, PYRO_E_INVALID, "sample_ssf must be 0, 1 or 2");
         require(!sample_ssf || ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
         require(!sample_ssf || ctx->has_ssf_map || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
                 "SSF sampling in tempering rounds needs a slot map or one accumulator slot per replica");
+        // SSF samples per round: one after the sweeps, or one after each sweep
+        const int spr = sample_ssf == 2 ? n_sweep : (sample_ssf ? 1 : 0);
+        require(!ctx->munu || spr <= ctx->munu_B, PYRO_E_INVALID,
+                "a round's SSF samples exceed the sublattice-resolved buffer");
         if (n_rounds == 0) return;
         const int strat = n_sweep > 0 ? ctx->choose_strategy(kind, n_overrelax) : 1;
         const unsigned long long ppr = (unsigned long long)n_sweep * (1 + n_overrelax);
@@ -1362,21 +1369,31 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
 
         // the round's last colour pass leaves the energy partials when it can
         const bool fused_energy = n_sweep > 0 && ctx->sweeps_leave_energy(strat);
+        auto ssf_round_sample = [&]() {
+            if (ctx->has_ssf_map) {
+                HIPCHK(pyro::launch_map_slots(ctx->slot_of.as<int32_t>(), ctx->ssf_map.as<int32_t>(),
+                                              ctx->ssf_slot.as<int32_t>(), ctx->R, ctx->stream));
+                ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>(), ctx->stream);
+            } else {
+                ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
+            }
+        };
         auto round_body = [&]() {
-            if (n_sweep > 0)
-                ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
-                                fused_energy ? ctx->epart.as<double>() : nullptr);
-            // One stream: running the SSF branch on a second stream beside the
-            // energy (fork and join by events inside the captured round) made
-            // each round 0.19 ms slower, more than the branch itself takes.
-            if (sample_ssf) {
-                if (ctx->has_ssf_map) {
-                    HIPCHK(pyro::launch_map_slots(ctx->slot_of.as<int32_t>(), ctx->ssf_map.as<int32_t>(),
-                                                  ctx->ssf_slot.as<int32_t>(), ctx->R, ctx->stream));
-                    ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>(), ctx->stream);
-                } else {
-                    ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
+            if (sample_ssf == 2) {
+                // an SSF sample after every sweep of the round
+                for (int sw = 0; sw < n_sweep; ++sw) {
+                    ctx->run_sweeps(strat, kind, n_overrelax, 1, ctr, (unsigned long long)sw * (1 + n_overrelax),
+                                    fused_energy && sw == n_sweep - 1 ? ctx->epart.as<double>() : nullptr);
+                    ssf_round_sample();
                 }
+            } else {
+                if (n_sweep > 0)
+                    ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
+                                    fused_energy ? ctx->epart.as<double>() : nullptr);
+                // One stream: running the SSF branch on a second stream beside the
+                // energy (fork and join by events inside the captured round) made
+                // each round 0.19 ms slower, more than the branch itself takes.
+                if (sample_ssf) ssf_round_sample();
             }
             if (fused_energy) ctx->energy_finish(true, pyro::pyro_colour_energy_blocks(int(ctx->nc())));
             else ctx->energy(true);
@@ -1423,12 +1440,14 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
             }
         }
         for (int i = 0; i < n_rounds; ++i) {
+            // flush the buffered rows before a round that would not fit
+            if (spr && ctx->munu && ctx->munu_pending + spr > ctx->munu_B) ctx->munu_flush();
             if (use_graph) HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
             else round_body();
             ctx->pass_count += ppr;
             ctx->round_count += 1;
-            if (sample_ssf && ctx->munu) {
-                ctx->munu_pending += 1;
+            if (spr && ctx->munu) {
+                ctx->munu_pending += spr;
                 if (ctx->munu_pending >= ctx->munu_B) ctx->munu_flush();
             }
         }
@@ -1691,7 +1710,8 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha, const in
             ctx->mbuf.alloc(sizeof(float) * fw * size_t(n_slots) * nc * ctx->munu_B);
             ctx->mcnt.alloc(sizeof(unsigned int) * n_slots);
             HIPCHK(hipMemset(ctx->mcnt.p, 0, ctx->mcnt.bytes));
-            const size_t mbytes = sizeof(double) * 2 * size_t(n_slots) * nc * n_f * n_f;
+            // the stored tile pairs only (Hermitian: 6 of the 9 tiles at 48 features)
+            const size_t mbytes = sizeof(double) * 2 * size_t(n_slots) * nc * pyro::munu_acc_pairs(ctx->munu_T) * 256;
             ctx->macc.alloc(mbytes);
             HIPCHK(hipMemset(ctx->macc.p, 0, mbytes));
             ctx->host_munu_count.assign(n_slots, 0);
@@ -1808,10 +1828,11 @@ int pyro_ssf_read_munu(pyro_ctx* ctx, int slot, int c, double* C) {
         require(c >= 0 && c < ctx->n_corr, PYRO_E_INVALID, "correlator index out of range");
         ctx->munu_flush();
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        const int n_sl = ctx->geom.n_sl, n_f = 3 * n_sl;
+        const int n_sl = ctx->geom.n_sl, T = ctx->munu_T;
         const int64_t nc = ctx->nc();
-        std::vector<double> full(size_t(nc) * n_f * n_f * 2);
-        ctx->d2h(full.data(), ctx->macc.as<double>() + size_t(slot) * full.size(), full.size() * 8);
+        const size_t per_k = size_t(pyro::munu_acc_pairs(T)) * 256 * 2;  // doubles
+        std::vector<double> packed(size_t(nc) * per_k);
+        ctx->d2h(packed.data(), ctx->macc.as<double>() + size_t(slot) * packed.size(), packed.size() * 8);
         const int al = ctx->h_alpha[c], be = ctx->h_beta[c];
         // The flush accumulates 16-feature tile pairs (I, J) with I <= J
         // only; an entry of a lower tile is the conjugate of its transpose
@@ -1819,13 +1840,14 @@ int pyro_ssf_read_munu(pyro_ctx* ctx, int slot, int c, double* C) {
         for (int64_t K = 0; K < nc; ++K)
             for (int mu = 0; mu < n_sl; ++mu)
                 for (int nu = 0; nu < n_sl; ++nu) {
-                    const int f = al * n_sl + mu, g = be * n_sl + nu;
+                    int f = al * n_sl + mu, g = be * n_sl + nu;
                     const bool lower = f / 16 > g / 16;
-                    const size_t src = lower ? ((size_t(K) * n_f + g) * n_f + f) * 2
-                                             : ((size_t(K) * n_f + f) * n_f + g) * 2;
+                    if (lower) std::swap(f, g);
+                    const size_t src = size_t(K) * per_k +
+                                       (size_t(pyro::munu_acc_pair(f / 16, g / 16, T)) * 256 + (f % 16) * 16 + g % 16) * 2;
                     const size_t dst = ((size_t(K) * n_sl + mu) * n_sl + nu) * 2;
-                    C[dst] = full[src];
-                    C[dst + 1] = lower ? -full[src + 1] : full[src + 1];
+                    C[dst] = packed[src];
+                    C[dst + 1] = lower ? -packed[src + 1] : packed[src + 1];
                 }
     });
 }
@@ -1843,7 +1865,7 @@ int pyro_ssf_reset(pyro_ctx* ctx, int slot) {
             if (ctx->store2) HIPCHK(pyro::launch_zero(ctx->sacc2.as<double>() + s * per, per * 2, ctx->stream));
             HIPCHK(pyro::launch_zero(ctx->sn.as<unsigned long long>() + s, 2, ctx->stream));
             if (ctx->munu) {
-                const size_t mper = size_t(2) * ctx->nc() * 9 * ctx->geom.n_sl * ctx->geom.n_sl;
+                const size_t mper = size_t(2) * ctx->nc() * pyro::munu_acc_pairs(ctx->munu_T) * 256;
                 HIPCHK(pyro::launch_zero(ctx->macc.as<double>() + s * mper, mper * 2, ctx->stream));
             }
         }
@@ -1915,16 +1937,21 @@ int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pas
                         const int o0 = ((phase & 1) + 2 * m0) * T[0];
                         const int o1 = (((phase >> 1) & 1) + 2 * m1) * T[1];
                         const int o2 = (((phase >> 2) & 1) + 2 * m2) * T[2];
-                        for (int k = 0; k <= n_overrelax; ++k)
+                        // fused: within each colour stage a site takes every
+                        // pass in a row; else pass by pass within the tile
+                        const int n_outer = ctx->fused_passes ? 1 : n_overrelax + 1;
+                        const int n_inner = ctx->fused_passes ? n_overrelax + 1 : 1;
+                        for (int ko = 0; ko < n_outer; ++ko)
                             for (int col = 0; col < ncol; ++col)
                                 for (int q = 0; q < 16 / ncol; ++q) {
                                     const int sl = ncol == 16 ? col : col + 4 * q;
                                     for (int i0 = 0; i0 < T[0]; ++i0)
                                         for (int i1 = 0; i1 < T[1]; ++i1)
-                                            for (int i2 = 0; i2 < T[2]; ++i2) {
-                                                site[o] = int32_t(sl * nc + ctx->geom.cell_index(o0 + i0, o1 + i1, o2 + i2));
-                                                pass_offset[o++] = k;
-                                            }
+                                            for (int i2 = 0; i2 < T[2]; ++i2)
+                                                for (int ki = 0; ki < n_inner; ++ki) {
+                                                    site[o] = int32_t(sl * nc + ctx->geom.cell_index(o0 + i0, o1 + i1, o2 + i2));
+                                                    pass_offset[o++] = ko + ki;
+                                                }
                                 }
                     }
     });
@@ -1944,6 +1971,14 @@ int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
         ctx->strategy = strategy;
         ctx->tuned_key = -1;
         ctx->invalidate_graph();
+    });
+}
+
+int pyro_set_fused_passes(pyro_ctx* ctx, int on) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        if (ctx->fused_passes != (on != 0)) ctx->invalidate_graph();
+        ctx->fused_passes = on != 0;
     });
 }
 

@@ -1119,8 +1119,9 @@ __global__ __launch_bounds__(256) void munu_flush_kernel(const float* __restrict
         }
     }
     // element v of a tile accumulator on this lane: row 16I + 4*(lane>>4) + v,
-    // column 16J + (lane&15)
-    double2* out = reinterpret_cast<double2*>(acc) + (size_t(slot) * n_k + K) * size_t(n_f) * n_f;
+    // column 16J + (lane&15); the stored tile pairs (I <= J) are packed,
+    // pair p at [p][row % 16][column % 16] (munu_acc_pairs)
+    double2* out = reinterpret_cast<double2*>(acc) + (size_t(slot) * n_k + K) * size_t(NP) * 256;
     int p = 0;
 #pragma unroll
     for (int I = 0; I < T; ++I)
@@ -1128,13 +1129,12 @@ __global__ __launch_bounds__(256) void munu_flush_kernel(const float* __restrict
         for (int J = I; J < T; ++J, ++p)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int f = 16 * I + 4 * (lane >> 4) + v;
-                const int g = 16 * J + (lane & 15);
-                if (f < n_f && g < n_f) {
-                    double2 m = out[size_t(f) * n_f + g];
+                const int r = 4 * (lane >> 4) + v, c = lane & 15;
+                if (16 * I + r < n_f && 16 * J + c < n_f) {
+                    double2 m = out[p * 256 + r * 16 + c];
                     m.x += double(RR[p][v]) + double(II[p][v]);
                     m.y += double(RI[p][v]) - double(IR[p][v]);
-                    out[size_t(f) * n_f + g] = m;
+                    out[p * 256 + r * 16 + c] = m;
                 }
             }
 }

@@ -123,9 +123,12 @@ constexpr size_t pyro_tile_lds(int T0, int T1, int T2, int real_size, bool lifte
     return size_t(3) * 16 * (T0 + 1) * (T1 + 1) * (T2 + 1) * size_t(real_size) +
            (lifted ? size_t(16) * 4 * ((T0 * T1 * T2 + 31) / 32) : 0);
 }
+// fused: every pass of a site back to back within its colour stage, from one
+// field evaluation (pyro_set_fused_passes), else pass by pass.
 template <typename Real>
 hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
-                             int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s);
+                             int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s,
+                             bool fused = false);
 
 // Local fields (no global field) of replica r into out[3][N].
 template <typename Real>
@@ -244,8 +247,13 @@ hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_be
 // replica are staged as one FP32 row X = [Re a | Im a] (a_f, f = a*n_sl+mu,
 // each half padded to 16*T) into buf[slot][k][cnt[slot]][32*T]; a flush
 // accumulates X^T X per (slot, k) on FP32 MFMA and adds
-// Re M = RR + II, Im M = RI - IR into acc[slot][k][n_f][n_f][2] (FP64),
-// M = sum conj(a)^T a, then clears the counts.
+// Re M = RR + II, Im M = RI - IR into the FP64 accumulators, M = sum
+// conj(a)^T a, then clears the counts.  M is Hermitian, so only its 16 x 16
+// tile pairs (I, J) with I <= J are kept: acc[slot][k][p][16][16][2] with
+// p = munu_acc_pair(I, J, T) (element (f, g) of a lower tile is the
+// conjugate of (g, f)).
+inline int munu_acc_pairs(int T) { return T * (T + 1) / 2; }
+inline int munu_acc_pair(int I, int J, int T) { return I * T - I * (I - 1) / 2 + (J - I); }
 struct MunuArgs {
     float* buf = nullptr;
     unsigned int* cnt = nullptr;   // [n_slots]

@@ -102,6 +102,17 @@ struct LowSums {
 // (colour = sublattice mod NCOL) into *low, from the same loads.  They are
 // sums of the loaded values only (no products), so the full field's
 // arithmetic is that of SPLIT = 0.
+// Whether stencil neighbour k of sublattice SL is the first one its shell
+// sums (the shell's sum starts from its value rather than from 0 + value).
+template <int SL, bool FWD>
+constexpr bool first_in_shell(int k) {
+    const int shell = k < 6 ? 0 : (k < 18 ? 1 : 2);
+    const int begin = shell == 0 ? 0 : (shell == 1 ? 6 : 18);
+    for (int j = begin; j < k; ++j)
+        if (!FWD || kStencil.n[SL][j].sl > SL) return false;
+    return true;
+}
+
 template <typename Real, int SHELLS, int SL, bool FWD = false, int SPLIT = 0, typename Fetch>
 __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplings<Real>& J, Real& hx,
                                                 Real& hy, Real& hz, LowSums<Real>* low = nullptr) {
@@ -117,9 +128,17 @@ __device__ __forceinline__ void pyro_field_with(Fetch&& fetch, const PyroCouplin
         if constexpr (!FWD || n.sl > SL) {
             Real x, y, z;
             fetch(n, x, y, z);
-            s[shell][0] += x;
-            s[shell][1] += y;
-            s[shell][2] += z;
+            // a shell's first term starts its sum: 0 + x would cost an add
+            // per component (it cannot be folded away, 0 + -0 being +0)
+            if constexpr (first_in_shell<SL, FWD>(k)) {
+                s[shell][0] = x;
+                s[shell][1] = y;
+                s[shell][2] = z;
+            } else {
+                s[shell][0] += x;
+                s[shell][1] += y;
+                s[shell][2] += z;
+            }
             constexpr int tp = n.sl % 4;
             if constexpr (shell == 0)
                 zs += Real(pyrostencil::kLink[tp][0]) * x + Real(pyrostencil::kLink[tp][1]) * y +
@@ -580,7 +599,18 @@ __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, cons
 // Workgroups map to tiles XCD by XCD: the 8 XCDs take contiguous chunks of
 // the (replica, tile) order, so consecutive tiles along n2, whose halo rows
 // share cache lines, are fetched through the same L2.
-template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2, int BS>
+// FUSED (site passes back to back): within a colour stage each site takes
+// every pass of the sweep in a row, the first kind then the n_or OR passes,
+// from one evaluation of its field.  No bonded site changes in between (a
+// colour class has no bond inside it), so the field is the one each pass
+// would compute: this is the sequential order "per tile, per colour, per
+// site, every pass", with half the field evaluations and barriers of a
+// lifted + OR sweep.  The site's uniforms for the sweep are then drawn in
+// order from its blocks: uniform j of the sweep is component j % 5 of the
+// block with pass index pass0 + j / 5, the first kind taking the first
+// kUniformsOf[FIRST] and each OR pass the next one (a lifted + OR sweep
+// draws one Philox block per site instead of two).
+template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2, int BS, bool FUSED>
 __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                          int phase, int n_or,
                                                          unsigned long long pass_index) {
@@ -635,6 +665,64 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     static_assert(NI <= BS, "one tile cell per thread");
     const int my_cell = kTileLanes<T0, T1, T2, BS>.cell[threadIdx.x];
     unsigned int my_acc = 0;
+    if constexpr (FUSED) {
+        for (int col = 0; col < NCOL; ++col) {
+            for (int q = 0; q < SPC; ++q) {
+                const int sl = NCOL == 16 ? col : col + 4 * q;
+                const int f = sl >> 2;
+                const int own = ((f == 2 || f == 3) * B1 + (f == 1 || f == 3)) * B2 + (f == 1 || f == 2);
+                if (my_cell >= 0) {
+                    const int m = my_cell;
+                    const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
+                    const int lc = (i0 * B1 + i1) * B2 + i2;
+                    Real hx, hy, hz;
+                    switch (sl) {
+#define PYRO_SL_CASE(X) \
+    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, BC>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
+                        PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+                        PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+                        PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+                        PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+                        default: hx = hy = hz = Real(0);
+                    }
+                    const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
+                    const int li = sl * BC + lc + own;
+                    Real x = lx[li], y = ly[li], z = lz[li];
+                    bool any = false, ch;
+                    int8_t d = 1, d0 = 1;
+                    unsigned int* wp = nullptr;
+                    const unsigned int bit = 1u << (m & 31);
+                    if constexpr (LIFTED) {
+                        wp = ldw + sl * W + (m >> 5);
+                        d0 = d = (*wp & bit) ? int8_t(-1) : int8_t(1);
+                    }
+                    Uniform5 ub = site_uniforms(st, site, r, pass0);
+                    my_acc += update_spin_u<Real, FIRST>(rp, x, y, z, ch, ub, hx, hy, hz, LIFTED ? &d : nullptr)
+                                  ? 1u : 0u;
+                    any = ch;
+                    if constexpr (LIFTED) {
+                        if (d != d0) atomicXor(wp, bit);
+                    }
+                    for (int k = 1; k <= n_or; ++k) {
+                        const int j = kUniformsOf[FIRST] + k - 1;  // the sweep's uniform for this pass
+                        const int c = j % 5;
+                        if (c == 0) ub = site_uniforms(st, site, r, pass0 + unsigned(j / 5));
+                        Uniform5 uo;
+                        uo.m[0] = c == 0 ? ub.m[0] : c == 1 ? ub.m[1] : c == 2 ? ub.m[2] : c == 3 ? ub.m[3] : ub.m[4];
+                        update_spin_u<Real, 3>(rp, x, y, z, ch, uo, hx, hy, hz, nullptr);
+                        any = any || ch;
+                    }
+                    if (any) {
+                        lx[li] = x;
+                        ly[li] = y;
+                        lz[li] = z;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    } else
     for (int k = 0; k <= n_or; ++k) {
         for (int col = 0; col < NCOL; ++col) {
             for (int q = 0; q < SPC; ++q) {
@@ -876,7 +964,8 @@ hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings
 
 template <typename Real>
 hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
-                             int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s) {
+                             int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s,
+                             bool fused) {
     auto go = [&](auto t0, auto t1, auto t2) -> hipError_t {
         constexpr int T0 = decltype(t0)::value, T1 = decltype(t1)::value, T2 = decltype(t2)::value;
         constexpr int BS = pyro_tile_threads(T0 * T1 * T2);
@@ -885,20 +974,22 @@ hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Re
         const dim3 grid(tiles * unsigned(st.R));  // (replica, tile), XCD-chunked in the kernel
         return dispatch_pyro(J, [&](auto shells) -> hipError_t {
             constexpr int SH = decltype(shells)::value;
-            switch (kind) {
-#define PYRO_TILED_LAUNCH(K)                                                                      \
-    case K: {                                                                                   \
-        auto* fn = pyro_tiled_kernel<Real, SH, K, T0, T1, T2, BS>;                              \
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),                            \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));        \
-        hipLaunchKernelGGL(fn, grid, dim3(BS), lds, s, st, J, phase, n_or, pass_index);         \
-        break;                                                                                  \
-    }
-                PYRO_TILED_LAUNCH(0) PYRO_TILED_LAUNCH(1) PYRO_TILED_LAUNCH(2)
-#undef PYRO_TILED_LAUNCH
-                default: return hipErrorInvalidValue;
-            }
-            return hipGetLastError();
+            auto launch = [&](auto kk, auto fu) -> hipError_t {
+                auto* fn = pyro_tiled_kernel<Real, SH, decltype(kk)::value, T0, T1, T2, BS, decltype(fu)::value>;
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+                hipLaunchKernelGGL(fn, grid, dim3(BS), lds, s, st, J, phase, n_or, pass_index);
+                return hipGetLastError();
+            };
+            auto by_kind = [&](auto fu) -> hipError_t {
+                switch (kind) {
+                    case 0: return launch(std::integral_constant<int, 0>{}, fu);
+                    case 1: return launch(std::integral_constant<int, 1>{}, fu);
+                    case 2: return launch(std::integral_constant<int, 2>{}, fu);
+                    default: return hipErrorInvalidValue;
+                }
+            };
+            return fused ? by_kind(std::true_type{}) : by_kind(std::false_type{});
         });
     };
     using I4 = std::integral_constant<int, 4>;

@@ -31,12 +31,13 @@ EPS_UPDATE = 1e-6  # one FP32 update, hardware transcendentals included
 SAFETY = 10.0
 
 
-def tiled_stages(ctx, site, koff):
+def tiled_stages(ctx, site, koff, fused=False):
     """Stage key of every update of a tiled sweep's order (pyro_sweep_order
-    under the tiled strategy): (checkerboard phase, pass, colour).  Tiles of
-    one phase share no bond and a colour class has no bond inside it, so
-    updates of one key are independent, and taking the keys in increasing
-    order preserves every dependency of the tile-by-tile order."""
+    under the tiled strategy): (checkerboard phase, pass, colour), or with
+    fused passes (phase, colour, pass).  Tiles of one phase share no bond
+    and a colour class has no bond inside it, so updates of one key are
+    independent, and taking the keys in increasing order preserves every
+    dependency of the tile-by-tile order."""
     T = np.array(ctx.tile_shape())
     L = np.array(ctx.Ls)
     nc = int(np.prod(L))
@@ -46,7 +47,10 @@ def tiled_stages(ctx, site, koff):
     phase = par[:, 0] + 2 * par[:, 1] + 4 * par[:, 2]
     n_col = ctx.info()["n_colours"]
     colour = ctx.colouring()[site]
-    return (phase.astype(np.int64) * (int(koff.max()) + 1) + koff) * n_col + colour
+    n_pass = int(koff.max()) + 1
+    if fused:
+        return (phase.astype(np.int64) * n_col + colour) * n_pass + koff
+    return (phase.astype(np.int64) * n_pass + koff) * n_col + colour
 
 
 def padded_adjacency(o, J=None):
@@ -101,8 +105,9 @@ def propagate(order_sites, stage, info, adj, T, scale, N):
 
 def check(gpu_S, orc_S, E, taint, seeds, n_updates, label=""):
     """Every untainted site within FP32_SITE_TOL + SAFETY * E, and possibly
-    flipped decisions rare (at most 1 in 500 updates, a handful on small
-    lattices); returns the largest ratio of deviation to allowance (printed
+    flipped decisions rare (at most 1 in 100 updates, a handful on small
+    lattices: candidates by the bound, which is conservative, not observed
+    flips); returns the largest ratio of deviation to allowance (printed
     for calibration).  The tainted cones may cover much of a large lattice
     (one flip at these temperatures flips downstream decisions with O(1)
     probability), so their size is printed, not bounded."""
@@ -114,6 +119,6 @@ def check(gpu_S, orc_S, E, taint, seeds, n_updates, label=""):
           f"{len(E)}, sites off by > 1e-3: {int((diff > 1e-3).sum())}, max dev {diff.max():.3g} (untainted "
           f"{diff[~taint].max() if (~taint).any() else 0:.3g}), max dev/allowance {ratio[worst]:.3g} "
           f"(site {worst}, bound {E[worst]:.3g})")
-    assert seeds <= max(4, n_updates // 500), seeds
+    assert seeds <= max(4, n_updates // 100), seeds
     assert ratio[worst] <= 1.0, (worst, diff[worst], E[worst])
     return float(ratio[worst])

@@ -666,14 +666,22 @@ def test_ssf_munu_mfma(gpu, oracle, prec):
                                                 (16, (1.0, 0.1, 0.0), 2, 1, (8, 8, 4)),
                                                 (24, (1.0, 0.1, 0.1), 1, 1, (4, 4, 4)),
                                                 # the large tempering run's sweep: lifted + OR on 6x6x4 tiles
-                                                (24, (1.0, 0.1, 0.1), 1, 1, (6, 6, 4))])
+                                                (24, (1.0, 0.1, 0.1), 1, 1, (6, 6, 4)),
+                                                # a site's passes back to back (pyro_set_fused_passes)
+                                                (24, (1.0, 0.1, 0.1), 1, 1, "fused"),
+                                                (8, (1.0, -0.05, 0.05), 0, 2, "fused"),
+                                                (16, (1.0, 0.1, 0.0), 2, 1, "fused")])
 def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, tile):
     """The tiled schedule (tiles in LDS, all passes per tile) equals the
-    sequential order pyro_sweep_order reports, for every tile shape; the
-    oracle replays it draw for draw."""
+    sequential order pyro_sweep_order reports, for every tile shape and
+    with fused passes; the oracle replays it draw for draw."""
     R, seed = 2, 0xBEEF
     T = [0.35, 1.7]
     ctx, os_ = make(gpu, oracle, L, J, 1.0, prec, R=R, seed=seed, B=(0.05, -0.1, 0.2))
+    fused = tile == "fused"
+    if fused:
+        ctx.set_fused_passes(True)
+        tile = None
     if tile is not None:
         try:
             ctx.set_tile_shape(tile)
@@ -697,8 +705,8 @@ def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, t
             assert e.code == 5  # tile + halo exceeds this device's LDS limit
             pytest.skip(str(e))
         for r, o in enumerate(os_):
-            a = o.sweep_order(kind, T[r], site, koff, seed, r, 0)
-            a += o.sweep_order(kind, T[r], site, koff, seed, r, 1 + n_or)
+            a = o.sweep_order(kind, T[r], site, koff, seed, r, 0, shared=fused)
+            a += o.sweep_order(kind, T[r], site, koff, seed, r, 1 + n_or, shared=fused)
             diff = np.abs(ctx.get_spins(r) - o.get_spins()).max(1)
             # every decision equal; rounding of the shell-grouped field sum
             # against the oracle's per-bond sum accumulates over the passes
@@ -711,7 +719,7 @@ def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, t
     # FP32: sweep by sweep from the GPU's own state, every site within its
     # first-order error bound
     adj = fp32_bound.padded_adjacency(os_[0], (*J, 1.0))
-    stage = fp32_bound.tiled_stages(ctx, site, koff)
+    stage = fp32_bound.tiled_stages(ctx, site, koff, fused)
     for sw in range(2):
         before = [(ctx.get_spins(r), ctx.get_lifted_dir(r)) for r in range(R)]
         p0 = ctx.pass_count()
@@ -724,7 +732,7 @@ def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, t
             o.set_spins(before[r][0])
             o.set_dir(before[r][1])
             info = np.empty((len(site), 4))
-            o.sweep_order(kind, T[r], site, koff, seed, r, p0, info=info)
+            o.sweep_order(kind, T[r], site, koff, seed, r, p0, info=info, shared=fused)
             E, taint, seeds = fp32_bound.propagate(site, stage, info, adj, T[r], field_scale(J, (0.05, -0.1, 0.2)),
                                                    ctx.N)
             fp32_bound.check(ctx.get_spins(r), o.get_spins(), E, taint, seeds, len(site),
@@ -799,3 +807,42 @@ def test_invalid_contexts_rejected(gpu):
     with pytest.raises(gpu.PyroError):
         ctx.set_temperatures([-1.0])
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_tempering_ssf_every_sweep(gpu, prec):
+    """Rounds of two sweeps with an SSF sample after each sweep (sample_ssf
+    = 2) visit the same states as one-sweep rounds with an exchange after
+    every second round: the same swaps and permutation, bit-identical
+    contracted SSF sums, and sublattice-resolved sums equal to FP32-MFMA
+    rounding although a 3-row buffer is flushed at different points (before
+    a round whose two samples would not fit)."""
+    out = []
+    for per_sweep in (True, False):
+        ctx = gpu.Context.pyrochlore(2, n_replicas=6, precision=prec, seed=17)
+        ctx.define_J1J2J3(1.0, -0.05, 0.05)
+        ctx.setup()
+        ctx.randomize(4)
+        ctx.pt_init(np.geomspace(2.0, 0.2, 6), 8)
+        ctx.ssf_configure(("xx", "zy"), n_slots=6, munu=True, munu_buffer=3)
+        if per_sweep:
+            ctx.pt_rounds(gpu.LIFTED, 1, 2, 2, 5)
+        else:
+            ctx.pt_exchange_interval(2)
+            ctx.pt_rounds(gpu.LIFTED, 1, 1, True, 10)
+        st = ctx.pt_stats()
+        ssf = [ctx.ssf_read(s) for s in range(6)]
+        munu = [ctx.ssf_read_munu(s, c) for s in range(6) for c in range(2)]
+        out.append((st, ctx.pt_permutation(), ssf, munu, [ctx.get_spins(r) for r in range(6)]))
+        ctx.close()
+    (sa, pa, fa, ma, Sa), (sb, pb, fb, mb, Sb) = out
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(sa["attempts"], sb["attempts"]) and np.array_equal(sa["accepts"], sb["accepts"])
+    assert np.all(sa["n"] == 5) and np.all(sb["n"] == 10)
+    for a, b in zip(Sa, Sb):
+        assert np.array_equal(a, b)
+    for (S1, Q1, n1), (S2, Q2, n2) in zip(fa, fb):
+        assert n1 == n2 == 10
+        assert np.array_equal(S1, S2) and np.array_equal(Q1, Q2)
+    for a, b in zip(ma, mb):
+        assert np.abs(a - b).max() <= 2e-5 * 8 * 10

@@ -6,7 +6,8 @@ Metropolis + one OR pass on the default 6x6x4 tiles, an exchange every 2
 sweeps, the 9 correlators with the sublattice-resolved matrices every sweep
 through the 192-sample FP32-MFMA buffer.
   - one sweep of two replicas replayed by the oracle in the exported tiled
-    order: FP64 decisions exact, FP32 within the first-order error bound;
+    order, pass by pass and with fused passes: FP64 decisions exact, FP32
+    within the first-order error bound;
   - tempering rounds: the sampled E sums equal the energies of the states
     the rounds leave, and the accumulated S(q) and C_{mu nu}(K) equal sums
     over rounds of products of the spins' own transforms, at a sample of K
@@ -43,11 +44,13 @@ def oracle48(oracle):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-def test_large_tempering_sweep_replay(gpu, oracle48, prec):
+@pytest.mark.parametrize("fused", [False, True])
+def test_large_tempering_sweep_replay(gpu, oracle48, prec, fused):
     seed = 0x48C5
     ctx = gpu.Context.pyrochlore(L5, n_replicas=R5, precision=prec, seed=seed)
     ctx.define_J1J2J3(*J5)
     ctx.setup()
+    ctx.set_fused_passes(fused)
     ctx.set_field(np.array(B5))
     ctx.set_temperatures(T5)
     ctx.randomize(17)
@@ -68,15 +71,15 @@ def test_large_tempering_sweep_replay(gpu, oracle48, prec):
         o.set_spins(before[r][0])
         o.set_dir(before[r][1])
         if prec == 64:
-            a = o.sweep_order(gpu.LIFTED, T5[r], site, koff, seed, r, p0)
+            a = o.sweep_order(gpu.LIFTED, T5[r], site, koff, seed, r, p0, shared=fused)
             assert int(acc[r]) == a
             assert np.abs(ctx.get_spins(r) - o.get_spins()).max() < 1e-9
             assert np.array_equal(ctx.get_lifted_dir(r), o.get_dir())
         else:
             info = np.empty((len(site), 4))
-            o.sweep_order(gpu.LIFTED, T5[r], site, koff, seed, r, p0, info=info)
+            o.sweep_order(gpu.LIFTED, T5[r], site, koff, seed, r, p0, info=info, shared=fused)
             adj = fp32_bound.padded_adjacency(o, (*J5, 1.0))
-            E, taint, seeds = fp32_bound.propagate(site, fp32_bound.tiled_stages(ctx, site, koff), info, adj,
+            E, taint, seeds = fp32_bound.propagate(site, fp32_bound.tiled_stages(ctx, site, koff, fused), info, adj,
                                                    T5[r], _scale(J5, B5), ctx.N)
             fp32_bound.check(ctx.get_spins(r), o.get_spins(), E, taint, seeds, len(site),
                              f"L=48 replica {r} T={T5[r]:.3g}")
