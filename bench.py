@@ -129,8 +129,9 @@ def munu_correlators(pyro, samples=384, buffer=192):
     sublattices (the 48 x 48 Hermitian matrix per k), one accumulator slot
     per replica.  Samples are staged as FP32 rows X = [Re | Im] of the 48
     transformed fields, `buffer` deep, and flushed as X^T X on FP32 MFMA
-    into FP64 sums (tile pairs I <= J of the Hermitian matrix).  192 rows
-    per flush: 130 GB of staging beside the 65 GB of FP64 accumulators, so
+    into FP64 sums (tile pairs I <= J of the Hermitian matrix), for the
+    kept half k2 <= L/2 of the K (M(-K) = conj(M(K)) for real spins).  192
+    rows per flush: 68 GB of staging beside 22.6 GB of FP64 accumulators, so
     the accumulator round trip is spread over three times as many samples as
     at 64.  Returns the flush kernel's timing and work."""
     R, L = 16, 48
@@ -147,7 +148,7 @@ def munu_correlators(pyro, samples=384, buffer=192):
     ctx.profile(False)
     ctx.close()
     flush_ms, n_flush = prof["munu_flush"]
-    n_k, T = L ** 3, 3  # 48 features = 3 tiles of 16
+    n_k, T = L * L * (L // 2 + 1), 3  # the kept K; 48 features = 3 tiles of 16
     pairs = T * (T + 1) // 2  # tile pairs I <= J of the Hermitian matrix
     # per row and tile pair: RR, II, RI, IR products, 16 x 16 x 2 flops each
     flops = R * n_k * buffer * pairs * 4 * 16 * 16 * 2

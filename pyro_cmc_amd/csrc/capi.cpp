@@ -232,6 +232,12 @@ struct pyro_ctx {
     std::vector<int32_t> h_alpha, h_beta;
     DevMem F, G, tw0, tw1, tw2, phase, d_alpha, d_beta, sacc, sacc2, sn, ssf_slot;
     DevMem mbuf, mcnt, macc;
+    // The DFT keeps k2 < ssf_H of every (k0, k1) line (n_kh values per
+    // field); the other K follow from -K (the fields are real).  The
+    // contraction also accumulates the K outside the kept half with a
+    // Nyquist index on axis 0 or 1 (nyquist_01); reads fill in the rest.
+    int ssf_H = 0;
+    int64_t n_kh = 0;
     DevMem ssf_map;            // tempering slot -> accumulator slot (-1 skip)
     bool has_ssf_map = false;
     bool dft_ready = false;
@@ -265,6 +271,23 @@ struct pyro_ctx {
 
     int64_t N() const { return geom.n_sites(); }
     int64_t nc() const { return geom.n_cells(); }
+    // K = (k0 L1 + k1) L2 + k2: its kept index (-1 outside the kept half),
+    // its partner -K, and whether its correlators are accumulated directly
+    int64_t kept_index(int64_t K) const {
+        const int L2 = geom.L[2];
+        const int64_t l = K / L2, k2 = K - l * L2;
+        return k2 < ssf_H ? l * ssf_H + k2 : -1;
+    }
+    int64_t neg_k(int64_t K) const {
+        const auto& L = geom.L;
+        const int64_t k0 = K / (int64_t(L[1]) * L[2]), k1 = (K / L[2]) % L[1], k2 = K % L[2];
+        return (((L[0] - k0) % L[0]) * L[1] + (L[1] - k1) % L[1]) * L[2] + (L[2] - k2) % L[2];
+    }
+    bool nyquist_01(int64_t K) const {
+        const auto& L = geom.L;
+        const int64_t k0 = K / (int64_t(L[1]) * L[2]), k1 = (K / L[2]) % L[1];
+        return (L[0] % 2 == 0 && 2 * k0 == L[0]) || (L[1] % 2 == 0 && 2 * k1 == L[1]);
+    }
     size_t real_size() const { return prec == 64 ? 8 : 4; }
 
     // Copies ordered on the context stream (kernels queued earlier finish
@@ -392,6 +415,8 @@ struct pyro_ctx {
         st.L0 = geom.L[0];
         st.L1 = geom.L[1];
         st.L2 = geom.L[2];
+        st.H = ssf_H;
+        st.n_kh = int(n_kh);
         return st;
     }
 
@@ -653,7 +678,7 @@ struct pyro_ctx {
         m.cnt = mcnt.as<unsigned int>();
         m.acc = macc.as<double>();
         m.Bmax = munu_B;
-        m.n_k = int(nc());
+        m.n_k = int(n_kh);
         m.n_slots = n_slots;
         m.n_f = 3 * geom.n_sl;
         m.T = munu_T;
@@ -672,7 +697,10 @@ struct pyro_ctx {
         if (dft_ready) return;
         const size_t rs = real_size();
         const int n_sl = geom.n_sl;
-        const size_t per_rep = size_t(3) * n_sl * nc() * 2 * rs;
+        const auto& L = geom.L;
+        ssf_H = pyro::dft_kept_k2(int(rs), L[0], L[1], L[2], 3ll * n_sl * nc());
+        n_kh = int64_t(L[0]) * L[1] * ssf_H;
+        const size_t per_rep = size_t(3) * n_sl * n_kh * 2 * rs;
         ssf_chunk = int(std::max<size_t>(1, std::min<size_t>(R, (size_t(2) << 30) / per_rep)));
         F.alloc(per_rep * ssf_chunk);
         G.alloc(per_rep * ssf_chunk);
@@ -685,7 +713,6 @@ struct pyro_ctx {
                 h2d(m.p, f.data(), f.size() * 4);
             }
         };
-        const auto& L = geom.L;
         DevMem* tws[3] = {&tw0, &tw1, &tw2};
         for (int a = 0; a < 3; ++a) {
             std::vector<double> tw(2 * L[a]);
@@ -1707,11 +1734,13 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha, const in
         if (ctx->munu) {
             ctx->munu_T = (n_f + 15) / 16;
             const size_t fw = size_t(32) * ctx->munu_T;
-            ctx->mbuf.alloc(sizeof(float) * fw * size_t(n_slots) * nc * ctx->munu_B);
+            ctx->mbuf.alloc(sizeof(float) * fw * size_t(n_slots) * ctx->n_kh * ctx->munu_B);
             ctx->mcnt.alloc(sizeof(unsigned int) * n_slots);
             HIPCHK(hipMemset(ctx->mcnt.p, 0, ctx->mcnt.bytes));
             // the stored tile pairs only (Hermitian: 6 of the 9 tiles at 48 features)
-            const size_t mbytes = sizeof(double) * 2 * size_t(n_slots) * nc * pyro::munu_acc_pairs(ctx->munu_T) * 256;
+            // (and the kept K only: M(-K) = conj(M(K)))
+            const size_t mbytes =
+                sizeof(double) * 2 * size_t(n_slots) * ctx->n_kh * pyro::munu_acc_pairs(ctx->munu_T) * 256;
             ctx->macc.alloc(mbytes);
             HIPCHK(hipMemset(ctx->macc.p, 0, mbytes));
             ctx->host_munu_count.assign(n_slots, 0);
@@ -1809,9 +1838,28 @@ int pyro_ssf_read(pyro_ctx* ctx, int slot, double* S, double* S2, uint64_t* n_sa
         require(slot >= 0 && slot < ctx->n_slots, PYRO_E_INVALID, "slot index out of range");
         require(!S2 || ctx->store2, PYRO_E_STATE, "error term was not configured");
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        const size_t per = size_t(2) * ctx->n_corr * ctx->nc();
+        const int64_t nc = ctx->nc();
+        const size_t per = size_t(2) * ctx->n_corr * nc;
         if (S) ctx->d2h(S, ctx->sacc.as<double>() + slot * per, per * 8);
         if (S2) ctx->d2h(S2, ctx->sacc2.as<double>() + slot * per, per * 8);
+        // K outside the kept half: S(K) = conj(S(-K)) per sample (b(K) =
+        // conj(b(-K)) wherever the centred q of K is minus that of -K); the
+        // squares of the parts are equal
+        for (int64_t K = 0; K < nc; ++K) {
+            if (ctx->kept_index(K) >= 0 || ctx->nyquist_01(K)) continue;
+            const int64_t P = ctx->neg_k(K);
+            for (int c = 0; c < ctx->n_corr; ++c) {
+                const size_t d = 2 * (size_t(c) * nc + K), q = 2 * (size_t(c) * nc + P);
+                if (S) {
+                    S[d] = S[q];
+                    S[d + 1] = -S[q + 1];
+                }
+                if (S2) {
+                    S2[d] = S2[q];
+                    S2[d + 1] = S2[q + 1];
+                }
+            }
+        }
         if (n_samples) {
             unsigned long long n = 0;
             ctx->d2h(&n, ctx->sn.as<unsigned long long>() + slot, 8);
@@ -1831,24 +1879,29 @@ int pyro_ssf_read_munu(pyro_ctx* ctx, int slot, int c, double* C) {
         const int n_sl = ctx->geom.n_sl, T = ctx->munu_T;
         const int64_t nc = ctx->nc();
         const size_t per_k = size_t(pyro::munu_acc_pairs(T)) * 256 * 2;  // doubles
-        std::vector<double> packed(size_t(nc) * per_k);
+        std::vector<double> packed(size_t(ctx->n_kh) * per_k);
         ctx->d2h(packed.data(), ctx->macc.as<double>() + size_t(slot) * packed.size(), packed.size() * 8);
         const int al = ctx->h_alpha[c], be = ctx->h_beta[c];
         // The flush accumulates 16-feature tile pairs (I, J) with I <= J
         // only; an entry of a lower tile is the conjugate of its transpose
-        // (the matrix sum_samples conj(A_f) A_g is Hermitian).
-        for (int64_t K = 0; K < nc; ++K)
+        // (the matrix sum_samples conj(A_f) A_g is Hermitian).  K outside the
+        // kept half: M(K) = conj(M(-K)), since A(K) = conj(A(-K)).
+        for (int64_t K = 0; K < nc; ++K) {
+            int64_t kh = ctx->kept_index(K);
+            const bool outside = kh < 0;
+            if (outside) kh = ctx->kept_index(ctx->neg_k(K));
             for (int mu = 0; mu < n_sl; ++mu)
                 for (int nu = 0; nu < n_sl; ++nu) {
                     int f = al * n_sl + mu, g = be * n_sl + nu;
                     const bool lower = f / 16 > g / 16;
                     if (lower) std::swap(f, g);
-                    const size_t src = size_t(K) * per_k +
+                    const size_t src = size_t(kh) * per_k +
                                        (size_t(pyro::munu_acc_pair(f / 16, g / 16, T)) * 256 + (f % 16) * 16 + g % 16) * 2;
                     const size_t dst = ((size_t(K) * n_sl + mu) * n_sl + nu) * 2;
                     C[dst] = packed[src];
-                    C[dst + 1] = lower ? -packed[src + 1] : packed[src + 1];
+                    C[dst + 1] = lower != outside ? -packed[src + 1] : packed[src + 1];
                 }
+        }
     });
 }
 
@@ -1865,7 +1918,7 @@ int pyro_ssf_reset(pyro_ctx* ctx, int slot) {
             if (ctx->store2) HIPCHK(pyro::launch_zero(ctx->sacc2.as<double>() + s * per, per * 2, ctx->stream));
             HIPCHK(pyro::launch_zero(ctx->sn.as<unsigned long long>() + s, 2, ctx->stream));
             if (ctx->munu) {
-                const size_t mper = size_t(2) * ctx->nc() * pyro::munu_acc_pairs(ctx->munu_T) * 256;
+                const size_t mper = size_t(2) * ctx->n_kh * pyro::munu_acc_pairs(ctx->munu_T) * 256;
                 HIPCHK(pyro::launch_zero(ctx->macc.as<double>() + s * mper, mper * 2, ctx->stream));
             }
         }
@@ -1879,17 +1932,29 @@ int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft) {
         ctx->need_setup();
         ctx->check_replica(replica, false);
         ctx->ensure_dft();
-        const size_t n = size_t(3) * ctx->geom.n_sl * ctx->nc() * 2;
+        const int64_t nc = ctx->nc(), nh = ctx->n_kh;
+        const size_t n = size_t(3) * ctx->geom.n_sl * nh * 2;  // the kept half
+        std::vector<double> h(n);
         if (ctx->prec == 64) {
             HIPCHK(pyro::launch_dft<double>(ctx->ssf_state<double>(), replica, 1, ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
-            ctx->d2h(ft, ctx->F.p, n * 8);
+            ctx->d2h(h.data(), ctx->F.p, n * 8);
         } else {
             HIPCHK(pyro::launch_dft<float>(ctx->ssf_state<float>(), replica, 1, ctx->stream));
             std::vector<float> f(n);
             HIPCHK(hipStreamSynchronize(ctx->stream));
             ctx->d2h(f.data(), ctx->F.p, n * 4);
-            for (size_t i = 0; i < n; ++i) ft[i] = f[i];
+            for (size_t i = 0; i < n; ++i) h[i] = f[i];
+        }
+        // every K: kept, or the conjugate of -K (real fields)
+        for (int64_t K = 0; K < nc; ++K) {
+            int64_t kh = ctx->kept_index(K);
+            const double sg = kh < 0 ? -1.0 : 1.0;
+            if (kh < 0) kh = ctx->kept_index(ctx->neg_k(K));
+            for (int f = 0; f < 3 * ctx->geom.n_sl; ++f) {
+                ft[2 * (size_t(f) * nc + K)] = h[2 * (size_t(f) * nh + kh)];
+                ft[2 * (size_t(f) * nc + K) + 1] = sg * h[2 * (size_t(f) * nh + kh) + 1];
+            }
         }
     });
 }

@@ -801,17 +801,22 @@ __global__ __launch_bounds__(256) void dft_ct_strided_kernel(const float* __rest
 // P'[k2][i1]; pass B then reads columns of P' with the same transposed
 // access.  The row stride 68 puts a wave's transposed reads on 64 distinct
 // banks for 48-point lines (4 ci + 3 kq mod 64 distinct).
+//
+// The input is real, so only the kept half k2 < H = L2 / 2 + 1 is carried
+// into pass B and written (plane stride L1 H): the waves of pass B take the
+// columns k2 < 16 ceil(H / 16).
 template <int M1, int M2>
 __global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict__ in, float* __restrict__ out,
                                                         const float* __restrict__ tw1,
                                                         const float* __restrict__ tw2, int n_sl, int L0) {
     constexpr int L1 = 16 * M1, L2 = 16 * M2, NP = L1 * L2;
+    constexpr int H = L2 / 2 + 1, NPH = L1 * H, WB = (H + 15) / 16;
     constexpr int XS = (M1 == 3 && M2 == 3) ? 68 : 71;
     constexpr int ROWS = L1 > L2 ? L1 : L2;
     constexpr int RND = (NP + 255) / 256;
     __shared__ float pr[ROWS * XS];
     __shared__ float pi[ROWS * XS];
-    const size_t base = size_t(blockIdx.x) * NP;  // plane p = field * L0 + n0 of the output
+    const size_t base = size_t(blockIdx.x) * NPH;  // plane p = field * L0 + n0 of the output
     // input: field = (r * 3 + a) * n_sl + mu of spins S[r][i][a], every third value
     const int field = int(blockIdx.x) / L0, n0 = int(blockIdx.x) - field * L0;
     const int mu = field % n_sl, ra = field / n_sl, a = ra % 3, r = ra / 3;
@@ -848,13 +853,15 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict_
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int k2 = 4 * kq + u + 16 * q;
-                pr[k2 * XS + 16 * w + ci] = yr[q][u];
-                pi[k2 * XS + 16 * w + ci] = yi[q][u];
+                if (16 * q < 16 * WB) {  // rows read by pass B
+                    pr[k2 * XS + 16 * w + ci] = yr[q][u];
+                    pi[k2 * XS + 16 * w + ci] = yi[q][u];
+                }
             }
     }
     __syncthreads();
-    // pass B: columns k2 = 16 w + ci (M2 waves), points along n1
-    if (w < M2) {
+    // pass B: columns k2 = 16 w + ci (WB waves), points along n1
+    if (w < WB) {
         float zr[M1][4], zi[M1][4];
         ct16_lines<false, M1>(
             [&](int, int, int n, float& br, float& bi) __attribute__((always_inline)) {
@@ -863,10 +870,13 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict_
             },
             tw1, kq, ci, zr, zi);
         float2* o = reinterpret_cast<float2*>(out) + base;
+        const int k2 = 16 * w + ci;
+        if (k2 < H) {
 #pragma unroll
-        for (int q = 0; q < M1; ++q)
+            for (int q = 0; q < M1; ++q)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) o[(4 * kq + u + 16 * q) * L2 + 16 * w + ci] = float2{zr[q][u], zi[q][u]};
+                for (int u = 0; u < 4; ++u) o[(4 * kq + u + 16 * q) * H + k2] = float2{zr[q][u], zi[q][u]};
+        }
     }
 }
 
@@ -874,22 +884,72 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const float* __restrict_
 // fold): b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K); S_c = conj(b^alpha) b^beta.
 constexpr int kContractBlock = 64;  // one wave per workgroup
 
+// full index K of kept index kh (SsfState::H)
+template <typename Real>
+__device__ __forceinline__ int full_k(const SsfState<Real>& st, int kh) {
+    const int l = kh / st.H;
+    return l * st.L2 + (kh - l * st.H);
+}
+
+// -K of a full index K
+template <typename Real>
+__device__ __forceinline__ int neg_full_k(const SsfState<Real>& st, int K) {
+    const int l = K / st.L2, k2 = K - l * st.L2;
+    const int k0 = l / st.L1, k1 = l - k0 * st.L1;
+    return ((k0 ? st.L0 - k0 : 0) * st.L1 + (k1 ? st.L1 - k1 : 0)) * st.L2 + (k2 ? st.L2 - k2 : 0);
+}
+
+// Whether the partner -K of kept index kh lies outside the kept half with a
+// Nyquist index on axis 0 or 1 (k0 = L0 / 2 or k1 = L1 / 2): there the
+// centred q of -K is not minus that of K, so S(-K) is not conj(S(K)) and is
+// accumulated directly, from A(-K) = conj(A(K)) folded with the phases of
+// -K.  The other K outside the kept half are filled in as conj(S(-K)) when
+// read.
+template <typename Real>
+__device__ __forceinline__ bool mirror_direct(const SsfState<Real>& st, int kh) {
+    const int l = kh / st.H, k2 = kh - l * st.H;
+    if (k2 < 1 || k2 > st.L2 - st.H) return false;
+    const int k0 = l / st.L1, k1 = l - k0 * st.L1;
+    return 2 * k0 == st.L0 || 2 * k1 == st.L1;
+}
+
+// S_c(K) += conj(b^al) b^be (and the squares of its parts) for the
+// correlators of one (slot, K).
+template <typename Real>
+__device__ __forceinline__ void add_correlators(const CorrArgs& c, int slot, size_t nc, int K, const Real (&br)[3],
+                                                const Real (&bi)[3]) {
+    for (int cc = 0; cc < c.n_corr; ++cc) {
+        const int al = c.alpha[cc], be = c.beta[cc];
+        const double re = double(br[al] * br[be] + bi[al] * bi[be]);
+        const double im = double(br[al] * bi[be] - bi[al] * br[be]);
+        double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+        A[0] += re;
+        A[1] += im;
+        if (c.acc2) {
+            double* A2 = c.acc2 + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+            A2[0] += re * re;
+            A2[1] += im * im;
+        }
+    }
+}
+
 template <typename Real>
 __global__ __launch_bounds__(kContractBlock) void contract_kernel(SsfState<Real> st, CorrArgs c,
                                                                   int r_begin) {
-    const int K = blockIdx.x * kContractBlock + threadIdx.x;
+    const int kh = blockIdx.x * kContractBlock + threadIdx.x;
     const int rl = blockIdx.y;
     const int r = r_begin + rl;
     const int slot = c.slot_of[r];
     if (slot < 0) return;
-    if (K < st.n_cells) {
+    if (kh < st.n_kh) {
+        const int K = full_k(st, kh);
         Real br[3], bi[3];
-        const size_t nc = size_t(st.n_cells);
+        const size_t nc = size_t(st.n_cells), nh = size_t(st.n_kh);
         for (int a = 0; a < 3; ++a) {
             Real sr = Real(0), si = Real(0);
-            const Real* F = st.F + 2 * ((size_t(rl) * 3 + a) * st.n_sl) * nc;
+            const Real* F = st.F + 2 * ((size_t(rl) * 3 + a) * st.n_sl) * nh;
             for (int mu = 0; mu < st.n_sl; ++mu) {
-                const Real fr = F[2 * (mu * nc + K)], fi = F[2 * (mu * nc + K) + 1];
+                const Real fr = F[2 * (mu * nh + kh)], fi = F[2 * (mu * nh + kh) + 1];
                 const Real pr = st.phase[2 * (mu * nc + K)], pi = st.phase[2 * (mu * nc + K) + 1];
                 sr += fr * pr - fi * pi;
                 si += fr * pi + fi * pr;
@@ -897,22 +957,27 @@ __global__ __launch_bounds__(kContractBlock) void contract_kernel(SsfState<Real>
             br[a] = sr;
             bi[a] = si;
         }
-        for (int cc = 0; cc < c.n_corr; ++cc) {
-            const int al = c.alpha[cc], be = c.beta[cc];
-            const double re = double(br[al] * br[be] + bi[al] * bi[be]);
-            const double im = double(br[al] * bi[be] - bi[al] * br[be]);
-            double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
-            A[0] += re;
-            A[1] += im;
-            if (c.acc2) {
-                double* A2 = c.acc2 + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
-                A2[0] += re * re;
-                A2[1] += im * im;
+        add_correlators(c, slot, nc, K, br, bi);
+        if (mirror_direct(st, kh)) {
+            const int Kn = neg_full_k(st, K);
+            for (int a = 0; a < 3; ++a) {
+                Real sr = Real(0), si = Real(0);
+                const Real* F = st.F + 2 * ((size_t(rl) * 3 + a) * st.n_sl) * nh;
+                for (int mu = 0; mu < st.n_sl; ++mu) {
+                    const Real fr = F[2 * (mu * nh + kh)], fi = -F[2 * (mu * nh + kh) + 1];
+                    const Real pr = st.phase[2 * (mu * nc + Kn)], pi = st.phase[2 * (mu * nc + Kn) + 1];
+                    sr += fr * pr - fi * pi;
+                    si += fr * pi + fi * pr;
+                }
+                br[a] = sr;
+                bi[a] = si;
             }
+            add_correlators(c, slot, nc, Kn, br, bi);
         }
     }
-    if (K == 0) c.n_samples[slot] += 1ull;
+    if (kh == 0) c.n_samples[slot] += 1ull;
 }
+
 
 // ---------------------------------------------------------------------------
 // Sublattice-resolved correlators C^{ab}_{mu nu}(K) = sum conj(A^a_mu) A^b_nu,
@@ -940,7 +1005,7 @@ __global__ __launch_bounds__(kBlock) void munu_stage_kernel(SsfState<Real> st,
     if (slot < 0) return;
     const unsigned int b = cnt[slot];
     if (b >= unsigned(Bmax)) return;
-    const int nc = st.n_cells;
+    const int nc = st.n_kh;  // rows of the kept K
     const int K0 = int(blockIdx.x) * 64;
     const int nf = 3 * st.n_sl;
     const int kl = threadIdx.x & 63;
@@ -981,7 +1046,7 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
     const int rl = blockIdx.y;
     const int slot = c.slot_of[r_begin + rl];
     if (slot < 0) return;
-    const int nc = st.n_cells;
+    const int nc = st.n_kh;  // kept K (fields and staged rows); accumulators by full K
     const int K0 = int(blockIdx.x) * 64;
     const int ns = st.n_sl, nf = 3 * ns;
     const int kl = threadIdx.x & 63;
@@ -1007,33 +1072,51 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
             dst[e] = xs[f][k];
         }
     }
-    const int K = K0 + kl;
+    const int kh = K0 + kl;
+    const int K = kh < nc ? full_k(st, kh) : 0;
+    const bool mir = kh < nc && mirror_direct(st, kh);
+    const int Kn = mir ? neg_full_k(st, K) : 0;
+    const size_t nfull = size_t(st.n_cells);
     const int w = int(threadIdx.x >> 6);
-    __shared__ float bsh[2][3][64];
-    if (w < 3 && K < nc) {
+    __shared__ float bsh[2][2][3][64];  // [K, -K][re, im][component][k]
+    if (w < 3 && kh < nc) {
         float sr = 0.f, si = 0.f;
         for (int mu = 0; mu < ns; ++mu) {
             const float fr = xs[w * ns + mu][kl], fi = xs[16 * T + w * ns + mu][kl];
-            const float pr = st.phase[2 * (mu * size_t(nc) + K)], pi = st.phase[2 * (mu * size_t(nc) + K) + 1];
+            const float pr = st.phase[2 * (mu * nfull + K)], pi = st.phase[2 * (mu * nfull + K) + 1];
             sr += fr * pr - fi * pi;
             si += fr * pi + fi * pr;
         }
-        bsh[0][w][kl] = sr;
-        bsh[1][w][kl] = si;
+        bsh[0][0][w][kl] = sr;
+        bsh[0][1][w][kl] = si;
+        if (mir) {  // A(-K) = conj(A(K)) with the phases of -K
+            sr = 0.f;
+            si = 0.f;
+            for (int mu = 0; mu < ns; ++mu) {
+                const float fr = xs[w * ns + mu][kl], fi = -xs[16 * T + w * ns + mu][kl];
+                const float pr = st.phase[2 * (mu * nfull + Kn)], pi = st.phase[2 * (mu * nfull + Kn) + 1];
+                sr += fr * pr - fi * pi;
+                si += fr * pi + fi * pr;
+            }
+            bsh[1][0][w][kl] = sr;
+            bsh[1][1][w][kl] = si;
+        }
     }
     __syncthreads();
-    if (K < nc) {
-        const float br[3] = {bsh[0][0][kl], bsh[0][1][kl], bsh[0][2][kl]};
-        const float bi[3] = {bsh[1][0][kl], bsh[1][1][kl], bsh[1][2][kl]};
+    for (int m = 0; m < (mir ? 2 : 1); ++m) {
+        if (kh >= nc) break;
+        const int Kw = m ? Kn : K;
+        const float br[3] = {bsh[m][0][0][kl], bsh[m][0][1][kl], bsh[m][0][2][kl]};
+        const float bi[3] = {bsh[m][1][0][kl], bsh[m][1][1][kl], bsh[m][1][2][kl]};
         for (int cc = w; cc < c.n_corr; cc += kBlock / 64) {
             const int al = c.alpha[cc], be = c.beta[cc];
             const double re = double(br[al] * br[be] + bi[al] * bi[be]);
             const double im = double(br[al] * bi[be] - bi[al] * br[be]);
-            double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+            double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nfull + Kw);
             A[0] += re;
             A[1] += im;
             if (c.acc2) {
-                double* A2 = c.acc2 + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
+                double* A2 = c.acc2 + 2 * ((size_t(slot) * c.n_corr + cc) * nfull + Kw);
                 A2[0] += re * re;
                 A2[1] += im * im;
             }
@@ -1337,8 +1420,9 @@ hipError_t launch_zero(void* p, size_t n_words32, hipStream_t s) {
 
 // Whole 3-D DFT of one (replica, component, sublattice) field in LDS for
 // cubic lattices of edge L = 4, 8, 12 or 16.  The field (L^3 reals) is read
-// once, transformed along axis 2, 1, 0 in place and written once as
-// F[r][a][mu][K], the layout of the per-axis path.  A thread owns one line
+// once, transformed along axis 2, 1, 0 in place and written once as the
+// kept half F[r][a][mu][kh] (k2 < L / 2 + 1; the other K are the conjugates
+// of their partners -K, since the field is real).  A thread owns one line
 // per pass: it keeps the line's L values in registers, transforms them with
 // dft_line (one mixed-radix step over 4-point DFTs, compile-time twiddles;
 // the first pass on a real line), then writes them back over its own line,
@@ -1477,10 +1561,11 @@ template <typename Real, int L>
 __global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __restrict__ spins,
                                                                  Real* __restrict__ F,
                                                                  const Real* __restrict__ tw, int n_sl) {
-    constexpr int NC = L * L * L, LINES = L * L, BS = dft3_threads<L>();
+    // passes 1 and 2 and the output cover the kept half k2 < H only
+    constexpr int NC = L * L * L, H = L / 2 + 1, NCH = L * L * H, BS = dft3_threads<L>();
     __shared__ Real x[2 * NC];
     (void)tw;  // twiddles are compile-time constants (dft_line)
-    const size_t f_off = (size_t(blockIdx.y) * 3 * n_sl + size_t(blockIdx.x)) * size_t(NC);
+    const size_t f_off = (size_t(blockIdx.y) * 3 * n_sl + size_t(blockIdx.x)) * size_t(NCH);
     // field (component a, sublattice mu) of spins S[r][i][a]: every third
     // value from 3 (mu NC) + a; the three components' workgroups are n_sl
     // apart in the grid, on one XCD when n_sl is a multiple of 8
@@ -1493,9 +1578,13 @@ __global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __r
     __syncthreads();
 #pragma unroll
     for (int pass = 0; pass < 3; ++pass) {
+        // pass 0: rows (n0, n1) along n2; pass 1: (n0, k2 < H) along n1;
+        // pass 2: (k1, k2 < H) along n0
         const int sa = pass == 0 ? 1 : (pass == 1 ? L : L * L);
-        for (int line = threadIdx.x; line < LINES; line += BS) {
-            const int base = (line / sa) * (sa * L) + line % sa;
+        const int lines = pass == 0 ? L * L : L * H;
+        for (int line = threadIdx.x; line < lines; line += BS) {
+            const int base = pass == 0 ? line * L
+                                       : (pass == 1 ? (line / H) * (L * L) + line % H : (line / H) * L + line % H);
             Real xr[L], xi[L];
 #pragma unroll
             for (int j = 0; j < L; ++j) {
@@ -1514,7 +1603,11 @@ __global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __r
         __syncthreads();
     }
     Real* dst = F + 2 * f_off;
-    for (int i = threadIdx.x; i < 2 * NC; i += BS) dst[i] = x[i];
+    for (int e = threadIdx.x; e < NCH; e += BS) {
+        const int src = (e / H) * L + e % H;
+        dst[2 * e] = x[2 * src];
+        dst[2 * e + 1] = x[2 * src + 1];
+    }
 }
 
 // One axis of the per-sublattice DFT: the LDS line kernel for La <= 64,
@@ -1579,12 +1672,27 @@ hipError_t dft_axis(const Real* in, Real* out, const Real* tw, int La, int strid
     return go(std::integral_constant<int, 16>{});
 }
 
+static bool dft3_applies(int L0, int L1, int L2) {
+    return L0 == L1 && L1 == L2 && (L0 == 4 || L0 == 8 || L0 == 12 || L0 == 16);
+}
+static bool ct_length(int L) { return L == 32 || L == 48 || L == 64; }
+
+int dft_kept_k2(int real_bytes, int L0, int L1, int L2, long long fields_per_replica_total) {
+    if (dft3_applies(L0, L1, L2)) return L2 / 2 + 1;
+    if (real_bytes == 4 && ct_length(L1) && ct_length(L2) && fields_per_replica_total < (1ll << 30))
+        return L2 / 2 + 1;
+    return L2;
+}
+
 template <typename Real>
 hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStream_t s) {
     const size_t nc = size_t(st.n_cells);
     const long long total = (long long)r_count * 3 * st.n_sl * (long long)nc;
     const Real* spins = st.S + size_t(r_begin) * 3 * st.n_sl * nc;
-    if (st.L0 == st.L1 && st.L1 == st.L2) {
+    if (st.H != dft_kept_k2(int(sizeof(Real)), st.L0, st.L1, st.L2, 3ll * st.n_sl * (long long)nc) ||
+        st.n_kh != st.L0 * st.L1 * st.H)
+        return hipErrorInvalidValue;
+    if (dft3_applies(st.L0, st.L1, st.L2)) {
         const dim3 grid(unsigned(3 * st.n_sl), unsigned(r_count));
         auto go = [&](auto l) -> hipError_t {
             constexpr int LL = decltype(l)::value;
@@ -1601,14 +1709,16 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
         }
     }
     if constexpr (std::is_same_v<Real, float>) {
-        // axes 2 and 1 of every plane in one pass (spins -> G), then axis 0: G -> F
+        // axes 2 and 1 of every plane in one pass (spins -> G, the kept half
+        // k2 < H), then axis 0 over those columns: G -> F
         auto plane = [&](auto m1, auto m2) -> hipError_t {
             constexpr int M1 = decltype(m1)::value, M2 = decltype(m2)::value;
             const long long planes = total / ((long long)st.L1 * st.L2);
             hipLaunchKernelGGL((dft_plane_kernel<M1, M2>), dim3(unsigned(planes)), dim3(256), 0, s, spins, st.G,
                                st.tw1, st.tw2, st.n_sl, st.L0);
             hipError_t e = hipGetLastError();
-            if (e == hipSuccess) e = dft_axis<Real, false>(st.G, st.F, st.tw0, st.L0, st.L1 * st.L2, total, s);
+            const long long total_h = planes * st.L1 * st.H;
+            if (e == hipSuccess) e = dft_axis<Real, false>(st.G, st.F, st.tw0, st.L0, st.L1 * st.H, total_h, s);
             return e;
         };
         auto m2 = [&](auto m1) -> hipError_t {
@@ -1618,8 +1728,7 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
                 default: return plane(m1, std::integral_constant<int, 4>{});
             }
         };
-        const bool ct1 = st.L1 == 32 || st.L1 == 48 || st.L1 == 64, ct2 = st.L2 == 32 || st.L2 == 48 || st.L2 == 64;
-        if (ct1 && ct2 && total < (1ll << 30)) {
+        if (st.H < st.L2) {  // the plane path (dft_kept_k2)
             switch (st.L1) {
                 case 32: return m2(std::integral_constant<int, 2>{});
                 case 48: return m2(std::integral_constant<int, 3>{});
@@ -1627,6 +1736,7 @@ hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStr
             }
         }
     }
+    if (st.H != st.L2) return hipErrorInvalidValue;
     // spins S[r][i][a] -> real fields G[r][a][mu][cell]; axis 2 (stride 1):
     // G -> F; axis 1 (stride L2): F -> G; axis 0: G -> F
     hipLaunchKernelGGL((deinterleave_kernel<Real>), dim3(unsigned((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
@@ -1644,7 +1754,7 @@ hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_be
     if constexpr (std::is_same_v<Real, float>) {
         // FP32: fields through LDS, the fold split across waves
         const int T = (3 * st.n_sl + 15) / 16;
-        const dim3 g64(unsigned((st.n_cells + 63) / 64), unsigned(r_count));
+        const dim3 g64(unsigned((st.n_kh + 63) / 64), unsigned(r_count));
         switch (T) {
             case 1: hipLaunchKernelGGL((contract_stage_kernel<1, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
             case 2: hipLaunchKernelGGL((contract_stage_kernel<2, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
@@ -1652,7 +1762,7 @@ hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_be
             default: break;
         }
     }
-    const dim3 grid(unsigned((st.n_cells + kContractBlock - 1) / kContractBlock), unsigned(r_count));
+    const dim3 grid(unsigned((st.n_kh + kContractBlock - 1) / kContractBlock), unsigned(r_count));
     hipLaunchKernelGGL(contract_kernel<Real>, grid, dim3(kContractBlock), 0, s, st, c, r_begin);
     return hipGetLastError();
 }
@@ -1660,7 +1770,7 @@ hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_be
 template <typename Real>
 hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
                              int r_count, const MunuArgs& m, hipStream_t s) {
-    const dim3 grid(unsigned((st.n_cells + 63) / 64), unsigned(r_count));
+    const dim3 grid(unsigned((st.n_kh + 63) / 64), unsigned(r_count));
     switch (m.T) {
         case 1: hipLaunchKernelGGL((munu_stage_kernel<Real, 1>), grid, dim3(kBlock), 0, s, st, slot_of, r_begin, m.buf, m.cnt, m.Bmax); break;
         case 2: hipLaunchKernelGGL((munu_stage_kernel<Real, 2>), grid, dim3(kBlock), 0, s, st, slot_of, r_begin, m.buf, m.cnt, m.Bmax); break;
@@ -1674,7 +1784,7 @@ hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, i
 
 hipError_t launch_contract_stage(const SsfState<float>& st, const CorrArgs& c, int r_begin, int r_count,
                                  const MunuArgs& m, hipStream_t s) {
-    const dim3 grid(unsigned((st.n_cells + 63) / 64), unsigned(r_count));
+    const dim3 grid(unsigned((st.n_kh + 63) / 64), unsigned(r_count));
     switch (m.T) {
         case 1: hipLaunchKernelGGL((contract_stage_kernel<1, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
         case 2: hipLaunchKernelGGL((contract_stage_kernel<2, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;

@@ -213,18 +213,31 @@ hipError_t launch_reduce_accepts(const unsigned int* acc_wave, int n, unsigned l
 // Static structure factor.
 // ---------------------------------------------------------------------------
 template <typename Real>
+// The fields are real, so A(-K) = conj(A(K)): the fast DFT paths keep only
+// k2 < H = L2 / 2 + 1 (the kept half, n_kh = L0 L1 H values per field, kept
+// index kh = (k0 L1 + k1) H + k2); H = L2 keeps every K.  Accumulators of the
+// contracted correlators are indexed by the full K: the contraction writes
+// the kept K and, where the centred q of -K is not minus that of K (a
+// Nyquist index on axis 0 or 1), -K too; the rest is conj(S(-K)) on read.
 struct SsfState {
     const Real* S = nullptr;         // [R][N][3]
-    Real* F = nullptr;               // [R][3][n_sl][n_cells] complex (2 Real)
+    Real* F = nullptr;               // [R][3][n_sl][n_kh] complex (2 Real)
     Real* G = nullptr;               // same, scratch
     const Real* tw0 = nullptr;       // twiddles per axis, [L_a] complex
     const Real* tw1 = nullptr;
     const Real* tw2 = nullptr;
-    const Real* phase = nullptr;     // [n_sl][n_cells] complex e^{-i q(K).r_mu}
+    const Real* phase = nullptr;     // [n_sl][n_cells] complex e^{-i q(K).r_mu}, full K
     int R = 0, n_sl = 0, n_cells = 0, L0 = 0, L1 = 0, L2 = 0;
+    int H = 0, n_kh = 0;             // kept k2 per (k0, k1) line, kept values per field
 };
 
-// Per-sublattice DFT of all replicas' spins: F[r][a][mu][K] (complex).
+// k2 values per (k0, k1) line that launch_dft produces for a lattice (L2 / 2
+// + 1 on the whole-field and plane paths, else L2); fields_per_replica_total
+// = 3 n_sl n_cells.
+int dft_kept_k2(int real_bytes, int L0, int L1, int L2, long long fields_per_replica_total);
+
+// Per-sublattice DFT of all replicas' spins: F[r][a][mu][kh] (complex), the
+// kept K of st.H.
 template <typename Real>
 hipError_t launch_dft(const SsfState<Real>& st, int r_begin, int r_count, hipStream_t s);
 
@@ -258,7 +271,7 @@ struct MunuArgs {
     float* buf = nullptr;
     unsigned int* cnt = nullptr;   // [n_slots]
     double* acc = nullptr;
-    int Bmax = 0, n_k = 0, n_slots = 0, n_f = 0, T = 0;
+    int Bmax = 0, n_k = 0, n_slots = 0, n_f = 0, T = 0;  // n_k: the kept K (SsfState::n_kh)
 };
 template <typename Real>
 hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
@@ -268,6 +281,7 @@ hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, i
 hipError_t launch_contract_stage(const SsfState<float>& st, const CorrArgs& c, int r_begin, int r_count,
                                  const MunuArgs& m, hipStream_t s);
 hipError_t launch_munu_flush(const MunuArgs& m, hipStream_t s);
+
 
 // Luttinger-Tisza spectrum over the L[0] x L[1] x L[2] grid of k with
 // q_a = 2 pi K'_a / (cell_a L_a), K' the centred index: builds the
