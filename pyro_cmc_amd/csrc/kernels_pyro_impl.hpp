@@ -648,14 +648,19 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
 
     load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
     int8_t* const dir_r = st.dir + size_t(r) * N;
+    // the interior cell mi = threadIdx.x (< NI) whose 16 sites this thread
+    // stages directions for and writes back, and its lattice cell
+    const int mi = int(threadIdx.x);
+    const bool has_mi = mi < NI;
+    const int mi2 = mi % T2, mi1 = (mi / T2) % T1, mi0 = mi / (T1 * T2);
+    const int cell_mi = ((o0 + mi0) * L1 + (o1 + mi1)) * L2 + (o2 + mi2);
     if constexpr (LIFTED) {
         for (int w = threadIdx.x; w < 16 * W; w += BS) ldw[w] = 0u;
         __syncthreads();
-        for (int e = threadIdx.x; e < 16 * NI; e += BS) {
-            const int sl = e / NI, m = e % NI;
-            const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
-            if (dir_r[sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2)] < 0)
-                atomicOr(ldw + sl * W + (m >> 5), 1u << (m & 31));
+        if (has_mi) {
+#pragma unroll
+            for (int sl = 0; sl < 16; ++sl)
+                if (dir_r[sl * nc + cell_mi] < 0) atomicOr(ldw + sl * W + (mi >> 5), 1u << (mi & 31));
         }
     }
     __syncthreads();
@@ -664,6 +669,11 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const ReplicaParams<Real> rp = replica_params(st, r);
     static_assert(NI <= BS, "one tile cell per thread");
     const int my_cell = kTileLanes<T0, T1, T2, BS>.cell[threadIdx.x];
+    // the cell this lane updates: its box index (before the sublattice's
+    // shift) and lattice cell
+    const int c2 = my_cell % T2, c1 = (my_cell / T2) % T1, c0 = my_cell / (T1 * T2);
+    const int my_lc = (c0 * B1 + c1) * B2 + c2;
+    const int my_gc = ((o0 + c0) * L1 + (o1 + c1)) * L2 + (o2 + c2);
     unsigned int my_acc = 0;
     if constexpr (FUSED) {
         for (int col = 0; col < NCOL; ++col) {
@@ -673,8 +683,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                 const int own = ((f == 2 || f == 3) * B1 + (f == 1 || f == 3)) * B2 + (f == 1 || f == 2);
                 if (my_cell >= 0) {
                     const int m = my_cell;
-                    const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
-                    const int lc = (i0 * B1 + i1) * B2 + i2;
+                    const int lc = my_lc;
                     Real hx, hy, hz;
                     switch (sl) {
 #define PYRO_SL_CASE(X) \
@@ -686,7 +695,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
 #undef PYRO_SL_CASE
                         default: hx = hy = hz = Real(0);
                     }
-                    const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
+                    const int site = sl * nc + my_gc;
                     const int li = sl * BC + lc + own;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool any = false, ch;
@@ -731,8 +740,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                 const int own = ((f == 2 || f == 3) * B1 + (f == 1 || f == 3)) * B2 + (f == 1 || f == 2);
                 if (my_cell >= 0) {
                     const int m = my_cell;
-                    const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
-                    const int lc = (i0 * B1 + i1) * B2 + i2;
+                    const int lc = my_lc;
                     // the field in a block of its own: drawing the uniforms
                     // next to the LDS gathers raises the register count past
                     // what three workgroups per CU allow
@@ -747,7 +755,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
 #undef PYRO_SL_CASE
                         default: hx = hy = hz = Real(0);
                     }
-                    const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
+                    const int site = sl * nc + my_gc;
                     const int li = sl * BC + lc + own;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool ch;
@@ -778,15 +786,17 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
         }
     }
 
-    for (int e = threadIdx.x; e < 16 * NI; e += BS) {
-        const int sl = e / NI, m = e % NI;
-        const int i2 = m % T2, i1 = (m / T2) % T1, i0 = m / (T1 * T2);
-        const int f = sl >> 2;
-        const int li = sl * BC + ((i0 + (f == 2 || f == 3)) * B1 + (i1 + (f == 1 || f == 3))) * B2 +
-                       (i2 + (f == 1 || f == 2));
-        const int site = sl * nc + ((o0 + i0) * L1 + (o1 + i1)) * L2 + (o2 + i2);
-        st3(Sr, unsigned(site), lx[li], ly[li], lz[li]);
-        if constexpr (LIFTED) dir_r[site] = ((ldw[sl * W + (m >> 5)] >> (m & 31)) & 1u) ? int8_t(-1) : int8_t(1);
+    if (has_mi) {
+#pragma unroll
+        for (int sl = 0; sl < 16; ++sl) {
+            const int f = sl >> 2;
+            const int li = sl * BC + ((mi0 + (f == 2 || f == 3)) * B1 + (mi1 + (f == 1 || f == 3))) * B2 +
+                           (mi2 + (f == 1 || f == 2));
+            const int site = sl * nc + cell_mi;
+            st3(Sr, unsigned(site), lx[li], ly[li], lz[li]);
+            if constexpr (LIFTED)
+                dir_r[site] = ((ldw[sl * W + (mi >> 5)] >> (mi & 31)) & 1u) ? int8_t(-1) : int8_t(1);
+        }
     }
 
     if (st.acc == nullptr) return;  // uniform: counts not requested
