@@ -497,8 +497,12 @@ def main():
             "bytes_per_update": b_alg,
             "updates_per_launch": upd_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,
+            # the HIP-event sample avg_launch_us comes from
+            "launch_timing": {"sweeps": n_sw, "launches": sweep_launches, "elapsed_ms": sweep_ms,
+                              "launches_per_round": sweep_launches // n_sw * N_SWEEP},
         },
         "kernel_ms": {k: v[0] for k, v in prof.items()},
+        "kernel_profiled_rounds": prof_steps,
         "kernel_launches": {k: v[1] for k, v in prof.items()},
         # 64 log-spaced rungs over two decades at 27,648 spins: adjacent rungs'
         # energy distributions barely overlap, so swaps are rare (about 1%

@@ -4,6 +4,7 @@
     python -m pyro_cmc_amd.rocprof_summary counter <results.db> <COUNTER>
     python -m pyro_cmc_amd.rocprof_summary traffic <fetch.db> <write.db> --kernel REGEX
            --tag TAG [--fetch-scale F] [--write-scale W]
+    python -m pyro_cmc_amd.rocprof_summary events <bench output>
 
 `trace` prints per-kernel call count, total and average duration (us) from
 a `rocprofv3 --kernel-trace --stats` run; `counter` prints the per-kernel
@@ -12,10 +13,16 @@ mean of one PMC counter per dispatch from a `rocprofv3 --pmc` run.
 into HBM bytes per launch of the kernels matching REGEX (dispatch-weighted
 mean), each counter scaled by what it reads for a plain copy of known size,
 and prints "traffic tag=TAG bytes_per_launch=B".
+`events` reads the JSON line `bench.py` prints and tabulates its HIP-event
+timing of the headline round: the sweep kernel's back-to-back launch sample
+(the source of roofline.avg_launch_us and frac), the launches per round
+against the measured round time, and the per-family event times of the
+profiled rounds.
 """
 from __future__ import annotations
 
 import argparse
+import json
 import re
 import sqlite3
 
@@ -67,9 +74,44 @@ def traffic(fetch_db: str, write_db: str, kernel: str, tag: str, fetch_scale: fl
             f"traffic tag={tag} bytes_per_launch={total:.0f}")
 
 
+def events(path: str) -> str:
+    line = None
+    for text in open(path):
+        text = text.strip()
+        if text.startswith("{") and '"metric"' in text:
+            line = json.loads(text)
+    if line is None:
+        raise SystemExit(f"no bench.py result line in {path}")
+    rf = line["roofline"]
+    lt = rf["launch_timing"]
+    avg_us = 1e3 * lt["elapsed_ms"] / lt["launches"]
+    per_round = lt["launches_per_round"]
+    gbs = rf["bytes_per_update"] * rf["updates_per_launch"] / (avg_us * 1e-6) / 1e9
+    out = [
+        f"sweep kernel {rf['kernel']}: {lt['sweeps']} sweeps back to back = {lt['launches']} launches in "
+        f"{lt['elapsed_ms']:.3f} ms (HIP events on the library stream) -> {avg_us:.3f} us per launch",
+        f"round: {per_round} sweep launches x {avg_us:.3f} us = {per_round * avg_us / 1e3:.4f} ms of the measured "
+        f"{line['ms_per_step']:.4f} ms per round (wall clock over {line['steps']} rounds)",
+        f"roofline: {rf['bytes_per_update']} B x {rf['updates_per_launch']:.0f} updates / {avg_us:.3f} us = "
+        f"{gbs:.1f} GB/s = {gbs / rf['peak']:.4f} of {rf['peak']:.0f} GB/s",
+        "",
+        f"per-family HIP-event times over {line.get('kernel_profiled_rounds', '?')} profiled rounds "
+        "(events around each family's launches):",
+        f"{'family':<12} {'launches':>10} {'total_ms':>12} {'avg_us':>10} {'us/round':>10}",
+    ]
+    n_rounds = line.get("kernel_profiled_rounds")
+    for fam, ms in line["kernel_ms"].items():
+        n = line["kernel_launches"][fam]
+        if not n:
+            continue
+        per = f"{1e3 * ms / n_rounds:10.2f}" if n_rounds else f"{'':>10}"
+        out.append(f"{fam:<12} {n:>10} {ms:>12.3f} {1e3 * ms / n:>10.3f} {per}")
+    return "\n".join(out)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counter", "traffic"])
+    ap.add_argument("mode", choices=["trace", "counter", "traffic", "events"])
     ap.add_argument("db")
     ap.add_argument("arg", nargs="?", help="COUNTER (counter mode) or the WRITE_SIZE db (traffic mode)")
     ap.add_argument("--top", type=int, default=20)
@@ -82,6 +124,8 @@ def main() -> None:
         print(trace(a.db, a.top))
     elif a.mode == "counter":
         print(counter(a.db, a.arg))
+    elif a.mode == "events":
+        print(events(a.db))
     else:
         print(traffic(a.db, a.arg, a.kernel, a.tag, a.fetch_scale, a.write_scale))
 
