@@ -293,6 +293,33 @@ def test_autotuner_keeps_results(gpu, oracle, stencil):
     assert out[0][2] == out[1][2]
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_autotuner_keeps_energy_sums(gpu, prec):
+    """Tempering rounds and anneal steps take their energies from the last
+    colour pass under strategies 3 and 4 alike: the same partial per wave of
+    64 cells and sublattice, summed by the same finish kernel, so E, E^2,
+    the swaps and the anneal history are bit-identical whichever of the
+    pair the tuner picks (L = 12: 27 partials per sublattice)."""
+    out = []
+    for strategy in (0, 3, 4):
+        ctx = gpu.Context.pyrochlore(12, n_replicas=8, precision=prec, seed=5)
+        ctx.define_J1J2J3(1.0, -0.05, 0.05)
+        ctx.setup()
+        ctx.set_strategy(strategy)
+        ctx.randomize(2)
+        ctx.pt_init(np.geomspace(2.0, 0.2, 8), 3)
+        ctx.pt_rounds(gpu.METROPOLIS, 2, 2, False, 6)
+        st = ctx.pt_stats()
+        ctx.anneal_schedule(np.geomspace(1.0, 0.1, 5))
+        ctx.anneal_steps(gpu.LIFTED, 1, 2, False, 5)
+        out.append((st["E"], st["E2"], st["accepts"], ctx.pt_permutation(), ctx.anneal_history(),
+                    ctx.get_spins(7)))
+        ctx.close()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("prec", [64, 32])
 def test_tempering_rounds(gpu, oracle, prec):
     L, R, swap_seed = 2, 6, 4242
