@@ -47,10 +47,7 @@ SOURCES = [
     "topology.cpp",
 ]
 
-# rocSOLVER's batched Hermitian eigensolver (and the rocBLAS handle it runs
-# on) for the Luttinger-Tisza spectrum.
-ROCM_LIB = Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib"
-LINK_LIBS = [f"-L{ROCM_LIB}", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM_LIB}"]
+LINK_LIBS: list[str] = []
 
 
 def _run(cmd: list[str]) -> None:

@@ -286,7 +286,9 @@ hipError_t launch_munu_flush(const MunuArgs& m, hipStream_t s);
 // Luttinger-Tisza spectrum over the L[0] x L[1] x L[2] grid of k with
 // q_a = 2 pi K'_a / (cell_a L_a), K' the centred index: builds the
 // (3 n_sl)^2 Hermitian M(k) = sum_bonds J e^{i q.d} on the device and
-// solves it with rocSOLVER.  Writes lambda_min[k] (flat order
+// finds its lowest eigenvalue there (Householder tridiagonalisation and
+// Sturm multisection per k); the eigenvector at k* is solved on the host.
+// n_sl must be 16.  Writes lambda_min[k] (flat order
 // (K0 L1 + K1) L2 + K2), the first k of the strict minimum into k_star and
 // the lowest eigenvector there into eigvec[3 n_sl][2].  Returns an empty
 // string or the error.

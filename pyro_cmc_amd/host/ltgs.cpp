@@ -4,8 +4,9 @@
 // eigenvalue_map [Nk], k_star_idx int64[3], k_star_vec [3], the scalar
 // E_min (= lambda_min / 2, the energy per spin), eigenvector [48][2],
 // /geometry {index_cell, recip_vectors} and the root attribute k_dims.  The
-// spectrum comes from pyro_lt_spectrum (device-built M(k), batched
-// rocSOLVER eigensolve) instead of a per-k Eigen loop.
+// spectrum comes from pyro_lt_spectrum (one workgroup per k builds M(k),
+// tridiagonalises it and finds its lowest eigenvalue on the device) instead
+// of a per-k Eigen loop.
 #include <cmath>
 #include <cstdio>
 #include <iostream>
