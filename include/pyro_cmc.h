@@ -339,7 +339,8 @@ int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pas
  * every launch is bracketed by events and its duration is added to the
  * counter of its kernel family:
  *   0 = sweep passes, 1 = energy, 2 = exchange, 3 = SSF transforms,
- *   4 = SSF contraction, 5 = sublattice-resolved staging of samples,
+ *   4 = SSF contraction (with the flush of buffered contracted samples),
+ *   5 = sublattice-resolved staging of samples,
  *   6 = sublattice-resolved MFMA products flushed into the accumulators.
  * ms[7], launches[7]. */
 int pyro_profile_enable(pyro_ctx* ctx, int on);

@@ -232,6 +232,7 @@ struct pyro_ctx {
     std::vector<int32_t> h_alpha, h_beta;
     DevMem F, G, tw0, tw1, tw2, phase, d_alpha, d_beta, sacc, sacc2, sn, ssf_slot;
     DevMem mbuf, mcnt, macc;
+    DevMem cbuf;               // buffered contracted correlators (FP32 with munu): [slot][b][c][n_kh]
     // The DFT keeps k2 < ssf_H of every (k0, k1) line (n_kh values per
     // field); the other K follow from -K (the fields are real).  The
     // contraction also accumulates the K outside the kept half with a
@@ -682,10 +683,22 @@ struct pyro_ctx {
         m.n_slots = n_slots;
         m.n_f = 3 * geom.n_sl;
         m.T = munu_T;
+        if (cbuf.p) {
+            m.cbuf = cbuf.as<float2>();
+            m.acc_s = sacc.as<double>();
+            m.acc_s2 = store2 ? sacc2.as<double>() : nullptr;
+            m.n_corr = n_corr;
+            m.H = ssf_H;
+            m.L2 = geom.L[2];
+            m.n_k_full = int(nc());
+        }
         return m;
     }
     void munu_flush() {
         if (!munu) return;
+        // the buffered contracted samples (timed with the contraction), then
+        // the sublattice-resolved products; the latter resets the row counts
+        if (cbuf.p) timed(kFamContract, [&] { return pyro::launch_corr_flush(munu_args(), stream); });
         timed(kFamMunuFlush, [&] { return pyro::launch_munu_flush(munu_args(), stream); });
         munu_pending = 0;
         std::fill(host_munu_count.begin(), host_munu_count.end(), 0);
@@ -1744,10 +1757,17 @@ int pyro_ssf_configure(pyro_ctx* ctx, int n_corr, const int32_t* alpha, const in
             ctx->macc.alloc(mbytes);
             HIPCHK(hipMemset(ctx->macc.p, 0, mbytes));
             ctx->host_munu_count.assign(n_slots, 0);
+            // FP32: the contraction buffers its products beside the rows and
+            // the flush adds them to the FP64 sums (no per-sample update)
+            if (ctx->prec == 32)
+                ctx->cbuf.alloc(sizeof(float) * 2 * size_t(n_slots) * ctx->munu_B * n_corr * ctx->n_kh);
+            else
+                ctx->cbuf.release();
         } else {
             ctx->mbuf.release();
             ctx->mcnt.release();
             ctx->macc.release();
+            ctx->cbuf.release();
         }
         HIPCHK(hipDeviceSynchronize());
         ctx->munu_pending = 0;
@@ -1837,6 +1857,7 @@ int pyro_ssf_read(pyro_ctx* ctx, int slot, double* S, double* S2, uint64_t* n_sa
         require(ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
         require(slot >= 0 && slot < ctx->n_slots, PYRO_E_INVALID, "slot index out of range");
         require(!S2 || ctx->store2, PYRO_E_STATE, "error term was not configured");
+        ctx->munu_flush();  // buffered contracted samples into the sums
         HIPCHK(hipStreamSynchronize(ctx->stream));
         const int64_t nc = ctx->nc();
         const size_t per = size_t(2) * ctx->n_corr * nc;

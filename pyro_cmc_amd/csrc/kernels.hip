@@ -920,8 +920,9 @@ __device__ __forceinline__ void add_correlators(const CorrArgs& c, int slot, siz
                                                 const Real (&bi)[3]) {
     for (int cc = 0; cc < c.n_corr; ++cc) {
         const int al = c.alpha[cc], be = c.beta[cc];
-        const double re = double(br[al] * br[be] + bi[al] * bi[be]);
-        const double im = double(br[al] * bi[be] - bi[al] * br[be]);
+        // explicit fma: every kernel rounds the product alike
+        const double re = double(fma(br[al], br[be], bi[al] * bi[be]));
+        const double im = double(fma(br[al], bi[be], -(bi[al] * br[be])));
         double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nc + K);
         A[0] += re;
         A[1] += im;
@@ -1037,10 +1038,18 @@ __global__ __launch_bounds__(kBlock) void munu_stage_kernel(SsfState<Real> st,
 // from LDS (b^a(K) = sum_mu e^{-i q.r_mu} A^a_mu(K), in the order of
 // contract_kernel) and every wave updates the correlators cc = w, w + 4, ...
 // of its slot.
+//
+// With cbuf (STAGE only) the products of the kept K go to the buffer as
+// FP32 pairs, cbuf[slot][b][c][kh] with b the row the staging writes, and
+// corr_flush_kernel adds them to the FP64 sums in sample order when the
+// rows are flushed: the same additions as the direct update, without an
+// FP64 read-modify-write per sample.  The directly accumulated partners -K
+// of the Nyquist planes are updated in place.
 template <int T, bool STAGE>
 __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> st, CorrArgs c, int r_begin,
                                                                 float* __restrict__ buf,
-                                                                const unsigned int* __restrict__ cnt, int Bmax) {
+                                                                const unsigned int* __restrict__ cnt, int Bmax,
+                                                                float2* __restrict__ cbuf) {
     constexpr int FW = 2 * 16 * T;
     __shared__ float xs[FW][65];  // [feature][k], padded against bank conflicts
     const int rl = blockIdx.y;
@@ -1103,6 +1112,7 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
         }
     }
     __syncthreads();
+    const bool buffered = STAGE && cbuf != nullptr && b < unsigned(Bmax);
     for (int m = 0; m < (mir ? 2 : 1); ++m) {
         if (kh >= nc) break;
         const int Kw = m ? Kn : K;
@@ -1110,8 +1120,14 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
         const float bi[3] = {bsh[m][1][0][kl], bsh[m][1][1][kl], bsh[m][1][2][kl]};
         for (int cc = w; cc < c.n_corr; cc += kBlock / 64) {
             const int al = c.alpha[cc], be = c.beta[cc];
-            const double re = double(br[al] * br[be] + bi[al] * bi[be]);
-            const double im = double(br[al] * bi[be] - bi[al] * br[be]);
+            const float rf = fmaf(br[al], br[be], bi[al] * bi[be]);
+            const float imf = fmaf(br[al], bi[be], -(bi[al] * br[be]));
+            if (buffered && m == 0) {
+                cbuf[((size_t(slot) * Bmax + b) * c.n_corr + cc) * size_t(nc) + kh] = float2{rf, imf};
+                continue;
+            }
+            const double re = double(rf);
+            const double im = double(imf);
             double* A = c.acc + 2 * ((size_t(slot) * c.n_corr + cc) * nfull + Kw);
             A[0] += re;
             A[1] += im;
@@ -1123,6 +1139,45 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
         }
     }
     if (K0 == 0 && threadIdx.x == 0) c.n_samples[slot] += 1ull;
+}
+
+// The buffered contracted samples b < cnt[slot] of every (slot, correlator,
+// kept K), added to the FP64 sums in sample order (the additions the direct
+// update makes, sample by sample).  One thread per (correlator, kept K).
+__global__ __launch_bounds__(kBlock) void corr_flush_kernel(const float2* __restrict__ cbuf,
+                                                            const unsigned int* __restrict__ cnt,
+                                                            double* __restrict__ acc, double* __restrict__ acc2,
+                                                            int n_corr, int n_kh, int H, int L2, int n_k, int Bmax) {
+    const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const int slot = int(blockIdx.y);
+    if (e >= (long long)n_corr * n_kh) return;
+    const int nb = int(cnt[slot]);
+    if (nb == 0) return;
+    const int cc = int(e / n_kh), kh = int(e - (long long)cc * n_kh);
+    const int l = kh / H, K = l * L2 + (kh - l * H);
+    const size_t at = 2 * ((size_t(slot) * n_corr + cc) * size_t(n_k) + K);
+    double a0 = acc[at], a1 = acc[at + 1];
+    double s0 = 0.0, s1 = 0.0;
+    if (acc2) {
+        s0 = acc2[at];
+        s1 = acc2[at + 1];
+    }
+    const float2* v = cbuf + (size_t(slot) * Bmax * n_corr + cc) * size_t(n_kh) + kh;
+    const size_t step = size_t(n_corr) * n_kh;
+    for (int b = 0; b < nb; ++b) {
+        const float2 x = v[b * step];
+        const double re = double(x.x), im = double(x.y);
+        a0 += re;
+        a1 += im;
+        s0 += re * re;
+        s1 += im * im;
+    }
+    acc[at] = a0;
+    acc[at + 1] = a1;
+    if (acc2) {
+        acc2[at] = s0;
+        acc2[at + 1] = s1;
+    }
 }
 
 __global__ void munu_count_kernel(const int32_t* __restrict__ slot_of, int r_begin, int r_count,
@@ -1756,9 +1811,9 @@ hipError_t launch_contract(const SsfState<Real>& st, const CorrArgs& c, int r_be
         const int T = (3 * st.n_sl + 15) / 16;
         const dim3 g64(unsigned((st.n_kh + 63) / 64), unsigned(r_count));
         switch (T) {
-            case 1: hipLaunchKernelGGL((contract_stage_kernel<1, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
-            case 2: hipLaunchKernelGGL((contract_stage_kernel<2, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
-            case 3: hipLaunchKernelGGL((contract_stage_kernel<3, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0); return hipGetLastError();
+            case 1: hipLaunchKernelGGL((contract_stage_kernel<1, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0, nullptr); return hipGetLastError();
+            case 2: hipLaunchKernelGGL((contract_stage_kernel<2, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0, nullptr); return hipGetLastError();
+            case 3: hipLaunchKernelGGL((contract_stage_kernel<3, false>), g64, dim3(kBlock), 0, s, st, c, r_begin, nullptr, nullptr, 0, nullptr); return hipGetLastError();
             default: break;
         }
     }
@@ -1786,12 +1841,21 @@ hipError_t launch_contract_stage(const SsfState<float>& st, const CorrArgs& c, i
                                  const MunuArgs& m, hipStream_t s) {
     const dim3 grid(unsigned((st.n_kh + 63) / 64), unsigned(r_count));
     switch (m.T) {
-        case 1: hipLaunchKernelGGL((contract_stage_kernel<1, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
-        case 2: hipLaunchKernelGGL((contract_stage_kernel<2, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
-        case 3: hipLaunchKernelGGL((contract_stage_kernel<3, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax); break;
+        case 1: hipLaunchKernelGGL((contract_stage_kernel<1, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax, m.cbuf); break;
+        case 2: hipLaunchKernelGGL((contract_stage_kernel<2, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax, m.cbuf); break;
+        case 3: hipLaunchKernelGGL((contract_stage_kernel<3, true>), grid, dim3(kBlock), 0, s, st, c, r_begin, m.buf, m.cnt, m.Bmax, m.cbuf); break;
         default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(munu_count_kernel, dim3(1), dim3(kBlock), 0, s, c.slot_of, r_begin, r_count, m.cnt, m.Bmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_corr_flush(const MunuArgs& m, hipStream_t s) {
+    if (!m.cbuf) return hipSuccess;
+    const long long n = (long long)m.n_corr * m.n_k;
+    hipLaunchKernelGGL(corr_flush_kernel, dim3(unsigned((n + kBlock - 1) / kBlock), unsigned(m.n_slots)),
+                       dim3(kBlock), 0, s, m.cbuf, m.cnt, m.acc_s, m.acc_s2, m.n_corr, m.n_k, m.H, m.L2, m.n_k_full,
+                       m.Bmax);
     return hipGetLastError();
 }
 

@@ -272,6 +272,13 @@ struct MunuArgs {
     unsigned int* cnt = nullptr;   // [n_slots]
     double* acc = nullptr;
     int Bmax = 0, n_k = 0, n_slots = 0, n_f = 0, T = 0;  // n_k: the kept K (SsfState::n_kh)
+    // buffered contracted correlators of the FP32 staging path (null: the
+    // contraction updates its FP64 sums directly): cbuf[slot][b][c][kh],
+    // added to acc_s / acc_s2 ([slot][c][n_k_full][2]) by the flush
+    float2* cbuf = nullptr;
+    double* acc_s = nullptr;
+    double* acc_s2 = nullptr;
+    int n_corr = 0, H = 0, L2 = 0, n_k_full = 0;
 };
 template <typename Real>
 hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, int r_begin,
@@ -281,6 +288,9 @@ hipError_t launch_munu_stage(const SsfState<Real>& st, const int32_t* slot_of, i
 hipError_t launch_contract_stage(const SsfState<float>& st, const CorrArgs& c, int r_begin, int r_count,
                                  const MunuArgs& m, hipStream_t s);
 hipError_t launch_munu_flush(const MunuArgs& m, hipStream_t s);
+// The buffered contracted samples (MunuArgs::cbuf) into the FP64 sums, in
+// sample order; before launch_munu_flush, which resets the row counts.
+hipError_t launch_corr_flush(const MunuArgs& m, hipStream_t s);
 
 
 // Luttinger-Tisza spectrum over the L[0] x L[1] x L[2] grid of k with
