@@ -69,10 +69,22 @@ def bytes_per_update(prec_bytes: int, n_nbr: int, r_share: int, lifted_frac: flo
     return 2 * 3 * prec_bytes + 4.0 * n_nbr / r_share + 2.0 * lifted_frac
 
 
-def make_c3(pyro, precision, R, seed=2024):
+# The tempering workload's sweeps run in the fused pass order
+# (pyro_set_fused_passes): per colour, each site's Metropolis update and its
+# first OR update in one launch from one field evaluation, then the second
+# OR pass on its own: 32 colour launches per sweep instead of 48.  The FP64
+# one-replica path it is compared with keeps the reference's pass-by-pass
+# order.
+C3_FUSED = True
+C3_ORDER = ("fused: per colour, each site's Metropolis and first OR update from one field evaluation, "
+            "then the second OR pass" if C3_FUSED else "pass by pass")
+
+
+def make_c3(pyro, precision, R, seed=2024, fused=False):
     ctx = pyro.Context.pyrochlore(L_LAT, n_replicas=R, precision=precision, seed=seed)
     ctx.define_J1J2J3(*J123)
     ctx.setup()
+    ctx.set_fused_passes(fused)
     ctx.randomize(seed + 1)
     ladder = np.geomspace(T_RANGE[0], T_RANGE[1], R) if R > 1 else np.array([T_RANGE[1]])
     ctx.pt_init(ladder, swap_seed=seed + 2)
@@ -410,7 +422,7 @@ def main():
         return
 
     # ---- headline: batched FP32 tempering ----
-    ctx = make_c3(pyro, pyro.FP32, R_REP)
+    ctx = make_c3(pyro, pyro.FP32, R_REP, fused=C3_FUSED)
     ctx.ssf_configure(CORR, n_slots=R_REP, store_error_term=True)
     ctx.set_strategy(args.strategy)
     wall, gpu_s = time_rounds(ctx, pyro, args.steps, args.warmup)
@@ -435,12 +447,20 @@ def main():
     ctx.sweep(pyro.METROPOLIS, N_OR, n_sw, count=False)
     ctx.mark(1)
     sweep_ms = ctx.elapsed_ms()
-    sweep_launches = {1: n_sw * (1 + N_OR) * info["n_colours"], 3: n_sw * (1 + N_OR) * info["n_colours"],
+    # colour launches per sweep: one per colour and pass, or in the fused
+    # order one per colour for the Metropolis + first OR update and one per
+    # colour for each further OR pass
+    groups = max(N_OR, 1) if C3_FUSED else 1 + N_OR
+    sweep_launches = {1: n_sw * (1 + N_OR) * info["n_colours"], 3: n_sw * groups * info["n_colours"],
                       2: 1, 4: 1, 5: 8 * n_sw}[strategy]
     upd_per_launch = N * R_REP * (1 + N_OR) * n_sw / sweep_launches
     avg_launch_s = sweep_ms / 1e3 / sweep_launches
+    # compulsory bytes per site a launch visits (its spin read and written
+    # once, whatever the number of updates it takes there, and its share of
+    # the neighbour row); sites per launch = the launch's colour class
     b_alg = bytes_per_update(4, info["n_slot"], R_REP)
-    achieved = b_alg * upd_per_launch / avg_launch_s
+    sites_per_launch = N * R_REP * n_sw * (groups if strategy == 3 else 1 + N_OR) / sweep_launches
+    achieved = b_alg * sites_per_launch / avg_launch_s
     traffic_c3 = profiled_traffic("sweep_c3_L12x64")
     st = ctx.pt_stats()
     acc_rate = (st["accepts"] / np.maximum(st["attempts"], 1)).tolist()
@@ -478,7 +498,7 @@ def main():
                         "T log-spaced 2.0..0.02, J1=1 J2=-0.05 J3=0.05, sweep = 1 Metropolis + "
                         "2 OR, exchange every 4 sweeps, SSF (9 correlators) every round",
             "L": L_LAT, "replicas": R_REP, "passes_per_sweep": 1 + N_OR,
-            "sweeps_per_step": N_SWEEP, "strategy": strategy,
+            "sweeps_per_step": N_SWEEP, "strategy": strategy, "pass_order": C3_ORDER,
         },
         "gpu_event_s": gpu_s,
         "fp64_one_replica_site_updates_per_s": fp64,
@@ -494,7 +514,8 @@ def main():
             "traffic_rate_frac": (traffic_c3 / avg_launch_s / PEAK_HBM) if traffic_c3 else None,
             "kernel": {1: "sweep_colour_kernel", 2: "sweep_resident_kernel", 3: "pyro_colour_kernel",
                        4: "pyro_resident_kernel", 5: "pyro_tiled_kernel"}[strategy],
-            "bytes_per_update": b_alg,
+            "bytes_per_site": b_alg,
+            "sites_per_launch": sites_per_launch,
             "updates_per_launch": upd_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,
             # the HIP-event sample avg_launch_us comes from

@@ -312,25 +312,32 @@ int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
  * is a fixed rule, never a timing choice.  pyro_get_tile_shape returns the
  * shape in use (0, 0, 0 when none applies). */
 int pyro_set_tile_shape(pyro_ctx* ctx, const int32_t T[3]);
-/* Pass order of the tiled sweep (strategy 5).  0 (default): pass by pass
- * within each tile, the order the reference's sweeps follow
- * (MC.cpp:283-303: every site's first-kind update, then every site's OR
- * update).  1: within each colour stage every site takes all passes of the
- * sweep back to back, the first kind then the OR passes, from one
- * evaluation of its local field (no bonded site changes in between), and
- * its uniforms for the sweep are drawn in order from its Philox blocks:
+/* Pass order of the pyrochlore stencil sweeps (strategies 3 and 5; the
+ * generic strategies 1 and 2 always run pass by pass, and a forced
+ * strategy 4 with OR passes in this order reports PYRO_E_UNSUPPORTED; the
+ * autotuner then keeps strategy 3).  0 (default): pass by pass, the order
+ * the reference's sweeps follow (MC.cpp:155-163, 283-303: every site's
+ * first-kind update, then every site's OR update).  1 (fused): within each
+ * colour stage every site takes its first-kind update and then its first
+ * OR update back to back, from one evaluation of its local field (no
+ * bonded site changes in between); OR passes 2..n_overrelax follow as
+ * passes of their own (two OR rotations about one field would make one).
+ * Its uniforms for the sweep are drawn in order from its Philox blocks:
  * uniform j is component j % 5 of the block with pass index p0 + j / 5 (p0
  * the sweep's first pass), the first kind taking 5 (Metropolis), 2
- * (lifted) or 1 (lifted rotation) and each OR pass the next one, so a
- * lifted + OR sweep draws one block per site.  Both orders are sequential
- * single-site schedules of the same updates; the order changes the Markov
- * chain, not its stationary distribution, and pyro_sweep_order reports it.
- * No counterpart in the reference. */
+ * (lifted) or 1 (lifted rotation) and OR pass k the next one, so a lifted
+ * + OR sweep draws one block per site and a Metropolis + 2 OR sweep two.
+ * Per-colour launches (strategy 3) then run 16 launches per sweep for the
+ * first kind with one OR pass, and 16 more per further OR pass.  Both
+ * orders are sequential single-site schedules of the same updates; the
+ * order changes the Markov chain, not its stationary distribution, and
+ * pyro_sweep_order reports it.  No counterpart in the reference. */
 int pyro_set_fused_passes(pyro_ctx* ctx, int on);
 int pyro_get_tile_shape(pyro_ctx* ctx, int32_t T[3]);
 /* The site visiting order of one sweep under the strategy the next sweep
- * will use: site[k] and pass_offset[k] (0 = the sweep's first pass kind,
- * 1..n_overrelax = OR passes) for k < N * (1 + n_overrelax).  Sites with
+ * will use and the pass order (pyro_set_fused_passes): site[k] and
+ * pass_offset[k] (0 = the sweep's first pass kind, 1..n_overrelax = OR
+ * passes) for k < N * (1 + n_overrelax).  Sites with
  * no bond between them may run in any relative order, so this is one of
  * the sequential orders the parallel sweep equals. */
 int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pass_offset);

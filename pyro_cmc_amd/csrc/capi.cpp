@@ -451,15 +451,19 @@ struct pyro_ctx {
     // (strategy 1 generic, 3 pyrochlore stencil).
     // epart (stencil kernels only): the pass also leaves each replica's
     // energy partials there (pyro::launch_pyro_colour).
+    // fused and ucomp (stencil kernels only): the fused pass order's launch
+    // of the first kind with the first OR update, and the uniform component
+    // of an OR pass of that order running on its own.
     template <typename Real>
     void pass_colours(int kind, const unsigned long long* pass_base, unsigned long long pass_index,
-                      bool use_spec, double* epart = nullptr) {
+                      bool use_spec, double* epart = nullptr, bool fused = false, int ucomp = 0) {
         const auto st = state<Real>(pass_base);
         if (use_spec) {
             const auto J = pyro_couplings<Real>();
             for (int c = 0; c < topo.n_colours; ++c)
-                timed(kFamSweep,
-                      [&] { return pyro::launch_pyro_colour<Real>(st, J, c, kind, pass_index, stream, epart); });
+                timed(kFamSweep, [&] {
+                    return pyro::launch_pyro_colour<Real>(st, J, c, kind, pass_index, stream, epart, fused, ucomp);
+                });
             return;
         }
         for (int c = 0; c < topo.n_colours; ++c) {
@@ -492,7 +496,7 @@ struct pyro_ctx {
                 for (int phase = 0; phase < 8; ++phase)
                     timed(kFamSweep, [&] {
                         return pyro::launch_pyro_tiled<Real>(st, J, tile_kind, phase, kind, n_or, p, stream,
-                                                             fused_passes);
+                                                             fused_order(5, n_or));
                     });
             return;
         }
@@ -506,6 +510,23 @@ struct pyro_ctx {
             return;
         }
         unsigned long long p = pass_index;
+        if (strat == 3 && fused_order(strat, n_or)) {
+            // the fused pass order: per colour the first kind and the first
+            // OR update of each site in one launch, then OR passes 2..n_or;
+            // OR pass k draws uniform j = kUniformsOf[kind] + k - 1 of the
+            // sweep, component j % 5 of block p + j / 5
+            const int n_first = pyro::kUniformsOf[kind];
+            for (int s = 0; s < n_sweeps; ++s, p += 1 + n_or) {
+                const bool last = s == n_sweeps - 1;
+                pass_colours<Real>(kind, pass_base, p, true, last && n_or == 1 ? epart_last : nullptr, true);
+                for (int k = 2; k <= n_or; ++k) {
+                    const int j = n_first + k - 1;
+                    pass_colours<Real>(3, pass_base, p + unsigned(j / 5), true,
+                                       last && k == n_or ? epart_last : nullptr, false, j % 5);
+                }
+            }
+            return;
+        }
         for (int s = 0; s < n_sweeps; ++s) {
             const bool last = s == n_sweeps - 1 && strat == 3;
             pass_colours<Real>(kind, pass_base, p++, strat == 3, last && n_or == 0 ? epart_last : nullptr);
@@ -513,6 +534,10 @@ struct pyro_ctx {
                 pass_colours<Real>(3, pass_base, p++, strat == 3, last && o == n_or - 1 ? epart_last : nullptr);
         }
     }
+    // Whether sweeps of strategy strat with n_or OR passes run in the fused
+    // pass order (pyro_set_fused_passes): the stencil strategies 3 and 5,
+    // with at least one OR pass to pair with the first kind.
+    bool fused_order(int strat, int n_or) const { return fused_passes && n_or >= 1 && (strat == 3 || strat == 5); }
     void run_sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
                     unsigned long long pass_index, double* epart_last = nullptr) {
         if (prec == 64) sweeps<double>(strat, kind, n_or, n_sweeps, pass_base, pass_index, epart_last);
@@ -538,6 +563,8 @@ struct pyro_ctx {
                     "the pyrochlore resident strategy needs at most 2048 cells");
             require(strategy != 5 || tile_kind > 0, PYRO_E_UNSUPPORTED,
                     "the tiled strategy needs each L divisible by twice the tile edge (8 or 12)");
+            require(strategy != 4 || !fused_passes || n_or == 0, PYRO_E_UNSUPPORTED,
+                    "the fused pass order runs with strategies 3 and 5, not 4");
             return strategy;
         }
         // Large lattices take the tiled schedule (a fixed rule, so results do
@@ -552,7 +579,8 @@ struct pyro_ctx {
         std::vector<int> cand;
         if (spec) {
             cand.push_back(3);
-            if (resident_fits()) cand.push_back(4);
+            // the fused pass order has no resident form: 3 alone
+            if (resident_fits() && !fused_order(3, n_or)) cand.push_back(4);
         } else {
             cand = {1, 2};
         }
@@ -2002,14 +2030,24 @@ int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pas
             const int n0 = int(c / L[1]);
             return s * P + ((n0 % t.period[0]) * t.period[1] + (n1 % t.period[1])) * t.period[2] + (n2 % t.period[2]);
         };
+        // the fused pass order: stage group 0 gives each site its first
+        // kind and its first OR update back to back, group g >= 1 OR pass
+        // g + 1 alone; otherwise group k is pass k alone
+        const bool fused = ctx->fused_order(strat, n_overrelax);
+        const int n_groups = fused ? std::max(n_overrelax, 1) : n_overrelax + 1;
+        auto emit = [&](int grp, int32_t i) {
+            site[o] = i;
+            pass_offset[o++] = fused && grp > 0 ? grp + 1 : grp;
+            if (fused && grp == 0) {
+                site[o] = i;
+                pass_offset[o++] = 1;
+            }
+        };
         if (strat != 5) {
-            for (int k = 0; k <= n_overrelax; ++k)
+            for (int g = 0; g < n_groups; ++g)
                 for (int c = 0; c < t.n_colours; ++c)
                     for (int64_t i = 0; i < ctx->N(); ++i)
-                        if (t.class_colour[class_of(i)] == c) {
-                            site[o] = int32_t(i);
-                            pass_offset[o++] = k;
-                        }
+                        if (t.class_colour[class_of(i)] == c) emit(g, int32_t(i));
             return;
         }
         require(ctx->tile_kind > 0, PYRO_E_UNSUPPORTED, "tiled strategy does not apply");
@@ -2023,21 +2061,15 @@ int pyro_sweep_order(pyro_ctx* ctx, int n_overrelax, int32_t* site, int32_t* pas
                         const int o0 = ((phase & 1) + 2 * m0) * T[0];
                         const int o1 = (((phase >> 1) & 1) + 2 * m1) * T[1];
                         const int o2 = (((phase >> 2) & 1) + 2 * m2) * T[2];
-                        // fused: within each colour stage a site takes every
-                        // pass in a row; else pass by pass within the tile
-                        const int n_outer = ctx->fused_passes ? 1 : n_overrelax + 1;
-                        const int n_inner = ctx->fused_passes ? n_overrelax + 1 : 1;
-                        for (int ko = 0; ko < n_outer; ++ko)
+                        // the stage groups within the tile, as above
+                        for (int g = 0; g < n_groups; ++g)
                             for (int col = 0; col < ncol; ++col)
                                 for (int q = 0; q < 16 / ncol; ++q) {
                                     const int sl = ncol == 16 ? col : col + 4 * q;
                                     for (int i0 = 0; i0 < T[0]; ++i0)
                                         for (int i1 = 0; i1 < T[1]; ++i1)
                                             for (int i2 = 0; i2 < T[2]; ++i2)
-                                                for (int ki = 0; ki < n_inner; ++ki) {
-                                                    site[o] = int32_t(sl * nc + ctx->geom.cell_index(o0 + i0, o1 + i1, o2 + i2));
-                                                    pass_offset[o++] = ko + ki;
-                                                }
+                                                emit(g, int32_t(sl * nc + ctx->geom.cell_index(o0 + i0, o1 + i1, o2 + i2)));
                                 }
                     }
     });
@@ -2063,7 +2095,10 @@ int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
 int pyro_set_fused_passes(pyro_ctx* ctx, int on) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
-        if (ctx->fused_passes != (on != 0)) ctx->invalidate_graph();
+        if (ctx->fused_passes != (on != 0)) {
+            ctx->invalidate_graph();
+            ctx->tuned_key = -1;  // the candidates depend on the order
+        }
         ctx->fused_passes = on != 0;
     });
 }

@@ -122,11 +122,6 @@ __device__ __forceinline__ Uniform5 site_uniforms(const DeviceState<Real>& st, i
         Words4{uint32_t(site), uint32_t(r), uint32_t(pass), uint32_t(pass >> 32)}, st.k0, st.k1));
 }
 
-// Uniforms one update of each kind consumes from its block: Metropolis 5
-// (two Box-Muller pairs and the acceptance), lifted 2 (the exponential
-// displacement and the acceptance), lifted rotation 1, overrelaxation 1.
-constexpr int kUniformsOf[4] = {5, 2, 1, 1};
-
 // What an update needs of its replica: the temperature of the slot the
 // replica occupies, the proposal width sqrt(T / T_ref) (computed in FP64 as
 // the reference does) and the global field.  Constant within a launch, so

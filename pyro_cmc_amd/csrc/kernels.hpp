@@ -49,6 +49,11 @@ struct ColourList {
     int n_cls = 0;
 };
 
+// Uniforms one update of each kind consumes from its block: Metropolis 5
+// (two Box-Muller pairs and the acceptance), lifted 2 (the exponential
+// displacement and the acceptance), lifted rotation 1, overrelaxation 1.
+constexpr int kUniformsOf[4] = {5, 2, 1, 1};
+
 // One pass of `kind` over one colour for every replica (one launch).
 // pass_index is added to *pass_base when pass_base is set.
 template <typename Real>
@@ -90,9 +95,14 @@ struct PyroCouplings {
 // pass of a sweep, every colour in order): also each replica's energy as
 // pyro_colour_energy_blocks(n_cells) partials epart[r][...], which
 // launch_energy_finish sums once the pass's last colour has run.
+// fused (the fused pass order, kind 0, 1 or 2): each site's first OR update
+// follows its `kind` update in the same launch, from the same field, with
+// the sweep's next uniform; ucomp (kind 3 only): the component of the
+// block that an OR pass of the fused order running on its own draws.
 template <typename Real>
 hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
-                              int kind, unsigned long long pass_index, hipStream_t s, double* epart = nullptr);
+                              int kind, unsigned long long pass_index, hipStream_t s, double* epart = nullptr,
+                              bool fused = false, int ucomp = 0);
 __host__ __device__ constexpr int pyro_colour_energy_blocks(int n_cells) { return 16 * ((n_cells + 63) / 64); }
 // Whole sweeps with one resident workgroup (1024 threads, two cells per
 // thread) per replica; lattices of at most 2048 cells.  With epart the last
@@ -123,8 +133,10 @@ constexpr size_t pyro_tile_lds(int T0, int T1, int T2, int real_size, bool lifte
     return size_t(3) * 16 * (T0 + 1) * (T1 + 1) * (T2 + 1) * size_t(real_size) +
            (lifted ? size_t(16) * 4 * ((T0 * T1 * T2 + 31) / 32) : 0);
 }
-// fused: every pass of a site back to back within its colour stage, from one
-// field evaluation (pyro_set_fused_passes), else pass by pass.
+// fused (pyro_set_fused_passes): within each colour stage a site takes its
+// `kind` update and its first OR update back to back, from one field
+// evaluation; further OR passes run pass by pass after them.  Else every
+// pass runs pass by pass.
 template <typename Real>
 hipError_t launch_pyro_tiled(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int tile_kind,
                              int phase, int kind, int n_or, unsigned long long pass_index, hipStream_t s,

@@ -301,15 +301,22 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // Over the pass's colours this counts every bond once, when its later
 // endpoint is visited, so the partials sum to the energy of the state the
 // pass leaves.  One partial per (one-wave) block: epart[r * e_stride + bx].
+// FUSED (the fused pass order): each site takes its KIND update and then
+// its first OR update back to back, from one evaluation of its field (no
+// bonded site changes in between: a colour class holds no bond), drawing
+// the OR's uniform as uniform kUniformsOf[KIND] of its sweep (component
+// j % 5 of the block with pass index pass + j / 5).  ucomp: an OR pass of
+// the fused order that runs on its own takes component ucomp of its block.
 constexpr int kPyroColourThreads = 64;
 
-template <typename Real, int SHELLS, int KIND, bool PER_REP, int BS, bool EN>
+template <typename Real, int SHELLS, int KIND, bool PER_REP, int BS, bool EN, bool FUSED>
 __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
                                                           PyroCouplings<Real> J, int colour,
-                                                          unsigned long long pass_index,
+                                                          unsigned long long pass_index, int ucomp,
                                                           double* __restrict__ epart, int e_stride) {
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     static_assert(!EN || BS == 64, "one energy partial per wave");
+    static_assert(!FUSED || KIND != 3, "the fused launch starts with the sweep's first kind");
     const int nc = st.n_cells;
     const int bps = (nc + BS - 1) / BS;
     unsigned bx, ru;
@@ -325,24 +332,30 @@ __global__ __launch_bounds__(BS) void pyro_colour_kernel(DeviceState<Real> st,
         const size_t N = size_t(st.N);
         Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
         const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
+        const PyroCouplings<Real> Jr = PER_REP ? J.at(r) : J;
         Real hx, hy, hz;
-        if constexpr (EN) {
-            LowSums<Real> lo;
-            const PyroCouplings<Real> Jr = PER_REP ? J.at(r) : J;
-            pyro_field_rt<Real, SHELLS, NCOL>(sl, Sr, nc, c, Jr, hx, hy, hz, &lo);
-            const ReplicaParams<Real> p = replica_params(st, r);
-            const int site = sl * nc + cell;
-            const Vec3<Real> v = ld3(Sr, unsigned(site));
-            Real x = v.x, y = v.y, z = v.z;
-            bool changed;
-            accepted = update_spin_p<Real, KIND>(p, st, x, y, z, changed, site, r, pass, hx, hy, hz,
-                                                 st.dir + size_t(r) * N + site);
-            if (changed) st3(Sr, unsigned(site), x, y, z);
-            e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, p, x, y, z);
-        } else {
-            pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, PER_REP ? J.at(r) : J, hx, hy, hz);
-            accepted = apply_update<Real, KIND>(st, Sr, sl * nc + cell, r, pass, hx, hy, hz);
+        LowSums<Real> lo;
+        if constexpr (EN) pyro_field_rt<Real, SHELLS, NCOL>(sl, Sr, nc, c, Jr, hx, hy, hz, &lo);
+        else pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, Jr, hx, hy, hz);
+        const ReplicaParams<Real> p = replica_params(st, r);
+        const int site = sl * nc + cell;
+        const Vec3<Real> v = ld3(Sr, unsigned(site));
+        Real x = v.x, y = v.y, z = v.z;
+        Uniform5 ub = site_uniforms(st, site, r, pass);
+        if constexpr (KIND == 3) ub.m[0] = uniform_at(ub, ucomp);
+        bool changed;
+        accepted = update_spin_u<Real, KIND>(p, x, y, z, changed, ub, hx, hy, hz, st.dir + size_t(r) * N + site);
+        if constexpr (FUSED) {
+            constexpr int j = kUniformsOf[KIND];  // the sweep's uniform of its first OR update
+            if constexpr (j % 5 == 0) ub = site_uniforms(st, site, r, pass + unsigned(j / 5));
+            Uniform5 uo;
+            uo.m[0] = ub.m[j % 5];
+            bool ch;
+            update_spin_u<Real, 3>(p, x, y, z, ch, uo, hx, hy, hz, nullptr);
+            changed = changed || ch;
         }
+        if (changed) st3(Sr, unsigned(site), x, y, z);
+        if constexpr (EN) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, p, x, y, z);
     }
     if constexpr (KIND != 3) count_accepts(st, r, bx, accepted);
     if constexpr (EN) {
@@ -599,17 +612,20 @@ __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, cons
 // Workgroups map to tiles XCD by XCD: the 8 XCDs take contiguous chunks of
 // the (replica, tile) order, so consecutive tiles along n2, whose halo rows
 // share cache lines, are fetched through the same L2.
-// FUSED (site passes back to back): within a colour stage each site takes
-// every pass of the sweep in a row, the first kind then the n_or OR passes,
-// from one evaluation of its field.  No bonded site changes in between (a
-// colour class has no bond inside it), so the field is the one each pass
-// would compute: this is the sequential order "per tile, per colour, per
-// site, every pass", with half the field evaluations and barriers of a
-// lifted + OR sweep.  The site's uniforms for the sweep are then drawn in
-// order from its blocks: uniform j of the sweep is component j % 5 of the
-// block with pass index pass0 + j / 5, the first kind taking the first
-// kUniformsOf[FIRST] and each OR pass the next one (a lifted + OR sweep
-// draws one Philox block per site instead of two).
+// FUSED (the fused pass order): within a colour stage each site takes its
+// first-kind update and then its first OR update back to back, from one
+// evaluation of its field.  No bonded site changes in between (a colour
+// class has no bond inside it), so the field is the one each pass would
+// compute: this is the sequential order "per tile, per colour, per site,
+// first kind then OR", with half the field evaluations and barriers of a
+// lifted + OR sweep.  Further OR passes (n_or >= 2) follow as colour stages
+// of their own: a second OR rotation about the same field would only
+// continue the first (the sum of two uniform angles is one uniform angle).
+// The site's uniforms for the sweep are drawn in order from its blocks:
+// uniform j of the sweep is component j % 5 of the block with pass index
+// pass0 + j / 5, the first kind taking the first kUniformsOf[FIRST] and OR
+// pass k uniform kUniformsOf[FIRST] + k - 1 (a lifted + OR sweep draws one
+// Philox block per site instead of two).
 template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2, int BS, bool FUSED>
 __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                          int phase, int n_or,
@@ -676,6 +692,10 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const int my_gc = ((o0 + c0) * L1 + (o1 + c1)) * L2 + (o2 + c2);
     unsigned int my_acc = 0;
     if constexpr (FUSED) {
+        // group 0: the first kind and the first OR update of every site of a
+        // colour back to back; group g >= 1: OR pass g + 1 on its own
+        const int n_groups = n_or > 1 ? n_or : 1;
+        for (int grp = 0; grp < n_groups; ++grp) {
         for (int col = 0; col < NCOL; ++col) {
             for (int q = 0; q < SPC; ++q) {
                 const int sl = NCOL == 16 ? col : col + 4 * q;
@@ -699,28 +719,36 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                     const int li = sl * BC + lc + own;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool any = false, ch;
-                    int8_t d = 1, d0 = 1;
-                    unsigned int* wp = nullptr;
-                    const unsigned int bit = 1u << (m & 31);
-                    if constexpr (LIFTED) {
-                        wp = ldw + sl * W + (m >> 5);
-                        d0 = d = (*wp & bit) ? int8_t(-1) : int8_t(1);
-                    }
-                    Uniform5 ub = site_uniforms(st, site, r, pass0);
-                    my_acc += update_spin_u<Real, FIRST>(rp, x, y, z, ch, ub, hx, hy, hz, LIFTED ? &d : nullptr)
-                                  ? 1u : 0u;
-                    any = ch;
-                    if constexpr (LIFTED) {
-                        if (d != d0) atomicXor(wp, bit);
-                    }
-                    for (int k = 1; k <= n_or; ++k) {
-                        const int j = kUniformsOf[FIRST] + k - 1;  // the sweep's uniform for this pass
-                        const int c = j % 5;
-                        if (c == 0) ub = site_uniforms(st, site, r, pass0 + unsigned(j / 5));
+                    if (grp == 0) {
+                        int8_t d = 1, d0 = 1;
+                        unsigned int* wp = nullptr;
+                        const unsigned int bit = 1u << (m & 31);
+                        if constexpr (LIFTED) {
+                            wp = ldw + sl * W + (m >> 5);
+                            d0 = d = (*wp & bit) ? int8_t(-1) : int8_t(1);
+                        }
+                        Uniform5 ub = site_uniforms(st, site, r, pass0);
+                        my_acc += update_spin_u<Real, FIRST>(rp, x, y, z, ch, ub, hx, hy, hz, LIFTED ? &d : nullptr)
+                                      ? 1u : 0u;
+                        any = ch;
+                        if constexpr (LIFTED) {
+                            if (d != d0) atomicXor(wp, bit);
+                        }
+                        if (n_or >= 1) {
+                            constexpr int j = kUniformsOf[FIRST];  // the sweep's uniform of its first OR update
+                            if constexpr (j % 5 == 0) ub = site_uniforms(st, site, r, pass0 + unsigned(j / 5));
+                            Uniform5 uo;
+                            uo.m[0] = ub.m[j % 5];
+                            update_spin_u<Real, 3>(rp, x, y, z, ch, uo, hx, hy, hz, nullptr);
+                            any = any || ch;
+                        }
+                    } else {
+                        const int j = kUniformsOf[FIRST] + grp;  // the sweep's uniform of OR pass grp + 1
+                        const Uniform5 ub = site_uniforms(st, site, r, pass0 + unsigned(j / 5));
                         Uniform5 uo;
-                        uo.m[0] = c == 0 ? ub.m[0] : c == 1 ? ub.m[1] : c == 2 ? ub.m[2] : c == 3 ? ub.m[3] : ub.m[4];
+                        uo.m[0] = uniform_at(ub, j % 5);
                         update_spin_u<Real, 3>(rp, x, y, z, ch, uo, hx, hy, hz, nullptr);
-                        any = any || ch;
+                        any = ch;
                     }
                     if (any) {
                         lx[li] = x;
@@ -730,6 +758,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                 }
             }
             __syncthreads();
+        }
         }
     } else
     for (int k = 0; k <= n_or; ++k) {
@@ -896,8 +925,11 @@ hipError_t dispatch_pyro(const PyroCouplings<Real>& J, F&& f) {
 
 template <typename Real>
 hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int colour,
-                              int kind, unsigned long long pass_index, hipStream_t s, double* epart) {
+                              int kind, unsigned long long pass_index, hipStream_t s, double* epart, bool fused,
+                              int ucomp) {
     const int ncol = J.shells == 3 ? 16 : 4;
+    if (fused && kind == 3) return hipErrorInvalidValue;
+    if (ucomp < 0 || ucomp > 4 || (ucomp != 0 && kind != 3)) return hipErrorInvalidValue;
     // One wave per workgroup: measured on the L=12 x 64 tempering lattice,
     // 64-thread workgroups take 5.17 us per colour launch against 5.39 us
     // for 128, 256 or 512 threads (same sites, same results).
@@ -908,23 +940,32 @@ hipError_t launch_pyro_colour(const DeviceState<Real>& st, const PyroCouplings<R
     // energy partials of colour c at [r][c * per_launch + block]: 16 bps per replica
     double* ep = epart ? epart + size_t(colour) * size_t(per_launch) : nullptr;
     const int es = pyro_colour_energy_blocks(st.n_cells);
-    auto go = [&](auto per_rep, auto en) -> hipError_t {
+    auto go = [&](auto per_rep, auto en, auto fu) -> hipError_t {
         constexpr bool PR = decltype(per_rep)::value;
         constexpr bool EN = decltype(en)::value;
+        constexpr bool FU = decltype(fu)::value;
         return dispatch_pyro(J, [&](auto shells) -> hipError_t {
             constexpr int SH = decltype(shells)::value;
             switch (kind) {
-                case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
-                case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
-                case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
-                case 3: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3, PR, BS, EN>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ep, es); break;
+                case 0: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 0, PR, BS, EN, FU>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ucomp, ep, es); break;
+                case 1: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 1, PR, BS, EN, FU>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ucomp, ep, es); break;
+                case 2: hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 2, PR, BS, EN, FU>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ucomp, ep, es); break;
+                case 3:
+                    if constexpr (!FU) {
+                        hipLaunchKernelGGL((pyro_colour_kernel<Real, SH, 3, PR, BS, EN, false>), grid, dim3(BS), 0, s, st, J, colour, pass_index, ucomp, ep, es);
+                        break;
+                    }
+                    return hipErrorInvalidValue;
                 default: return hipErrorInvalidValue;
             }
             return hipGetLastError();
         });
     };
-    if (epart) return J.per_replica ? go(std::true_type{}, std::true_type{}) : go(std::false_type{}, std::true_type{});
-    return J.per_replica ? go(std::true_type{}, std::false_type{}) : go(std::false_type{}, std::false_type{});
+    auto by_fused = [&](auto per_rep, auto en) -> hipError_t {
+        return fused ? go(per_rep, en, std::true_type{}) : go(per_rep, en, std::false_type{});
+    };
+    if (epart) return J.per_replica ? by_fused(std::true_type{}, std::true_type{}) : by_fused(std::false_type{}, std::true_type{});
+    return J.per_replica ? by_fused(std::true_type{}, std::false_type{}) : by_fused(std::false_type{}, std::false_type{});
 }
 
 template <typename Real>

@@ -58,6 +58,11 @@ __host__ __device__ inline Uniform5 split_uniform5(Words4 w) {
     return u;
 }
 
+// Component c (0..4) of u for a c known only at run time.
+__host__ __device__ inline uint32_t uniform_at(const Uniform5& u, int c) {
+    return c == 0 ? u.m[0] : c == 1 ? u.m[1] : c == 2 ? u.m[2] : c == 3 ? u.m[3] : u.m[4];
+}
+
 template <typename Real>
 __host__ __device__ inline Real u24(uint32_t m) {
     return (Real(m) + Real(0.5)) * Real(5.9604644775390625e-08);  // 2^-24

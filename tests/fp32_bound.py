@@ -31,13 +31,20 @@ EPS_UPDATE = 1e-6  # one FP32 update, hardware transcendentals included
 SAFETY = 10.0
 
 
+def _group(koff, fused):
+    """Stage group of each update: with fused passes the first kind and the
+    first OR pass share group 0 and OR pass k >= 2 is group k - 1; else
+    pass k is group k."""
+    return np.maximum(koff - 1, 0) if fused else koff
+
+
 def tiled_stages(ctx, site, koff, fused=False):
     """Stage key of every update of a tiled sweep's order (pyro_sweep_order
     under the tiled strategy): (checkerboard phase, pass, colour), or with
-    fused passes (phase, colour, pass).  Tiles of one phase share no bond
-    and a colour class has no bond inside it, so updates of one key are
-    independent, and taking the keys in increasing order preserves every
-    dependency of the tile-by-tile order."""
+    fused passes (phase, stage group, colour, pass).  Tiles of one phase
+    share no bond and a colour class has no bond inside it, so updates of
+    one key are independent, and taking the keys in increasing order
+    preserves every dependency of the tile-by-tile order."""
     T = np.array(ctx.tile_shape())
     L = np.array(ctx.Ls)
     nc = int(np.prod(L))
@@ -48,9 +55,18 @@ def tiled_stages(ctx, site, koff, fused=False):
     n_col = ctx.info()["n_colours"]
     colour = ctx.colouring()[site]
     n_pass = int(koff.max()) + 1
-    if fused:
-        return (phase.astype(np.int64) * n_col + colour) * n_pass + koff
-    return (phase.astype(np.int64) * n_pass + koff) * n_col + colour
+    grp = _group(koff, fused)
+    return ((phase.astype(np.int64) * n_pass + grp) * n_col + colour) * n_pass + koff
+
+
+def colour_stages(ctx, site, koff, fused=False):
+    """Stage key of every update of a per-colour sweep's order
+    (pyro_sweep_order under strategy 3): (pass, colour), or with fused
+    passes (stage group, colour, pass)."""
+    n_col = ctx.info()["n_colours"]
+    colour = ctx.colouring()[site]
+    n_pass = int(koff.max()) + 1
+    return (_group(koff, fused).astype(np.int64) * n_col + colour) * n_pass + koff
 
 
 def padded_adjacency(o, J=None):

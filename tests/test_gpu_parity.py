@@ -396,6 +396,10 @@ def test_tempering_exchange_interval(gpu, oracle):
         (3, 4, (1.0, 0.1, 0.0), 1, 1),  # 4 colours, lifted
         (4, 3, (1.0, 0.1, 0.1), 0, 0),  # no OR pass: the Metropolis pass is the last
         (12, 64, (1.0, -0.05, 0.05), 0, 2),  # the L=12 x 64 tempering lattice, 27 blocks
+        # the fused pass order: the energy pass is OR pass 2 on its own, or
+        # with one OR pass the launch of the first kind and the OR update
+        (12, 64, (1.0, -0.05, 0.05), 0, -2),
+        (3, 4, (1.0, 0.1, 0.0), 1, -1),
     ],
 )
 def test_tempering_fused_energy(gpu, oracle, prec, L, R, J, kind, n_or):
@@ -407,6 +411,9 @@ def test_tempering_fused_energy(gpu, oracle, prec, L, R, J, kind, n_or):
     T = np.geomspace(2.0, 0.05, R)
     ctx, os_ = make(gpu, oracle, L, J, 0.8, prec, R=R, B=(0.1, -0.2, 0.3))
     ctx.set_strategy(3)
+    if n_or < 0:  # negative: that many OR passes in the fused pass order
+        n_or = -n_or
+        ctx.set_fused_passes(True)
     ctx.randomize(11)
     ctx.pt_init(T, 9)
     ctx.pt_rounds(kind, n_or, 2, n_rounds=1)
@@ -437,7 +444,7 @@ def test_tempering_fused_energy(gpu, oracle, prec, L, R, J, kind, n_or):
 @pytest.mark.parametrize("prec", [32, 64])
 @pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 0), (2, 1), (1, 1), (0, 0), (2, 0)])
 @pytest.mark.parametrize("per_rep", [True, False])
-@pytest.mark.parametrize("strategy", [3, 4])
+@pytest.mark.parametrize("strategy", [3, 4, "3-fused"])
 def test_energy_pass_is_bit_identical(gpu, prec, kind, n_or, per_rep, strategy):
     """The kernels that also sum the energy (the last pass of an anneal step
     or tempering round; per-colour launches and resident workgroups) update
@@ -460,7 +467,11 @@ def test_energy_pass_is_bit_identical(gpu, prec, kind, n_or, per_rep, strategy):
                 ctx.set_field(np.array(B), r)
         else:
             ctx.set_field(np.array([0.1, -0.2, 0.3]))
-        ctx.set_strategy(strategy)
+        if strategy == "3-fused":  # the fused pass order's energy pass
+            ctx.set_strategy(3)
+            ctx.set_fused_passes(True)
+        else:
+            ctx.set_strategy(strategy)
         ctx.randomize(8)
         if fused:
             ctx.anneal_schedule(np.array([1.5, 0.6]))
@@ -764,6 +775,105 @@ def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, t
                                                    ctx.N)
             fp32_bound.check(ctx.get_spins(r), o.get_spins(), E, taint, seeds, len(site),
                              f"L {L} tile {ctx.tile_shape()} replica {r}")
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("kind,n_or", [(0, 1), (0, 2), (1, 1), (2, 1), (1, 2), (0, 3), (1, 0)])
+@pytest.mark.parametrize("J,Delta,B", [CONFIGS[1], CONFIGS[3]])
+def test_colour_fused_order_against_oracle(gpu, oracle, prec, kind, n_or, J, Delta, B):
+    """Per-colour launches (strategy 3) in the fused pass order: per colour
+    each site's first-kind and first OR update in one launch from one field,
+    then OR passes 2..n_or on their own, the sweep's uniforms drawn in order
+    from its blocks.  pyro_sweep_order reports that order, and the oracle
+    replays it draw for draw: FP64 decisions, accept counts and lifted
+    directions exact, FP32 every site within its first-order error bound."""
+    L, R, seed = 3, 3, 0x5EED1234
+    T = np.array([0.3, 0.8, 2.5])
+    ctx, os_ = make(gpu, oracle, L, J, Delta, prec, R=R, seed=seed, B=B, T_ref=0.7)
+    ctx.set_strategy(3)
+    ctx.set_fused_passes(True)
+    ctx.set_temperatures(T)
+    rng = np.random.default_rng(17)
+    for r in range(R):
+        S = random_spins(ctx.N, rng)
+        d = rng.choice(np.array([-1, 1], np.int8), ctx.N)
+        ctx.set_spins(S, r)
+        ctx.set_lifted_dir(d, r)
+        os_[r].set_spins(S)
+        os_[r].set_dir(d)
+    site, koff = ctx.sweep_order(n_or)
+    assert len(site) == ctx.N * (1 + n_or)
+    for k in range(1 + n_or):
+        assert np.array_equal(np.sort(site[koff == k]), np.arange(ctx.N))
+    if n_or >= 1:  # each site's first OR update right after its first-kind update
+        first = np.flatnonzero(koff == 0)
+        assert np.all(koff[first + 1] == 1) and np.array_equal(site[first + 1], site[first])
+    p0 = ctx.pass_count()
+    if prec == 64:
+        acc = ctx.sweep(kind, n_or, 3)
+        assert ctx.pass_count() == p0 + 3 * (1 + n_or)
+        for r, o in enumerate(os_):
+            a = sum(o.sweep_order(kind, T[r], site, koff, seed, r, p0 + sw * (1 + n_or), shared=True)
+                    for sw in range(3))
+            assert int(acc[r]) == a
+            # every decision equal; rounding of the shell-grouped field sum
+            # against the oracle's per-bond sum grows over the 3 sweeps of up
+            # to 4 passes (most at lifted moves close to the field axis, where
+            # the displacement divides by 1 - u^2), as on the tiled order
+            assert np.max(np.abs(ctx.get_spins(r) - o.get_spins())) < 1e-10
+            assert np.array_equal(ctx.get_lifted_dir(r), o.get_dir())
+        return
+    adj = fp32_bound.padded_adjacency(os_[0], (*J, Delta))
+    stage = fp32_bound.colour_stages(ctx, site, koff, True)
+    for sw in range(3):
+        before = [(ctx.get_spins(r), ctx.get_lifted_dir(r)) for r in range(R)]
+        p0 = ctx.pass_count()
+        ctx.sweep(kind, n_or, 1)
+        for r, o in enumerate(os_):
+            o.set_spins(before[r][0])
+            o.set_dir(before[r][1])
+            info = np.empty((len(site), 4))
+            o.sweep_order(kind, T[r], site, koff, seed, r, p0, info=info, shared=True)
+            E, taint, seeds = fp32_bound.propagate(site, stage, info, adj, T[r], field_scale(J, B), ctx.N)
+            fp32_bound.check(ctx.get_spins(r), o.get_spins(), E, taint, seeds, len(site),
+                             f"fused colour order kind {kind} n_or {n_or} sweep {sw} replica {r}")
+
+
+def test_fused_order_strategy_rules(gpu):
+    """The fused pass order runs with per-colour launches: the autotuner
+    keeps strategy 3 (the resident strategy has no fused form), a forced
+    strategy 4 is refused for sweeps with OR passes, and a sweep without OR
+    passes is the same in either order."""
+    ctx = gpu.Context.pyrochlore(2, n_replicas=4, precision=32, seed=3)
+    ctx.define_J1J2J3(1.0, -0.05, 0.05)
+    ctx.setup()
+    ctx.randomize(1)
+    ctx.set_temperatures(np.linspace(0.3, 2.0, 4))
+    ctx.set_fused_passes(True)
+    ctx.sweep(gpu.METROPOLIS, 2, 1)
+    assert ctx.strategy() == 3
+    ctx.set_strategy(4)
+    with pytest.raises(gpu.PyroError) as e:
+        ctx.sweep(gpu.METROPOLIS, 1, 1)
+    assert e.value.code == 5
+    ctx.sweep(gpu.METROPOLIS, 0, 1)  # no OR pass: nothing to fuse
+    ctx.close()
+    out = []
+    for fused in (True, False):
+        ctx = gpu.Context.pyrochlore(2, n_replicas=4, precision=32, seed=3)
+        ctx.define_J1J2J3(1.0, -0.05, 0.05)
+        ctx.setup()
+        ctx.randomize(1)
+        ctx.set_temperatures(np.linspace(0.3, 2.0, 4))
+        ctx.set_strategy(3)
+        ctx.set_fused_passes(fused)
+        acc = ctx.sweep(gpu.LIFTED, 0, 3)
+        out.append((acc, [ctx.get_spins(r) for r in range(4)], ctx.sweep_order(0)))
+        ctx.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][2][0], out[1][2][0]) and np.array_equal(out[0][2][1], out[1][2][1])
 
 
 def test_tile_shape_rules(gpu):

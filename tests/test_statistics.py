@@ -76,10 +76,11 @@ def _reference(kind, n_or, n_seeds=32, n_burn=300, n_meas=2000, ssf_every=4):
 
 
 @functools.lru_cache(maxsize=None)
-def _gpu(pyro_mod, prec, kind, n_or, R=256, n_burn=300, n_meas=2000, ssf_every=4, seed=31337):
+def _gpu(pyro_mod, prec, kind, n_or, R=256, n_burn=300, n_meas=2000, ssf_every=4, seed=31337, fused=False):
     ctx = pyro_mod.Context.pyrochlore(L, n_replicas=R, precision=prec, seed=seed)
     ctx.define_J1J2J3(*J)
     ctx.setup()
+    ctx.set_fused_passes(fused)
     ctx.set_temperatures(np.full(R, T))
     ctx.randomize(seed + 1)
     ctx.ssf_configure(("xx", "yy", "zz"), n_slots=R, store_error_term=False)
@@ -115,15 +116,18 @@ def _compare(a, b, n_sigma=N_SIGMA):
 
 
 @pytest.mark.parametrize(
-    "prec,kind,n_or",
-    [(32, 0, 1), (32, 0, 2), (32, 1, 1), (32, 2, 1), (64, 0, 1), (64, 1, 1)],
+    "prec,kind,n_or,fused",
+    [(32, 0, 1, False), (32, 0, 2, False), (32, 1, 1, False), (32, 2, 1, False), (64, 0, 1, False),
+     (64, 1, 1, False), (32, 0, 2, True), (32, 1, 1, True)],
     ids=["fp32-metropolis+or", "fp32-metropolis+2or", "fp32-lifted+or", "fp32-lifted_rot+or",
-         "fp64-metropolis+or", "fp64-lifted+or"],
+         "fp64-metropolis+or", "fp64-lifted+or", "fp32-metropolis+2or-fused", "fp32-lifted+or-fused"],
 )
-def test_sampled_observables_match_reference_sampler(gpu, prec, kind, n_or):
+def test_sampled_observables_match_reference_sampler(gpu, prec, kind, n_or, fused):
+    """The fused cases run the per-colour launches in the fused pass order
+    (pyro_set_fused_passes), the order of the tempering workload."""
     ref = _reference(kind, n_or)
-    got = _gpu(gpu, prec, kind, n_or)
-    print(f"GPU FP{prec} kind {kind} n_or {n_or} vs reference sampler:")
+    got = _gpu(gpu, prec, kind, n_or, fused=fused)
+    print(f"GPU FP{prec} kind {kind} n_or {n_or}{' fused order' if fused else ''} vs reference sampler:")
     bad = _compare(got, ref)
     assert not bad, f"outside {N_SIGMA} sigma: {bad}"
     # the error bars must be small enough for the comparison to mean something
