@@ -247,6 +247,12 @@ def large_tempering_run(pyro, rounds=192, buffer=192):
     }
 
 
+# The uneven batch's Metropolis + OR sweeps in the fused pass order: one
+# stage per colour (a launch per colour on the small lattices, a colour
+# stage per tile on L = 16 and 24) instead of two.
+BATCH_FUSED = True
+
+
 def uneven_batch(pyro):
     """256 independent anneals with L drawn from {4, 8, 12, 16, 24}, J1 = 1,
     J2, J3 uniform in [-0.2, 0.2], |B| uniform in [0, 0.5] along a random
@@ -257,10 +263,12 @@ def uneven_batch(pyro):
     from pyro_cmc_amd import batch
 
     items = batch.uneven_items(256, seed=4)
-    _, st = batch.run_anneal_batch(items, seed=4)
+    _, st = batch.run_anneal_batch(items, seed=4, fused=BATCH_FUSED)
     return {
         "workload": "256 anneals, L in {4,8,12,16,24}, random J2/J3 in [-0.2,0.2], |B| in [0,0.5], "
                     "T 10->0.05 over 100 steps, 16 sweeps/step, 64 SSF samples at T_cold, FP32",
+        "pass_order": "fused: per colour, each site's Metropolis and OR update from one field evaluation"
+                      if BATCH_FUSED else "pass by pass",
         "site_updates_per_s": st["site_updates_per_s"],
         "wall_s": st["wall_s"],
         "site_updates": st["site_updates"],

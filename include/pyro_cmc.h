@@ -312,10 +312,9 @@ int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
  * is a fixed rule, never a timing choice.  pyro_get_tile_shape returns the
  * shape in use (0, 0, 0 when none applies). */
 int pyro_set_tile_shape(pyro_ctx* ctx, const int32_t T[3]);
-/* Pass order of the pyrochlore stencil sweeps (strategies 3 and 5; the
- * generic strategies 1 and 2 always run pass by pass, and a forced
- * strategy 4 with OR passes in this order reports PYRO_E_UNSUPPORTED; the
- * autotuner then keeps strategy 3).  0 (default): pass by pass, the order
+/* Pass order of the pyrochlore stencil sweeps (strategies 3, 4 and 5; the
+ * generic strategies 1 and 2 always run pass by pass; 3 and 4 stay
+ * bit-identical in either order).  0 (default): pass by pass, the order
  * the reference's sweeps follow (MC.cpp:155-163, 283-303: every site's
  * first-kind update, then every site's OR update).  1 (fused): within each
  * colour stage every site takes its first-kind update and then its first

@@ -84,8 +84,11 @@ def uneven_items(n=256, sizes=(4, 8, 12, 16, 24), seed=0):
 
 def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_burn_in=16, n_sample=64,
                      n_overrelax=1, kind=METROPOLIS, precision=FP32, correlators=("xx", "yy", "zz"),
-                     seed=0, keep_spins=False, strategy=0, T_sample=None):
-    """Run every item's anneal; returns (results in item order, stats)."""
+                     seed=0, keep_spins=False, strategy=0, T_sample=None, fused=False):
+    """Run every item's anneal; returns (results in item order, stats).
+    fused: the sweeps in the fused pass order (pyro_set_fused_passes: per
+    colour stage each site's first-kind and first OR update from one field
+    evaluation) instead of pass by pass."""
     T_sample = [T_cold] if T_sample is None else list(T_sample)
     grid, s_idx = generate_T_profile(T_hot, T_cold, T_sample, n_steps)
     n_ts = len(s_idx)
@@ -117,6 +120,7 @@ def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_
         c.define_J1J2J3(1.0, 1.0, 1.0)
         c.setup()
         c.set_strategy(strategy)
+        c.set_fused_passes(fused)
         c.set_replica_couplings(np.array([items[i].J for i in idx], np.float64))
         for r, i in enumerate(idx):
             c.set_field(np.array(items[i].B, np.float64), r)

@@ -505,7 +505,7 @@ struct pyro_ctx {
             const auto J = pyro_couplings<Real>();
             timed(kFamSweep, [&] {
                 return pyro::launch_pyro_resident<Real>(st, J, kind, n_or, n_sweeps, pass_index, stream,
-                                                        epart_last);
+                                                        epart_last, fused_order(4, n_or));
             });
             return;
         }
@@ -535,9 +535,9 @@ struct pyro_ctx {
         }
     }
     // Whether sweeps of strategy strat with n_or OR passes run in the fused
-    // pass order (pyro_set_fused_passes): the stencil strategies 3 and 5,
+    // pass order (pyro_set_fused_passes): the stencil strategies 3, 4 and 5,
     // with at least one OR pass to pair with the first kind.
-    bool fused_order(int strat, int n_or) const { return fused_passes && n_or >= 1 && (strat == 3 || strat == 5); }
+    bool fused_order(int strat, int n_or) const { return fused_passes && n_or >= 1 && strat >= 3; }
     void run_sweeps(int strat, int kind, int n_or, int n_sweeps, const unsigned long long* pass_base,
                     unsigned long long pass_index, double* epart_last = nullptr) {
         if (prec == 64) sweeps<double>(strat, kind, n_or, n_sweeps, pass_base, pass_index, epart_last);
@@ -563,8 +563,6 @@ struct pyro_ctx {
                     "the pyrochlore resident strategy needs at most 2048 cells");
             require(strategy != 5 || tile_kind > 0, PYRO_E_UNSUPPORTED,
                     "the tiled strategy needs each L divisible by twice the tile edge (8 or 12)");
-            require(strategy != 4 || !fused_passes || n_or == 0, PYRO_E_UNSUPPORTED,
-                    "the fused pass order runs with strategies 3 and 5, not 4");
             return strategy;
         }
         // Large lattices take the tiled schedule (a fixed rule, so results do
@@ -579,8 +577,7 @@ struct pyro_ctx {
         std::vector<int> cand;
         if (spec) {
             cand.push_back(3);
-            // the fused pass order has no resident form: 3 alone
-            if (resident_fits() && !fused_order(3, n_or)) cand.push_back(4);
+            if (resident_fits()) cand.push_back(4);
         } else {
             cand = {1, 2};
         }
@@ -2097,7 +2094,7 @@ int pyro_set_fused_passes(pyro_ctx* ctx, int on) {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
         if (ctx->fused_passes != (on != 0)) {
             ctx->invalidate_graph();
-            ctx->tuned_key = -1;  // the candidates depend on the order
+            ctx->tuned_key = -1;  // time the candidates again in the new order
         }
         ctx->fused_passes = on != 0;
     });

@@ -108,10 +108,11 @@ __host__ __device__ constexpr int pyro_colour_energy_blocks(int n_cells) { retur
 // thread) per replica; lattices of at most 2048 cells.  With epart the last
 // pass also leaves the energy partials launch_pyro_colour leaves, bit for bit.
 constexpr int kPyroResidentMaxCells = 2048;
+// fused: the fused pass order of launch_pyro_colour, bit for bit.
 template <typename Real>
 hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,
                                 int n_or, int n_sweeps, unsigned long long pass_index,
-                                hipStream_t s, double* epart = nullptr);
+                                hipStream_t s, double* epart = nullptr, bool fused = false);
 
 // Tiled sweeps: one call launches one of the 8 checkerboard phases of a
 // sweep; its workgroups hold a tile + one-sided halo in LDS and run every

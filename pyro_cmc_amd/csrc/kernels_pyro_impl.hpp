@@ -432,13 +432,48 @@ __global__ __launch_bounds__(256) void pyro_fields_kernel(DeviceState<Real> st, 
     out[2 * N + i] = hz;
 }
 
+// The update of one site by a resident workgroup, with the draws and
+// arithmetic of pyro_colour_kernel: KIND from block `pass` (an OR pass
+// alone: component ucomp of it) and with FUSED (n_or >= 1) the first OR
+// update after it from the sweep's uniform kUniformsOf[KIND]; the spin
+// (x, y, z) is updated and stored when it changed.  Returns whether the
+// KIND proposal was accepted.
+template <typename Real, int KIND, bool FUSED>
+__device__ __forceinline__ bool resident_update(const ReplicaParams<Real>& p, const DeviceState<Real>& st, Real& x,
+                                                Real& y, Real& z, int site, int r, unsigned long long pass, int ucomp,
+                                                int n_or, Real hx, Real hy, Real hz, Real* __restrict__ Sr,
+                                                size_t N) {
+    Uniform5 ub = site_uniforms(st, site, r, pass);
+    if constexpr (KIND == 3) ub.m[0] = uniform_at(ub, ucomp);
+    bool changed;
+    const bool acc = update_spin_u<Real, KIND>(p, x, y, z, changed, ub, hx, hy, hz, st.dir + size_t(r) * N + site);
+    if constexpr (FUSED && KIND != 3) {
+        if (n_or >= 1) {
+            constexpr int j = kUniformsOf[KIND];
+            if constexpr (j % 5 == 0) ub = site_uniforms(st, site, r, pass + unsigned(j / 5));
+            Uniform5 uo;
+            uo.m[0] = ub.m[j % 5];
+            bool ch;
+            update_spin_u<Real, 3>(p, x, y, z, ch, uo, hx, hy, hz, nullptr);
+            changed = changed || ch;
+        }
+    }
+    if (changed) st3(Sr, unsigned(site), x, y, z);
+    return acc;
+}
+
 // One workgroup per replica; thread t owns cells t, t + BS, ... (CPT cells).
 // EN: the last pass of the last sweep also leaves the energy partials of
 // pyro_colour_kernel<..., EN>, at the same indices and with the same
 // per-site arithmetic and wave sums: wave w of the workgroup covers cells
 // 64 w .. 64 w + 63 + j BS, which is block j (BS / 64) + w of the per-colour
 // launch, so the two strategies give bit-identical energies.
-template <typename Real, int SHELLS, int FIRST, int BS, int CPT, bool EN>
+// FUSED: the fused pass order of pyro_colour_kernel<..., FUSED> (stage
+// group 0: each site's FIRST update and its first OR update; group g >= 1:
+// OR pass g + 1, drawing uniform kUniformsOf[FIRST] + g of the sweep), with
+// the same draws and arithmetic, so it too is bit-identical to the
+// per-colour launches.
+template <typename Real, int SHELLS, int FIRST, int BS, int CPT, bool EN, bool FUSED>
 __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                                                            PyroCouplings<Real> J, int n_or,
                                                            int n_sweeps,
@@ -451,7 +486,7 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     const int nc = st.n_cells;
     const size_t N = size_t(st.N);
     Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
-    unsigned long long pass = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    const unsigned long long pass_start = pass_index + (st.pass_base ? *st.pass_base : 0ull);
     const ReplicaParams<Real> rp = replica_params(st, r);
     unsigned int my_acc = 0;
     static_assert(CPT == 2, "the resident kernel owns two cells per thread");
@@ -460,9 +495,22 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
     const CellAxes ax_b = cell_axes_of(cell_b, st.L0, st.L1, st.L2);
     const int bps = (nc + 63) / 64;
     const size_t es = size_t(pyro_colour_energy_blocks(nc));
+    // stage groups per sweep: every pass alone, or (fused) the first kind
+    // with the first OR update, then the further OR passes
+    const int n_groups = FUSED ? (n_or > 1 ? n_or : 1) : n_or + 1;
     for (int sw = 0; sw < n_sweeps; ++sw) {
-        for (int ps = 0; ps <= n_or; ++ps, ++pass) {
-            const bool en_pass = EN && sw == n_sweeps - 1 && ps == n_or;  // uniform
+        const unsigned long long p0 = pass_start + (unsigned long long)sw * unsigned(1 + n_or);
+        for (int grp = 0; grp < n_groups; ++grp) {
+            const bool en_pass = EN && sw == n_sweeps - 1 && grp == n_groups - 1;  // uniform
+            // this group's block and, for an OR pass alone, the component
+            // of it that the pass draws
+            int ucomp = 0;
+            unsigned long long pass = p0 + unsigned(grp);
+            if (FUSED && grp > 0) {
+                const int j = kUniformsOf[FIRST] + grp;
+                pass = p0 + unsigned(j / 5);
+                ucomp = j % 5;
+            }
             for (int col = 0; col < NCOL; ++col) {
                 for (int q = 0; q < SPC; ++q) {
                     const int sl = NCOL == 16 ? col : col + 4 * q;
@@ -489,14 +537,13 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                                     const int site = sl * nc + cell;
                                     const Vec3<Real> v = ld3(Sr, unsigned(site));
                                     Real x = v.x, y = v.y, z = v.z;
-                                    bool changed;
-                                    const bool a = ps == 0
-                                        ? update_spin_p<Real, FIRST>(rp, st, x, y, z, changed, site, r, pass, hx, hy,
-                                                                     hz, st.dir + size_t(r) * N + site)
-                                        : update_spin_p<Real, 3>(rp, st, x, y, z, changed, site, r, pass, hx, hy,
-                                                                 hz, st.dir + size_t(r) * N + site);
-                                    if (ps == 0 && a) ++my_acc;
-                                    if (changed) st3(Sr, unsigned(site), x, y, z);
+                                    if (grp == 0 && FIRST != 3)
+                                        my_acc += resident_update<Real, FIRST, FUSED>(rp, st, x, y, z, site, r, pass,
+                                                                                     0, n_or, hx, hy, hz, Sr, N)
+                                                      ? 1u : 0u;
+                                    else
+                                        resident_update<Real, 3, false>(rp, st, x, y, z, site, r, pass, ucomp, n_or, hx,
+                                                                        hy, hz, Sr, N);
                                     e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, x, y, z);
                                 }
                                 const double t = wave_sum_f64(e);
@@ -509,12 +556,16 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
                         if (cell >= nc) break;
                         Real hx, hy, hz;
                         pyro_field_rt<Real, SHELLS>(sl, Sr, nc, c, Jr, hx, hy, hz);
-                        if (ps == 0)
-                            my_acc += apply_update_p<Real, FIRST>(rp, st, Sr, sl * nc + cell, r, pass, hx, hy, hz)
-                                          ? 1u
-                                          : 0u;
+                        const int site = sl * nc + cell;
+                        const Vec3<Real> v = ld3(Sr, unsigned(site));
+                        Real x = v.x, y = v.y, z = v.z;
+                        if (grp == 0 && FIRST != 3)
+                            my_acc += resident_update<Real, FIRST, FUSED>(rp, st, x, y, z, site, r, pass, 0, n_or, hx,
+                                                                         hy, hz, Sr, N)
+                                          ? 1u : 0u;
                         else
-                            apply_update_p<Real, 3>(rp, st, Sr, sl * nc + cell, r, pass, hx, hy, hz);
+                            resident_update<Real, 3, false>(rp, st, x, y, z, site, r, pass, ucomp, n_or, hx, hy, hz,
+                                                            Sr, N);
                     }
                 }
                 __syncthreads();
@@ -993,24 +1044,28 @@ hipError_t launch_pyro_fields(const DeviceState<Real>& st, const PyroCouplings<R
 template <typename Real>
 hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,
                                 int n_or, int n_sweeps, unsigned long long pass_index,
-                                hipStream_t s, double* epart) {
+                                hipStream_t s, double* epart, bool fused) {
     constexpr int BS = 1024, CPT = 2;
     if (st.n_cells > BS * CPT) return hipErrorInvalidValue;
     const dim3 grid(unsigned(st.R));
-    auto go = [&](auto en) -> hipError_t {
+    auto go = [&](auto en, auto fu) -> hipError_t {
         constexpr bool EN = decltype(en)::value;
+        constexpr bool FU = decltype(fu)::value;
         return dispatch_pyro(J, [&](auto shells) -> hipError_t {
             constexpr int SH = decltype(shells)::value;
             switch (kind) {
-                case 0: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 0, BS, CPT, EN>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
-                case 1: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 1, BS, CPT, EN>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
-                case 2: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 2, BS, CPT, EN>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
+                case 0: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 0, BS, CPT, EN, FU>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
+                case 1: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 1, BS, CPT, EN, FU>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
+                case 2: hipLaunchKernelGGL((pyro_resident_kernel<Real, SH, 2, BS, CPT, EN, FU>), grid, dim3(BS), 0, s, st, J, n_or, n_sweeps, pass_index, epart); break;
                 default: return hipErrorInvalidValue;
             }
             return hipGetLastError();
         });
     };
-    return epart ? go(std::true_type{}) : go(std::false_type{});
+    auto by_fused = [&](auto en) -> hipError_t {
+        return fused && n_or >= 1 ? go(en, std::true_type{}) : go(en, std::false_type{});
+    };
+    return epart ? by_fused(std::true_type{}) : by_fused(std::false_type{});
 }
 
 template <typename Real>

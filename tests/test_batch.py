@@ -149,14 +149,16 @@ def _anneal_eager(pyro, ctx, sched, kind, n_or, n_sweep, ssf_from):
     return np.array(E).T
 
 
-@pytest.mark.parametrize("prec,strategy", [(32, 3), (64, 4), (32, 5)])
-def test_anneal_steps_equal_eager_steps(gpu, oracle, prec, strategy):
+@pytest.mark.parametrize("prec,strategy,fused", [(32, 3, False), (64, 4, False), (32, 5, False), (32, 3, True),
+                                                (64, 4, True), (32, 5, True)])
+def test_anneal_steps_equal_eager_steps(gpu, oracle, prec, strategy, fused):
     L = 8 if strategy == 5 else 3
     sched = np.geomspace(3.0, 0.1, 9)
     out = []
     for graph in (True, False):
         ctx, _ = _batch(gpu, oracle, L, prec, seed=77)
         ctx.set_strategy(strategy)
+        ctx.set_fused_passes(fused)
         ctx.randomize(5)
         ctx.ssf_configure(("xx", "zz", "xy"), n_slots=ctx.R)
         if graph:

@@ -194,18 +194,27 @@ def test_sweep_fp32_tracks_fp64(gpu, oracle, kind, n_or):
     assert np.allclose(np.linalg.norm(ctx.get_spins(), axis=1), 1.0, atol=1e-6)
 
 
-@pytest.mark.parametrize("pair,J,R,prec", [((1, 2), (1.0, 0.1, 0.1), 4, 32), ((3, 4), (1.0, 0.1, 0.1), 4, 32),
-                                            ((3, 4), (1.0, -0.2, 0.0), 4, 32), ((3, 4), (-1.0, 0.0, 0.0), 4, 32),
-                                            ((3, 4), (1.0, -0.05, 0.05), 16, 64), ((1, 2), (1.0, 0.1, 0.0), 8, 64)])
-def test_strategies_bit_identical(gpu, pair, J, R, prec):
+@pytest.mark.parametrize("pair,J,R,prec,fused", [((1, 2), (1.0, 0.1, 0.1), 4, 32, False),
+                                                  ((3, 4), (1.0, 0.1, 0.1), 4, 32, False),
+                                                  ((3, 4), (1.0, -0.2, 0.0), 4, 32, False),
+                                                  ((3, 4), (-1.0, 0.0, 0.0), 4, 32, False),
+                                                  ((3, 4), (1.0, -0.05, 0.05), 16, 64, False),
+                                                  ((1, 2), (1.0, 0.1, 0.0), 8, 64, False),
+                                                  # the fused pass order
+                                                  ((3, 4), (1.0, -0.05, 0.05), 8, 32, True),
+                                                  ((3, 4), (1.0, 0.1, 0.0), 4, 32, True),
+                                                  ((3, 4), (1.0, -0.05, 0.05), 4, 64, True)])
+def test_strategies_bit_identical(gpu, pair, J, R, prec, fused):
     """Per-colour launches (1, 3) and resident workgroups (2, 4): spins,
-    lifted directions and accept counts of both pass mixes agree bit for bit."""
+    lifted directions and accept counts of both pass mixes agree bit for
+    bit, pass by pass and (3, 4) in the fused pass order."""
     res = []
     T = np.geomspace(0.1, 3.0, R)
     for strategy in pair:
         ctx = gpu.Context.pyrochlore(3, n_replicas=R, precision=prec, seed=99)
         ctx.define_J1J2J3(*J)
         ctx.setup()
+        ctx.set_fused_passes(fused)
         ctx.set_strategy(strategy)
         ctx.set_temperatures(T)
         ctx.randomize(5)
@@ -444,7 +453,7 @@ def test_tempering_fused_energy(gpu, oracle, prec, L, R, J, kind, n_or):
 @pytest.mark.parametrize("prec", [32, 64])
 @pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 0), (2, 1), (1, 1), (0, 0), (2, 0)])
 @pytest.mark.parametrize("per_rep", [True, False])
-@pytest.mark.parametrize("strategy", [3, 4, "3-fused"])
+@pytest.mark.parametrize("strategy", [3, 4, "3-fused", "4-fused"])
 def test_energy_pass_is_bit_identical(gpu, prec, kind, n_or, per_rep, strategy):
     """The kernels that also sum the energy (the last pass of an anneal step
     or tempering round; per-colour launches and resident workgroups) update
@@ -467,8 +476,8 @@ def test_energy_pass_is_bit_identical(gpu, prec, kind, n_or, per_rep, strategy):
                 ctx.set_field(np.array(B), r)
         else:
             ctx.set_field(np.array([0.1, -0.2, 0.3]))
-        if strategy == "3-fused":  # the fused pass order's energy pass
-            ctx.set_strategy(3)
+        if isinstance(strategy, str):  # the fused pass order's energy pass
+            ctx.set_strategy(int(strategy[0]))
             ctx.set_fused_passes(True)
         else:
             ctx.set_strategy(strategy)
@@ -777,20 +786,21 @@ def test_tiled_sweep_against_oracle_order(gpu, oracle, prec, L, J, kind, n_or, t
                              f"L {L} tile {ctx.tile_shape()} replica {r}")
 
 
+@pytest.mark.parametrize("strategy", [3, 4])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("kind,n_or", [(0, 1), (0, 2), (1, 1), (2, 1), (1, 2), (0, 3), (1, 0)])
 @pytest.mark.parametrize("J,Delta,B", [CONFIGS[1], CONFIGS[3]])
-def test_colour_fused_order_against_oracle(gpu, oracle, prec, kind, n_or, J, Delta, B):
-    """Per-colour launches (strategy 3) in the fused pass order: per colour
-    each site's first-kind and first OR update in one launch from one field,
-    then OR passes 2..n_or on their own, the sweep's uniforms drawn in order
-    from its blocks.  pyro_sweep_order reports that order, and the oracle
+def test_colour_fused_order_against_oracle(gpu, oracle, strategy, prec, kind, n_or, J, Delta, B):
+    """Per-colour launches (strategy 3) and resident workgroups (strategy 4)
+    in the fused pass order: per colour each site's first-kind and first OR
+    update in one stage from one field, then OR passes 2..n_or on their own,
+    the sweep's uniforms drawn in order from its blocks.  pyro_sweep_order reports that order, and the oracle
     replays it draw for draw: FP64 decisions, accept counts and lifted
     directions exact, FP32 every site within its first-order error bound."""
     L, R, seed = 3, 3, 0x5EED1234
     T = np.array([0.3, 0.8, 2.5])
     ctx, os_ = make(gpu, oracle, L, J, Delta, prec, R=R, seed=seed, B=B, T_ref=0.7)
-    ctx.set_strategy(3)
+    ctx.set_strategy(strategy)
     ctx.set_fused_passes(True)
     ctx.set_temperatures(T)
     rng = np.random.default_rng(17)
@@ -840,24 +850,8 @@ def test_colour_fused_order_against_oracle(gpu, oracle, prec, kind, n_or, J, Del
 
 
 def test_fused_order_strategy_rules(gpu):
-    """The fused pass order runs with per-colour launches: the autotuner
-    keeps strategy 3 (the resident strategy has no fused form), a forced
-    strategy 4 is refused for sweeps with OR passes, and a sweep without OR
-    passes is the same in either order."""
-    ctx = gpu.Context.pyrochlore(2, n_replicas=4, precision=32, seed=3)
-    ctx.define_J1J2J3(1.0, -0.05, 0.05)
-    ctx.setup()
-    ctx.randomize(1)
-    ctx.set_temperatures(np.linspace(0.3, 2.0, 4))
-    ctx.set_fused_passes(True)
-    ctx.sweep(gpu.METROPOLIS, 2, 1)
-    assert ctx.strategy() == 3
-    ctx.set_strategy(4)
-    with pytest.raises(gpu.PyroError) as e:
-        ctx.sweep(gpu.METROPOLIS, 1, 1)
-    assert e.value.code == 5
-    ctx.sweep(gpu.METROPOLIS, 0, 1)  # no OR pass: nothing to fuse
-    ctx.close()
+    """A sweep without OR passes is the same in either pass order, and
+    switching the order re-times the autotuner's pair."""
     out = []
     for fused in (True, False):
         ctx = gpu.Context.pyrochlore(2, n_replicas=4, precision=32, seed=3)
@@ -874,6 +868,17 @@ def test_fused_order_strategy_rules(gpu):
     for a, b in zip(out[0][1], out[1][1]):
         assert np.array_equal(a, b)
     assert np.array_equal(out[0][2][0], out[1][2][0]) and np.array_equal(out[0][2][1], out[1][2][1])
+    ctx = gpu.Context.pyrochlore(2, n_replicas=4, precision=32, seed=3)
+    ctx.define_J1J2J3(1.0, -0.05, 0.05)
+    ctx.setup()
+    ctx.randomize(1)
+    ctx.set_temperatures(np.linspace(0.3, 2.0, 4))
+    ctx.sweep(gpu.METROPOLIS, 2, 1)
+    assert ctx.strategy() in (3, 4)
+    ctx.set_fused_passes(True)
+    ctx.sweep(gpu.METROPOLIS, 2, 1)
+    assert ctx.strategy() in (3, 4)
+    ctx.close()
 
 
 def test_tile_shape_rules(gpu):

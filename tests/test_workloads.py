@@ -144,13 +144,17 @@ def test_large_tempering_rounds_energy_and_munu(gpu):
     ctx.close()
 
 
-def test_uneven_batch_every_item_against_its_oracle(gpu, oracle):
+@pytest.mark.parametrize("fused", [False, True])
+def test_uneven_batch_every_item_against_its_oracle(gpu, oracle, fused):
+    """All 256 items of the uneven batch, pass by pass and in the fused pass
+    order the benchmark runs: each item's final energy against an oracle of
+    its own model."""
     from pyro_cmc_amd import batch
 
     items = batch.uneven_items(256)
     assert sorted({it.L for it in items}) == [4, 8, 12, 16, 24]
     res, stats = batch.run_anneal_batch(items, n_steps=8, n_sweep=4, n_burn_in=4, n_sample=2, keep_spins=True,
-                                        seed=11)
+                                        seed=11, fused=fused)
     assert stats["contexts"] == {L: sum(it.L == L for it in items) for L in (4, 8, 12, 16, 24)}
     orcs = {}
     worst = 0.0
