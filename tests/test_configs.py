@@ -19,7 +19,15 @@ seed), with checks that do not depend on this implementation:
   benchmark's fused pass order and FP64 pass by pass;
   and the FP32 run against the reference sampler (the oracle in reference
   mode, 16 independent chains per rung, tests/golden/c3_reference.json) at
-  T = 2.0, 0.62 and 0.19.
+  T = 2.0, 0.62 and 0.19;
+- the large tempering workload, L = 48 x 16 rungs, J = (1, 0.1, 0.1),
+  B = (0, 0, 0.2), T log-spaced in [0.05, 1], sweeps of 1 lifted Metropolis
+  + 1 OR, exchange every 2 sweeps, six independent ladders per precision:
+  FP32 in the fused pass order (the benchmark's sweep) against FP64 pass by
+  pass at every rung for <E>/N, C_v/N and S(q) at three wavevectors, within
+  3 sigma of the seed-to-seed errors.  The CPU reference sampler would take
+  minutes per sweep of a ladder at this size, so the all-FP64 GPU path is
+  the reference here.
 """
 import numpy as np
 import pytest
@@ -175,3 +183,68 @@ def test_headline_tempering_matches_reference_sampler(headline_runs):
     zs = np.array(zs)
     # 15 comparisons: one beyond 3 sigma is a 2-3% event; none beyond 4
     assert np.sum(zs > 3) <= 1 and zs.max() < 4, zs
+
+
+def _large_tempering_copies(gpu, precision, seed, fused, n_copies=N_COPIES, n_burn=64, n_rounds=192):
+    """The large tempering workload for n_copies independent ladders stacked
+    in one context: L = 48, 16 rungs per ladder at T log-spaced in
+    [0.05, 1], J = (1, 0.1, 0.1), B = (0, 0, 0.2), rounds of 2 sweeps (1
+    lifted Metropolis + 1 OR each) with an exchange and one SSF sample per
+    round.  The pairs joining two ladders sit between T = 0.05 and T = 1,
+    where a swap needs exp(-19 (E_hot - E_cold) L^3) and never happens
+    (checked).  Returns T[16] and per ladder and rung <E>/cell, <E^2>/cell^2
+    and S_xx + S_yy + S_zz at K = (0, 0, 0), (0, 0, 1), (0, 1, 1)."""
+    L, R0 = 48, 16
+    R = R0 * n_copies
+    T = np.geomspace(1.0, 0.05, R0)
+    ctx = gpu.Context.pyrochlore(L, n_replicas=R, precision=precision, seed=seed)
+    ctx.define_J1J2J3(1.0, 0.1, 0.1)
+    ctx.setup()
+    ctx.set_fused_passes(fused)
+    ctx.set_field(np.array([0.0, 0.0, 0.2]))
+    ctx.randomize(seed + 1)
+    ctx.pt_init(np.tile(T, n_copies), seed + 2)
+    ctx.ssf_configure(("xx", "yy", "zz"), n_slots=R, store_error_term=False)
+    ctx.pt_rounds(gpu.LIFTED, 1, 2, True, n_burn)
+    ctx.pt_reset_stats()
+    ctx.ssf_reset()
+    ctx.pt_rounds(gpu.LIFTED, 1, 2, True, n_rounds)
+    st = ctx.pt_stats()
+    assert np.all(st["n"] == n_rounds)
+    boundary = np.arange(R0 - 1, R - 1, R0)
+    assert np.all(st["accepts"][boundary] == 0) and np.all(st["attempts"][boundary] > 0)
+    ks = [0, 1, L + 1]
+    S = []
+    for slot in range(R):
+        Ss, _, n = ctx.ssf_read(slot, with_sq=False)
+        assert n == n_rounds
+        S.append(Ss[:3, ks, 0].sum(axis=0) / n)
+    ctx.close()
+    shape = (n_copies, R0)
+    return (T, (st["E"] / n_rounds).reshape(shape), (st["E2"] / n_rounds).reshape(shape),
+            np.array(S).reshape(n_copies, R0, 3), L ** 3)
+
+
+@pytest.mark.gpu
+def test_large_tempering_fp32_matches_fp64(gpu):
+    """FP32 (fused pass order, as the benchmark runs it) against FP64 (pass
+    by pass) at all 16 rungs of the large tempering workload, each side's
+    error the spread of its independent ladders.  384 measured sweeps do
+    not equilibrate the coldest rungs of a 1.8-million-spin lattice: there
+    E / N still drifts during the measurement and the "C_v" of the run is
+    mostly that drift, which the two precisions must then show alike."""
+    (T, E32, E232, S32, nc) = _large_tempering_copies(gpu, gpu.FP32, 61, True)
+    (_, E64, E264, S64, _) = _large_tempering_copies(gpu, gpu.FP64, 81, False)
+    a, b = _stats(T, E32, E232, S32, nc), _stats(T, E64, E264, S64, nc)
+    z = np.array([np.abs(m1 - m2) / np.sqrt(s1 ** 2 + s2 ** 2 + 1e-300) for (m1, s1), (m2, s2) in zip(a, b)])
+    assert np.all(np.isfinite(z))
+    worst = np.unravel_index(np.argmax(z), z.shape)
+    print(f"L=48 FP32 vs FP64, {z.size} comparisons over all {len(T)} rungs: {int((z > 3).sum())} beyond 3 "
+          f"sigma, largest {z.max():.2f} (observable {worst[0]}, T = {T[worst[1]]:.3g})")
+    for name, (m1, s1), (m2, s2) in zip(("E/N", "Cv/N", "S0/N", "S1/N", "S2/N"), a, b):
+        for i in (0, 8, 15):
+            print(f"  T = {T[i]:.4g} {name}: FP32 {m1[i]:.6g}({s1[i]:.2g}) FP64 {m2[i]:.6g}({s2[i]:.2g})")
+    # 6 ladders per side (t with about 10 degrees of freedom): 80
+    # comparisons, about one beyond 3 sigma expected
+    assert np.mean(z > 3) < 0.05, (np.sort(z.ravel())[-8:], z.size)
+    assert z.max() < 6, np.sort(z.ravel())[-8:]
