@@ -452,7 +452,13 @@ def main():
     n_sw = max(16, min(args.steps, 256))
     ctx.synchronize()
     ctx.mark(0)
-    ctx.sweep(pyro.METROPOLIS, N_OR, n_sw, count=False)
+    if strategy == 6:
+        # the XCD-resident kernel runs a round's sweeps in one launch: time
+        # back-to-back launches of N_SWEEP sweeps, as the rounds issue them
+        for _ in range(n_sw // N_SWEEP):
+            ctx.sweep(pyro.METROPOLIS, N_OR, N_SWEEP, count=False)
+    else:
+        ctx.sweep(pyro.METROPOLIS, N_OR, n_sw, count=False)
     ctx.mark(1)
     sweep_ms = ctx.elapsed_ms()
     # colour launches per sweep: one per colour and pass, or in the fused
@@ -460,16 +466,16 @@ def main():
     # colour for each further OR pass
     groups = max(N_OR, 1) if C3_FUSED else 1 + N_OR
     sweep_launches = {1: n_sw * (1 + N_OR) * info["n_colours"], 3: n_sw * groups * info["n_colours"],
-                      2: 1, 4: 1, 5: 8 * n_sw}[strategy]
+                      2: 1, 4: 1, 5: 8 * n_sw, 6: n_sw // N_SWEEP}[strategy]
     upd_per_launch = N * R_REP * (1 + N_OR) * n_sw / sweep_launches
     avg_launch_s = sweep_ms / 1e3 / sweep_launches
     # compulsory bytes per site a launch visits (its spin read and written
     # once, whatever the number of updates it takes there, and its share of
     # the neighbour row); sites per launch = the launch's colour class
     b_alg = bytes_per_update(4, info["n_slot"], R_REP)
-    sites_per_launch = N * R_REP * n_sw * (groups if strategy == 3 else 1 + N_OR) / sweep_launches
+    sites_per_launch = N * R_REP * n_sw * (groups if strategy in (3, 6) else 1 + N_OR) / sweep_launches
     achieved = b_alg * sites_per_launch / avg_launch_s
-    traffic_c3 = profiled_traffic("sweep_c3_L12x64")
+    traffic_c3 = profiled_traffic("sweep_c3_L12x64" + ("_xcd" if strategy == 6 else ""))
     st = ctx.pt_stats()
     acc_rate = (st["accepts"] / np.maximum(st["attempts"], 1)).tolist()
 
@@ -521,14 +527,14 @@ def main():
             # the measured bytes per launch moved in the measured launch time
             "traffic_rate_frac": (traffic_c3 / avg_launch_s / PEAK_HBM) if traffic_c3 else None,
             "kernel": {1: "sweep_colour_kernel", 2: "sweep_resident_kernel", 3: "pyro_colour_kernel",
-                       4: "pyro_resident_kernel", 5: "pyro_tiled_kernel"}[strategy],
+                       4: "pyro_resident_kernel", 5: "pyro_tiled_kernel", 6: "pyro_xcd_kernel"}[strategy],
             "bytes_per_site": b_alg,
             "sites_per_launch": sites_per_launch,
             "updates_per_launch": upd_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,
             # the HIP-event sample avg_launch_us comes from
             "launch_timing": {"sweeps": n_sw, "launches": sweep_launches, "elapsed_ms": sweep_ms,
-                              "launches_per_round": sweep_launches // n_sw * N_SWEEP},
+                              "launches_per_round": sweep_launches * N_SWEEP // n_sw},
         },
         "kernel_ms": {k: v[0] for k, v in prof.items()},
         "kernel_profiled_rounds": prof_steps,

@@ -291,17 +291,29 @@ int pyro_spin_ft(pyro_ctx* ctx, int replica, double* ft);
 /* Execution control and timing.                                        */
 /* ------------------------------------------------------------------ */
 int pyro_synchronize(pyro_ctx* ctx);
-/* Sweep launch strategy: 0 = autotuned (default), 1 = one launch per
+/* Sweep launch strategy: 0 = automatic (default), 1 = one launch per
  * colour (neighbour table), 2 = one resident workgroup per replica looping
  * over all colours of a sweep (neighbour table), 3 = one launch per colour
  * with the compile-time pyrochlore stencil, 4 = resident workgroup per
- * replica with the stencil (at most 2048 cells).  3 and 4 apply when the
- * couplings were defined by pyro_define_J1J2J3 alone.  Strategies 1 and 2
- * give bit-identical results, as do 3 and 4 (same colour order, per-site
- * counters and arithmetic); the autotuner only chooses within a pair, so it
- * never changes results.  Sweeps report PYRO_E_UNSUPPORTED for a forced
- * strategy that does not apply. */
+ * replica with the stencil (at most 2048 cells), 5 = tiled sweep (tiles and
+ * their halo in LDS, 8 checkerboard phases; lattices a tile shape divides),
+ * 6 = XCD-resident sweep (the per-colour work of 3 for whole sweeps in one
+ * launch, every workgroup of a replica on one XCD, the colour stages
+ * separated by waits on counters in that XCD's L2; when the grid fits on
+ * the device at once).  3 to 6 apply when the couplings were defined by
+ * pyro_define_J1J2J3 alone.  Strategies 1 and 2 give bit-identical results,
+ * as do 3, 4 and 6 (same colour order, per-site counters and arithmetic);
+ * the automatic choice takes 5 on lattices of at least 4096 cells that a
+ * tile shape divides, and otherwise times the members of its bit-identical
+ * set and keeps the fastest, so it never changes results.  Sweeps report
+ * PYRO_E_UNSUPPORTED for a forced strategy that does not apply. */
 int pyro_set_strategy(pyro_ctx* ctx, int strategy);
+/* Bound, in seconds, on each wait between the colour stages of the
+ * XCD-resident sweep (strategy 6; default 0.25 s).  A wait past it ends the
+ * launch, and the next call that reads results reports PYRO_E_HIP ("a stage
+ * wait timed out") instead of returning them.  No counterpart in the
+ * reference (its sweeps have no waits). */
+int pyro_set_wait_timeout(pyro_ctx* ctx, double seconds);
 int pyro_get_strategy(pyro_ctx* ctx, int* strategy);
 /* Tile shape of the tiled sweep (strategy 5), in cells per axis: one of
  * 4x4x4, 6x6x4, 8x4x4, 4x4x8, 8x8x4, applicable when each L_a is divisible

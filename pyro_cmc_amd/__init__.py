@@ -78,6 +78,7 @@ _SIGS = {
     "pyro_get_strategy": [_vp, C.POINTER(C.c_int)],
     "pyro_set_tile_shape": [_vp, _vp],
     "pyro_set_fused_passes": [_vp, C.c_int],
+    "pyro_set_wait_timeout": [_vp, C.c_double],
     "pyro_get_tile_shape": [_vp, _i32p],
     "pyro_profile_enable": [_vp, C.c_int],
     "pyro_profile_read": [_vp, _f64p, _u64p],
@@ -438,6 +439,11 @@ class Context:
         """Tile shape (cells per axis) of the tiled sweep; None = automatic."""
         t = None if T is None else np.ascontiguousarray(T, np.int32)
         _check(lib().pyro_set_tile_shape(self._h, _ptr(t)))
+
+    def set_wait_timeout(self, seconds):
+        """Bound on each stage wait of the XCD-resident sweep (strategy 6)
+        (pyro_set_wait_timeout)."""
+        _check(lib().pyro_set_wait_timeout(self._h, float(seconds)))
 
     def set_fused_passes(self, on=True):
         """Tiled sweeps: every pass of a site back to back within its colour

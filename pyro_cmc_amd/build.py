@@ -42,10 +42,19 @@ SOURCES = [
     "kernels_pyro_resident_f64.hip",
     "kernels_pyro_tiled_f32.hip",
     "kernels_pyro_tiled_f64.hip",
+    "kernels_pyro_xcd_f32.hip",
+    "kernels_pyro_xcd_f64.hip",
     "lt.hip",
     "capi.cpp",
     "topology.cpp",
 ]
+
+# per-source flags: the XCD-resident kernels use the threadgroup-split
+# memory model (a workgroup-scope acquire invalidates the CU's L1)
+EXTRA_FLAGS = {
+    "kernels_pyro_xcd_f32.hip": ["-mtgsplit"],
+    "kernels_pyro_xcd_f64.hip": ["-mtgsplit"],
+}
 
 LINK_LIBS: list[str] = []
 
@@ -64,7 +73,7 @@ def build_library() -> Path:
     for src in SOURCES:
         obj = BUILD / (src + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)])
+        cmds.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         list(ex.map(_run, cmds))

@@ -185,6 +185,12 @@ struct pyro_ctx {
     DevMem acc_wave;          // [R][acc_waves] per-wave accept counters
     int acc_waves = 0;        // wave slots per replica (largest per-colour grid)
     bool count_acc = false;   // sweeps count accepted updates (pyro_sweep with a buffer)
+    // XCD-resident sweeps (strategy 6): per-replica stage counters, the
+    // kernel's error flag, its wait bound in wall-clock ticks, and whether
+    // such a launch has run on this context (then reads check the flag)
+    DevMem xcd_bar, xcd_err;
+    unsigned long long xcd_ticks = 0;
+    bool xcd_used = false;
     DevMem epart, e, Esum, E2sum, nE, att, accp;
     DevMem S_save, dir_save, fields;
     std::vector<int32_t> h_cls;          // all class descriptors, colour-major
@@ -300,6 +306,19 @@ struct pyro_ctx {
     void d2h(void* dst, const void* src, size_t n) {
         HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
+        check_xcd();
+    }
+    // Results read after XCD-resident sweeps are valid only if none of their
+    // waits timed out and every workgroup ran on its XCD.
+    void check_xcd() {
+        if (!xcd_used) return;  // set for good once strategy 6 has run (graph replays included)
+        int flag = 0;
+        HIPCHK(hipMemcpyAsync(&flag, xcd_err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (flag == 0) return;
+        HIPCHK(hipMemsetAsync(xcd_err.p, 0, sizeof(int), stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        throw Status(PYRO_E_HIP, "XCD-resident sweep: a stage wait timed out; results are invalid");
     }
 
     hipEvent_t get_event() {
@@ -509,6 +528,21 @@ struct pyro_ctx {
             });
             return;
         }
+        if (strat == 6) {
+            const auto st = state<Real>(pass_base);
+            const auto J = pyro_couplings<Real>();
+            xcd_used = true;
+            timed(kFamSweep, [&] {
+                const hipError_t e = pyro::launch_pyro_xcd<Real>(st, J, kind, n_or, n_sweeps, pass_index, stream,
+                                                                 epart_last, fused_order(6, n_or),
+                                                                 xcd_bar.as<unsigned int>(), xcd_err.as<int>(),
+                                                                 xcd_ticks);
+                require(e != hipErrorNotSupported, PYRO_E_UNSUPPORTED,
+                        "the XCD-resident strategy needs its whole grid resident on the device at once");
+                return e;
+            });
+            return;
+        }
         unsigned long long p = pass_index;
         if (strat == 3 && fused_order(strat, n_or)) {
             // the fused pass order: per colour the first kind and the first
@@ -548,7 +582,9 @@ struct pyro_ctx {
     // 3 and 4, which leave identical partials (so the autotuner's choice
     // between them never changes a sampled energy), with the observables on
     // the stencil too.
-    bool sweeps_leave_energy(int strat) const { return (strat == 3 || strat == 4) && stencil_observables(); }
+    bool sweeps_leave_energy(int strat) const {
+        return (strat == 3 || strat == 4 || strat == 6) && stencil_observables();
+    }
 
     // Pick the faster launch strategy for (kind, n_or) by timing both on a
     // saved copy of the state, then restore it (results do not depend on the
@@ -556,9 +592,9 @@ struct pyro_ctx {
     int choose_strategy(int kind, int n_or) {
         if (strategy != 0) {
             require(strategy <= 2 || spec, PYRO_E_UNSUPPORTED,
-                    "strategies 3 and 4 need couplings defined by define_J1J2J3 alone");
+                    "strategies 3 to 6 need couplings defined by define_J1J2J3 alone");
             require(strategy >= 3 || !has_Jrep, PYRO_E_UNSUPPORTED,
-                    "per-replica couplings need the pyrochlore strategies 3, 4 or 5");
+                    "per-replica couplings need the pyrochlore strategies 3 to 6");
             require(strategy != 4 || resident_fits(), PYRO_E_UNSUPPORTED,
                     "the pyrochlore resident strategy needs at most 2048 cells");
             require(strategy != 5 || tile_kind > 0, PYRO_E_UNSUPPORTED,
@@ -578,6 +614,7 @@ struct pyro_ctx {
         if (spec) {
             cand.push_back(3);
             if (resident_fits()) cand.push_back(4);
+            cand.push_back(6);  // dropped below if its grid cannot be resident at once
         } else {
             cand = {1, 2};
         }
@@ -600,11 +637,31 @@ struct pyro_ctx {
         float best = 1e30f;
         int best_s = spec ? 3 : 1;
         for (int s : cand) {
-            run_sweeps(s, kind, n_or, 1, nullptr, pass_count);  // warm-up
+            if (s == 6) {
+                // the XCD-resident kernel applies when its whole grid fits on
+                // the device at once and its waits complete
+                try {
+                    run_sweeps(s, kind, n_or, 1, nullptr, pass_count);
+                    HIPCHK(hipStreamSynchronize(stream));
+                    check_xcd();
+                } catch (const Status&) {
+                    (void)hipGetLastError();
+                    continue;
+                }
+            } else {
+                run_sweeps(s, kind, n_or, 1, nullptr, pass_count);  // warm-up
+            }
             HIPCHK(hipEventRecord(a, stream));
             run_sweeps(s, kind, n_or, 3, nullptr, pass_count);
             HIPCHK(hipEventRecord(b, stream));
             HIPCHK(hipEventSynchronize(b));
+            if (s == 6) {
+                try {
+                    check_xcd();
+                } catch (const Status&) {
+                    continue;
+                }
+            }
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, a, b));
             if (ms < best) {
@@ -1009,6 +1066,17 @@ int pyro_setup_lattice(pyro_ctx* ctx) {
             ctx->acc_wave.alloc(sizeof(unsigned int) * size_t(R) * size_t(ctx->acc_waves));
         }
         ctx->counters.alloc(sizeof(unsigned long long) * 3);  // pass base, round, anneal step
+        {
+            // XCD-resident sweeps: stage counters, error flag, a 0.25 s bound
+            // on every wait in ticks of the device's wall clock
+            ctx->xcd_bar.alloc(sizeof(unsigned int) * 32 * (R + 8));  // R stage counters, 8 XCD tickets, 128 B apart
+            ctx->xcd_err.alloc(sizeof(int));
+            HIPCHK(hipMemset(ctx->xcd_err.p, 0, sizeof(int)));
+            int dev = 0, khz = 0;
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+            ctx->xcd_ticks = (unsigned long long)(khz > 0 ? khz : 100000) * 250ull;
+        }
         ctx->epart.alloc(sizeof(double) * R *
                          std::max({pyro::energy_blocks(int(N)), pyro::pyro_energy_blocks(int(N)),
                                    pyro::pyro_colour_energy_blocks(int(ctx->nc()))}));
@@ -2082,10 +2150,24 @@ int pyro_synchronize(pyro_ctx* ctx) {
 int pyro_set_strategy(pyro_ctx* ctx, int strategy) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
-        require(strategy >= 0 && strategy <= 5, PYRO_E_INVALID, "strategy must be 0..5");
+        require(strategy >= 0 && strategy <= 6, PYRO_E_INVALID, "strategy must be 0..6");
         ctx->strategy = strategy;
         ctx->tuned_key = -1;
         ctx->invalidate_graph();
+    });
+}
+
+int pyro_set_wait_timeout(pyro_ctx* ctx, double seconds) {
+    return guarded([&] {
+        require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
+        ctx->need_setup();
+        require(seconds > 0.0 && seconds <= 60.0, PYRO_E_INVALID, "the wait bound must lie in (0, 60] seconds");
+        int dev = 0, khz = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        const double ticks = seconds * 1e3 * double(khz > 0 ? khz : 100000);
+        ctx->xcd_ticks = ticks < 1.0 ? 1ull : (unsigned long long)ticks;
+        ctx->invalidate_graph();  // the bound is a kernel argument of captured rounds
     });
 }
 

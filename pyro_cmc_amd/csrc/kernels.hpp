@@ -108,6 +108,26 @@ __host__ __device__ constexpr int pyro_colour_energy_blocks(int n_cells) { retur
 // thread) per replica; lattices of at most 2048 cells.  With epart the last
 // pass also leaves the energy partials launch_pyro_colour leaves, bit for bit.
 constexpr int kPyroResidentMaxCells = 2048;
+// Whole sweeps with the per-colour work of launch_pyro_colour held on one
+// XCD per replica in one launch, the colour stages separated by
+// per-replica counters in that XCD's L2 (bar[32 (R + 8)]: R counters and 8
+// XCD tickets, 128 bytes apart, zeroed by the launch), spins read past the
+// L1: bit-identical to
+// launch_pyro_colour pass after pass (epart and fused as for
+// launch_pyro_resident).  Every wait is bounded by timeout_ticks of the
+// device's wall clock; a timeout sets *err (then the results are
+// invalid).  hipErrorNotSupported when the grid
+// cannot be resident at once.
+// Workgroups of one XCD-resident launch: 8 XCDs x ceil(R / 8) replicas x
+// blocks per replica and colour stage.
+inline int pyro_xcd_workgroups(int R, int n_cells, int shells) {
+    const int bpr = ((n_cells + 63) / 64) * (shells == 3 ? 1 : 4);
+    return 8 * ((R + 7) / 8) * bpr;
+}
+template <typename Real>
+hipError_t launch_pyro_xcd(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind, int n_or,
+                           int n_sweeps, unsigned long long pass_index, hipStream_t s, double* epart, bool fused,
+                           unsigned int* bar, int* err, unsigned long long timeout_ticks);
 // fused: the fused pass order of launch_pyro_colour, bit for bit.
 template <typename Real>
 hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,

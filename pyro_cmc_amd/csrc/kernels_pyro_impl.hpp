@@ -211,16 +211,25 @@ __device__ __forceinline__ double low_site_energy(const LowSums<Real>& lo, const
     return double(bond) - double(bs);
 }
 
-// Field from spins in global memory (Sr = S + r * 3 N), neighbour cells
-// from the wrapped axes, one 3-component load per neighbour.
-template <typename Real, int SHELLS, int SL, int SPLIT = 0>
-__device__ __forceinline__ void pyro_field(const Real* __restrict__ Sr, int nc, const CellAxes& c,
-                                           const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
-                                           LowSums<Real>* low = nullptr) {
+// A replica's spins in global memory (Sr = S + r * 3 N) read through the
+// caches as usual: load(i) is site i's record.
+template <typename Real>
+struct GlobalSpins {
+    const Real* __restrict__ Sr;
+    __device__ __forceinline__ Vec3<Real> operator()(unsigned i) const { return ld3(Sr, i); }
+};
+
+// Field from spins in global memory read by `load` (GlobalSpins, or a
+// loader that bypasses the L1), neighbour cells from the wrapped axes, one
+// 3-component load per neighbour.
+template <typename Real, int SHELLS, int SL, int SPLIT = 0, typename Load>
+__device__ __forceinline__ void pyro_field_l(const Load& load, int nc, const CellAxes& c,
+                                             const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
+                                             LowSums<Real>* low = nullptr) {
     pyro_field_with<Real, SHELLS, SL, false, SPLIT>(
         [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
             const unsigned j = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]);
-            const Vec3<Real> v = ld3(Sr, j);
+            const Vec3<Real> v = load(j);
             x = v.x;
             y = v.y;
             z = v.z;
@@ -247,14 +256,14 @@ __device__ __forceinline__ void pyro_field_lds(const Real* lx, const Real* ly, c
         J, hx, hy, hz);
 }
 
-// Runtime sublattice -> compile-time stencil.
-template <typename Real, int SHELLS, int SPLIT = 0>
-__device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sr, int nc, const CellAxes& c,
-                                              const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
-                                              LowSums<Real>* low = nullptr) {
+// Runtime sublattice -> compile-time stencil, spins read by `load`.
+template <typename Real, int SHELLS, int SPLIT = 0, typename Load>
+__device__ __forceinline__ void pyro_field_rt_l(int sl, const Load& load, int nc, const CellAxes& c,
+                                                const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
+                                                LowSums<Real>* low = nullptr) {
     switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field<Real, SHELLS, X, SPLIT>(Sr, nc, c, J, hx, hy, hz, low); break;
+    case X: pyro_field_l<Real, SHELLS, X, SPLIT>(load, nc, c, J, hx, hy, hz, low); break;
         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
         PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
         PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -262,6 +271,49 @@ __device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sr, int nc, co
 #undef PYRO_SL_CASE
         default: hx = hy = hz = Real(0);
     }
+}
+// pyro_field_rt_l with every neighbour record loaded before the sums, so
+// that all 30 loads are in flight at once whatever the scheduler would make
+// of the interleaved loads and sums.
+template <typename Real, int SHELLS, int SPLIT = 0, typename Load>
+__device__ __forceinline__ void pyro_field_rt_gathered(int sl, const Load& load, int nc, const CellAxes& c,
+                                                       const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
+                                                       LowSums<Real>* low = nullptr) {
+    constexpr int NS = pyrostencil::slots_for(SHELLS);
+    switch (sl) {
+#define PYRO_SL_CASE(X)                                                                                          \
+    case X: {                                                                                                    \
+        Vec3<Real> v[NS];                                                                                        \
+        static_for<NS>([&](auto kk) __attribute__((always_inline)) {                                            \
+            constexpr pyrostencil::Nbr n = kStencil.n[X][decltype(kk)::value];                                  \
+            v[decltype(kk)::value] =                                                                             \
+                load(unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]));             \
+        });                                                                                                      \
+        int k = 0;                                                                                               \
+        pyro_field_with<Real, SHELLS, X, false, SPLIT>(                                                         \
+            [&](const pyrostencil::Nbr&, Real& x, Real& y, Real& z) __attribute__((always_inline)) {            \
+                x = v[k].x;                                                                                      \
+                y = v[k].y;                                                                                      \
+                z = v[k].z;                                                                                      \
+                ++k;                                                                                             \
+            },                                                                                                   \
+            J, hx, hy, hz, low);                                                                                 \
+        break;                                                                                                   \
+    }
+        PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+        PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+        PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+        PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+        default: hx = hy = hz = Real(0);
+    }
+}
+// The same with the spins of replica block Sr read through the caches.
+template <typename Real, int SHELLS, int SPLIT = 0>
+__device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sr, int nc, const CellAxes& c,
+                                              const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz,
+                                              LowSums<Real>* low = nullptr) {
+    pyro_field_rt_l<Real, SHELLS, SPLIT>(sl, GlobalSpins<Real>{Sr}, nc, c, J, hx, hy, hz, low);
 }
 
 // Sum of v over the 64 lanes of a wave (every lane active), the same in
@@ -583,6 +635,190 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
         for (int i = 0; i < BS / 64; ++i) t += red[i];
         atomicAdd(st.acc + r, t);
     }
+}
+
+// Persistent per-colour sweeps held on one XCD per replica (strategy 6).
+//
+// Every launch of pyro_colour_kernel begins with the L2 writeback and
+// invalidation that keep the eight XCDs' L2 caches coherent, so each of the
+// 16 colour stages of a pass fetches the replica set again from beyond L2.
+// Here one launch runs whole sweeps: the workgroups of replica r all run on
+// XCD r % 8 (each workgroup reads its XCD from the XCC_ID register and
+// takes a ticket among that XCD's workgroups) and wait for one another
+// between colour stages on a per-replica counter, so a replica's spins stay
+// in its XCD's L2 across the stages.
+//
+// Coherence between the CUs of an XCD goes through its L2, the point of
+// coherence of those CUs.  A workgroup counts a finished stage with an
+// atomic add after its spin stores have reached the L2 (a release, which
+// waits for them; the unit is built with -mtgsplit, whose workgroup-scope
+// release waits for stores to complete), and polls the count with an atomic
+// OR of 0: atomics execute in the L2 of the issuing XCD, never in an L1.
+// Every read of the spins in this kernel is a device-scope buffer load
+// (cache policy sc1), which misses the CU's L1 and is served by the L2.
+// Measured alternatives: workgroup-scope loads of the counters or spins (sc0)
+// and an L1 invalidation by a workgroup-scope acquire read stale values;
+// a device-scope acquire (which also invalidates the L2) was about four
+// times slower than the sc1 loads.
+//
+// Workgroup with ticket t on XCD x (one wave) -> replica x + 8 (t / bpr),
+// block b = t % bpr of each colour stage, the same cells and block
+// index as pyro_colour_kernel's block b (bpr = its blocks per replica), and
+// the same per-site counters, draws and arithmetic (the pass-by-pass or the
+// fused pass order of pyro_resident_kernel), so the results equal strategy
+// 3's bit for bit.  Every wait is bounded: after `timeout_ticks` of the
+// device's wall clock the workgroup sets *err and returns, and the host
+// reports the error instead of the results.
+constexpr int kHwRegXccId = 20;  // hwreg(HW_REG_XCC_ID), bits 3:0 = the XCD
+
+__device__ __forceinline__ int xcc_id() {
+    return int(__builtin_amdgcn_s_getreg((31 << 11) | kHwRegXccId) & 0xFu);
+}
+
+// A replica's spins read with device-scope buffer loads (cache policy sc1:
+// past the L1, from the L2).
+constexpr int kBufferWord3 = 0x00020000;  // raw-buffer descriptor word 3 of gfx9, 32-bit data
+constexpr int kPolicySc1 = 16;            // the sc1 bit of a buffer load's cache policy
+constexpr int kXcdPad = 32;               // counters 128 bytes apart: one cache line each
+template <typename Real>
+struct L2Spins;
+template <>
+struct L2Spins<float> {
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ __forceinline__ Vec3<float> operator()(unsigned i) const {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(rs, i * 12u, 0, kPolicySc1);
+        return Vec3<float>{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
+    }
+};
+template <>
+struct L2Spins<double> {
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ __forceinline__ Vec3<double> operator()(unsigned i) const {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b64(rs, i * 24u, 0, kPolicySc1);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, i * 24u + 8u, 0, kPolicySc1);
+        const auto c = __builtin_amdgcn_raw_buffer_load_b64(rs, i * 24u + 16u, 0, kPolicySc1);
+        return Vec3<double>{__hiloint2double(int(a[1]), int(a[0])), __hiloint2double(int(b[1]), int(b[0])),
+                            __hiloint2double(int(c[1]), int(c[0]))};
+    }
+};
+
+// Stage counter of replica r: wait until it reaches `target` (device-scope
+// reads, past the L1); false (and *err set) once the wait has lasted
+// `timeout` ticks of the wall clock.
+__device__ __forceinline__ bool xcd_wait(unsigned int* ctr, unsigned int target, int* err,
+                                         unsigned long long timeout, unsigned int zero) {
+    unsigned long long deadline = 0;
+    for (;;) {
+        // an atomic read-modify-write executes in the L2 (never the L1);
+        // `zero` (0, a kernel argument) keeps the compiler from folding an
+        // OR with 0 into a plain load, which the L1 may serve stale
+        const unsigned int v = __hip_atomic_fetch_or(ctr, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (v >= target) return true;
+        const unsigned long long now = wall_clock64();
+        if (deadline == 0) {
+            deadline = now + timeout;
+        } else if (now > deadline) {
+            atomicOr(err, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__global__ void xcd_zero_kernel(unsigned int* __restrict__ p, int n) {
+    for (int i = int(threadIdx.x); i < n; i += int(blockDim.x)) p[i] = 0u;
+}
+
+template <typename Real, int SHELLS, int FIRST, bool PER_REP, bool EN, bool FUSED>
+__global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, PyroCouplings<Real> J, int n_or,
+                                                      int n_sweeps, unsigned long long pass_index,
+                                                      double* __restrict__ epart, unsigned int* __restrict__ bar,
+                                                      int* __restrict__ err, unsigned long long timeout_ticks,
+                                                      unsigned int zero) {
+    constexpr int NCOL = pyrostencil::colours_for(SHELLS);
+    if (*err) return;  // an earlier launch failed: the host reports it at its next read
+    // the XCD this workgroup runs on and its place among that XCD's
+    // workgroups: a ticket from the XCD's own counter bar[R + x] (the
+    // dispatcher hands consecutive workgroups to consecutive XCDs from a
+    // starting XCD that varies from launch to launch, so each XCD receives
+    // a whole share of the grid, but not always the same ids)
+    const int x = xcc_id() & 7;
+    unsigned int tk = 0;
+    if (threadIdx.x == 0)
+        tk = __hip_atomic_fetch_add(bar + size_t(st.R + x) * kXcdPad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const int t = int(__builtin_amdgcn_readfirstlane(tk));
+    const int nc = st.n_cells;
+    const int bps = (nc + 63) / 64;
+    const int bpr = bps * (16 / NCOL);  // blocks per replica and colour stage
+    const int r = x + 8 * (t / bpr);
+    if (r >= st.R) return;  // no replica for this workgroup: it takes no part in any count
+    const int b = t - (t / bpr) * bpr;
+    const int q = b / bps;
+    const int cell = (b - q * bps) * 64 + int(threadIdx.x);
+    const bool live = cell < nc;
+    const size_t N = size_t(st.N);
+    Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
+    const L2Spins<Real> ld{__builtin_amdgcn_make_buffer_rsrc(Sr, 0, int(3 * N * sizeof(Real)), kBufferWord3)};
+    const PyroCouplings<Real> Jr = PER_REP ? J.at(r) : J;
+    const ReplicaParams<Real> rp = replica_params(st, r);
+    const unsigned long long pass_start = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    const int per_launch = bpr;
+    const size_t es = size_t(pyro_colour_energy_blocks(nc));
+    unsigned int* ctr = bar + size_t(r) * kXcdPad;
+    const int n_groups = FUSED ? (n_or > 1 ? n_or : 1) : n_or + 1;
+    unsigned int n_acc = 0;
+    unsigned int stage = 0;
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+        const unsigned long long p0 = pass_start + (unsigned long long)sw * unsigned(1 + n_or);
+        for (int grp = 0; grp < n_groups; ++grp) {
+            const bool en_pass = EN && sw == n_sweeps - 1 && grp == n_groups - 1;  // uniform
+            int ucomp = 0;
+            unsigned long long pass = p0 + unsigned(grp);
+            if (FUSED && grp > 0) {
+                const int j = kUniformsOf[FIRST] + grp;
+                pass = p0 + unsigned(j / 5);
+                ucomp = j % 5;
+            }
+            for (int col = 0; col < NCOL; ++col, ++stage) {
+                if (stage > 0 && !xcd_wait(ctr, stage * unsigned(bpr), err, timeout_ticks, zero)) return;
+                const int sl = NCOL == 16 ? col : col + 4 * q;
+                bool a = false;
+                double e = 0.0;
+                if (live) {
+                    // the cell's neighbour offsets, recomputed per stage: held
+                    // across the loop (16 stencils of 30) they would take
+                    // hundreds of registers; the empty asm keeps the compiler
+                    // from hoisting them
+                    int cv = cell;
+                    asm volatile("" : "+v"(cv));
+                    const CellAxes c = cell_axes_of(cv, st.L0, st.L1, st.L2);
+                    Real hx, hy, hz;
+                    LowSums<Real> lo;
+                    if (en_pass) pyro_field_rt_gathered<Real, SHELLS, NCOL>(sl, ld, nc, c, Jr, hx, hy, hz, &lo);
+                    else pyro_field_rt_gathered<Real, SHELLS>(sl, ld, nc, c, Jr, hx, hy, hz);
+                    const int site = sl * nc + cell;
+                    const Vec3<Real> v = ld(unsigned(site));
+                    Real sx = v.x, sy = v.y, sz = v.z;
+                    if (grp == 0 && FIRST != 3)
+                        a = resident_update<Real, FIRST, FUSED>(rp, st, sx, sy, sz, site, r, pass, 0, n_or, hx, hy,
+                                                                hz, Sr, N);
+                    else
+                        resident_update<Real, 3, false>(rp, st, sx, sy, sz, site, r, pass, ucomp, n_or, hx, hy, hz,
+                                                        Sr, N);
+                    if (en_pass) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, sx, sy, sz);
+                }
+                if (grp == 0 && FIRST != 3 && st.acc_wave) n_acc += unsigned(__popcll(__ballot(a)));
+                if (en_pass) {
+                    const double tot = wave_sum_f64(e);
+                    if (threadIdx.x == 0) epart[size_t(col) * unsigned(per_launch) + size_t(r) * es + unsigned(b)] = tot;
+                }
+                // release: this wave's stores reach the L2 before its count
+                if (threadIdx.x == 0)
+                    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+    if (st.acc_wave && threadIdx.x == 0 && n_acc) st.acc_wave[size_t(r) * unsigned(st.acc_waves) + unsigned(b)] += n_acc;
 }
 
 // Lane -> tile cell of the tiled sweep.  A colour stage reads, for every
@@ -1066,6 +1302,51 @@ hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings
         return fused && n_or >= 1 ? go(en, std::true_type{}) : go(en, std::false_type{});
     };
     return epart ? by_fused(std::true_type{}) : by_fused(std::false_type{});
+}
+
+template <typename Real>
+hipError_t launch_pyro_xcd(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind, int n_or,
+                           int n_sweeps, unsigned long long pass_index, hipStream_t s, double* epart, bool fused,
+                           unsigned int* bar, int* err, unsigned long long timeout_ticks) {
+    if (kind < 0 || kind > 2) return hipErrorInvalidValue;
+    const int G = pyro_xcd_workgroups(st.R, st.n_cells, J.shells);
+    const bool fu = fused && n_or >= 1;
+    auto go = [&](auto per_rep, auto en, auto fuc) -> hipError_t {
+        constexpr bool PR = decltype(per_rep)::value;
+        constexpr bool EN = decltype(en)::value;
+        constexpr bool FU = decltype(fuc)::value;
+        return dispatch_pyro(J, [&](auto shells) -> hipError_t {
+            constexpr int SH = decltype(shells)::value;
+            auto one = [&](auto kk) -> hipError_t {
+                auto* fn = pyro_xcd_kernel<Real, SH, decltype(kk)::value, PR, EN, FU>;
+                // every workgroup of a replica must be resident at once (they
+                // wait for one another): refuse a grid beyond what fits
+                int dev = 0, cus = 0, per_cu = 0;
+                hipError_t e = hipGetDevice(&dev);
+                if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+                if (e == hipSuccess)
+                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), 64, 0);
+                if (e != hipSuccess) return e;
+                if (cus % 8 != 0 || size_t(per_cu) * size_t(cus) < size_t(G)) return hipErrorNotSupported;
+                hipLaunchKernelGGL(xcd_zero_kernel, dim3(1), dim3(256), 0, s, bar, (st.R + 8) * kXcdPad);
+                hipLaunchKernelGGL(fn, dim3(unsigned(G)), dim3(64), 0, s, st, J, n_or, n_sweeps, pass_index, epart,
+                                   bar, err, timeout_ticks, 0u);
+                return hipGetLastError();
+            };
+            switch (kind) {
+                case 0: return one(std::integral_constant<int, 0>{});
+                case 1: return one(std::integral_constant<int, 1>{});
+                default: return one(std::integral_constant<int, 2>{});
+            }
+        });
+    };
+    auto by_fused = [&](auto per_rep, auto en) -> hipError_t {
+        return fu ? go(per_rep, en, std::true_type{}) : go(per_rep, en, std::false_type{});
+    };
+    auto by_en = [&](auto per_rep) -> hipError_t {
+        return epart ? by_fused(per_rep, std::true_type{}) : by_fused(per_rep, std::false_type{});
+    };
+    return J.per_replica ? by_en(std::true_type{}) : by_en(std::false_type{});
 }
 
 template <typename Real>
