@@ -725,6 +725,30 @@ __device__ __forceinline__ bool xcd_wait(unsigned int* ctr, unsigned int target,
     }
 }
 
+// resident_update with the uniforms drawn beforehand: u0 the block of the
+// KIND update (for an OR pass alone, its component already in m[0]) and u1
+// the block holding the fused OR update's uniform.
+template <typename Real, int KIND, bool FUSED>
+__device__ __forceinline__ bool drawn_update(const ReplicaParams<Real>& p, const DeviceState<Real>& st, Real& x,
+                                             Real& y, Real& z, int site, int r, int n_or, const Uniform5& u0,
+                                             const Uniform5& u1, Real hx, Real hy, Real hz, Real* __restrict__ Sr,
+                                             size_t N) {
+    bool changed;
+    const bool acc = update_spin_u<Real, KIND>(p, x, y, z, changed, u0, hx, hy, hz, st.dir + size_t(r) * N + site);
+    if constexpr (FUSED && KIND != 3) {
+        if (n_or >= 1) {
+            constexpr int j = kUniformsOf[KIND];
+            Uniform5 uo;
+            uo.m[0] = (j % 5 == 0 ? u1 : u0).m[j % 5];
+            bool ch;
+            update_spin_u<Real, 3>(p, x, y, z, ch, uo, hx, hy, hz, nullptr);
+            changed = changed || ch;
+        }
+    }
+    if (changed) st3(Sr, unsigned(site), x, y, z);
+    return acc;
+}
+
 __global__ void xcd_zero_kernel(unsigned int* __restrict__ p, int n) {
     for (int i = int(threadIdx.x); i < n; i += int(blockDim.x)) p[i] = 0u;
 }
@@ -759,6 +783,7 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
     const size_t N = size_t(st.N);
     Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
     const L2Spins<Real> ld{__builtin_amdgcn_make_buffer_rsrc(Sr, 0, int(3 * N * sizeof(Real)), kBufferWord3)};
+    const CellAxes axes = cell_axes_of(live ? cell : 0, st.L0, st.L1, st.L2);
     const PyroCouplings<Real> Jr = PER_REP ? J.at(r) : J;
     const ReplicaParams<Real> rp = replica_params(st, r);
     const unsigned long long pass_start = pass_index + (st.pass_base ? *st.pass_base : 0ull);
@@ -780,31 +805,43 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
                 ucomp = j % 5;
             }
             for (int col = 0; col < NCOL; ++col, ++stage) {
-                if (stage > 0 && !xcd_wait(ctr, stage * unsigned(bpr), err, timeout_ticks, zero)) return;
                 const int sl = NCOL == 16 ? col : col + 4 * q;
+                const int site = sl * nc + cell;
+                // this stage's uniforms, drawn before the wait: they depend
+                // on the site and the pass alone, and the rounds then run
+                // while the other workgroups finish the previous stage
+                const bool first = grp == 0 && FIRST != 3;
+                Uniform5 u0 = site_uniforms(st, site, r, pass), u1 = u0;
+                if (first) {
+                    if constexpr (FUSED && kUniformsOf[FIRST] % 5 == 0)
+                        u1 = site_uniforms(st, site, r, pass + unsigned(kUniformsOf[FIRST] / 5));
+                } else {
+                    u0.m[0] = uniform_at(u0, ucomp);
+                }
+                asm volatile("" : "+v"(u0.m[0]), "+v"(u0.m[1]), "+v"(u0.m[2]), "+v"(u0.m[3]), "+v"(u0.m[4]),
+                             "+v"(u1.m[0]));
+                if (stage > 0 && !xcd_wait(ctr, stage * unsigned(bpr), err, timeout_ticks, zero)) return;
                 bool a = false;
                 double e = 0.0;
                 if (live) {
-                    // the cell's neighbour offsets, recomputed per stage: held
-                    // across the loop (16 stencils of 30) they would take
-                    // hundreds of registers; the empty asm keeps the compiler
-                    // from hoisting them
-                    int cv = cell;
-                    asm volatile("" : "+v"(cv));
-                    const CellAxes c = cell_axes_of(cv, st.L0, st.L1, st.L2);
+                    // the neighbour offsets from the cell's axes afresh in each
+                    // stage: hoisted out of the loop (16 stencils of 30) they
+                    // would take hundreds of registers; the empty asm hides
+                    // the axes' values from the compiler so it cannot hoist
+                    CellAxes c = axes;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(c.a0[k]), "+v"(c.a1[k]), "+v"(c.a2[k]));
                     Real hx, hy, hz;
                     LowSums<Real> lo;
                     if (en_pass) pyro_field_rt_gathered<Real, SHELLS, NCOL>(sl, ld, nc, c, Jr, hx, hy, hz, &lo);
                     else pyro_field_rt_gathered<Real, SHELLS>(sl, ld, nc, c, Jr, hx, hy, hz);
-                    const int site = sl * nc + cell;
                     const Vec3<Real> v = ld(unsigned(site));
                     Real sx = v.x, sy = v.y, sz = v.z;
-                    if (grp == 0 && FIRST != 3)
-                        a = resident_update<Real, FIRST, FUSED>(rp, st, sx, sy, sz, site, r, pass, 0, n_or, hx, hy,
-                                                                hz, Sr, N);
+                    if (first)
+                        a = drawn_update<Real, FIRST, FUSED>(rp, st, sx, sy, sz, site, r, n_or, u0, u1, hx, hy, hz, Sr,
+                                                             N);
                     else
-                        resident_update<Real, 3, false>(rp, st, sx, sy, sz, site, r, pass, ucomp, n_or, hx, hy, hz,
-                                                        Sr, N);
+                        drawn_update<Real, 3, false>(rp, st, sx, sy, sz, site, r, n_or, u0, u1, hx, hy, hz, Sr, N);
                     if (en_pass) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, sx, sy, sz);
                 }
                 if (grp == 0 && FIRST != 3 && st.acc_wave) n_acc += unsigned(__popcll(__ballot(a)));
