@@ -258,12 +258,16 @@ def uneven_batch(pyro):
     J2, J3 uniform in [-0.2, 0.2], |B| uniform in [0, 0.5] along a random
     direction; T 10 -> 0.05 over 100 log steps, 16 burn-in sweeps, 16 sweeps
     per step, 64 SSF samples (xx, yy, zz) at T_cold; FP32 spins, FP64 sums;
-    Metropolis + OR.  Items of equal L share a context; the five contexts run
-    concurrently on their own streams."""
+    Metropolis + OR.  Items of equal L share a context; the five contexts
+    step together as one Batch (one graph per step: L = 4, 8, 12 in one
+    persistent launch, L = 16 and 24 tiled beside it).  The same anneals with
+    each context replaying its own graph on its own stream are timed after
+    it for comparison."""
     from pyro_cmc_amd import batch
 
     items = batch.uneven_items(256, seed=4)
     _, st = batch.run_anneal_batch(items, seed=4, fused=BATCH_FUSED)
+    _, own = batch.run_anneal_batch(items, seed=4, fused=BATCH_FUSED, persistent=False)
     return {
         "workload": "256 anneals, L in {4,8,12,16,24}, random J2/J3 in [-0.2,0.2], |B| in [0,0.5], "
                     "T 10->0.05 over 100 steps, 16 sweeps/step, 64 SSF samples at T_cold, FP32",
@@ -273,6 +277,8 @@ def uneven_batch(pyro):
         "wall_s": st["wall_s"],
         "site_updates": st["site_updates"],
         "items_per_L": st["contexts"],
+        "persistent_launch": st.get("persistent_launch"),
+        "own_graphs_site_updates_per_s": own["site_updates_per_s"],
     }
 
 

@@ -250,6 +250,31 @@ int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep,
  * (zero for steps not yet run). */
 int pyro_anneal_history(pyro_ctx* ctx, double* E);
 
+/* A batch of contexts stepped together, e.g. one context per lattice size
+ * of an uneven batch of anneals.  No counterpart in the reference, which
+ * runs one anneal per process and merges them afterwards
+ * (acc_runs.py:13-57).  The contexts stay owned by the caller and must
+ * outlive the batch. */
+typedef struct pyro_batch pyro_batch;
+int pyro_batch_create(pyro_ctx* const* ctxs, int n_ctx, pyro_batch** out);
+/* pyro_anneal_steps(ctx, ...) for every context of the batch, with the
+ * same results bit for bit, as one HIP graph per step: the contexts whose
+ * sweeps take the per-colour stencil strategies under the automatic choice
+ * sweep together in one persistent launch (every replica of every such
+ * context, whole sweeps, each replica's workgroups on one XCD, stage waits
+ * bounded as set by pyro_set_wait_timeout), the others (the tiled strategy
+ * on large lattices, forced strategies) sweep on their own in parallel
+ * branches of the same graph.  Every context needs an anneal schedule with
+ * n_steps steps left.  Only enqueued: reading calls on a context wait for
+ * the batch's steps. */
+int pyro_batch_anneal_steps(pyro_batch* b, int kind, int n_overrelax, int n_sweep, int sample_ssf,
+                            int n_steps);
+/* After pyro_batch_anneal_steps: the contexts in the persistent launch,
+ * the cell blocks each of its workgroups takes per colour stage, and its
+ * workgroups (0, 0, 0 without one). */
+int pyro_batch_info(pyro_batch* b, int* launch_members, int* blocks_per_worker, int* workgroups);
+int pyro_batch_destroy(pyro_batch* b);
+
 /* ------------------------------------------------------------------ */
 /* Static structure factor (ssf_manager.hpp:42-57, 260-293).            */
 /* ------------------------------------------------------------------ */
@@ -309,8 +334,9 @@ int pyro_synchronize(pyro_ctx* ctx);
  * PYRO_E_UNSUPPORTED for a forced strategy that does not apply. */
 int pyro_set_strategy(pyro_ctx* ctx, int strategy);
 /* Bound, in seconds, on each wait between the colour stages of the
- * XCD-resident sweep (strategy 6; default 0.25 s).  A wait past it ends the
- * launch, and the next call that reads results reports PYRO_E_HIP ("a stage
+ * XCD-resident sweep (strategy 6) and of batch launches (pyro_batch_*;
+ * default 0.25 s), counted from the workgroup's own arrival at the end of
+ * the previous stage.  A wait past it ends the launch, and the next call that reads results reports PYRO_E_HIP ("a stage
  * wait timed out") instead of returning them.  No counterpart in the
  * reference (its sweeps have no waits). */
 int pyro_set_wait_timeout(pyro_ctx* ctx, double seconds);

@@ -85,6 +85,10 @@ _SIGS = {
     "pyro_mark": [_vp, C.c_int],
     "pyro_elapsed_ms": [_vp, C.POINTER(C.c_double)],
     "pyro_lt_spectrum": [_i32p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _i32p, _f64p],
+    "pyro_batch_create": [_vp, C.c_int, C.POINTER(_vp)],
+    "pyro_batch_anneal_steps": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "pyro_batch_info": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "pyro_batch_destroy": [_vp],
 }
 
 _lib: Optional[C.CDLL] = None
@@ -476,3 +480,42 @@ class Context:
         v = C.c_double()
         _check(lib().pyro_elapsed_ms(self._h, C.byref(v)))
         return float(v.value)
+
+
+class Batch:
+    """Contexts stepped together (pyro_batch_*): one graph per anneal step,
+    the stencil-strategy contexts sweeping in one persistent launch.  The
+    contexts stay open while the batch is."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        hs = (_vp * len(self.contexts))(*[c._h for c in self.contexts])
+        self._h = _vp()
+        _check(lib().pyro_batch_create(hs, len(self.contexts), C.byref(self._h)))
+
+    def anneal_steps(self, kind=METROPOLIS, n_overrelax=1, n_sweep=1, sample_ssf=False, n_steps=1):
+        _check(lib().pyro_batch_anneal_steps(self._h, kind, n_overrelax, n_sweep, int(bool(sample_ssf)), n_steps))
+
+    def info(self):
+        """(contexts in the persistent launch, cell blocks per workgroup and
+        colour stage, workgroups of the launch)."""
+        v = [C.c_int() for _ in range(3)]
+        _check(lib().pyro_batch_info(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def close(self):
+        if self._h:
+            _check(lib().pyro_batch_destroy(self._h))
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -3,10 +3,12 @@
 The reference runs one anneal per process (anneal.cpp:33-248) and merges
 seeds afterwards (acc_runs.py:13-57).  Here a batch of models with different
 lattice sizes, couplings and fields runs as one job: items of equal L share a
-context (one replica per item, per-replica couplings and field), every
-context follows the same device-side temperature schedule as a replayed HIP
-graph on its own stream, and the contexts' steps are enqueued in turn so the
-small lattices fill the GPU beside the large ones.
+context (one replica per item, per-replica couplings and field), and every
+context follows the same device-side temperature schedule.  The contexts
+step together as a Batch: each anneal step of the whole batch is one
+replayed HIP graph, in which the contexts on the per-colour stencil
+strategies (the small lattices) sweep in one persistent launch and the
+large lattices run their tiled sweeps beside it.
 
 Schedule per item, as in anneal.cpp:193-221: n_burn_in sweeps at T_hot;
 then for every T of generate_T_profile(T_hot, T_cold, T_sample, n_steps)
@@ -22,7 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import FP32, METROPOLIS, Context
+from . import FP32, METROPOLIS, Batch, Context
 
 
 @dataclass
@@ -84,11 +86,14 @@ def uneven_items(n=256, sizes=(4, 8, 12, 16, 24), seed=0):
 
 def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_burn_in=16, n_sample=64,
                      n_overrelax=1, kind=METROPOLIS, precision=FP32, correlators=("xx", "yy", "zz"),
-                     seed=0, keep_spins=False, strategy=0, T_sample=None, fused=False):
+                     seed=0, keep_spins=False, strategy=0, T_sample=None, fused=False, persistent=True):
     """Run every item's anneal; returns (results in item order, stats).
     fused: the sweeps in the fused pass order (pyro_set_fused_passes: per
     colour stage each site's first-kind and first OR update from one field
-    evaluation) instead of pass by pass."""
+    evaluation) instead of pass by pass.  persistent: the contexts step
+    together as a Batch (one graph per step, the small lattices sweeping in
+    one persistent launch) instead of each replaying its own graph; the
+    results are the same bit for bit."""
     T_sample = [T_cold] if T_sample is None else list(T_sample)
     grid, s_idx = generate_T_profile(T_hot, T_cold, T_sample, n_steps)
     n_ts = len(s_idx)
@@ -129,19 +134,26 @@ def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_
         c.ssf_configure(correlators, n_slots=n_ts * len(idx), store_error_term=True)
         c.anneal_schedule(sched)
         ctxs.append((c, idx))
+    batch = Batch([c for c, _ in ctxs]) if persistent else None
     for c, _ in ctxs:
         c.synchronize()
     # timed: everything from the first step on, including each context's
     # strategy choice and graph capture on its first call
     t0 = time.perf_counter()
     for n_sw, ssf, steps, j in phases:
-        for c, _ in ctxs:  # enqueue only: the contexts' streams run side by side
+        for c, _ in ctxs:
             if ssf and n_ts > 1:  # (setting a map waits for the context's queued steps)
                 c.pt_ssf_map(np.arange(c.R, dtype=np.int32) + j * c.R)
-            c.anneal_steps(kind, n_overrelax, n_sw, ssf, steps)
+            if batch is None:  # enqueue only: the contexts' streams run side by side
+                c.anneal_steps(kind, n_overrelax, n_sw, ssf, steps)
+        if batch is not None:
+            batch.anneal_steps(kind, n_overrelax, n_sw, ssf, steps)
     for c, _ in ctxs:
         c.synchronize()
     wall = time.perf_counter() - t0
+    launch = batch.info() if batch is not None else None
+    if batch is not None:
+        batch.close()
     results = [None] * len(items)
     updates = 0
     e_at = np.array(e_at)
@@ -158,4 +170,7 @@ def run_anneal_batch(items, T_hot=10.0, T_cold=0.05, n_steps=100, n_sweep=16, n_
         c.close()
     stats = {"wall_s": wall, "site_updates": updates, "site_updates_per_s": updates / wall,
              "contexts": {int(L): len(idx) for L, idx in sorted(groups.items())}, "schedule_steps": len(sched)}
+    if launch is not None:
+        stats["persistent_launch"] = {"contexts": launch[0], "blocks_per_workgroup": launch[1],
+                                      "workgroups": launch[2]}
     return results, stats

@@ -44,16 +44,21 @@ SOURCES = [
     "kernels_pyro_tiled_f64.hip",
     "kernels_pyro_xcd_f32.hip",
     "kernels_pyro_xcd_f64.hip",
+    "kernels_pyro_batch_f32.hip",
+    "kernels_pyro_batch_f64.hip",
     "lt.hip",
     "capi.cpp",
     "topology.cpp",
 ]
 
-# per-source flags: the XCD-resident kernels use the threadgroup-split
-# memory model (a workgroup-scope acquire invalidates the CU's L1)
+# per-source flags: the XCD-resident and batch kernels use the
+# threadgroup-split memory model, under which a workgroup-scope release
+# waits for the workgroup's stores to complete
 EXTRA_FLAGS = {
     "kernels_pyro_xcd_f32.hip": ["-mtgsplit"],
     "kernels_pyro_xcd_f64.hip": ["-mtgsplit"],
+    "kernels_pyro_batch_f32.hip": ["-mtgsplit"],
+    "kernels_pyro_batch_f64.hip": ["-mtgsplit"],
 }
 
 LINK_LIBS: list[str] = []

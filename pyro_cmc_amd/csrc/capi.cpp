@@ -136,6 +136,26 @@ struct DevMem {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Captures what body enqueues on s, and on the streams it forks from s and
+// joins back, as one instantiated graph.
+template <typename F>
+hipGraphExec_t capture_graph(hipStream_t s, F&& body) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {
+        if (hipStreamEndCapture(s, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(ie);
+    return exec;
+}
+
 // Scratch of the allocation-free entry point (pyro_energy_per_cell_noalloc):
 // the energy partials and the field, in a file-scope device array that every
 // GPU holds until the process ends.  In host code `scratch` names a host-side
@@ -219,6 +239,7 @@ struct pyro_ctx {
     int n_sched = 0, anneal_step = 0;
     hipGraphExec_t agexec = nullptr;
     int a_kind = -1, a_nor = -1, a_nsw = -1, a_ssf = -1, a_strategy = -1;
+    unsigned long long graph_gen = 0;  // counts invalidate_graph() calls: captured work is stale when it moves
 
     // requested: 0 auto, 1 generic per-colour, 2 generic resident,
     // 3 pyrochlore per-colour, 4 pyrochlore resident, 5 pyrochlore tiled
@@ -842,11 +863,153 @@ struct pyro_ctx {
         gexec = nullptr;
         if (agexec) (void)hipGraphExecDestroy(agexec);
         agexec = nullptr;
+        ++graph_gen;
+    }
+
+    // Anneal steps (pyro_anneal_steps, pyro_batch_anneal_steps): argument
+    // checks, the device counters (pass base, round, schedule step) set from
+    // the host's, the step's temperatures from the schedule, and after the
+    // sweeps the step's SSF sample, energy and energy record.
+    void anneal_check(int kind, int n_or, int n_sweep, int sample_ssf, int n_steps) const {
+        need_setup();
+        require(n_sched > 0, PYRO_E_STATE, "anneal_schedule() has not been called");
+        require(kind == PYRO_METROPOLIS || kind == PYRO_LIFTED || kind == PYRO_LIFTED_ROT, PYRO_E_INVALID,
+                "sweep kind must be METROPOLIS, LIFTED or LIFTED_ROT");
+        require(n_or >= 0 && n_sweep >= 0 && n_steps >= 0, PYRO_E_INVALID, "negative count");
+        require(anneal_step + n_steps <= n_sched, PYRO_E_INVALID, "steps beyond the end of the schedule");
+        require(!sample_ssf || ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
+        require(!sample_ssf || has_ssf_map || n_slots >= R, PYRO_E_INVALID,
+                "SSF sampling in anneal steps needs a slot map or one accumulator slot per replica");
+    }
+    void anneal_upload_counters() {
+        const unsigned long long hc[3] = {pass_count, round_count, (unsigned long long)anneal_step};
+        h2d(counters.p, hc, sizeof(hc));
+    }
+    void anneal_set_T() {
+        HIPCHK(pyro::launch_anneal_set_T(T_slot.as<double>(), R, sched.as<double>(), n_sched,
+                                         counters.as<unsigned long long>() + 2, stream));
+    }
+    void anneal_observe(int sample_ssf, bool fused_energy, unsigned long long ppr) {
+        auto* ctr = counters.as<unsigned long long>();
+        if (sample_ssf) {
+            if (has_ssf_map) {  // replica r (slot r) -> accumulator map[r]
+                HIPCHK(pyro::launch_map_slots(slot_of.as<int32_t>(), ssf_map.as<int32_t>(), ssf_slot.as<int32_t>(),
+                                              R, stream));
+                ssf_sample_device(ssf_slot.as<int32_t>(), stream);
+            } else {
+                ssf_sample_device(slot_of.as<int32_t>(), stream);
+            }
+        }
+        if (fused_energy) energy_finish(false, pyro::pyro_colour_energy_blocks(int(nc())));
+        else energy(false);
+        HIPCHK(pyro::launch_anneal_record(e.as<double>(), R, ehist.as<double>(), n_sched, ctr + 2, ctr, ppr, stream));
+    }
+    // Host bookkeeping after one step is enqueued; true when the mu-nu
+    // buffer is full and must be flushed before the next step.
+    bool anneal_advance(int sample_ssf, unsigned long long ppr) {
+        pass_count += ppr;
+        anneal_step += 1;
+        if (sample_ssf && munu) {
+            munu_pending += 1;
+            return munu_pending >= munu_B;
+        }
+        return false;
     }
     // Stencil kernels for field and energy: pyrochlore couplings, unless a
     // table strategy (1, 2) is forced without per-replica couplings.
     bool stencil_observables() const { return spec && (has_Jrep || strategy == 0 || strategy >= 3); }
 };
+
+// A batch of contexts stepped together (pyro_batch_anneal_steps): one graph
+// per step for all of them, in which the members that take the per-colour
+// stencil strategies sweep in one persistent launch (pyro::launch_pyro_batch)
+// and the others (the tiled strategy on large lattices, forced strategies)
+// run their own sweeps in parallel branches.  Each member's work runs on its
+// own stream, forked from the batch's stream and joined back.
+struct pyro_batch {
+    std::vector<pyro_ctx*> ctx;
+    hipStream_t stream = nullptr;
+    // fork/join events: fork, the persistent launch's end, the batch's end,
+    // then per member its pre-sweep end and its step end
+    std::vector<hipEvent_t> ev;
+    DevMem members, work, bar;
+    std::vector<unsigned char> h_members;  // the uploaded member table, bytes
+    std::vector<int32_t> h_work;           // the uploaded work table
+    std::vector<char> in_launch;           // member sweeps in the persistent launch
+    int per_xcd = 0, n_items = 0, blocks_per_worker = 0, launch_members = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<long long> g_key;          // what gexec was captured for
+    ~pyro_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Whether member c can sweep in the batch's persistent launch: a context of
+// the per-colour stencil strategies left to the automatic choice (not the
+// tiled strategy's large lattices, whose fixed rule gives a different
+// visiting order).
+bool batch_launch_eligible(const pyro_ctx* c) {
+    return c->spec && c->strategy == 0 && !(c->tile_kind > 0 && c->nc() >= 4096);
+}
+
+// Places every replica of the launch's members whole on one XCD: worker
+// counts ceil(bpr / k) for the fewest blocks per worker k that fit, the
+// replicas (largest first) onto the least loaded XCD.  Fills the work
+// table; false when not even one worker per replica fits.
+bool batch_plan(const std::vector<pyro_ctx*>& m, int capacity, std::vector<int32_t>& work, int& per_xcd, int& k_out) {
+    struct Item { int member, r, bpr; };
+    std::vector<Item> items;
+    int max_bpr = 1;
+    for (int i = 0; i < int(m.size()); ++i) {
+        const int bpr = int((m[i]->nc() + 63) / 64) * (m[i]->spec_shells == 3 ? 1 : 4);
+        max_bpr = std::max(max_bpr, bpr);
+        for (int r = 0; r < m[i]->R; ++r) items.push_back({i, r, bpr});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.bpr > b.bpr; });
+    for (int k = 1; k <= max_bpr; ++k) {
+        std::vector<std::vector<int32_t>> xcd(8);
+        std::vector<long long> load(8, 0);
+        for (const Item& it : items) {
+            const int nw = (it.bpr + k - 1) / k;
+            const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += nw;
+            for (int w = 0; w < nw; ++w) xcd[x].insert(xcd[x].end(), {it.member, it.r, w, nw});
+        }
+        const long long px = *std::max_element(load.begin(), load.end());
+        if (8 * px > capacity) continue;
+        per_xcd = int(px);
+        k_out = k;
+        work.assign(size_t(8) * size_t(px) * 4, -1);
+        for (int x = 0; x < 8; ++x) std::copy(xcd[x].begin(), xcd[x].end(), work.begin() + size_t(x) * px * 4);
+        return true;
+    }
+    return false;
+}
+
+template <typename Real>
+void batch_members(pyro_batch* b, const std::vector<pyro_ctx*>& m, std::vector<unsigned char>& bytes) {
+    std::vector<pyro::BatchMember<Real>> t(m.size());
+    int item = 0;
+    for (size_t i = 0; i < m.size(); ++i) {
+        t[i].st = m[i]->state<Real>(m[i]->counters.as<unsigned long long>());
+        t[i].J = m[i]->pyro_couplings<Real>();
+        t[i].epart = m[i]->epart.as<double>();
+        t[i].err = m[i]->xcd_err.as<int>();
+        t[i].timeout = m[i]->xcd_ticks;
+        t[i].pass_index = 0;
+        t[i].item0 = item;
+        item += m[i]->R;
+    }
+    b->n_items = item;
+    const auto* p = reinterpret_cast<const unsigned char*>(t.data());
+    bytes.assign(p, p + sizeof(pyro::BatchMember<Real>) * t.size());
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1553,18 +1716,7 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
             if (!ctx->gexec || ctx->g_kind != kind || ctx->g_nor != n_overrelax || ctx->g_nsw != n_sweep ||
                 ctx->g_ssf != sample_ssf || ctx->g_strategy != strat) {
                 ctx->invalidate_graph();
-                hipGraph_t graph;
-                HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                try {
-                    round_body();
-                } catch (...) {
-                    (void)hipStreamEndCapture(ctx->stream, &graph);
-                    throw;
-                }
-                HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
-                hipError_t ie = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                HIPCHK(ie);
+                ctx->gexec = capture_graph(ctx->stream, round_body);
                 ctx->g_kind = kind;
                 ctx->g_nor = n_overrelax;
                 ctx->g_nsw = n_sweep;
@@ -1659,60 +1811,26 @@ int pyro_anneal_schedule(pyro_ctx* ctx, const double* T, int n_steps) {
 int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sample_ssf, int n_steps) {
     return guarded([&] {
         require(ctx != nullptr, PYRO_E_INVALID, "NULL context");
-        ctx->need_setup();
-        require(ctx->n_sched > 0, PYRO_E_STATE, "anneal_schedule() has not been called");
-        require(kind == PYRO_METROPOLIS || kind == PYRO_LIFTED || kind == PYRO_LIFTED_ROT, PYRO_E_INVALID,
-                "sweep kind must be METROPOLIS, LIFTED or LIFTED_ROT");
-        require(n_overrelax >= 0 && n_sweep >= 0 && n_steps >= 0, PYRO_E_INVALID, "negative count");
-        require(ctx->anneal_step + n_steps <= ctx->n_sched, PYRO_E_INVALID, "steps beyond the end of the schedule");
-        require(!sample_ssf || ctx->ssf_ready, PYRO_E_STATE, "ssf_configure() has not been called");
-        require(!sample_ssf || ctx->has_ssf_map || ctx->n_slots >= ctx->R, PYRO_E_INVALID,
-                "SSF sampling in anneal steps needs a slot map or one accumulator slot per replica");
+        ctx->anneal_check(kind, n_overrelax, n_sweep, sample_ssf, n_steps);
         if (n_steps == 0) return;
         const int strat = n_sweep > 0 ? ctx->choose_strategy(kind, n_overrelax) : 1;
         const unsigned long long ppr = (unsigned long long)n_sweep * (1 + n_overrelax);
-        auto* ctr = ctx->counters.as<unsigned long long>();
-        const unsigned long long hc[3] = {ctx->pass_count, ctx->round_count, (unsigned long long)ctx->anneal_step};
-        ctx->h2d(ctx->counters.p, hc, sizeof(hc));
+        ctx->anneal_upload_counters();
         // the step's last colour pass leaves the energy partials when it can
         const bool fused_energy = n_sweep > 0 && ctx->sweeps_leave_energy(strat);
         auto step_body = [&]() {
-            HIPCHK(pyro::launch_anneal_set_T(ctx->T_slot.as<double>(), ctx->R, ctx->sched.as<double>(), ctx->n_sched,
-                                             ctr + 2, ctx->stream));
+            ctx->anneal_set_T();
             if (n_sweep > 0)
-                ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
+                ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctx->counters.as<unsigned long long>(), 0ull,
                                 fused_energy ? ctx->epart.as<double>() : nullptr);
-            if (sample_ssf) {
-                if (ctx->has_ssf_map) {  // replica r (slot r) -> accumulator map[r]
-                    HIPCHK(pyro::launch_map_slots(ctx->slot_of.as<int32_t>(), ctx->ssf_map.as<int32_t>(),
-                                                  ctx->ssf_slot.as<int32_t>(), ctx->R, ctx->stream));
-                    ctx->ssf_sample_device(ctx->ssf_slot.as<int32_t>(), ctx->stream);
-                } else {
-                    ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
-                }
-            }
-            if (fused_energy) ctx->energy_finish(false, pyro::pyro_colour_energy_blocks(int(ctx->nc())));
-            else ctx->energy(false);
-            HIPCHK(pyro::launch_anneal_record(ctx->e.as<double>(), ctx->R, ctx->ehist.as<double>(), ctx->n_sched,
-                                              ctr + 2, ctr, ppr, ctx->stream));
+            ctx->anneal_observe(sample_ssf, fused_energy, ppr);
         };
         const bool use_graph = !ctx->prof;
         if (use_graph && (!ctx->agexec || ctx->a_kind != kind || ctx->a_nor != n_overrelax ||
                           ctx->a_nsw != n_sweep || ctx->a_ssf != sample_ssf || ctx->a_strategy != strat)) {
             if (ctx->agexec) (void)hipGraphExecDestroy(ctx->agexec);
             ctx->agexec = nullptr;
-            hipGraph_t graph;
-            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            try {
-                step_body();
-            } catch (...) {
-                (void)hipStreamEndCapture(ctx->stream, &graph);
-                throw;
-            }
-            HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
-            hipError_t ie = hipGraphInstantiate(&ctx->agexec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            HIPCHK(ie);
+            ctx->agexec = capture_graph(ctx->stream, step_body);
             ctx->a_kind = kind;
             ctx->a_nor = n_overrelax;
             ctx->a_nsw = n_sweep;
@@ -1723,12 +1841,7 @@ int pyro_anneal_steps(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int
         for (int i = 0; i < n_steps; ++i) {
             if (use_graph) HIPCHK(hipGraphLaunch(ctx->agexec, ctx->stream));
             else step_body();
-            ctx->pass_count += ppr;
-            ctx->anneal_step += 1;
-            if (sample_ssf && ctx->munu) {
-                ctx->munu_pending += 1;
-                if (ctx->munu_pending >= ctx->munu_B) ctx->munu_flush();
-            }
+            if (ctx->anneal_advance(sample_ssf, ppr)) ctx->munu_flush();
         }
         ctx->h_T.assign(ctx->R, ctx->h_sched[ctx->anneal_step - 1]);
     });
@@ -2275,6 +2388,193 @@ int pyro_lt_spectrum(const int32_t L[3], double J1, double J2, double J3, double
         const int Li[3] = {L[0], L[1], L[2]}, cell[3] = {8, 8, 8};
         const std::string err = pyro::lt_spectrum(table, 16, Li, cell, lambda_min, k_star, eigvec);
         require(err.empty(), PYRO_E_HIP, err);
+    });
+}
+
+int pyro_batch_create(pyro_ctx* const* ctxs, int n_ctx, pyro_batch** out) {
+    return guarded([&] {
+        require(ctxs && out && n_ctx >= 1, PYRO_E_INVALID, "a batch needs at least one context");
+        *out = nullptr;
+        std::set<const pyro_ctx*> seen;
+        for (int i = 0; i < n_ctx; ++i) {
+            require(ctxs[i] != nullptr, PYRO_E_INVALID, "NULL context");
+            ctxs[i]->need_setup();
+            require(seen.insert(ctxs[i]).second, PYRO_E_INVALID, "a context appears twice in the batch");
+            require(ctxs[i]->device == ctxs[0]->device, PYRO_E_INVALID, "the batch's contexts must share a device");
+        }
+        auto b = std::make_unique<pyro_batch>();
+        b->ctx.assign(ctxs, ctxs + n_ctx);
+        HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->ev.resize(3 + 2 * size_t(n_ctx), nullptr);
+        for (auto& e : b->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        *out = b.release();
+    });
+}
+
+int pyro_batch_destroy(pyro_batch* b) {
+    return guarded([&] {
+        if (!b) return;
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        delete b;
+    });
+}
+
+
+int pyro_batch_anneal_steps(pyro_batch* b, int kind, int n_overrelax, int n_sweep, int sample_ssf, int n_steps) {
+    return guarded([&] {
+        require(b != nullptr, PYRO_E_INVALID, "NULL batch");
+        for (pyro_ctx* c : b->ctx) c->anneal_check(kind, n_overrelax, n_sweep, sample_ssf, n_steps);
+        if (n_steps == 0) return;
+        const int n = int(b->ctx.size());
+        const unsigned long long ppr = (unsigned long long)n_sweep * (1 + n_overrelax);
+        // the persistent launch takes the eligible members that share the
+        // first one's template arguments (precision, shells, per-replica
+        // couplings, pass order); the rest sweep on their own
+        b->in_launch.assign(size_t(n), 0);
+        std::vector<pyro_ctx*> lm;
+        if (n_sweep > 0)
+            for (int i = 0; i < n; ++i) {
+                pyro_ctx* c = b->ctx[i];
+                if (!batch_launch_eligible(c)) continue;
+                const pyro_ctx* f = lm.empty() ? c : lm[0];
+                if (c->prec != f->prec || c->spec_shells != f->spec_shells || c->has_Jrep != f->has_Jrep ||
+                    c->fused_order(3, n_overrelax) != f->fused_order(3, n_overrelax))
+                    continue;
+                lm.push_back(c);
+                b->in_launch[i] = 1;
+            }
+        const bool fused = !lm.empty() && lm[0]->fused_order(3, n_overrelax);
+        const bool f64 = !lm.empty() && lm[0]->prec == 64;
+        const int shells = lm.empty() ? 3 : lm[0]->spec_shells;
+        const bool per_rep = !lm.empty() && lm[0]->has_Jrep;
+        std::vector<int32_t> work;
+        std::vector<unsigned char> mem;
+        if (!lm.empty()) {
+            const int cap = f64 ? pyro::pyro_batch_capacity<double>(shells, per_rep, kind, true, fused)
+                                : pyro::pyro_batch_capacity<float>(shells, per_rep, kind, true, fused);
+            int px = 0, k = 0;
+            if (batch_plan(lm, cap, work, px, k)) {
+                b->per_xcd = px;
+                b->blocks_per_worker = k;
+                if (f64) batch_members<double>(b, lm, mem);
+                else batch_members<float>(b, lm, mem);
+            } else {  // too many replicas to hold at once: every member sweeps on its own
+                lm.clear();
+                std::fill(b->in_launch.begin(), b->in_launch.end(), 0);
+            }
+        }
+        b->launch_members = int(lm.size());
+        // strategies of the members that sweep on their own
+        std::vector<int> strat(size_t(n), 0);
+        for (int i = 0; i < n; ++i)
+            if (!b->in_launch[i]) strat[i] = n_sweep > 0 ? b->ctx[i]->choose_strategy(kind, n_overrelax) : 1;
+        for (pyro_ctx* c : b->ctx) c->anneal_upload_counters();
+        if (!lm.empty() && (mem != b->h_members || work != b->h_work)) {
+            if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+            HIPCHK(hipStreamSynchronize(b->stream));
+            if (mem.size() > b->members.bytes) b->members.alloc(mem.size());
+            if (work.size() * sizeof(int32_t) > b->work.bytes) b->work.alloc(work.size() * sizeof(int32_t));
+            if (size_t(32) * (b->n_items + 8) * sizeof(unsigned int) > b->bar.bytes)
+                b->bar.alloc(size_t(32) * (b->n_items + 8) * sizeof(unsigned int));
+            HIPCHK(hipMemcpyAsync(b->members.p, mem.data(), mem.size(), hipMemcpyHostToDevice, b->stream));
+            HIPCHK(hipMemcpyAsync(b->work.p, work.data(), work.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                  b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+            b->h_members = mem;
+            b->h_work = work;
+        }
+        for (pyro_ctx* c : lm) c->xcd_used = true;
+        // the batch's work follows what each member has queued
+        hipEvent_t* ev = b->ev.data();
+        hipEvent_t* ev_pre = ev + 3;
+        hipEvent_t* ev_end = ev + 3 + n;
+        for (int i = 0; i < n; ++i) {
+            HIPCHK(hipEventRecord(ev_end[i], b->ctx[i]->stream));
+            HIPCHK(hipStreamWaitEvent(b->stream, ev_end[i], 0));
+        }
+        auto step_body = [&]() {
+            HIPCHK(hipEventRecord(ev[0], b->stream));
+            for (int i = 0; i < n; ++i) {
+                pyro_ctx* c = b->ctx[i];
+                HIPCHK(hipStreamWaitEvent(c->stream, ev[0], 0));
+                c->anneal_set_T();
+                if (!b->in_launch[i]) {
+                    const bool fe = n_sweep > 0 && c->sweeps_leave_energy(strat[i]);
+                    if (n_sweep > 0)
+                        c->run_sweeps(strat[i], kind, n_overrelax, n_sweep, c->counters.as<unsigned long long>(), 0ull,
+                                      fe ? c->epart.as<double>() : nullptr);
+                    c->anneal_observe(sample_ssf, fe, ppr);
+                }
+                HIPCHK(hipEventRecord(ev_pre[i], c->stream));
+            }
+            if (!lm.empty()) {
+                for (int i = 0; i < n; ++i)
+                    if (b->in_launch[i]) HIPCHK(hipStreamWaitEvent(b->stream, ev_pre[i], 0));
+                const hipError_t e =
+                    f64 ? pyro::launch_pyro_batch<double>(b->members.as<pyro::BatchMember<double>>(), b->work.as<int4>(),
+                                                          b->per_xcd, b->n_items, shells, per_rep, kind, n_overrelax,
+                                                          n_sweep, true, fused, b->bar.as<unsigned int>(), b->stream)
+                        : pyro::launch_pyro_batch<float>(b->members.as<pyro::BatchMember<float>>(), b->work.as<int4>(),
+                                                         b->per_xcd, b->n_items, shells, per_rep, kind, n_overrelax,
+                                                         n_sweep, true, fused, b->bar.as<unsigned int>(), b->stream);
+                require(e != hipErrorNotSupported, PYRO_E_UNSUPPORTED,
+                        "the batch launch needs its whole grid resident on the device at once");
+                HIPCHK(e);
+                HIPCHK(hipEventRecord(ev[1], b->stream));
+                for (int i = 0; i < n; ++i) {
+                    if (!b->in_launch[i]) continue;
+                    pyro_ctx* c = b->ctx[i];
+                    HIPCHK(hipStreamWaitEvent(c->stream, ev[1], 0));
+                    c->anneal_observe(sample_ssf, true, ppr);
+                    HIPCHK(hipEventRecord(ev_end[i], c->stream));
+                }
+            }
+            for (int i = 0; i < n; ++i)
+                HIPCHK(hipStreamWaitEvent(b->stream, b->in_launch[i] ? ev_end[i] : ev_pre[i], 0));
+        };
+        bool use_graph = true;
+        std::vector<long long> key = {kind, n_overrelax, n_sweep, sample_ssf};
+        for (int i = 0; i < n; ++i) {
+            const pyro_ctx* c = b->ctx[i];
+            use_graph = use_graph && !c->prof;
+            key.insert(key.end(), {(long long)c->graph_gen, strat[i], b->in_launch[i]});
+        }
+        if (use_graph && (!b->gexec || key != b->g_key)) {
+            if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+            b->gexec = capture_graph(b->stream, step_body);
+            b->g_key = key;
+        }
+        for (int s = 0; s < n_steps; ++s) {
+            if (use_graph) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+            else step_body();
+            for (int i = 0; i < n; ++i) {
+                pyro_ctx* c = b->ctx[i];
+                if (!c->anneal_advance(sample_ssf, ppr)) continue;
+                // a full mu-nu buffer is flushed on the member's stream between steps
+                HIPCHK(hipEventRecord(ev[2], b->stream));
+                HIPCHK(hipStreamWaitEvent(c->stream, ev[2], 0));
+                c->munu_flush();
+                HIPCHK(hipEventRecord(ev_end[i], c->stream));
+                HIPCHK(hipStreamWaitEvent(b->stream, ev_end[i], 0));
+            }
+        }
+        // what each member does next follows the batch's steps
+        HIPCHK(hipEventRecord(ev[2], b->stream));
+        for (pyro_ctx* c : b->ctx) {
+            HIPCHK(hipStreamWaitEvent(c->stream, ev[2], 0));
+            c->h_T.assign(c->R, c->h_sched[c->anneal_step - 1]);
+        }
+    });
+}
+
+int pyro_batch_info(pyro_batch* b, int* launch_members, int* blocks_per_worker, int* workgroups) {
+    return guarded([&] {
+        require(b != nullptr, PYRO_E_INVALID, "NULL batch");
+        if (launch_members) *launch_members = b->launch_members;
+        if (blocks_per_worker) *blocks_per_worker = b->launch_members ? b->blocks_per_worker : 0;
+        if (workgroups) *workgroups = b->launch_members ? 8 * b->per_xcd : 0;
     });
 }
 

@@ -115,8 +115,8 @@ constexpr int kPyroResidentMaxCells = 2048;
 // L1: bit-identical to
 // launch_pyro_colour pass after pass (epart and fused as for
 // launch_pyro_resident).  Every wait is bounded by timeout_ticks of the
-// device's wall clock; a timeout sets *err (then the results are
-// invalid).  hipErrorNotSupported when the grid
+// device's wall clock from the workgroup's arrival at the previous stage; a
+// timeout sets *err (then the results are invalid).  hipErrorNotSupported when the grid
 // cannot be resident at once.
 // Workgroups of one XCD-resident launch: 8 XCDs x ceil(R / 8) replicas x
 // blocks per replica and colour stage.
@@ -128,6 +128,35 @@ template <typename Real>
 hipError_t launch_pyro_xcd(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind, int n_or,
                            int n_sweeps, unsigned long long pass_index, hipStream_t s, double* epart, bool fused,
                            unsigned int* bar, int* err, unsigned long long timeout_ticks);
+
+// One context of a batch launch (launch_pyro_batch), in device memory.
+template <typename Real>
+struct BatchMember {
+    DeviceState<Real> st;
+    PyroCouplings<Real> J;
+    double* epart = nullptr;             // energy partials as launch_pyro_colour leaves them (EN launches)
+    int* err = nullptr;                  // the context's wait-timeout flag
+    unsigned long long timeout = 0;      // the context's wait bound, in wall-clock ticks
+    unsigned long long pass_index = 0;   // added to *st.pass_base
+    int item0 = 0;                       // counter index of its replica 0 in the launch
+};
+// Whole sweeps of several contexts in one persistent launch: the members'
+// replicas placed whole on XCDs by a work table of per_xcd int4 entries per
+// XCD (member, replica, worker, workers; member < 0 idle), grid 8 per_xcd,
+// one-wave workgroups; bar[32 (n_items + 8)] holds the n_items replica
+// counters and 8 XCD tickets and is zeroed by the launch.  Every member
+// shares the template arguments (precision, coupling shells, per-replica
+// couplings, kind, fused order, energy partials).  Each member's results
+// equal launch_pyro_colour's pass after pass.  hipErrorNotSupported when
+// the grid cannot be resident at once.
+template <typename Real>
+hipError_t launch_pyro_batch(const BatchMember<Real>* members, const int4* work, int per_xcd, int n_items, int shells,
+                             bool per_rep, int kind, int n_or, int n_sweeps, bool energy, bool fused,
+                             unsigned int* bar, hipStream_t s);
+// Workgroups of launch_pyro_batch that can be resident at once on the
+// current device (0 on error).
+template <typename Real>
+int pyro_batch_capacity(int shells, bool per_rep, int kind, bool energy, bool fused);
 // fused: the fused pass order of launch_pyro_colour, bit for bit.
 template <typename Real>
 hipError_t launch_pyro_resident(const DeviceState<Real>& st, const PyroCouplings<Real>& J, int kind,

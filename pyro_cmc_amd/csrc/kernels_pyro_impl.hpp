@@ -666,9 +666,10 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
 // index as pyro_colour_kernel's block b (bpr = its blocks per replica), and
 // the same per-site counters, draws and arithmetic (the pass-by-pass or the
 // fused pass order of pyro_resident_kernel), so the results equal strategy
-// 3's bit for bit.  Every wait is bounded: after `timeout_ticks` of the
-// device's wall clock the workgroup sets *err and returns, and the host
-// reports the error instead of the results.
+// 3's bit for bit.  Every wait is bounded: when the workgroup sees its
+// replica's count later than `timeout_ticks` of the device's wall clock
+// after it counted its own previous stage, it sets *err and returns, and
+// the host reports the error instead of the results.
 constexpr int kHwRegXccId = 20;  // hwreg(HW_REG_XCC_ID), bits 3:0 = the XCD
 
 __device__ __forceinline__ int xcc_id() {
@@ -703,24 +704,22 @@ struct L2Spins<double> {
 };
 
 // Stage counter of replica r: wait until it reaches `target` (device-scope
-// reads, past the L1); false (and *err set) once the wait has lasted
-// `timeout` ticks of the wall clock.
+// reads, past the L1); false (and *err set) when the count is seen only
+// after `timeout` ticks of the wall clock from `arrived`, the time this
+// workgroup counted its previous stage.
 __device__ __forceinline__ bool xcd_wait(unsigned int* ctr, unsigned int target, int* err,
-                                         unsigned long long timeout, unsigned int zero) {
-    unsigned long long deadline = 0;
+                                         unsigned long long arrived, unsigned long long timeout,
+                                         unsigned int zero) {
     for (;;) {
         // an atomic read-modify-write executes in the L2 (never the L1);
         // `zero` (0, a kernel argument) keeps the compiler from folding an
         // OR with 0 into a plain load, which the L1 may serve stale
         const unsigned int v = __hip_atomic_fetch_or(ctr, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (v >= target) return true;
-        const unsigned long long now = wall_clock64();
-        if (deadline == 0) {
-            deadline = now + timeout;
-        } else if (now > deadline) {
+        if (wall_clock64() - arrived > timeout) {
             atomicOr(err, 1);
             return false;
         }
+        if (v >= target) return true;
         __builtin_amdgcn_s_sleep(1);
     }
 }
@@ -793,6 +792,7 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
     const int n_groups = FUSED ? (n_or > 1 ? n_or : 1) : n_or + 1;
     unsigned int n_acc = 0;
     unsigned int stage = 0;
+    unsigned long long arrived = 0;
     for (int sw = 0; sw < n_sweeps; ++sw) {
         const unsigned long long p0 = pass_start + (unsigned long long)sw * unsigned(1 + n_or);
         for (int grp = 0; grp < n_groups; ++grp) {
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
                 }
                 asm volatile("" : "+v"(u0.m[0]), "+v"(u0.m[1]), "+v"(u0.m[2]), "+v"(u0.m[3]), "+v"(u0.m[4]),
                              "+v"(u1.m[0]));
-                if (stage > 0 && !xcd_wait(ctr, stage * unsigned(bpr), err, timeout_ticks, zero)) return;
+                if (stage > 0 && !xcd_wait(ctr, stage * unsigned(bpr), err, arrived, timeout_ticks, zero)) return;
                 bool a = false;
                 double e = 0.0;
                 if (live) {
@@ -852,10 +852,119 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
                 // release: this wave's stores reach the L2 before its count
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                arrived = wall_clock64();
             }
         }
     }
     if (st.acc_wave && threadIdx.x == 0 && n_acc) st.acc_wave[size_t(r) * unsigned(st.acc_waves) + unsigned(b)] += n_acc;
+}
+
+// Whole sweeps of a batch of contexts (lattices of different sizes, each
+// with its own replicas, temperatures and couplings) in one persistent
+// launch: pyro_xcd_kernel's scheme over a work table instead of one
+// context's replicas.  The table lists, per XCD, the workgroups of the
+// replicas placed there: work[x * per_xcd + t] = (member, replica, worker,
+// workers) for the workgroup holding ticket t on XCD x (member < 0: idle).
+// Worker w of a replica's nw takes blocks w, w + nw, ... of each colour
+// stage (the blocks of pyro_colour_kernel), and the replica's counter
+// reaches stage * nw when the stage is done.  Each site sees the same
+// counters, draws and arithmetic as in pyro_colour_kernel, so every
+// member's results equal its own strategy-3 sweeps bit for bit.  A wait
+// that runs past the member's bound sets the member's error flag.
+template <typename Real, int SHELLS, int FIRST, bool PER_REP, bool EN, bool FUSED>
+__global__ __launch_bounds__(64) void pyro_batch_kernel(const BatchMember<Real>* __restrict__ members,
+                                                        const int4* __restrict__ work, int per_xcd, int n_items,
+                                                        int n_or, int n_sweeps, unsigned int* __restrict__ bar,
+                                                        unsigned int zero) {
+    constexpr int NCOL = pyrostencil::colours_for(SHELLS);
+    const int x = xcc_id() & 7;
+    unsigned int tk = 0;
+    if (threadIdx.x == 0)
+        tk = __hip_atomic_fetch_add(bar + size_t(n_items + x) * kXcdPad, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+    const int t = int(__builtin_amdgcn_readfirstlane(tk));
+    if (t >= per_xcd) return;  // the dispatcher gives every XCD per_xcd workgroups: never taken
+    const int4 wk = work[size_t(x) * unsigned(per_xcd) + unsigned(t)];
+    if (wk.x < 0) return;  // an idle entry: no replica, no count
+    const BatchMember<Real>& M = members[wk.x];
+    int* const err = M.err;
+    if (*err) return;  // an earlier launch of this member failed: its next read reports it
+    const unsigned long long timeout_ticks = M.timeout;
+    const DeviceState<Real> st = M.st;
+    const int r = wk.y, w = wk.z, nw = wk.w;
+    const int nc = st.n_cells;
+    const int bps = (nc + 63) / 64;
+    const int bpr = bps * (16 / NCOL);
+    const size_t N = size_t(st.N);
+    Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
+    const L2Spins<Real> ld{__builtin_amdgcn_make_buffer_rsrc(Sr, 0, int(3 * N * sizeof(Real)), kBufferWord3)};
+    const PyroCouplings<Real> Jr = PER_REP ? M.J.at(r) : M.J;
+    const ReplicaParams<Real> rp = replica_params(st, r);
+    const unsigned long long pass_start = M.pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    const size_t es = size_t(pyro_colour_energy_blocks(nc));
+    unsigned int* ctr = bar + size_t(M.item0 + r) * kXcdPad;
+    const int n_groups = FUSED ? (n_or > 1 ? n_or : 1) : n_or + 1;
+    unsigned int n_acc = 0;
+    unsigned int stage = 0;
+    unsigned long long arrived = 0;
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+        const unsigned long long p0 = pass_start + (unsigned long long)sw * unsigned(1 + n_or);
+        for (int grp = 0; grp < n_groups; ++grp) {
+            const bool en_pass = EN && sw == n_sweeps - 1 && grp == n_groups - 1;  // uniform
+            int ucomp = 0;
+            unsigned long long pass = p0 + unsigned(grp);
+            if (FUSED && grp > 0) {
+                const int j = kUniformsOf[FIRST] + grp;
+                pass = p0 + unsigned(j / 5);
+                ucomp = j % 5;
+            }
+            const bool first = grp == 0 && FIRST != 3;
+            for (int col = 0; col < NCOL; ++col, ++stage) {
+                if (stage > 0 && !xcd_wait(ctr, stage * unsigned(nw), err, arrived, timeout_ticks, zero)) return;
+                for (int b = w; b < bpr; b += nw) {
+                    const int q = b / bps;
+                    const int cell = (b - q * bps) * 64 + int(threadIdx.x);
+                    const bool live = cell < nc;
+                    const int sl = NCOL == 16 ? col : col + 4 * q;
+                    const int site = sl * nc + cell;
+                    bool a = false;
+                    double e = 0.0;
+                    if (live) {
+                        Uniform5 u0 = site_uniforms(st, site, r, pass), u1 = u0;
+                        if (first) {
+                            if constexpr (FUSED && kUniformsOf[FIRST] % 5 == 0)
+                                u1 = site_uniforms(st, site, r, pass + unsigned(kUniformsOf[FIRST] / 5));
+                        } else {
+                            u0.m[0] = uniform_at(u0, ucomp);
+                        }
+                        const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
+                        Real hx, hy, hz;
+                        LowSums<Real> lo;
+                        if (en_pass) pyro_field_rt_gathered<Real, SHELLS, NCOL>(sl, ld, nc, c, Jr, hx, hy, hz, &lo);
+                        else pyro_field_rt_gathered<Real, SHELLS>(sl, ld, nc, c, Jr, hx, hy, hz);
+                        const Vec3<Real> v = ld(unsigned(site));
+                        Real sx = v.x, sy = v.y, sz = v.z;
+                        if (first)
+                            a = drawn_update<Real, FIRST, FUSED>(rp, st, sx, sy, sz, site, r, n_or, u0, u1, hx, hy,
+                                                                 hz, Sr, N);
+                        else
+                            drawn_update<Real, 3, false>(rp, st, sx, sy, sz, site, r, n_or, u0, u1, hx, hy, hz, Sr,
+                                                         N);
+                        if (en_pass) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, sx, sy, sz);
+                    }
+                    if (first && st.acc_wave) n_acc += unsigned(__popcll(__ballot(a)));
+                    if (en_pass) {
+                        const double tot = wave_sum_f64(e);
+                        if (threadIdx.x == 0) M.epart[size_t(col) * unsigned(bpr) + size_t(r) * es + unsigned(b)] = tot;
+                    }
+                }
+                // release: this wave's stores reach the L2 before its count
+                if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                arrived = wall_clock64();
+            }
+        }
+    }
+    if (st.acc_wave && threadIdx.x == 0 && n_acc) st.acc_wave[size_t(r) * unsigned(st.acc_waves) + unsigned(w)] += n_acc;
 }
 
 // Lane -> tile cell of the tiled sweep.  A colour stage reads, for every
@@ -1384,6 +1493,70 @@ hipError_t launch_pyro_xcd(const DeviceState<Real>& st, const PyroCouplings<Real
         return epart ? by_fused(per_rep, std::true_type{}) : by_fused(per_rep, std::false_type{});
     };
     return J.per_replica ? by_en(std::true_type{}) : by_en(std::false_type{});
+}
+
+namespace {
+// Calls f with the batch kernel instance of the given template arguments.
+template <typename Real, typename F>
+hipError_t with_batch_kernel(int shells, bool per_rep, int kind, bool energy, bool fused, F&& f) {
+    if (kind < 0 || kind > 2) return hipErrorInvalidValue;
+    auto go = [&](auto sh, auto pr, auto en, auto fu) -> hipError_t {
+        constexpr int SH = decltype(sh)::value;
+        constexpr bool PR = decltype(pr)::value, EN = decltype(en)::value, FU = decltype(fu)::value;
+        switch (kind) {
+            case 0: return f(pyro_batch_kernel<Real, SH, 0, PR, EN, FU>);
+            case 1: return f(pyro_batch_kernel<Real, SH, 1, PR, EN, FU>);
+            default: return f(pyro_batch_kernel<Real, SH, 2, PR, EN, FU>);
+        }
+    };
+    auto by_fu = [&](auto sh, auto pr, auto en) -> hipError_t {
+        return fused ? go(sh, pr, en, std::true_type{}) : go(sh, pr, en, std::false_type{});
+    };
+    auto by_en = [&](auto sh, auto pr) -> hipError_t {
+        return energy ? by_fu(sh, pr, std::true_type{}) : by_fu(sh, pr, std::false_type{});
+    };
+    auto by_pr = [&](auto sh) -> hipError_t {
+        return per_rep ? by_en(sh, std::true_type{}) : by_en(sh, std::false_type{});
+    };
+    switch (shells) {
+        case 1: return by_pr(std::integral_constant<int, 1>{});
+        case 2: return by_pr(std::integral_constant<int, 2>{});
+        case 3: return by_pr(std::integral_constant<int, 3>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+}  // namespace
+
+template <typename Real>
+int pyro_batch_capacity(int shells, bool per_rep, int kind, bool energy, bool fused) {
+    int cap = 0;
+    const hipError_t e = with_batch_kernel<Real>(shells, per_rep, kind, energy, fused, [&](auto fn) -> hipError_t {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t r = hipGetDevice(&dev);
+        if (r == hipSuccess) r = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (r == hipSuccess)
+            r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), 64, 0);
+        if (r == hipSuccess && cus % 8 == 0) cap = per_cu * cus;
+        return r;
+    });
+    return e == hipSuccess ? cap : 0;
+}
+
+template <typename Real>
+hipError_t launch_pyro_batch(const BatchMember<Real>* members, const int4* work, int per_xcd, int n_items, int shells,
+                             bool per_rep, int kind, int n_or, int n_sweeps, bool energy, bool fused,
+                             unsigned int* bar, hipStream_t s) {
+    const bool fu = fused && n_or >= 1;
+    // every workgroup of a replica must be resident at once (they wait for
+    // one another): refuse a grid beyond what fits
+    const int cap = pyro_batch_capacity<Real>(shells, per_rep, kind, energy, fu);
+    if (per_xcd <= 0 || cap < 8 * per_xcd) return hipErrorNotSupported;
+    return with_batch_kernel<Real>(shells, per_rep, kind, energy, fu, [&](auto fn) -> hipError_t {
+        hipLaunchKernelGGL(xcd_zero_kernel, dim3(1), dim3(256), 0, s, bar, (n_items + 8) * kXcdPad);
+        hipLaunchKernelGGL(fn, dim3(unsigned(8 * per_xcd)), dim3(64), 0, s, members, work, per_xcd, n_items, n_or,
+                           n_sweeps, bar, 0u);
+        return hipGetLastError();
+    });
 }
 
 template <typename Real>

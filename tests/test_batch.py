@@ -7,7 +7,9 @@ replica's model (fields and energy to 1e-12 / 1e-5, FP64 trajectories draw
 for draw in colour order and in the tiled order).  The anneal schedule
 (temperature steps run as a replayed HIP graph) must equal the same steps
 issued one call at a time, and contexts of different sizes run concurrently
-on their own streams must give what each gives alone.
+on their own streams must give what each gives alone, and a Batch of
+contexts (one graph per step, the small lattices in one persistent launch)
+what each gives with its own steps.
 """
 import numpy as np
 import pytest
@@ -234,6 +236,60 @@ def test_uneven_batch_driver(gpu, oracle):
         e = o.energy_per_cell()
         assert abs(r.E_final - e) <= 1e-5 * 16 * _scale(it.J, it.B)
         assert np.all(np.isfinite(r.S)) and r.S.shape == (1, 3, it.L ** 3, 2)
+
+
+@pytest.mark.parametrize("prec,fused,kind", [(32, True, 0), (32, False, 1), (64, True, 1), (64, False, 0)])
+def test_batch_launch_equals_own_graphs(gpu, oracle, prec, fused, kind):
+    """A Batch steps every context as its own anneal steps would, bit for
+    bit: the L = 2, 4, 6 contexts sweep in the one persistent launch (the
+    per-colour stencil order), the L = 16 context (tiled, a large lattice)
+    in its own branch of the step's graph.  Energy histories, final spins and
+    SSF sums at two sampling temperatures are identical."""
+    from pyro_cmc_amd import batch
+
+    items = batch.uneven_items(n=12, sizes=(2, 4, 6, 16), seed=2)
+    kw = dict(T_hot=5.0, T_cold=0.1, n_steps=5, n_sweep=2, n_burn_in=2, n_sample=2, seed=2, keep_spins=True,
+              precision=prec, fused=fused, kind=kind, T_sample=[1.0, 0.1])
+    a, sa = batch.run_anneal_batch(items, persistent=True, **kw)
+    b, sb = batch.run_anneal_batch(items, persistent=False, **kw)
+    assert sa["contexts"] == {2: 3, 4: 5, 6: 2, 16: 2}
+    launch = sa["persistent_launch"]
+    assert launch["contexts"] == 3 and launch["blocks_per_workgroup"] == 1
+    # replicas placed whole on XCDs: 3 + 5 (one block each) and 2 x 4 blocks
+    assert launch["workgroups"] % 8 == 0 and launch["workgroups"] >= 3 + 5 + 2 * 4
+    for ra, rb in zip(a, b):
+        assert np.array_equal(ra.E, rb.E)
+        assert np.array_equal(ra.spins, rb.spins)
+        assert np.array_equal(ra.S, rb.S) and np.array_equal(ra.S2, rb.S2)
+        assert np.array_equal(ra.n_samples, rb.n_samples) and np.array_equal(ra.n_samples, [2, 2])
+
+
+def test_batch_wait_timeout_is_reported(gpu, oracle):
+    """The persistent launch bounds each member's stage waits by that
+    member's own bound: with one member's set to a nanosecond its next read
+    reports the timeout, the other member reads normally, and once the bound
+    is restored the batch steps and reads again."""
+    sched = np.geomspace(3.0, 0.2, 8)
+    ctxs = []
+    for i, L in enumerate((4, 6)):
+        c, _ = _batch(gpu, oracle, L, 32, seed=50 + i)
+        c.randomize(9 + i)
+        c.anneal_schedule(sched)
+        ctxs.append(c)
+    b = gpu.Batch(ctxs)
+    ctxs[0].set_wait_timeout(1e-9)
+    b.anneal_steps(gpu.METROPOLIS, 1, 3, False, 2)
+    assert b.info()[0] == 2
+    with pytest.raises(gpu.PyroError, match="timed out"):
+        ctxs[0].anneal_history()
+    assert np.all(np.isfinite(ctxs[1].anneal_history()[:, :2]))
+    ctxs[0].set_wait_timeout(0.25)
+    b.anneal_steps(gpu.METROPOLIS, 1, 3, False, 2)
+    E = ctxs[0].anneal_history()  # (steps 0 and 1 summed partials the aborted launch left unwritten)
+    assert np.all(np.isfinite(E[:, 2:4])) and np.all(E[:, 2:4] != 0)
+    b.close()
+    for c in ctxs:
+        c.close()
 
 
 def test_batch_samples_every_T_sample(gpu, oracle):
