@@ -369,6 +369,10 @@ def large_lattice_sweep_line(pyro):
             "bound": "hbm", "achieved": b_big * upl / (launch_ms / 1e3) / 1e9,
             "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": b_big * upl / (launch_ms / 1e3) / PEAK_HBM,
+            # a tiled launch reads and writes each spin (and its direction
+            # byte) once for both passes
+            "frac_per_site_visit": bytes_per_update(4, 30, 16, lifted_frac=1.0) * (upl / 2)
+                                   / (launch_ms / 1e3) / PEAK_HBM,
             "traffic": traffic,
             # the measured bytes per launch moved in the measured launch time
             "traffic_rate_frac": (traffic / (launch_ms / 1e3) / PEAK_HBM) if traffic else None,
@@ -475,12 +479,14 @@ def main():
                       2: 1, 4: 1, 5: 8 * n_sw, 6: n_sw // N_SWEEP}[strategy]
     upd_per_launch = N * R_REP * (1 + N_OR) * n_sw / sweep_launches
     avg_launch_s = sweep_ms / 1e3 / sweep_launches
-    # compulsory bytes per site a launch visits (its spin read and written
-    # once, whatever the number of updates it takes there, and its share of
-    # the neighbour row); sites per launch = the launch's colour class
+    # compulsory bytes per site-update (its spin read and written, its share
+    # of the neighbour row) x the launch's site updates; beside it the
+    # stricter count per site visited: a fused stage reads and writes a spin
+    # once for its two updates
     b_alg = bytes_per_update(4, info["n_slot"], R_REP)
     sites_per_launch = N * R_REP * n_sw * (groups if strategy in (3, 6) else 1 + N_OR) / sweep_launches
-    achieved = b_alg * sites_per_launch / avg_launch_s
+    achieved = b_alg * upd_per_launch / avg_launch_s
+    achieved_visit = b_alg * sites_per_launch / avg_launch_s
     traffic_c3 = profiled_traffic("sweep_c3_L12x64" + ("_xcd" if strategy == 6 else ""))
     st = ctx.pt_stats()
     acc_rate = (st["accepts"] / np.maximum(st["attempts"], 1)).tolist()
@@ -529,14 +535,15 @@ def main():
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM,
+            "frac_per_site_visit": achieved_visit / PEAK_HBM,
             "traffic": traffic_c3,
             # the measured bytes per launch moved in the measured launch time
             "traffic_rate_frac": (traffic_c3 / avg_launch_s / PEAK_HBM) if traffic_c3 else None,
             "kernel": {1: "sweep_colour_kernel", 2: "sweep_resident_kernel", 3: "pyro_colour_kernel",
                        4: "pyro_resident_kernel", 5: "pyro_tiled_kernel", 6: "pyro_xcd_kernel"}[strategy],
-            "bytes_per_site": b_alg,
-            "sites_per_launch": sites_per_launch,
+            "bytes_per_update": b_alg,
             "updates_per_launch": upd_per_launch,
+            "sites_visited_per_launch": sites_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,
             # the HIP-event sample avg_launch_us comes from
             "launch_timing": {"sweeps": n_sw, "launches": sweep_launches, "elapsed_ms": sweep_ms,

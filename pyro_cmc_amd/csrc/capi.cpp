@@ -951,9 +951,10 @@ namespace {
 // Whether member c can sweep in the batch's persistent launch: a context of
 // the per-colour stencil strategies left to the automatic choice (not the
 // tiled strategy's large lattices, whose fixed rule gives a different
-// visiting order).
+// visiting order), with replicas below 2^24 sites (the launch's 24-bit
+// byte offsets).
 bool batch_launch_eligible(const pyro_ctx* c) {
-    return c->spec && c->strategy == 0 && !(c->tile_kind > 0 && c->nc() >= 4096);
+    return c->spec && c->strategy == 0 && !(c->tile_kind > 0 && c->nc() >= 4096) && c->N() < (int64_t(1) << 24);
 }
 
 // Places every replica of the launch's members whole on one XCD: worker

@@ -145,7 +145,7 @@ struct BatchMember {
 // XCD (member, replica, worker, workers; member < 0 idle), grid 8 per_xcd,
 // one-wave workgroups; bar[32 (n_items + 8)] holds the n_items replica
 // counters and 8 XCD tickets and is zeroed by the launch.  Every member
-// shares the template arguments (precision, coupling shells, per-replica
+// has replicas below 2^24 sites and shares the template arguments (precision, coupling shells, per-replica
 // couplings, kind, fused order, energy partials).  Each member's results
 // equal launch_pyro_colour's pass after pass.  hipErrorNotSupported when
 // the grid cannot be resident at once.

@@ -677,7 +677,10 @@ __device__ __forceinline__ int xcc_id() {
 }
 
 // A replica's spins read with device-scope buffer loads (cache policy sc1:
-// past the L1, from the L2).
+// past the L1, from the L2).  The byte offset of site i is a 24-bit
+// multiply (full rate, where a 32-bit one is quarter rate): the launches
+// that use it hold every workgroup resident at once, which keeps a
+// replica far below 2^24 sites (and the launchers check it).
 constexpr int kBufferWord3 = 0x00020000;  // raw-buffer descriptor word 3 of gfx9, 32-bit data
 constexpr int kPolicySc1 = 16;            // the sc1 bit of a buffer load's cache policy
 constexpr int kXcdPad = 32;               // counters 128 bytes apart: one cache line each
@@ -687,7 +690,7 @@ template <>
 struct L2Spins<float> {
     __amdgpu_buffer_rsrc_t rs;
     __device__ __forceinline__ Vec3<float> operator()(unsigned i) const {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b96(rs, i * 12u, 0, kPolicySc1);
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(rs, __umul24(i, 12u), 0, kPolicySc1);
         return Vec3<float>{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
     }
 };
@@ -695,9 +698,10 @@ template <>
 struct L2Spins<double> {
     __amdgpu_buffer_rsrc_t rs;
     __device__ __forceinline__ Vec3<double> operator()(unsigned i) const {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b64(rs, i * 24u, 0, kPolicySc1);
-        const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, i * 24u + 8u, 0, kPolicySc1);
-        const auto c = __builtin_amdgcn_raw_buffer_load_b64(rs, i * 24u + 16u, 0, kPolicySc1);
+        const unsigned o = __umul24(i, 24u);
+        const auto a = __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, kPolicySc1);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, o + 8u, 0, kPolicySc1);
+        const auto c = __builtin_amdgcn_raw_buffer_load_b64(rs, o + 16u, 0, kPolicySc1);
         return Vec3<double>{__hiloint2double(int(a[1]), int(a[0])), __hiloint2double(int(b[1]), int(b[0])),
                             __hiloint2double(int(c[1]), int(c[0]))};
     }
@@ -1473,7 +1477,8 @@ hipError_t launch_pyro_xcd(const DeviceState<Real>& st, const PyroCouplings<Real
                 if (e == hipSuccess)
                     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), 64, 0);
                 if (e != hipSuccess) return e;
-                if (cus % 8 != 0 || size_t(per_cu) * size_t(cus) < size_t(G)) return hipErrorNotSupported;
+                if (cus % 8 != 0 || size_t(per_cu) * size_t(cus) < size_t(G) || st.N >= (1 << 24))
+                    return hipErrorNotSupported;
                 hipLaunchKernelGGL(xcd_zero_kernel, dim3(1), dim3(256), 0, s, bar, (st.R + 8) * kXcdPad);
                 hipLaunchKernelGGL(fn, dim3(unsigned(G)), dim3(64), 0, s, st, J, n_or, n_sweeps, pass_index, epart,
                                    bar, err, timeout_ticks, 0u);

@@ -22,17 +22,16 @@ struct Words4 {
     uint32_t x, y, z, w;
 };
 
-__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
-    return uint32_t((uint64_t(a) * uint64_t(b)) >> 32);
-}
-
 __host__ __device__ inline Words4 philox4x32_10(Words4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
-        const uint32_t lo0 = 0xD2511F53u * c.x;
-        const uint32_t hi0 = mulhi32(0xD2511F53u, c.x);
-        const uint32_t lo1 = 0xCD9E8D57u * c.z;
-        const uint32_t hi1 = mulhi32(0xCD9E8D57u, c.z);
+        // each 32x32 -> 64-bit product as one multiply whose halves are both
+        // used (one v_mad_u64_u32 on the GPU instead of a low and a high
+        // multiply, all three quarter rate)
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c.x;
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c.z;
+        const uint32_t lo0 = uint32_t(p0), hi0 = uint32_t(p0 >> 32);
+        const uint32_t lo1 = uint32_t(p1), hi1 = uint32_t(p1 >> 32);
         c = Words4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;

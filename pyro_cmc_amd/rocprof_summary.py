@@ -86,20 +86,23 @@ def events(path: str) -> str:
     lt = rf["launch_timing"]
     avg_us = 1e3 * lt["elapsed_ms"] / lt["launches"]
     per_round = lt["launches_per_round"]
-    # compulsory bytes per site a launch visits x sites per launch (older
-    # lines: per update x updates, one update per site and launch)
-    b_unit = rf.get("bytes_per_site", rf.get("bytes_per_update"))
-    n_unit = rf.get("sites_per_launch", rf.get("updates_per_launch"))
-    unit = "sites" if "sites_per_launch" in rf else "updates"
-    gbs = b_unit * n_unit / (avg_us * 1e-6) / 1e9
+    # compulsory bytes per site-update x site updates per launch, and the
+    # same bytes per site visited (a fused stage reads and writes a spin
+    # once for two updates) x sites visited per launch
+    b_unit = rf["bytes_per_update"]
+    n_upd = rf["updates_per_launch"]
+    n_vis = rf.get("sites_visited_per_launch", n_upd)
+    gbs = b_unit * n_upd / (avg_us * 1e-6) / 1e9
+    gbs_v = b_unit * n_vis / (avg_us * 1e-6) / 1e9
     out = [
         f"sweep kernel {rf['kernel']}: {lt['sweeps']} sweeps back to back = {lt['launches']} launches in "
         f"{lt['elapsed_ms']:.3f} ms (HIP events on the library stream) -> {avg_us:.3f} us per launch",
         f"round: {per_round} sweep launches x {avg_us:.3f} us = {per_round * avg_us / 1e3:.4f} ms of the measured "
         f"{line['ms_per_step']:.4f} ms per round (wall clock over {line['steps']} rounds)",
-        f"roofline: {b_unit} B x {n_unit:.0f} {unit} / {avg_us:.3f} us = "
-        f"{gbs:.1f} GB/s = {gbs / rf['peak']:.4f} of {rf['peak']:.0f} GB/s"
-        + (f" ({rf['updates_per_launch']:.0f} site updates per launch)" if unit == "sites" else ""),
+        f"roofline: {b_unit} B x {n_upd:.0f} site updates / {avg_us:.3f} us = "
+        f"{gbs:.1f} GB/s = {gbs / rf['peak']:.4f} of {rf['peak']:.0f} GB/s",
+        f"  per site visited: {b_unit} B x {n_vis:.0f} sites / {avg_us:.3f} us = "
+        f"{gbs_v:.1f} GB/s = {gbs_v / rf['peak']:.4f} of {rf['peak']:.0f} GB/s",
         "",
         f"per-family HIP-event times over {line.get('kernel_profiled_rounds', '?')} profiled rounds "
         "(events around each family's launches):",
