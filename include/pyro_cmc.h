@@ -259,8 +259,8 @@ typedef struct pyro_batch pyro_batch;
 int pyro_batch_create(pyro_ctx* const* ctxs, int n_ctx, pyro_batch** out);
 /* pyro_anneal_steps(ctx, ...) for every context of the batch, with the
  * same results bit for bit, as one HIP graph per step: the contexts whose
- * sweeps take the per-colour stencil strategies under the automatic choice
- * sweep together in one persistent launch (every replica of every such
+ * sweeps take the per-colour stencil strategies under the automatic choice,
+ * or that are forced to strategy 6, sweep together in one persistent launch (every replica of every such
  * context, whole sweeps, each replica's workgroups on one XCD, stage waits
  * bounded as set by pyro_set_wait_timeout), the others (the tiled strategy
  * on large lattices, forced strategies) sweep on their own in parallel

@@ -948,13 +948,16 @@ struct pyro_batch {
 
 namespace {
 
-// Whether member c can sweep in the batch's persistent launch: a context of
+// Whether member c sweeps in the batch's persistent launch: a context of
 // the per-colour stencil strategies left to the automatic choice (not the
 // tiled strategy's large lattices, whose fixed rule gives a different
-// visiting order), with replicas below 2^24 sites (the launch's 24-bit
-// byte offsets).
+// visiting order), or one forced to the XCD-resident sweep (whose results
+// the launch reproduces, and which beside it would be a second persistent
+// kernel competing for residency), with replicas below 2^24 sites (the
+// launch's 24-bit byte offsets).
 bool batch_launch_eligible(const pyro_ctx* c) {
-    return c->spec && c->strategy == 0 && !(c->tile_kind > 0 && c->nc() >= 4096) && c->N() < (int64_t(1) << 24);
+    const bool autom = c->strategy == 0 && !(c->tile_kind > 0 && c->nc() >= 4096);
+    return c->spec && (autom || c->strategy == 6) && c->N() < (int64_t(1) << 24);
 }
 
 // Places every replica of the launch's members whole on one XCD: worker

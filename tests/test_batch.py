@@ -264,6 +264,39 @@ def test_batch_launch_equals_own_graphs(gpu, oracle, prec, fused, kind):
         assert np.array_equal(ra.n_samples, rb.n_samples) and np.array_equal(ra.n_samples, [2, 2])
 
 
+def test_batch_takes_forced_xcd_members_into_its_launch(gpu, oracle):
+    """A member forced to the XCD-resident sweep (strategy 6) sweeps in the
+    batch's persistent launch, whose results it reproduces, rather than as
+    a second persistent kernel beside it; a member forced to per-colour
+    launches (3) keeps its own.  Both end where their own steps end."""
+    sched = np.geomspace(3.0, 0.2, 6)
+
+    def build():
+        cs = []
+        for i, (L, strat) in enumerate(((4, 6), (6, 3), (3, 0))):
+            c, _ = _batch(gpu, oracle, L, 32, seed=70 + i)
+            c.set_strategy(strat)
+            c.randomize(30 + i)
+            c.anneal_schedule(sched)
+            cs.append(c)
+        return cs
+
+    a = build()
+    b = gpu.Batch(a)
+    b.anneal_steps(gpu.METROPOLIS, 1, 2, False, 6)
+    assert b.info()[0] == 2  # the strategy-6 and the automatic member
+    own = build()
+    for c in own:
+        c.anneal_steps(gpu.METROPOLIS, 1, 2, False, 6)
+    for x, y in zip(a, own):
+        assert np.array_equal(x.anneal_history(), y.anneal_history())
+        for r in range(x.R):
+            assert np.array_equal(x.get_spins(r), y.get_spins(r))
+    b.close()
+    for c in a + own:
+        c.close()
+
+
 def test_batch_wait_timeout_is_reported(gpu, oracle):
     """The persistent launch bounds each member's stage waits by that
     member's own bound: with one member's set to a nanosecond its next read
