@@ -287,6 +287,7 @@ struct pyro_ctx {
     ~pyro_ctx() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (agexec) (void)hipGraphExecDestroy(agexec);
+
         for (auto& p : pending) {
             (void)hipEventDestroy(p.a);
             (void)hipEventDestroy(p.b);
@@ -750,6 +751,7 @@ struct pyro_ctx {
     // SSF sample of replicas [0, R) using the device slot array `slots`.
     // SSF sample on stream s (kernel timing records on the context stream,
     // so s is the context stream whenever timing is on).
+    // One SSF sample of every replica into accumulator slots[r].
     void ssf_sample_device(const int32_t* slots, hipStream_t s) {
         for (int r0 = 0; r0 < R; r0 += ssf_chunk) {
             const int rc = std::min(ssf_chunk, R - r0);
@@ -1678,23 +1680,8 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
                 ctx->ssf_sample_device(ctx->slot_of.as<int32_t>(), ctx->stream);
             }
         };
-        auto round_body = [&]() {
-            if (sample_ssf == 2) {
-                // an SSF sample after every sweep of the round
-                for (int sw = 0; sw < n_sweep; ++sw) {
-                    ctx->run_sweeps(strat, kind, n_overrelax, 1, ctr, (unsigned long long)sw * (1 + n_overrelax),
-                                    fused_energy && sw == n_sweep - 1 ? ctx->epart.as<double>() : nullptr);
-                    ssf_round_sample();
-                }
-            } else {
-                if (n_sweep > 0)
-                    ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
-                                    fused_energy ? ctx->epart.as<double>() : nullptr);
-                // One stream: running the SSF branch on a second stream beside the
-                // energy (fork and join by events inside the captured round) made
-                // each round 0.19 ms slower, more than the branch itself takes.
-                if (sample_ssf) ssf_round_sample();
-            }
+        // the round's energy sample and exchange, after its sweeps and SSF
+        auto energy_exchange = [&]() {
             if (fused_energy) ctx->energy_finish(true, pyro::pyro_colour_energy_blocks(int(ctx->nc())));
             else ctx->energy(true);
             pyro::ExchangeArgs a;
@@ -1714,7 +1701,27 @@ int pyro_pt_rounds(pyro_ctx* ctx, int kind, int n_overrelax, int n_sweep, int sa
             a.R = ctx->R;
             ctx->timed(kFamExchange, [&] { return pyro::launch_exchange(a, ctx->stream); });
         };
-
+        auto round_body = [&]() {
+            if (sample_ssf == 2) {
+                // an SSF sample after every sweep of the round
+                for (int sw = 0; sw < n_sweep; ++sw) {
+                    ctx->run_sweeps(strat, kind, n_overrelax, 1, ctr, (unsigned long long)sw * (1 + n_overrelax),
+                                    fused_energy && sw == n_sweep - 1 ? ctx->epart.as<double>() : nullptr);
+                    ssf_round_sample();
+                }
+            } else {
+                if (n_sweep > 0)
+                    ctx->run_sweeps(strat, kind, n_overrelax, n_sweep, ctr, 0ull,
+                                    fused_energy ? ctx->epart.as<double>() : nullptr);
+                // One stream.  The SSF on a second stream beside the energy
+                // (fork and join by events inside the captured round) made each
+                // round 0.19 ms slower; sampled from a snapshot beside the next
+                // round's XCD-resident sweep it slowed that sweep by more than it
+                // saved, and the snapshot copy cost as much again.
+                if (sample_ssf) ssf_round_sample();
+            }
+            energy_exchange();
+        };
         const bool use_graph = !ctx->prof;
         if (use_graph) {
             if (!ctx->gexec || ctx->g_kind != kind || ctx->g_nor != n_overrelax || ctx->g_nsw != n_sweep ||

@@ -465,6 +465,44 @@ def test_tempering_fused_energy(gpu, oracle, prec, L, R, J, kind, n_or, strategy
 
 
 @pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("fused,mapped", [(True, False), (False, True), (True, True)])
+def test_ssf_rounds_equal_across_strategies(gpu, oracle, prec, fused, mapped):
+    """Tempering rounds with SSF samples, over calls of 1, 2 and 5 rounds
+    (the captured round replayed across calls): the XCD-resident sweep and per-colour launches sweep bit-identically, so
+    the SSF sums, the energy and swap statistics and the spins are
+    identical, with and without a slot map (one that reverses the ladder
+    into the upper accumulators and skips the hottest slot)."""
+    L, R = 6, 16
+    T = np.geomspace(2.0, 0.05, R)
+    n_slots = 2 * R if mapped else R
+    out = []
+    for strategy in (6, 3):
+        ctx, _ = make(gpu, oracle, L, (1.0, -0.05, 0.05), 0.8, prec, R=R, B=(0.1, -0.2, 0.3))
+        ctx.set_strategy(strategy)
+        ctx.set_fused_passes(fused)
+        ctx.randomize(21)
+        ctx.pt_init(T, 5)
+        ctx.ssf_configure(("xx", "zz", "xy"), n_slots=n_slots, store_error_term=True)
+        if mapped:
+            m = (2 * R - 1 - np.arange(R)).astype(np.int32)
+            m[0] = -1
+            ctx.pt_ssf_map(m)
+        for n in (1, 2, 5):
+            ctx.pt_rounds(gpu.METROPOLIS, 2, 2, True, n)
+        out.append((ctx.pt_stats(), [ctx.ssf_read(s) for s in range(n_slots)],
+                    [ctx.get_spins(r) for r in range(R)]))
+        ctx.close()
+    (sa, qa, pa), (sb, qb, pb) = out
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    for x, y in zip(qa, qb):
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) and x[2] == y[2]
+    assert sum(q[2] for q in qa) == 8 * (R - 1 if mapped else R)  # every round sampled
+    for x, y in zip(pa, pb):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
 @pytest.mark.parametrize("kind,n_or", [(0, 2), (1, 0), (2, 1), (1, 1), (0, 0), (2, 0)])
 @pytest.mark.parametrize("per_rep", [True, False])
 @pytest.mark.parametrize("strategy", [3, 4, 6, "3-fused", "4-fused", "6-fused"])
