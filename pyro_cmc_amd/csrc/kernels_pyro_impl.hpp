@@ -308,6 +308,77 @@ __device__ __forceinline__ void pyro_field_rt_gathered(int sl, const Load& load,
         default: hx = hy = hz = Real(0);
     }
 }
+// pyro_field_rt_gathered with its loads split around wait(), for a colour
+// stage that must wait for the stage before it: the neighbours outside the
+// previous stage's colour (final since an earlier stage) are loaded first,
+// then wait() runs (false: give up, and so does this), then the neighbours
+// of that colour.  The sums then follow in the stencil's order, so the field
+// equals pyro_field_rt_gathered's bit for bit.  Stages run the colours in
+// order, so the previous stage's colour is colour(sl) - 1 (mod NCOL), with
+// colour(s) = s for 16 colours and s % 4 for 4.
+template <typename Real, int SHELLS, int SPLIT = 0, typename Load, typename Wait>
+__device__ __forceinline__ bool pyro_field_rt_staged(int sl, const Load& load, int nc, const CellAxes& c,
+                                                     const PyroCouplings<Real>& J, Wait&& wait, Real& hx, Real& hy,
+                                                     Real& hz, LowSums<Real>* low = nullptr) {
+    constexpr int NS = pyrostencil::slots_for(SHELLS);
+    constexpr int NCOL = pyrostencil::colours_for(SHELLS);
+    static_assert(NS <= 32, "one late-load bit per neighbour");
+    // the neighbours' site indices and which of them have the previous
+    // stage's colour (the stencil of sublattice sl, looked up once)
+    unsigned idx[NS];
+    unsigned late = 0;
+    switch (sl) {
+#define PYRO_SL_CASE(X)                                                                                          \
+    case X:                                                                                                      \
+        static_for<NS>([&](auto kk) __attribute__((always_inline)) {                                            \
+            constexpr pyrostencil::Nbr n = kStencil.n[X][decltype(kk)::value];                                  \
+            idx[decltype(kk)::value] = unsigned(n.sl * nc + c.a0[n.d[0] + 1] + c.a1[n.d[1] + 1] + c.a2[n.d[2] + 1]); \
+            if constexpr (n.sl % NCOL == (X % NCOL + NCOL - 1) % NCOL) late |= 1u << decltype(kk)::value;        \
+        });                                                                                                      \
+        break;
+        PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+        PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+        PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+        PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+        default:
+            static_for<NS>([&](auto kk) __attribute__((always_inline)) { idx[decltype(kk)::value] = 0u; });
+    }
+    Vec3<Real> v[NS];
+    static_for<NS>([&](auto kk) __attribute__((always_inline)) {
+        if (!((late >> decltype(kk)::value) & 1u)) v[decltype(kk)::value] = load(idx[decltype(kk)::value]);
+    });
+    // the early loads stay ahead of the wait: the compiler may not move
+    // memory accesses across this, and nothing here waits for them
+    asm volatile("" ::: "memory");
+    if (!wait()) return false;
+    static_for<NS>([&](auto kk) __attribute__((always_inline)) {
+        if ((late >> decltype(kk)::value) & 1u) v[decltype(kk)::value] = load(idx[decltype(kk)::value]);
+    });
+    switch (sl) {
+#define PYRO_SL_CASE(X)                                                                                          \
+    case X: {                                                                                                    \
+        int k = 0;                                                                                               \
+        pyro_field_with<Real, SHELLS, X, false, SPLIT>(                                                         \
+            [&](const pyrostencil::Nbr&, Real& x, Real& y, Real& z) __attribute__((always_inline)) {            \
+                x = v[k].x;                                                                                      \
+                y = v[k].y;                                                                                      \
+                z = v[k].z;                                                                                      \
+                ++k;                                                                                             \
+            },                                                                                                   \
+            J, hx, hy, hz, low);                                                                                 \
+        break;                                                                                                   \
+    }
+        PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
+        PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
+        PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
+        PYRO_SL_CASE(12) PYRO_SL_CASE(13) PYRO_SL_CASE(14) PYRO_SL_CASE(15)
+#undef PYRO_SL_CASE
+        default: hx = hy = hz = Real(0);
+    }
+    return true;
+}
+
 // The same with the spins of replica block Sr read through the caches.
 template <typename Real, int SHELLS, int SPLIT = 0>
 __device__ __forceinline__ void pyro_field_rt(int sl, const Real* Sr, int nc, const CellAxes& c,
@@ -656,6 +727,10 @@ __global__ __launch_bounds__(BS) void pyro_resident_kernel(DeviceState<Real> st,
 // OR of 0: atomics execute in the L2 of the issuing XCD, never in an L1.
 // Every read of the spins in this kernel is a device-scope buffer load
 // (cache policy sc1), which misses the CU's L1 and is served by the L2.
+// Of a stage's loads only those of the previous stage's colour wait for
+// that stage: the site's own spin and its other neighbours are final
+// already, so they are issued before the wait and arrive during it
+// (pyro_field_rt_staged).
 // Measured alternatives: workgroup-scope loads of the counters or spins (sc0)
 // and an L1 invalidation by a workgroup-scope acquire read stale values;
 // a device-scope acquire (which also invalidates the L2) was about four
@@ -824,22 +899,30 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
                 }
                 asm volatile("" : "+v"(u0.m[0]), "+v"(u0.m[1]), "+v"(u0.m[2]), "+v"(u0.m[3]), "+v"(u0.m[4]),
                              "+v"(u1.m[0]));
-                if (stage > 0 && !xcd_wait(ctr, stage * unsigned(bpr), err, arrived, timeout_ticks, zero)) return;
+                // the neighbour offsets from the cell's axes afresh in each
+                // stage: hoisted out of the loop (16 stencils of 30) they
+                // would take hundreds of registers; the empty asm hides the
+                // axes' values from the compiler so it cannot hoist
+                CellAxes c = axes;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(c.a0[k]), "+v"(c.a1[k]), "+v"(c.a2[k]));
+                // the site's own spin and the neighbours outside the previous
+                // stage's colour are final already: they are loaded before
+                // the wait for that stage, the rest after it (lanes past the
+                // last cell load the values of cell 0 and update nothing)
+                const Vec3<Real> v = ld(unsigned(live ? site : sl * nc));
+                auto wait = [&]() __attribute__((always_inline)) {
+                    return stage == 0 || xcd_wait(ctr, stage * unsigned(bpr), err, arrived, timeout_ticks, zero);
+                };
+                Real hx, hy, hz;
+                LowSums<Real> lo;
+                const bool ok = en_pass
+                                    ? pyro_field_rt_staged<Real, SHELLS, NCOL>(sl, ld, nc, c, Jr, wait, hx, hy, hz, &lo)
+                                    : pyro_field_rt_staged<Real, SHELLS>(sl, ld, nc, c, Jr, wait, hx, hy, hz);
+                if (!ok) return;
                 bool a = false;
                 double e = 0.0;
                 if (live) {
-                    // the neighbour offsets from the cell's axes afresh in each
-                    // stage: hoisted out of the loop (16 stencils of 30) they
-                    // would take hundreds of registers; the empty asm hides
-                    // the axes' values from the compiler so it cannot hoist
-                    CellAxes c = axes;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(c.a0[k]), "+v"(c.a1[k]), "+v"(c.a2[k]));
-                    Real hx, hy, hz;
-                    LowSums<Real> lo;
-                    if (en_pass) pyro_field_rt_gathered<Real, SHELLS, NCOL>(sl, ld, nc, c, Jr, hx, hy, hz, &lo);
-                    else pyro_field_rt_gathered<Real, SHELLS>(sl, ld, nc, c, Jr, hx, hy, hz);
-                    const Vec3<Real> v = ld(unsigned(site));
                     Real sx = v.x, sy = v.y, sz = v.z;
                     if (first)
                         a = drawn_update<Real, FIRST, FUSED>(rp, st, sx, sy, sz, site, r, n_or, u0, u1, hx, hy, hz, Sr,
