@@ -456,35 +456,38 @@ def main():
     ctx.pt_rounds(pyro.METROPOLIS, N_OR, N_SWEEP, True, prof_steps)
     prof = ctx.profile_read()
     ctx.profile(False)
-    # Sweep-kernel launch duration: HIP events on the context stream around
-    # back-to-back launches of whole sweeps (no per-launch events, no
-    # accept counting), as the tempering graph issues them.
+    # Sweep-kernel launch duration as the timed rounds run it: the profiling
+    # mode's HIP events around each round's sweep launches (the energy
+    # variant, after the previous round's SSF, with the XCD-resident sweep's
+    # counter clear included), on the context stream.
+    sweep_prof_ms, sweep_prof_launches = prof["sweep"]
+    launches_per_round = sweep_prof_launches / prof_steps
+    avg_launch_s = sweep_prof_ms / 1e3 / sweep_prof_launches
+    # colour stages per sweep: one per colour and pass, or in the fused order
+    # one per colour for the Metropolis + first OR update and one per colour
+    # for each further OR pass
+    groups = max(N_OR, 1) if C3_FUSED else 1 + N_OR
+    upd_per_launch = N * R_REP * (1 + N_OR) * N_SWEEP / launches_per_round
+    sites_per_launch = N * R_REP * N_SWEEP * (groups if strategy in (3, 6) else 1 + N_OR) / launches_per_round
+    # Beside it: back-to-back launches of a round's sweeps with no per-launch
+    # events (the plain variant, warm caches), as a second sample.
     n_sw = max(16, min(args.steps, 256))
     ctx.synchronize()
     ctx.mark(0)
     if strategy == 6:
-        # the XCD-resident kernel runs a round's sweeps in one launch: time
-        # back-to-back launches of N_SWEEP sweeps, as the rounds issue them
         for _ in range(n_sw // N_SWEEP):
             ctx.sweep(pyro.METROPOLIS, N_OR, N_SWEEP, count=False)
     else:
         ctx.sweep(pyro.METROPOLIS, N_OR, n_sw, count=False)
     ctx.mark(1)
     sweep_ms = ctx.elapsed_ms()
-    # colour launches per sweep: one per colour and pass, or in the fused
-    # order one per colour for the Metropolis + first OR update and one per
-    # colour for each further OR pass
-    groups = max(N_OR, 1) if C3_FUSED else 1 + N_OR
-    sweep_launches = {1: n_sw * (1 + N_OR) * info["n_colours"], 3: n_sw * groups * info["n_colours"],
-                      2: 1, 4: 1, 5: 8 * n_sw, 6: n_sw // N_SWEEP}[strategy]
-    upd_per_launch = N * R_REP * (1 + N_OR) * n_sw / sweep_launches
-    avg_launch_s = sweep_ms / 1e3 / sweep_launches
+    b2b_launches = {1: n_sw * (1 + N_OR) * info["n_colours"], 3: n_sw * groups * info["n_colours"],
+                    2: 1, 4: 1, 5: 8 * n_sw, 6: n_sw // N_SWEEP}[strategy]
     # compulsory bytes per site-update (its spin read and written, its share
     # of the neighbour row) x the launch's site updates; beside it the
     # stricter count per site visited: a fused stage reads and writes a spin
     # once for its two updates
     b_alg = bytes_per_update(4, info["n_slot"], R_REP)
-    sites_per_launch = N * R_REP * n_sw * (groups if strategy in (3, 6) else 1 + N_OR) / sweep_launches
     achieved = b_alg * upd_per_launch / avg_launch_s
     achieved_visit = b_alg * sites_per_launch / avg_launch_s
     traffic_c3 = profiled_traffic("sweep_c3_L12x64" + ("_xcd" if strategy == 6 else ""))
@@ -546,8 +549,11 @@ def main():
             "sites_visited_per_launch": sites_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,
             # the HIP-event sample avg_launch_us comes from
-            "launch_timing": {"sweeps": n_sw, "launches": sweep_launches, "elapsed_ms": sweep_ms,
-                              "launches_per_round": sweep_launches * N_SWEEP // n_sw},
+            "launch_timing": {"rounds": prof_steps, "launches": sweep_prof_launches, "elapsed_ms": sweep_prof_ms,
+                              "launches_per_round": launches_per_round},
+            # back-to-back launches of the plain variant, no per-launch events
+            "back_to_back": {"sweeps": n_sw, "launches": b2b_launches, "elapsed_ms": sweep_ms,
+                             "avg_launch_us": 1e3 * sweep_ms / b2b_launches},
         },
         "kernel_ms": {k: v[0] for k, v in prof.items()},
         "kernel_profiled_rounds": prof_steps,

@@ -316,6 +316,32 @@ __device__ __forceinline__ void pyro_field_rt_gathered(int sl, const Load& load,
 // equals pyro_field_rt_gathered's bit for bit.  Stages run the colours in
 // order, so the previous stage's colour is colour(sl) - 1 (mod NCOL), with
 // colour(s) = s for 16 colours and s % 4 for 4.
+// The SPLIT sums of pyro_field_with for sublattice SL (its neighbours of
+// earlier colours, per shell and for shell 0 per target pyro_sl) from the
+// neighbours' values v[k] in stencil order: the same additions in the same
+// order, from zero.
+template <typename Real, int SHELLS, int SL, int SPLIT>
+__device__ __forceinline__ void low_sums(const Vec3<Real>* v, LowSums<Real>& low) {
+    constexpr int NS = pyrostencil::slots_for(SHELLS);
+    low = LowSums<Real>{};
+    static_for<NS>([&](auto kk) __attribute__((always_inline)) {
+        constexpr int k = decltype(kk)::value;
+        constexpr pyrostencil::Nbr n = kStencil.n[SL][k];
+        constexpr int shell = k < 6 ? 0 : (k < 18 ? 1 : 2);
+        if constexpr (n.sl % SPLIT < SL % SPLIT) {
+            low.s[shell][0] += v[k].x;
+            low.s[shell][1] += v[k].y;
+            low.s[shell][2] += v[k].z;
+            if constexpr (shell == 0) {
+                constexpr int tp = n.sl % 4;
+                low.v[tp][0] += v[k].x;
+                low.v[tp][1] += v[k].y;
+                low.v[tp][2] += v[k].z;
+            }
+        }
+    });
+}
+
 template <typename Real, int SHELLS, int SPLIT = 0, typename Load, typename Wait>
 __device__ __forceinline__ bool pyro_field_rt_staged(int sl, const Load& load, int nc, const CellAxes& c,
                                                      const PyroCouplings<Real>& J, Wait&& wait, Real& hx, Real& hy,
@@ -355,18 +381,23 @@ __device__ __forceinline__ bool pyro_field_rt_staged(int sl, const Load& load, i
     static_for<NS>([&](auto kk) __attribute__((always_inline)) {
         if ((late >> decltype(kk)::value) & 1u) v[decltype(kk)::value] = load(idx[decltype(kk)::value]);
     });
+    // the field, then (SPLIT) the sums over the earlier colours' neighbours
+    // in a pass of their own over the loaded values: the same additions in
+    // the same order as pyro_field_with's SPLIT sums, kept apart from the
+    // field's so the field and the update do not wait for them
     switch (sl) {
 #define PYRO_SL_CASE(X)                                                                                          \
     case X: {                                                                                                    \
         int k = 0;                                                                                               \
-        pyro_field_with<Real, SHELLS, X, false, SPLIT>(                                                         \
+        pyro_field_with<Real, SHELLS, X, false, 0>(                                                             \
             [&](const pyrostencil::Nbr&, Real& x, Real& y, Real& z) __attribute__((always_inline)) {            \
                 x = v[k].x;                                                                                      \
                 y = v[k].y;                                                                                      \
                 z = v[k].z;                                                                                      \
                 ++k;                                                                                             \
             },                                                                                                   \
-            J, hx, hy, hz, low);                                                                                 \
+            J, hx, hy, hz);                                                                                      \
+        if constexpr (SPLIT > 0) low_sums<Real, SHELLS, X, SPLIT>(v, *low);                                      \
         break;                                                                                                   \
     }
         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
@@ -872,6 +903,7 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
     unsigned int n_acc = 0;
     unsigned int stage = 0;
     unsigned long long arrived = 0;
+    double part = 0.0;  // EN: lane col, the energy partial of colour col's last stage
     for (int sw = 0; sw < n_sweeps; ++sw) {
         const unsigned long long p0 = pass_start + (unsigned long long)sw * unsigned(1 + n_or);
         for (int grp = 0; grp < n_groups; ++grp) {
@@ -914,11 +946,15 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
                 auto wait = [&]() __attribute__((always_inline)) {
                     return stage == 0 || xcd_wait(ctr, stage * unsigned(bpr), err, arrived, timeout_ticks, zero);
                 };
+                // EN: every stage sums the earlier colours' neighbours and
+                // the site's energy, used by the last group's stages only.
+                // One code path for all stages: a path of their own would
+                // run once per launch from a cold instruction cache, which
+                // cost more (about 1.6 us per stage) than the sums
                 Real hx, hy, hz;
                 LowSums<Real> lo;
-                const bool ok = en_pass
-                                    ? pyro_field_rt_staged<Real, SHELLS, NCOL>(sl, ld, nc, c, Jr, wait, hx, hy, hz, &lo)
-                                    : pyro_field_rt_staged<Real, SHELLS>(sl, ld, nc, c, Jr, wait, hx, hy, hz);
+                const bool ok = pyro_field_rt_staged<Real, SHELLS, EN ? NCOL : 0>(sl, ld, nc, c, Jr, wait, hx, hy, hz,
+                                                                                 EN ? &lo : nullptr);
                 if (!ok) return;
                 bool a = false;
                 double e = 0.0;
@@ -929,12 +965,12 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
                                                              N);
                     else
                         drawn_update<Real, 3, false>(rp, st, sx, sy, sz, site, r, n_or, u0, u1, hx, hy, hz, Sr, N);
-                    if (en_pass) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, sx, sy, sz);
+                    if constexpr (EN) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, sx, sy, sz);
                 }
                 if (grp == 0 && FIRST != 3 && st.acc_wave) n_acc += unsigned(__popcll(__ballot(a)));
                 if (en_pass) {
                     const double tot = wave_sum_f64(e);
-                    if (threadIdx.x == 0) epart[size_t(col) * unsigned(per_launch) + size_t(r) * es + unsigned(b)] = tot;
+                    if (int(threadIdx.x) == col) part = tot;
                 }
                 // release: this wave's stores reach the L2 before its count
                 if (threadIdx.x == 0)
@@ -943,6 +979,11 @@ __global__ __launch_bounds__(64) void pyro_xcd_kernel(DeviceState<Real> st, Pyro
             }
         }
     }
+    // the energy partials of the last group's stages, lane col holding
+    // colour col's: stored here rather than in their stages, whose counts
+    // would otherwise wait for these partial-line stores to complete
+    if (EN && n_sweeps > 0 && int(threadIdx.x) < NCOL)
+        epart[size_t(threadIdx.x) * unsigned(per_launch) + size_t(r) * es + unsigned(b)] = part;
     if (st.acc_wave && threadIdx.x == 0 && n_acc) st.acc_wave[size_t(r) * unsigned(st.acc_waves) + unsigned(b)] += n_acc;
 }
 
@@ -994,6 +1035,7 @@ __global__ __launch_bounds__(64) void pyro_batch_kernel(const BatchMember<Real>*
     unsigned int n_acc = 0;
     unsigned int stage = 0;
     unsigned long long arrived = 0;
+    double part = 0.0;  // EN: lane jb NCOL + col, block jb's energy partial of colour col
     for (int sw = 0; sw < n_sweeps; ++sw) {
         const unsigned long long p0 = pass_start + (unsigned long long)sw * unsigned(1 + n_or);
         for (int grp = 0; grp < n_groups; ++grp) {
@@ -1008,7 +1050,7 @@ __global__ __launch_bounds__(64) void pyro_batch_kernel(const BatchMember<Real>*
             const bool first = grp == 0 && FIRST != 3;
             for (int col = 0; col < NCOL; ++col, ++stage) {
                 if (stage > 0 && !xcd_wait(ctr, stage * unsigned(nw), err, arrived, timeout_ticks, zero)) return;
-                for (int b = w; b < bpr; b += nw) {
+                for (int b = w, jb = 0; b < bpr; b += nw, ++jb) {
                     const int q = b / bps;
                     const int cell = (b - q * bps) * 64 + int(threadIdx.x);
                     const bool live = cell < nc;
@@ -1027,8 +1069,10 @@ __global__ __launch_bounds__(64) void pyro_batch_kernel(const BatchMember<Real>*
                         const CellAxes c = cell_axes_of(cell, st.L0, st.L1, st.L2);
                         Real hx, hy, hz;
                         LowSums<Real> lo;
-                        if (en_pass) pyro_field_rt_gathered<Real, SHELLS, NCOL>(sl, ld, nc, c, Jr, hx, hy, hz, &lo);
-                        else pyro_field_rt_gathered<Real, SHELLS>(sl, ld, nc, c, Jr, hx, hy, hz);
+                        // EN: the earlier colours' sums and the site energy in
+                        // every stage, one code path (as in pyro_xcd_kernel)
+                        pyro_field_rt_gathered<Real, SHELLS, EN ? NCOL : 0>(sl, ld, nc, c, Jr, hx, hy, hz,
+                                                                            EN ? &lo : nullptr);
                         const Vec3<Real> v = ld(unsigned(site));
                         Real sx = v.x, sy = v.y, sz = v.z;
                         if (first)
@@ -1037,12 +1081,17 @@ __global__ __launch_bounds__(64) void pyro_batch_kernel(const BatchMember<Real>*
                         else
                             drawn_update<Real, 3, false>(rp, st, sx, sy, sz, site, r, n_or, u0, u1, hx, hy, hz, Sr,
                                                          N);
-                        if (en_pass) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, sx, sy, sz);
+                        if constexpr (EN) e = low_site_energy<Real, SHELLS>(lo, Jr, sl % 4, rp, sx, sy, sz);
                     }
                     if (first && st.acc_wave) n_acc += unsigned(__popcll(__ballot(a)));
                     if (en_pass) {
                         const double tot = wave_sum_f64(e);
-                        if (threadIdx.x == 0) M.epart[size_t(col) * unsigned(bpr) + size_t(r) * es + unsigned(b)] = tot;
+                        const int slot = jb * NCOL + col;  // this block's partial of colour col
+                        if (slot < 64) {
+                            if (int(threadIdx.x) == slot) part = tot;
+                        } else if (threadIdx.x == 0) {
+                            M.epart[size_t(col) * unsigned(bpr) + size_t(r) * es + unsigned(b)] = tot;
+                        }
                     }
                 }
                 // release: this wave's stores reach the L2 before its count
@@ -1050,6 +1099,13 @@ __global__ __launch_bounds__(64) void pyro_batch_kernel(const BatchMember<Real>*
                 arrived = wall_clock64();
             }
         }
+    }
+    // the energy partials held in lanes (block jb's colour col in lane
+    // jb NCOL + col), stored after the last stage as in pyro_xcd_kernel
+    if (EN && n_sweeps > 0) {
+        const int jb = int(threadIdx.x) / NCOL, col = int(threadIdx.x) % NCOL;
+        const int b = w + jb * nw;
+        if (b < bpr) M.epart[size_t(col) * unsigned(bpr) + size_t(r) * es + unsigned(b)] = part;
     }
     if (st.acc_wave && threadIdx.x == 0 && n_acc) st.acc_wave[size_t(r) * unsigned(st.acc_waves) + unsigned(w)] += n_acc;
 }

@@ -95,14 +95,21 @@ def events(path: str) -> str:
     gbs = b_unit * n_upd / (avg_us * 1e-6) / 1e9
     gbs_v = b_unit * n_vis / (avg_us * 1e-6) / 1e9
     out = [
-        f"sweep kernel {rf['kernel']}: {lt['sweeps']} sweeps back to back = {lt['launches']} launches in "
-        f"{lt['elapsed_ms']:.3f} ms (HIP events on the library stream) -> {avg_us:.3f} us per launch",
-        f"round: {per_round} sweep launches x {avg_us:.3f} us = {per_round * avg_us / 1e3:.4f} ms of the measured "
+        f"sweep kernel {rf['kernel']} in the timed rounds: {lt['launches']} launches over {lt['rounds']} rounds "
+        f"in {lt['elapsed_ms']:.3f} ms (HIP events around each round's sweep launches, profiling mode) -> "
+        f"{avg_us:.3f} us per launch",
+        f"round: {per_round:g} sweep launches x {avg_us:.3f} us = {per_round * avg_us / 1e3:.4f} ms of the measured "
         f"{line['ms_per_step']:.4f} ms per round (wall clock over {line['steps']} rounds)",
         f"roofline: {b_unit} B x {n_upd:.0f} site updates / {avg_us:.3f} us = "
         f"{gbs:.1f} GB/s = {gbs / rf['peak']:.4f} of {rf['peak']:.0f} GB/s",
         f"  per site visited: {b_unit} B x {n_vis:.0f} sites / {avg_us:.3f} us = "
         f"{gbs_v:.1f} GB/s = {gbs_v / rf['peak']:.4f} of {rf['peak']:.0f} GB/s",
+    ]
+    bb = rf.get("back_to_back")
+    if bb:
+        out.append(f"back to back (plain variant, {bb['sweeps']} sweeps, no per-launch events): "
+                   f"{bb['launches']} launches in {bb['elapsed_ms']:.3f} ms -> {bb['avg_launch_us']:.3f} us per launch")
+    out += [
         "",
         f"per-family HIP-event times over {line.get('kernel_profiled_rounds', '?')} profiled rounds "
         "(events around each family's launches):",
