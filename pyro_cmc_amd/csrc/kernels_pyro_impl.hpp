@@ -1152,13 +1152,13 @@ inline constexpr TileLanes<T0, T1, T2, BS> kTileLanes{};
 // shifted down one cell on the axes where its halo is on the lower side
 // (halo_lo(sl, a) = 1 iff R_fcc[sl / 4] has 4 on axis a), coordinates
 // wrapped periodically.
-template <typename Real, int T0, int T1, int T2, int BS, bool UNROLL = false>
+template <typename Real, int T0, int T1, int T2, int BS, int UNROLL = 1>
 __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, const Real* __restrict__ Sr, int o0,
                                               int o1, int o2, int L0, int L1, int L2, int nc) {
     constexpr int B1 = T1 + 1, B2 = T2 + 1, BC = (T0 + 1) * B1 * B2;
-    // UNROLL: every load of the thread issued before the first LDS store
-    // (the few loads per thread of a wide workgroup); else a loop
-    constexpr int kUnroll = UNROLL ? (16 * BC + BS - 1) / BS : 1;
+    // UNROLL iterations at a time, so their loads are in flight together
+    // (0: all of the thread's loads, for a wide workgroup's few)
+    constexpr int kUnroll = UNROLL > 0 ? UNROLL : (16 * BC + BS - 1) / BS;
 #pragma unroll kUnroll
     for (int e = threadIdx.x; e < 16 * BC; e += BS) {
         const int sl = e / BC, bc = e % BC;
@@ -1242,7 +1242,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const size_t N = size_t(st.N);
     Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
 
-    load_tile_box<Real, T0, T1, T2, BS>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
+    load_tile_box<Real, T0, T1, T2, BS, 4>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
     int8_t* const dir_r = st.dir + size_t(r) * N;
     // the interior cell mi = threadIdx.x (< NI) whose 16 sites this thread
     // stages directions for and writes back, and its lattice cell
@@ -1448,8 +1448,8 @@ __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real>
     const int m1 = t12 % q1n, m0 = t12 / q1n;
     const size_t N = size_t(st.N);
     const Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
-    load_tile_box<Real, T0, T1, T2, BS, true>(lx, ly, lz, Sr, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1, st.L2,
-                                              st.n_cells);
+    load_tile_box<Real, T0, T1, T2, BS, 0>(lx, ly, lz, Sr, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1, st.L2,
+                                           st.n_cells);
     __syncthreads();
     const PyroCouplings<Real> Jr = J.at(r);
     const Real bx = st.B[3 * r], by = st.B[3 * r + 1], bz = st.B[3 * r + 2];
