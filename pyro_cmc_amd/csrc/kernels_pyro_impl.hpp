@@ -1242,6 +1242,11 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const size_t N = size_t(st.N);
     Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
 
+    // the pass counter and the replica's parameters read before the box, so
+    // their loads (two of them dependent) complete under its loads instead
+    // of ahead of the first stage
+    const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
+    const ReplicaParams<Real> rp = replica_params(st, r);
     load_tile_box<Real, T0, T1, T2, BS, 4>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
     int8_t* const dir_r = st.dir + size_t(r) * N;
     // the interior cell mi = threadIdx.x (< NI) whose 16 sites this thread
@@ -1261,8 +1266,6 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     }
     __syncthreads();
 
-    const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
-    const ReplicaParams<Real> rp = replica_params(st, r);
     static_assert(NI <= BS, "one tile cell per thread");
     const int my_cell = kTileLanes<T0, T1, T2, BS>.cell[threadIdx.x];
     // the cell this lane updates: its box index (before the sublattice's
