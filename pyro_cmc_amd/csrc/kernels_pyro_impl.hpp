@@ -1247,7 +1247,6 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     // of ahead of the first stage
     const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
     const ReplicaParams<Real> rp = replica_params(st, r);
-    load_tile_box<Real, T0, T1, T2, BS, 4>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
     int8_t* const dir_r = st.dir + size_t(r) * N;
     // the interior cell mi = threadIdx.x (< NI) whose 16 sites this thread
     // stages directions for and writes back, and its lattice cell
@@ -1255,13 +1254,27 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const bool has_mi = mi < NI;
     const int mi2 = mi % T2, mi1 = (mi / T2) % T1, mi0 = mi / (T1 * T2);
     const int cell_mi = ((o0 + mi0) * L1 + (o1 + mi1)) * L2 + (o2 + mi2);
+    // the interior's directions are read together with the box (set: -1)
+    bool neg[LIFTED ? 16 : 1] = {};
     if constexpr (LIFTED) {
-        for (int w = threadIdx.x; w < 16 * W; w += BS) ldw[w] = 0u;
-        __syncthreads();
         if (has_mi) {
 #pragma unroll
-            for (int sl = 0; sl < 16; ++sl)
-                if (dir_r[sl * nc + cell_mi] < 0) atomicOr(ldw + sl * W + (mi >> 5), 1u << (mi & 31));
+            for (int sl = 0; sl < 16; ++sl) neg[sl] = dir_r[sl * nc + cell_mi] < 0;
+        }
+    }
+    load_tile_box<Real, T0, T1, T2, BS, 4>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
+    if constexpr (LIFTED) {
+        // wave v holds cells 64 v .. 64 v + 63, the bits of words 2 v and
+        // 2 v + 1 of each sublattice: a ballot gives both words whole (cells
+        // past the tile read 0), and BS >= NI means every word is written
+        const int v = int(threadIdx.x) >> 6;
+#pragma unroll
+        for (int sl = 0; sl < 16; ++sl) {
+            const unsigned long long b = __ballot(neg[sl]);
+            if ((threadIdx.x & 63) == 0) {
+                if (2 * v < W) ldw[sl * W + 2 * v] = unsigned(b);
+                if (2 * v + 1 < W) ldw[sl * W + 2 * v + 1] = unsigned(b >> 32);
+            }
         }
     }
     __syncthreads();
