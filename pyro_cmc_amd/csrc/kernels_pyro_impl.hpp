@@ -1228,7 +1228,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica and phase
     unsigned g = blockIdx.x;
     if ((gridDim.x & 7u) == 0u) g = (g & 7u) * (gridDim.x >> 3) + (g >> 3);
-    const int r = int(g / tiles);
+    const int r = __builtin_amdgcn_readfirstlane(int(g / tiles));  // uniform: the replica's loads are scalar
     const int tile = int(g - unsigned(r) * tiles);
     const PyroCouplings<Real> Jr = J.at(r);
     const int L0 = st.L0, L1 = st.L1, L2 = st.L2, nc = st.n_cells;
@@ -1242,9 +1242,11 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     const size_t N = size_t(st.N);
     Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
 
-    // the pass counter and the replica's parameters read before the box, so
-    // their loads (two of them dependent) complete under its loads instead
-    // of ahead of the first stage
+    // the lane's cell (a constant table), the pass counter and the
+    // replica's parameters read before the box, so their loads (two of them
+    // dependent) complete under its loads instead of ahead of the first stage
+    static_assert(NI <= BS, "one tile cell per thread");
+    const int my_cell = kTileLanes<T0, T1, T2, BS>.cell[threadIdx.x];
     const unsigned long long pass0 = pass_index + (st.pass_base ? *st.pass_base : 0ull);
     const ReplicaParams<Real> rp = replica_params(st, r);
     int8_t* const dir_r = st.dir + size_t(r) * N;
@@ -1279,8 +1281,6 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
     }
     __syncthreads();
 
-    static_assert(NI <= BS, "one tile cell per thread");
-    const int my_cell = kTileLanes<T0, T1, T2, BS>.cell[threadIdx.x];
     // the cell this lane updates: its box index (before the sublattice's
     // shift) and lattice cell
     const int c2 = my_cell % T2, c1 = (my_cell / T2) % T1, c0 = my_cell / (T1 * T2);
