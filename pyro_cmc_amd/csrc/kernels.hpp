@@ -176,12 +176,28 @@ inline void pyro_tile_dims(int tile_kind, int dims[3]) {
 }
 // Threads per tiled workgroup: the tile's cells rounded up to whole waves.
 constexpr int pyro_tile_threads(int cells) { return cells >= 256 ? 256 : (cells + 63) / 64 * 64; }
-// LDS bytes of a tiled workgroup: spins of the tile + one-sided halo
-// (3 x 16 x (T0+1)(T1+1)(T2+1) reals) and, for the lifted kinds, the
+// LDS bytes of a tiled workgroup with sublattice stride ls: spins of the
+// tile + one-sided halo (3 x 16 x ls reals) and, for the lifted kinds, the
 // interior's directions as bits (16 x ceil(T0 T1 T2 / 32) words).
-constexpr size_t pyro_tile_lds(int T0, int T1, int T2, int real_size, bool lifted) {
-    return size_t(3) * 16 * (T0 + 1) * (T1 + 1) * (T2 + 1) * size_t(real_size) +
+constexpr size_t pyro_tile_lds_at(int ls, int T0, int T1, int T2, int real_size, bool lifted) {
+    return size_t(3) * 16 * size_t(ls) * size_t(real_size) +
            (lifted ? size_t(16) * 4 * ((T0 * T1 * T2 + 31) / 32) : 0);
+}
+// Sublattice stride of a tiled workgroup's LDS arrays, in entries: the box
+// of (T0+1)(T1+1)(T2+1) cells rounded up to a multiple of 64, so that one
+// cell of every sublattice and component lies a multiple of 256 B from the
+// others and the gathers pair up as ds_read2st64 from a few base addresses,
+// unless the padding would cost a workgroup per CU (160 KB of LDS).
+constexpr int pyro_tile_stride(int T0, int T1, int T2, int real_size, bool lifted) {
+    const int bc = (T0 + 1) * (T1 + 1) * (T2 + 1), padded = (bc + 63) / 64 * 64;
+    const size_t cu = 163840;
+    return cu / pyro_tile_lds_at(padded, T0, T1, T2, real_size, lifted) ==
+                   cu / pyro_tile_lds_at(bc, T0, T1, T2, real_size, lifted)
+               ? padded
+               : bc;
+}
+constexpr size_t pyro_tile_lds(int T0, int T1, int T2, int real_size, bool lifted) {
+    return pyro_tile_lds_at(pyro_tile_stride(T0, T1, T2, real_size, lifted), T0, T1, T2, real_size, lifted);
 }
 // fused (pyro_set_fused_passes): within each colour stage a site takes its
 // `kind` update and its first OR update back to back, from one field

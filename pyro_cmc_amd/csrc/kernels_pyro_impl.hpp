@@ -237,17 +237,18 @@ __device__ __forceinline__ void pyro_field_l(const Load& load, int nc, const Cel
         J, hx, hy, hz, low);
 }
 
-// Field from a tile in LDS laid out [sl][box cell], every sublattice's box
-// B0 x B1 x B2 = (T0 + 1) x (T1 + 1) x (T2 + 1) cells shifted down one cell
-// on the axes where its halo is on the lower side (pyrostencil::halo_lo);
-// lc = (i0 * B1 + i1) * B2 + i2 for the site's tile cell (i0, i1, i2).
-template <typename Real, int SHELLS, int SL, int B1, int B2, int BC, bool FWD = false>
+// Field from a tile in LDS laid out [sl][box cell] with sublattice stride
+// LS, every sublattice's box B0 x B1 x B2 = (T0 + 1) x (T1 + 1) x (T2 + 1)
+// cells shifted down one cell on the axes where its halo is on the lower
+// side (pyrostencil::halo_lo); lc = (i0 * B1 + i1) * B2 + i2 for the site's
+// tile cell (i0, i1, i2).
+template <typename Real, int SHELLS, int SL, int B1, int B2, int LS, bool FWD = false>
 __device__ __forceinline__ void pyro_field_lds(const Real* lx, const Real* ly, const Real* lz, int lc,
                                                const PyroCouplings<Real>& J, Real& hx, Real& hy, Real& hz) {
     pyro_field_with<Real, SHELLS, SL, FWD>(
         [&](const pyrostencil::Nbr& n, Real& x, Real& y, Real& z) __attribute__((always_inline)) {
             using pyrostencil::halo_lo;
-            const int j = lc + (n.sl * BC + (n.d[0] + halo_lo(n.sl, 0)) * B1 * B2 +
+            const int j = lc + (n.sl * LS + (n.d[0] + halo_lo(n.sl, 0)) * B1 * B2 +
                                 (n.d[1] + halo_lo(n.sl, 1)) * B2 + (n.d[2] + halo_lo(n.sl, 2)));
             x = lx[j];
             y = ly[j];
@@ -1148,14 +1149,15 @@ template <int T0, int T1, int T2, int BS>
 inline constexpr TileLanes<T0, T1, T2, BS> kTileLanes{};
 
 // Copies the tile at cell (o0, o1, o2) with its one-sided halo into LDS,
-// [sl][box cell]: sublattice sl's box of (T0 + 1)(T1 + 1)(T2 + 1) cells is
-// shifted down one cell on the axes where its halo is on the lower side
-// (halo_lo(sl, a) = 1 iff R_fcc[sl / 4] has 4 on axis a), coordinates
-// wrapped periodically.
-template <typename Real, int T0, int T1, int T2, int BS, int UNROLL = 1>
+// [sl][box cell] with sublattice stride LS: sublattice sl's box of
+// (T0 + 1)(T1 + 1)(T2 + 1) cells is shifted down one cell on the axes where
+// its halo is on the lower side (halo_lo(sl, a) = 1 iff R_fcc[sl / 4] has 4
+// on axis a), coordinates wrapped periodically.
+template <typename Real, int T0, int T1, int T2, int BS, int LS, int UNROLL = 1>
 __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, const Real* __restrict__ Sr, int o0,
                                               int o1, int o2, int L0, int L1, int L2, int nc) {
     constexpr int B1 = T1 + 1, B2 = T2 + 1, BC = (T0 + 1) * B1 * B2;
+    static_assert(LS >= BC, "the stride holds a sublattice's box");
     // UNROLL iterations at a time, so their loads are in flight together
     // (0: all of the thread's loads, for a wide workgroup's few)
     constexpr int kUnroll = UNROLL > 0 ? UNROLL : (16 * BC + BS - 1) / BS;
@@ -1172,9 +1174,10 @@ __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, cons
         g2 += g2 < 0 ? L2 : 0;
         g2 -= g2 >= L2 ? L2 : 0;
         const Vec3<Real> v = ld3(Sr, unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2));
-        lx[e] = v.x;
-        ly[e] = v.y;
-        lz[e] = v.z;
+        const int le = sl * LS + bc;
+        lx[le] = v.x;
+        ly[le] = v.y;
+        lz[le] = v.z;
     }
 }
 
@@ -1210,20 +1213,21 @@ template <typename Real, int SHELLS, int FIRST, int T0, int T1, int T2, int BS, 
 __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                          int phase, int n_or,
                                                          unsigned long long pass_index) {
-    constexpr int B0 = T0 + 1, B1 = T1 + 1, B2 = T2 + 1, BC = B0 * B1 * B2;
+    constexpr int B1 = T1 + 1, B2 = T2 + 1;
     constexpr int NI = T0 * T1 * T2;
     constexpr int NCOL = pyrostencil::colours_for(SHELLS);
     constexpr int SPC = 16 / NCOL;
     constexpr bool LIFTED = FIRST == 1 || FIRST == 2;
+    constexpr int LS = pyro_tile_stride(T0, T1, T2, int(sizeof(Real)), LIFTED);  // LDS sublattice stride
     extern __shared__ __align__(16) unsigned char pyro_smem[];
     Real* lx = reinterpret_cast<Real*>(pyro_smem);
-    Real* ly = lx + 16 * BC;
-    Real* lz = ly + 16 * BC;
+    Real* ly = lx + 16 * LS;
+    Real* lz = ly + 16 * LS;
     // lifted directions of the interior, one bit per site (set: -1), W words
     // per sublattice; sites of one sublattice flip their own bits with LDS
     // atomics, so lanes sharing a word never overwrite each other
     constexpr int W = (NI + 31) / 32;
-    unsigned int* ldw = reinterpret_cast<unsigned int*>(lz + 16 * BC);  // [16][W]
+    unsigned int* ldw = reinterpret_cast<unsigned int*>(lz + 16 * LS);  // [16][W]
 
     const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica and phase
     unsigned g = blockIdx.x;
@@ -1264,7 +1268,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
             for (int sl = 0; sl < 16; ++sl) neg[sl] = dir_r[sl * nc + cell_mi] < 0;
         }
     }
-    load_tile_box<Real, T0, T1, T2, BS, 4>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
+    load_tile_box<Real, T0, T1, T2, BS, LS, 4>(lx, ly, lz, Sr, o0, o1, o2, L0, L1, L2, nc);
     if constexpr (LIFTED) {
         // wave v holds cells 64 v .. 64 v + 63, the bits of words 2 v and
         // 2 v + 1 of each sublattice: a ballot gives both words whole (cells
@@ -1303,7 +1307,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                     Real hx, hy, hz;
                     switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, BC>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
+    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, LS>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
                         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
                         PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
                         PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -1312,7 +1316,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                         default: hx = hy = hz = Real(0);
                     }
                     const int site = sl * nc + my_gc;
-                    const int li = sl * BC + lc + own;
+                    const int li = sl * LS + lc + own;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool any = false, ch;
                     if (grp == 0) {
@@ -1372,7 +1376,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                     Real hx, hy, hz;
                     switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, BC>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
+    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, LS>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
                         PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
                         PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
                         PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -1381,7 +1385,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
                         default: hx = hy = hz = Real(0);
                     }
                     const int site = sl * nc + my_gc;
-                    const int li = sl * BC + lc + own;
+                    const int li = sl * LS + lc + own;
                     Real x = lx[li], y = ly[li], z = lz[li];
                     bool ch;
                     if (k == 0) {
@@ -1415,7 +1419,7 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
 #pragma unroll
         for (int sl = 0; sl < 16; ++sl) {
             const int f = sl >> 2;
-            const int li = sl * BC + ((mi0 + (f == 2 || f == 3)) * B1 + (mi1 + (f == 1 || f == 3))) * B2 +
+            const int li = sl * LS + ((mi0 + (f == 2 || f == 3)) * B1 + (mi1 + (f == 1 || f == 3))) * B2 +
                            (mi2 + (f == 1 || f == 2));
             const int site = sl * nc + cell_mi;
             st3(Sr, unsigned(site), lx[li], ly[li], lz[li]);
@@ -1448,11 +1452,12 @@ __global__ __launch_bounds__(BS) void pyro_tiled_kernel(DeviceState<Real> st, Py
 template <typename Real, int SHELLS, int T0, int T1, int T2, int BS>
 __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real> st, PyroCouplings<Real> J,
                                                                double* __restrict__ partial) {
-    constexpr int B1 = T1 + 1, B2 = T2 + 1, BC = (T0 + 1) * B1 * B2;
+    constexpr int B1 = T1 + 1, B2 = T2 + 1;
+    constexpr int LS = pyro_tile_stride(T0, T1, T2, int(sizeof(Real)), false);  // LDS sublattice stride
     extern __shared__ __align__(16) unsigned char pyro_smem[];
     Real* lx = reinterpret_cast<Real*>(pyro_smem);
-    Real* ly = lx + 16 * BC;
-    Real* lz = ly + 16 * BC;
+    Real* ly = lx + 16 * LS;
+    Real* lz = ly + 16 * LS;
     const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica
     unsigned g = blockIdx.x;
     if ((gridDim.x & 7u) == 0u) g = (g & 7u) * (gridDim.x >> 3) + (g >> 3);
@@ -1464,7 +1469,7 @@ __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real>
     const int m1 = t12 % q1n, m0 = t12 / q1n;
     const size_t N = size_t(st.N);
     const Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
-    load_tile_box<Real, T0, T1, T2, BS, 0>(lx, ly, lz, Sr, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1, st.L2,
+    load_tile_box<Real, T0, T1, T2, BS, LS, 0>(lx, ly, lz, Sr, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1, st.L2,
                                            st.n_cells);
     __syncthreads();
     const PyroCouplings<Real> Jr = J.at(r);
@@ -1482,7 +1487,7 @@ __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real>
             Real hx = Real(0), hy = Real(0), hz = Real(0);
             switch (sl) {
 #define PYRO_SL_CASE(X) \
-    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, BC, true>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
+    case X: pyro_field_lds<Real, SHELLS, X, B1, B2, LS, true>(lx, ly, lz, lc, Jr, hx, hy, hz); break;
                 PYRO_SL_CASE(0) PYRO_SL_CASE(1) PYRO_SL_CASE(2) PYRO_SL_CASE(3)
                 PYRO_SL_CASE(4) PYRO_SL_CASE(5) PYRO_SL_CASE(6) PYRO_SL_CASE(7)
                 PYRO_SL_CASE(8) PYRO_SL_CASE(9) PYRO_SL_CASE(10) PYRO_SL_CASE(11)
@@ -1490,7 +1495,7 @@ __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real>
 #undef PYRO_SL_CASE
                 default: break;
             }
-            const int li = sl * BC + lc + own;
+            const int li = sl * LS + lc + own;
             const Real sx = lx[li], sy = ly[li], sz = lz[li];
             const Real bond = sx * hx + sy * hy + sz * hz;
             const Real bs = sx * bx + sy * by + sz * bz;
