@@ -1059,17 +1059,32 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
     const int K0 = int(blockIdx.x) * 64;
     const int ns = st.n_sl, nf = 3 * ns;
     const int kl = threadIdx.x & 63;
-    for (int f = int(threadIdx.x >> 6); f < 16 * T; f += kBlock / 64) {
-        float re = 0.f, im = 0.f;
+    {
+        // a wave's fields f = w, w + 4, ...: every load issued before any
+        // value is stored, so that they are in flight together
+        constexpr int NFW = 16 * T / (kBlock / 64);
+        static_assert(16 * T % (kBlock / 64) == 0, "whole fields per wave");
+        float re[NFW], im[NFW];
         const int K = K0 + kl;
-        if (f < nf && K < nc) {
-            const int a = f / ns, mu = f % ns;
-            const float* F = st.F + 2 * (((size_t(rl) * 3 + a) * ns + mu) * size_t(nc) + K);
-            re = F[0];
-            im = F[1];
+#pragma unroll
+        for (int i = 0; i < NFW; ++i) {
+            const int f = int(threadIdx.x >> 6) + i * (kBlock / 64);
+            re[i] = 0.f;
+            im[i] = 0.f;
+            if (f < nf && K < nc) {
+                const int a = f / ns, mu = f % ns;
+                const float2 v = *reinterpret_cast<const float2*>(
+                    st.F + 2 * (((size_t(rl) * 3 + a) * ns + mu) * size_t(nc) + K));
+                re[i] = v.x;
+                im[i] = v.y;
+            }
         }
-        xs[f][kl] = re;
-        xs[16 * T + f][kl] = im;
+#pragma unroll
+        for (int i = 0; i < NFW; ++i) {
+            const int f = int(threadIdx.x >> 6) + i * (kBlock / 64);
+            xs[f][kl] = re[i];
+            xs[16 * T + f][kl] = im[i];
+        }
     }
     __syncthreads();
     const unsigned int b = STAGE ? cnt[slot] : 0u;
@@ -1089,23 +1104,39 @@ __global__ __launch_bounds__(kBlock) void contract_stage_kernel(SsfState<float> 
     const int w = int(threadIdx.x >> 6);
     __shared__ float bsh[2][2][3][64];  // [K, -K][re, im][component][k]
     if (w < 3 && kh < nc) {
+        // the phases of every sublattice loaded together (sublattices past
+        // n_sl read sublattice 0's and add nothing), the sums in mu order
+        constexpr int MAXNS = 16 * T / 3;  // n_sl <= 16 T / 3 (T = ceil(3 n_sl / 16))
+        float2 ph[MAXNS];
+#pragma unroll
+        for (int mu = 0; mu < MAXNS; ++mu)
+            ph[mu] = *reinterpret_cast<const float2*>(st.phase + 2 * ((mu < ns ? mu : 0) * nfull + K));
         float sr = 0.f, si = 0.f;
-        for (int mu = 0; mu < ns; ++mu) {
-            const float fr = xs[w * ns + mu][kl], fi = xs[16 * T + w * ns + mu][kl];
-            const float pr = st.phase[2 * (mu * nfull + K)], pi = st.phase[2 * (mu * nfull + K) + 1];
-            sr += fr * pr - fi * pi;
-            si += fr * pi + fi * pr;
+#pragma unroll
+        for (int mu = 0; mu < MAXNS; ++mu) {
+            if (mu < ns) {
+                const float fr = xs[w * ns + mu][kl], fi = xs[16 * T + w * ns + mu][kl];
+                const float pr = ph[mu].x, pi = ph[mu].y;
+                sr += fr * pr - fi * pi;
+                si += fr * pi + fi * pr;
+            }
         }
         bsh[0][0][w][kl] = sr;
         bsh[0][1][w][kl] = si;
         if (mir) {  // A(-K) = conj(A(K)) with the phases of -K
+#pragma unroll
+            for (int mu = 0; mu < MAXNS; ++mu)
+                ph[mu] = *reinterpret_cast<const float2*>(st.phase + 2 * ((mu < ns ? mu : 0) * nfull + Kn));
             sr = 0.f;
             si = 0.f;
-            for (int mu = 0; mu < ns; ++mu) {
-                const float fr = xs[w * ns + mu][kl], fi = -xs[16 * T + w * ns + mu][kl];
-                const float pr = st.phase[2 * (mu * nfull + Kn)], pi = st.phase[2 * (mu * nfull + Kn) + 1];
-                sr += fr * pr - fi * pi;
-                si += fr * pi + fi * pr;
+#pragma unroll
+            for (int mu = 0; mu < MAXNS; ++mu) {
+                if (mu < ns) {
+                    const float fr = xs[w * ns + mu][kl], fi = -xs[16 * T + w * ns + mu][kl];
+                    const float pr = ph[mu].x, pi = ph[mu].y;
+                    sr += fr * pr - fi * pi;
+                    si += fr * pi + fi * pr;
+                }
             }
             bsh[1][0][w][kl] = sr;
             bsh[1][1][w][kl] = si;
