@@ -1657,9 +1657,24 @@ __global__ __launch_bounds__(dft3_threads<L>()) void dft3_kernel(const Real* __r
     // apart in the grid, on one XCD when n_sl is a multiple of 8
     const int a = int(blockIdx.x) / n_sl, mu = int(blockIdx.x) - a * n_sl;
     const Real* src = spins + (size_t(blockIdx.y) * n_sl + mu) * 3 * size_t(NC) + a;
-    for (int i = threadIdx.x; i < NC; i += BS) {
-        x[2 * i] = src[3 * i];
-        x[2 * i + 1] = Real(0);
+    {
+        // every load of the thread issued before any value is stored, so
+        // that they are in flight together
+        constexpr int KN = (NC + BS - 1) / BS;
+        Real v[KN];
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+            const int i = int(threadIdx.x) + k * BS;
+            v[k] = i < NC ? src[3 * i] : Real(0);
+        }
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+            const int i = int(threadIdx.x) + k * BS;
+            if (i < NC) {
+                x[2 * i] = v[k];
+                x[2 * i + 1] = Real(0);
+            }
+        }
     }
     __syncthreads();
 #pragma unroll
