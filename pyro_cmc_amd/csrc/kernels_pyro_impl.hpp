@@ -1158,11 +1158,7 @@ __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, cons
                                               int o1, int o2, int L0, int L1, int L2, int nc) {
     constexpr int B1 = T1 + 1, B2 = T2 + 1, BC = (T0 + 1) * B1 * B2;
     static_assert(LS >= BC, "the stride holds a sublattice's box");
-    // UNROLL iterations at a time, so their loads are in flight together
-    // (0: all of the thread's loads, for a wide workgroup's few)
-    constexpr int kUnroll = UNROLL > 0 ? UNROLL : (16 * BC + BS - 1) / BS;
-#pragma unroll kUnroll
-    for (int e = threadIdx.x; e < 16 * BC; e += BS) {
+    auto site = [&](int e) -> unsigned {
         const int sl = e / BC, bc = e % BC;
         const int a2 = bc % B2, a1 = (bc / B2) % B1, a0 = bc / (B1 * B2);
         const int f = sl >> 2;
@@ -1173,8 +1169,35 @@ __device__ __forceinline__ void load_tile_box(Real* lx, Real* ly, Real* lz, cons
         g1 -= g1 >= L1 ? L1 : 0;
         g2 += g2 < 0 ? L2 : 0;
         g2 -= g2 >= L2 ? L2 : 0;
-        const Vec3<Real> v = ld3(Sr, unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2));
-        const int le = sl * LS + bc;
+        return unsigned(sl * nc + (g0 * L1 + g1) * L2 + g2);
+    };
+    if constexpr (UNROLL == 0) {
+        // all of the thread's loads (a wide workgroup's few) issued before
+        // any value is stored, a fixed count of them with the last predicated
+        constexpr int KB = (16 * BC + BS - 1) / BS;
+        Vec3<Real> v[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const int e = int(threadIdx.x) + k * BS;
+            if (e < 16 * BC) v[k] = ld3(Sr, site(e));
+        }
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const int e = int(threadIdx.x) + k * BS;
+            if (e < 16 * BC) {
+                const int le = e + (e / BC) * (LS - BC);
+                lx[le] = v[k].x;
+                ly[le] = v[k].y;
+                lz[le] = v[k].z;
+            }
+        }
+        return;
+    }
+    // UNROLL iterations at a time, so their loads are in flight together
+#pragma unroll UNROLL
+    for (int e = threadIdx.x; e < 16 * BC; e += BS) {
+        const Vec3<Real> v = ld3(Sr, site(e));
+        const int le = e + (e / BC) * (LS - BC);
         lx[le] = v.x;
         ly[le] = v.y;
         lz[le] = v.z;
@@ -1461,7 +1484,7 @@ __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real>
     const unsigned tiles = gridDim.x / unsigned(st.R);  // tiles per replica
     unsigned g = blockIdx.x;
     if ((gridDim.x & 7u) == 0u) g = (g & 7u) * (gridDim.x >> 3) + (g >> 3);
-    const int r = int(g / tiles);
+    const int r = __builtin_amdgcn_readfirstlane(int(g / tiles));  // uniform: the replica's loads are scalar
     const int tile = int(g - unsigned(r) * tiles);
     const int q1n = st.L1 / T1, q2n = st.L2 / T2;
     const int m2 = tile % q2n;
@@ -1469,11 +1492,13 @@ __global__ __launch_bounds__(BS) void pyro_energy_tiled_kernel(DeviceState<Real>
     const int m1 = t12 % q1n, m0 = t12 / q1n;
     const size_t N = size_t(st.N);
     const Real* __restrict__ Sr = st.S + size_t(r) * 3 * N;
-    load_tile_box<Real, T0, T1, T2, BS, LS, 0>(lx, ly, lz, Sr, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1, st.L2,
-                                           st.n_cells);
-    __syncthreads();
+    // the replica's couplings and field read before the box, so that their
+    // loads complete under its loads
     const PyroCouplings<Real> Jr = J.at(r);
     const Real bx = st.B[3 * r], by = st.B[3 * r + 1], bz = st.B[3 * r + 2];
+    load_tile_box<Real, T0, T1, T2, BS, LS, 0>(lx, ly, lz, Sr, m0 * T0, m1 * T1, m2 * T2, st.L0, st.L1, st.L2,
+                                               st.n_cells);
+    __syncthreads();
     double e = 0.0;
     constexpr int NI = T0 * T1 * T2, CH = (NI + 63) / 64, WAVES = BS / 64;
     for (int item = threadIdx.x >> 6; item < 16 * CH; item += WAVES) {
