@@ -934,6 +934,10 @@ __device__ __forceinline__ void add_correlators(const CorrArgs& c, int slot, siz
     }
 }
 
+// Sublattices whose values contract_kernel loads together (the pyrochlore
+// cell's 16); lattices with more take the rest one by one.
+constexpr int kMaxSl = 16;
+
 template <typename Real>
 __global__ __launch_bounds__(kContractBlock) void contract_kernel(SsfState<Real> st, CorrArgs c,
                                                                   int r_begin) {
@@ -946,33 +950,51 @@ __global__ __launch_bounds__(kContractBlock) void contract_kernel(SsfState<Real>
         const int K = full_k(st, kh);
         Real br[3], bi[3];
         const size_t nc = size_t(st.n_cells), nh = size_t(st.n_kh);
-        for (int a = 0; a < 3; ++a) {
-            Real sr = Real(0), si = Real(0);
-            const Real* F = st.F + 2 * ((size_t(rl) * 3 + a) * st.n_sl) * nh;
-            for (int mu = 0; mu < st.n_sl; ++mu) {
-                const Real fr = F[2 * (mu * nh + kh)], fi = F[2 * (mu * nh + kh) + 1];
-                const Real pr = st.phase[2 * (mu * nc + K)], pi = st.phase[2 * (mu * nc + K) + 1];
-                sr += fr * pr - fi * pi;
-                si += fr * pi + fi * pr;
+        // the first kMaxSl sublattices' phase and field values loaded
+        // together (those past n_sl read sublattice 0's and add nothing), any
+        // further sublattices one by one after them; the sums in mu order
+        const int ns = st.n_sl;
+        auto fold = [&](int Kp, Real sign_i) {
+            Real pr[kMaxSl], pi[kMaxSl];
+#pragma unroll
+            for (int mu = 0; mu < kMaxSl; ++mu) {
+                const size_t at = 2 * (size_t(mu < ns ? mu : 0) * nc + size_t(Kp));
+                pr[mu] = st.phase[at];
+                pi[mu] = st.phase[at + 1];
             }
-            br[a] = sr;
-            bi[a] = si;
-        }
-        add_correlators(c, slot, nc, K, br, bi);
-        if (mirror_direct(st, kh)) {
-            const int Kn = neg_full_k(st, K);
             for (int a = 0; a < 3; ++a) {
+                const Real* F = st.F + 2 * ((size_t(rl) * 3 + a) * ns) * nh;
+                Real fr[kMaxSl], fi[kMaxSl];
+#pragma unroll
+                for (int mu = 0; mu < kMaxSl; ++mu) {
+                    const size_t at = 2 * (size_t(mu < ns ? mu : 0) * nh + size_t(kh));
+                    fr[mu] = F[at];
+                    fi[mu] = F[at + 1];
+                }
                 Real sr = Real(0), si = Real(0);
-                const Real* F = st.F + 2 * ((size_t(rl) * 3 + a) * st.n_sl) * nh;
-                for (int mu = 0; mu < st.n_sl; ++mu) {
-                    const Real fr = F[2 * (mu * nh + kh)], fi = -F[2 * (mu * nh + kh) + 1];
-                    const Real pr = st.phase[2 * (mu * nc + Kn)], pi = st.phase[2 * (mu * nc + Kn) + 1];
-                    sr += fr * pr - fi * pi;
-                    si += fr * pi + fi * pr;
+#pragma unroll
+                for (int mu = 0; mu < kMaxSl; ++mu) {
+                    if (mu < ns) {
+                        const Real f_r = fr[mu], f_i = sign_i * fi[mu];
+                        sr += f_r * pr[mu] - f_i * pi[mu];
+                        si += f_r * pi[mu] + f_i * pr[mu];
+                    }
+                }
+                for (int mu = kMaxSl; mu < ns; ++mu) {
+                    const Real f_r = F[2 * (mu * nh + kh)], f_i = sign_i * F[2 * (mu * nh + kh) + 1];
+                    const Real p_r = st.phase[2 * (mu * nc + Kp)], p_i = st.phase[2 * (mu * nc + Kp) + 1];
+                    sr += f_r * p_r - f_i * p_i;
+                    si += f_r * p_i + f_i * p_r;
                 }
                 br[a] = sr;
                 bi[a] = si;
             }
+        };
+        fold(K, Real(1));
+        add_correlators(c, slot, nc, K, br, bi);
+        if (mirror_direct(st, kh)) {  // A(-K) = conj(A(K)) with the phases of -K
+            const int Kn = neg_full_k(st, K);
+            fold(Kn, Real(-1));
             add_correlators(c, slot, nc, Kn, br, bi);
         }
     }
@@ -1010,17 +1032,33 @@ __global__ __launch_bounds__(kBlock) void munu_stage_kernel(SsfState<Real> st,
     const int K0 = int(blockIdx.x) * 64;
     const int nf = 3 * st.n_sl;
     const int kl = threadIdx.x & 63;
-    for (int f = int(threadIdx.x >> 6); f < 16 * T; f += kBlock / 64) {
-        float re = 0.f, im = 0.f;
+    {
+        // a wave's fields f = w, w + 4, ...: every load issued before any
+        // value is stored, so that they are in flight together
+        constexpr int NFW = 16 * T / (kBlock / 64);
+        static_assert(16 * T % (kBlock / 64) == 0, "whole fields per wave");
+        // the values kept in their own precision until stored, so that no
+        // conversion waits on its load ahead of the next load
+        Real re[NFW], im[NFW];
         const int K = K0 + kl;
-        if (f < nf && K < nc) {
-            const int a = f / st.n_sl, mu = f % st.n_sl;
-            const Real* F = st.F + 2 * (((size_t(rl) * 3 + a) * st.n_sl + mu) * size_t(nc) + K);
-            re = float(F[0]);
-            im = float(F[1]);
+#pragma unroll
+        for (int i = 0; i < NFW; ++i) {
+            const int f = int(threadIdx.x >> 6) + i * (kBlock / 64);
+            re[i] = Real(0);
+            im[i] = Real(0);
+            if (f < nf && K < nc) {
+                const int a = f / st.n_sl, mu = f % st.n_sl;
+                const Real* F = st.F + 2 * (((size_t(rl) * 3 + a) * st.n_sl + mu) * size_t(nc) + K);
+                re[i] = F[0];
+                im[i] = F[1];
+            }
         }
-        xs[f][kl] = re;
-        xs[16 * T + f][kl] = im;
+#pragma unroll
+        for (int i = 0; i < NFW; ++i) {
+            const int f = int(threadIdx.x >> 6) + i * (kBlock / 64);
+            xs[f][kl] = float(re[i]);
+            xs[16 * T + f][kl] = float(im[i]);
+        }
     }
     __syncthreads();
     const int nk = min(64, nc - K0);
