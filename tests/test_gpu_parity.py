@@ -635,6 +635,38 @@ def test_ssf_contracted(gpu, oracle, prec):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
+def test_ssf_contracted_beyond_sixteen_sublattices(gpu, prec):
+    """A cell of 20 sublattices: more than the 16 whose values the
+    contraction loads together (the rest go one by one), and in FP32 more
+    than the staged kernel's 3 x 16 features, so the generic contraction
+    runs.  The correlators against conj(b^a) b^b with
+    b^a(K) = sum_mu e^{-i q(K).r_mu} A^a_mu(K), from the per-sublattice
+    transform A, q(K) = 2 pi K_c / (cell L) for the centred index K_c."""
+    n_sl, cell, L = 20, np.array([4, 4, 4]), np.array([3, 4, 5])
+    pos = np.array([(i % 4, (i // 4) % 4, i // 16) for i in range(n_sl)])
+    ctx = gpu.Context(pos, [0] * n_sl, cell, L, n_replicas=1, precision=prec, seed=5)
+    ctx.define_coupling("J1", [[(4, 0, 0), (-4, 0, 0)]], np.eye(3))
+    ctx.setup()
+    corr = ("xx", "xy", "zy", "yz")
+    ctx.ssf_configure(corr, n_slots=1, store_error_term=False)
+    ctx.set_spins(random_spins(ctx.N, np.random.default_rng(20)), 0)
+    ctx.ssf_sample([0])
+    S, _, n = ctx.ssf_read(0, with_sq=False)
+    assert n == 1
+    ft = ctx.spin_ft(0)
+    A = ft[..., 0] + 1j * ft[..., 1]  # [a][mu][K]
+    k = np.stack(np.meshgrid(*[np.arange(l) for l in L], indexing="ij"), -1).reshape(-1, 3)
+    kc = np.where(2 * k < L, k, k - L)
+    arg = 2 * np.pi * (kc / (cell * L)) @ pos.T  # [K][mu]
+    b = np.einsum("km,amk->ak", np.exp(-1j * arg), A)
+    ax = {"x": 0, "y": 1, "z": 2}
+    for c, (al, be) in enumerate(corr):
+        want = np.conj(b[ax[al]]) * b[ax[be]]
+        assert np.max(np.abs(S[c, :, 0] - want.real)) <= TOL[prec] * ctx.N ** 2 * 3
+        assert np.max(np.abs(S[c, :, 1] - want.imag)) <= TOL[prec] * ctx.N ** 2 * 3
+
+
+@pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("L", [4, 5, 8, 12, 16])
 def test_spin_ft_against_direct_dft(gpu, oracle, prec, L):
     """Per-sublattice DFT (save_ft_spin_state layout): through the per-axis
